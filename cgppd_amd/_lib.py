@@ -1,0 +1,92 @@
+"""ctypes binding of libcgppd_hip.so (include/cgppd_hip.h).
+
+The product path has no CPU fallback: if the library is missing, loading
+raises.  Build it with ``python -m cgppd_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+HIP_LIB_PATH = _PKG / "libcgppd_hip.so"
+
+_i32p = C.POINTER(C.c_int)
+_i64p = C.POINTER(C.c_int64)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_ctxp = C.c_void_p
+
+# name -> (restype, argtypes); mirrors include/cgppd_hip.h
+SIGNATURES = {
+    "cgppd_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "cgppd_destroy": (C.c_int, [_ctxp]),
+    "cgppd_last_error": (C.c_char_p, [_ctxp]),
+    "cgppd_device_count": (C.c_int, [_i32p]),
+    "cgppd_set_force_field": (C.c_int, [_ctxp, _f32p, _f32p, _f32p]),
+    "cgppd_set_box": (C.c_int, [_ctxp, C.c_float]),
+    "cgppd_set_system": (C.c_int, [_ctxp, C.c_int, _i32p, _i32p, C.c_int]),
+    "cgppd_set_replicas": (C.c_int, [_ctxp, C.c_int, _f32p, _f32p, _f64p, _f32p, _f32p, _i32p, C.c_uint64]),
+    "cgppd_upload_positions": (C.c_int, [_ctxp, C.c_int, _f32p]),
+    "cgppd_update_molecule": (C.c_int, [_ctxp, C.c_int, C.c_int, _f32p]),
+    "cgppd_energy": (C.c_int, [_ctxp, C.c_int, _f64p, _f64p, _f64p]),
+    "cgppd_energy_nc": (C.c_int, [_ctxp, C.c_int, _f64p, _f64p, _f64p]),
+    "cgppd_energy_async": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_energy_wait": (C.c_int, [_ctxp, C.c_int, _f64p, _f64p, _f64p]),
+    "cgppd_energy_all": (C.c_int, [_ctxp, _f64p, _f64p, _f64p, _f64p]),
+    "cgppd_mc_prepare": (C.c_int, [_ctxp]),
+    "cgppd_mc_steps": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_exchange": (C.c_int, [_ctxp]),
+    "cgppd_run": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int]),
+    "cgppd_synchronize": (C.c_int, [_ctxp]),
+    "cgppd_get_replica_stats": (C.c_int, [_ctxp, _f64p, _f64p, _i32p, _i64p, _i64p, _i64p]),
+    "cgppd_get_pair_evaluations": (C.c_int, [_ctxp, _i64p]),
+    "cgppd_get_coordinates": (C.c_int, [_ctxp, C.c_int, _f32p, _f32p, _f32p]),
+    "cgppd_get_energy_matrix": (C.c_int, [_ctxp, C.c_int, _f64p]),
+    "cgppd_set_launch": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_set_timing": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_last_kernel_ms": (C.c_int, [_ctxp, _f64p, _i64p]),
+    "cgppd_model_constants": (C.c_int, [_f32p, _f32p, _f32p]),
+    "cgppd_get_sizes": (C.c_int, [_ctxp, _i32p, _i32p, _i32p]),
+}
+
+_lib = None
+
+
+def load(path: Path | None = None):
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else HIP_LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built; run `python -m cgppd_amd.build` (no CPU fallback exists)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

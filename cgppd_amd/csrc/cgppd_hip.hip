@@ -1,0 +1,1497 @@
+// cgppd_hip.hip -- gfx950 kernels and C ABI for cgppd's REMC energy hot path.
+//
+// Design (one GPU, HBM-resident replicas):
+//   * Topology (molecule offsets, residue types) is shared by all replicas;
+//     per replica the device holds float4 positions {x,y,z,code}, float4
+//     relative positions, float4 molecule centres, the molecule-pair energy
+//     matrix (FP64, kcal/mol), the running total, temperature, step sizes,
+//     move counters and the Philox step counter.
+//   * One persistent pair kernel serves both the full energy (rows I vs all
+//     J>I) and the MC step (row of the moved molecule vs all J).  Work items
+//     are (replica, i-block of 64 residues, j-chunk of 256 residues); a 256-
+//     thread workgroup stages the j-chunk and the 20x20 pair table in LDS,
+//     each wave owns 64 i residues (one per lane) and 64 of the j residues,
+//     and every j is read as a wave-uniform LDS broadcast.  Pair terms are
+//     FP32 with Kahan compensation per lane; each (i-block, molecule J)
+//     segment is reduced across the wave in FP64 and added atomically to the
+//     per-replica row buffer.
+//   * MC steps are decided on the device: every workgroup re-derives the
+//     replica's proposal from the counter-based Philox stream, the workgroup
+//     completing a replica's last item performs Metropolis in FP64, applies
+//     the move and updates the matrix row/column.  A sweep is one launch per
+//     MC step; an exchange interval is captured once in a hipGraph.
+//   * No MFMA: the pair sum is a table-lookup n-body reduction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cgppd_hip.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerWG = 4;
+constexpr int kWG = kWave * kWavesPerWG;  // threads per workgroup
+constexpr int kJChunk = kWG;              // j residues staged per item
+constexpr int kIBlock = kWave;            // i residues per item (one per lane)
+constexpr int kRowShift = 9;              // LDS table row stride: 32 entries x 16 B
+constexpr int kColShift = 4;              // 16-byte entries
+constexpr int kTableBytes = CGPPD_AA_COUNT << kRowShift;
+constexpr uint32_t kRowMask = 0x1Fu << kRowShift;
+constexpr uint32_t kChargedBit = 0x80000000u;
+constexpr uint32_t kTagMC = 0x4D430000u;
+constexpr uint32_t kTagEX = 0x45580000u;
+
+// Model constants, as the reference evaluates them in float
+// (src/constants.h:8-26): T_room*Rgas/kcal and e^2 N_A/(4 pi eps0 kcal Ang D).
+constexpr float kRgas = 8.3144621f;
+constexpr float kKcal = 4184.0f;
+constexpr float kLambda = 0.159f;
+constexpr float kE0 = -2.27f;
+// exp(-r/xi) = exp2(r * kDHExp2), xi = 10 A
+constexpr float kDHExp2 = -0.14426950408889634f;
+
+struct Coef {
+    float sig2;  // sigma_ij^2
+    float A;     // -4 E_ij (raw RT) or -4 E_ij * RT_to_kcal (scaled)
+    float rc2;   // (sigma_ij * 2^(1/6))^2 when E_ij > 0, else -1
+    float qq;    // q_i q_j (raw) or q_i q_j * DH factor (scaled)
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                       uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        uint32_t lo0 = 0xD2511F53u * c0;
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+__device__ __forceinline__ double unit32(uint32_t w) { return (double)w * (1.0 / 4294967296.0); }
+__device__ __forceinline__ double unit53(uint32_t hi, uint32_t lo)
+{
+    return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+struct Proposal {
+    int mol;
+    int rotate;
+    double axis[3];  // raw (u-0.5, 0.5-u, u-0.5)
+    double u;        // Metropolis uniform
+};
+
+// The proposal of MC step `step` of replica r.  Molecule pick as
+// src/Replica.cpp:355, Bernoulli(0.5) translate/rotate as
+// src/Molecule.cpp:461, axis draw as src/Molecule.cpp:210-221.
+__device__ __forceinline__ Proposal draw_proposal(uint64_t seed, int r, int64_t step, int n_mol)
+{
+    uint32_t w[4], v[4];
+    const uint32_t s_lo = (uint32_t)step, s_hi = (uint32_t)((uint64_t)step >> 32);
+    philox(s_lo, s_hi, (uint32_t)r, kTagMC, (uint32_t)seed, (uint32_t)(seed >> 32), w);
+    philox(s_lo, s_hi, (uint32_t)r, kTagMC | 1u, (uint32_t)seed, (uint32_t)(seed >> 32), v);
+    Proposal p;
+    p.mol = (int)(((uint64_t)w[0] * (uint64_t)n_mol) >> 32);
+    p.rotate = w[1] < 0x80000000u ? 1 : 0;
+    p.axis[0] = unit32(v[0]) - 0.5;
+    p.axis[1] = 0.5 - unit32(v[1]);
+    p.axis[2] = unit32(v[2]) - 0.5;
+    p.u = unit53(w[2], w[3]);
+    return p;
+}
+
+struct Move {
+    int rotate;
+    float c[3];   // new centre
+    double q[4];  // normalised quaternion (w, x, y, z) for rotations
+};
+
+// Builds the rigid move from a proposal.  Translation: src/Molecule.cpp:262-266
+// (float unit vector scaled by the step, src/vector3f.cpp:58-62,75-78), new
+// centre wrapped with fmod (src/Molecule.cpp:146-159,234-241).  Rotation:
+// src/Molecule.cpp:193-208,216-221 with src/Quaternion.cpp:12-38.  No FMA
+// contraction, so the result is bit-identical to the CPU restatement.
+__device__ Move make_move(const Proposal &p, float4 c_old, float tstep, double rcos, double rsin, float box)
+{
+#pragma clang fp contract(off)
+    Move mv;
+    mv.rotate = p.rotate;
+    if (!p.rotate) {
+        float n0 = (float)p.axis[0], n1 = (float)p.axis[1], n2 = (float)p.axis[2];
+        float mag = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+        if (mag != 0.0f) {
+            n0 = n0 / mag;
+            n1 = n1 / mag;
+            n2 = n2 / mag;
+        }
+        mv.c[0] = fmodf(c_old.x + (float)((double)n0 * (double)tstep), box);
+        mv.c[1] = fmodf(c_old.y + (float)((double)n1 * (double)tstep), box);
+        mv.c[2] = fmodf(c_old.z + (float)((double)n2 * (double)tstep), box);
+        mv.q[0] = 1.0;
+        mv.q[1] = mv.q[2] = mv.q[3] = 0.0;
+    } else {
+        double ax = p.axis[0], ay = p.axis[1], az = p.axis[2];
+        double m = sqrt(ax * ax + ay * ay + az * az);
+        ax /= m;
+        ay /= m;
+        az /= m;
+        double w = rcos, x = rsin * ax, y = rsin * ay, z = rsin * az;
+        double qm = sqrt(w * w + x * x + y * y + z * z);
+        mv.q[0] = w / qm;
+        mv.q[1] = x / qm;
+        mv.q[2] = y / qm;
+        mv.q[3] = z / qm;
+        mv.c[0] = c_old.x;
+        mv.c[1] = c_old.y;
+        mv.c[2] = c_old.z;
+    }
+    return mv;
+}
+
+// New relative and absolute position of one residue under the move:
+// src/Quaternion.cpp:95-112 and src/Residue.cpp:93-99 for rotations,
+// src/Molecule.cpp:183-191 for translations.
+__device__ void move_residue(const Move &mv, float4 rel, float4 &rel_new, float4 &pos_new)
+{
+#pragma clang fp contract(off)
+    if (!mv.rotate) {
+        rel_new = rel;
+        pos_new.x = rel.x + mv.c[0];
+        pos_new.y = rel.y + mv.c[1];
+        pos_new.z = rel.z + mv.c[2];
+    } else {
+        const double w = mv.q[0], x = mv.q[1], y = mv.q[2], z = mv.q[3];
+        const double w2 = w * w, x2 = x * x, y2 = y * y, z2 = z * z;
+        const double wx = w * x, wy = w * y, wz = w * z, xy = x * y, xz = x * z, yz = y * z;
+        const double vx = rel.x, vy = rel.y, vz = rel.z;
+        rel_new.x = (float)(vx * (w2 + x2 - y2 - z2) + (vy * (xy - wz) + vz * (wy + xz)) * 2.0);
+        rel_new.y = (float)((vx * (wz + xy) + vz * (yz - wx)) * 2.0 + vy * (w2 - x2 + y2 - z2));
+        rel_new.z = (float)((vx * (xz - wy) + vy * (yz + wx)) * 2.0 + vz * (w2 - x2 - y2 + z2));
+        rel_new.w = rel.w;
+        pos_new.x = mv.c[0] + rel_new.x;
+        pos_new.y = mv.c[1] + rel_new.y;
+        pos_new.z = mv.c[2] + rel_new.z;
+    }
+    pos_new.w = rel.w;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel parameters
+// ---------------------------------------------------------------------------
+struct PairParams {
+    // topology
+    int n_mol, n_res, stride;
+    int n_active;       // molecules taking part (n_mol, or n_noncrowder)
+    int ib_max;         // i-blocks of the largest molecule
+    int jc_count;       // j-chunks over n_res
+    const int *mol_off;  // n_mol + 1
+    const int *mol_of;   // n_res
+    // replicas
+    int rep_begin, rep_count;
+    int xcd_map;  // items of replica r stay on XCD r % 8
+    const Coef *table;   // 20 x 20 entries, row = j type
+    float box, inv_box;
+    float4 *pos;         // R * stride
+    float4 *rel;         // R * stride
+    float4 *center;      // R * n_mol
+    // full-energy outputs (raw LJ/DH)
+    double *out_lj, *out_dh;  // R
+    double *out_mat;          // R * M * M (may be null)
+    float rt_to_kcal, dh_factor;
+    // MC state
+    uint64_t seed;
+    double *rowbuf;       // R * M
+    unsigned *done;       // R
+    double *emat;         // R * M * M
+    double *energy;       // R
+    const double *temperature;
+    const float *tstep;
+    const double *rcos, *rsin;
+    long long *counters;  // R * 3
+    long long *steps;     // R
+    unsigned long long *pair_evals;  // 1: inter-molecular residue pairs evaluated by MC steps
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v)
+{
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Loads the 20x20 table into LDS with the row/column layout used by the
+// inner loop: byte offset (type_j << 9) | (type_i << 4).
+__device__ __forceinline__ void load_table(const Coef *__restrict__ g, char *lds)
+{
+    for (int e = threadIdx.x; e < CGPPD_AA_COUNT * CGPPD_AA_COUNT; e += kWG) {
+        const int tj = e / CGPPD_AA_COUNT, ti = e % CGPPD_AA_COUNT;
+        *reinterpret_cast<Coef *>(lds + ((tj << kRowShift) | (ti << kColShift))) = g[e];
+    }
+}
+
+// Inner loop over j in [js, je) (wave-uniform) for this lane's residue i.
+// SEPARATE: LJ and DH accumulated apart (raw table); otherwise one total.
+template <bool SEPARATE>
+__device__ __forceinline__ void pair_segment(const float4 *__restrict__ jrec, int js, int je, float xi, float yi,
+                                             float zi, uint32_t lane_off, const char *__restrict__ tab, float box,
+                                             float inv_box, float &s0, float &c0, float &s1, float &c1)
+{
+    for (int j = js; j < je; ++j) {
+        const float4 rj = jrec[j];
+        const uint32_t code = __float_as_uint(rj.w);
+        float dx = xi - rj.x;
+        float dy = yi - rj.y;
+        float dz = zi - rj.z;
+        dx = fmaf(-box, rintf(dx * inv_box), dx);
+        dy = fmaf(-box, rintf(dy * inv_box), dy);
+        dz = fmaf(-box, rintf(dz * inv_box), dz);
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const Coef cf = *reinterpret_cast<const Coef *>(tab + ((code & kRowMask) | lane_off));
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float rinv2 = rinv * rinv;
+        const float s2 = cf.sig2 * rinv2;
+        const float s6 = s2 * s2 * s2;
+        float lj = fmaf(cf.A, s6, -cf.A) * s6;
+        const float rep = fmaf(cf.A, -0.5f, -lj);
+        lj = r2 < cf.rc2 ? rep : lj;
+        if (SEPARATE) {
+            kahan_add(s0, c0, lj);
+            if (code & kChargedBit) {
+                const float r = r2 * rinv;
+                const float dh = cf.qq * __builtin_amdgcn_exp2f(r * kDHExp2) * rinv;
+                kahan_add(s1, c1, dh);
+            }
+        } else {
+            float e = lj;
+            if (code & kChargedBit) {
+                const float r = r2 * rinv;
+                e += cf.qq * __builtin_amdgcn_exp2f(r * kDHExp2) * rinv;
+            }
+            kahan_add(s0, c0, e);
+        }
+    }
+}
+
+// Maps a persistent workgroup's k-th item to (replica, item-in-replica).
+// With xcd_map, workgroup b runs on XCD b % 8 and only takes items of
+// replicas r with r % 8 == b % 8, so a replica's coordinates live in one L2.
+struct ItemIter {
+    int k, step, count, per_rep, xcd, base;
+    __device__ ItemIter(const PairParams &p, int per_rep_)
+    {
+        per_rep = per_rep_;
+        const int G = gridDim.x, b = blockIdx.x;
+        if (p.xcd_map) {
+            xcd = b & 7;
+            k = b >> 3;
+            step = G >> 3;
+            count = (p.rep_count >> 3) * per_rep;
+        } else {
+            xcd = -1;
+            k = b;
+            step = G;
+            count = p.rep_count * per_rep;
+        }
+        base = p.rep_begin;
+    }
+    __device__ bool valid() const { return k < count; }
+    __device__ void next() { k += step; }
+    __device__ int replica() const { return base + (xcd < 0 ? k / per_rep : xcd + 8 * (k / per_rep)); }
+    __device__ int item() const { return k % per_rep; }
+};
+
+// ---------------------------------------------------------------------------
+// Full energy: rows I of every replica in [rep_begin, rep_begin+rep_count)
+// against all J > I (J < n_active).  SCALED: single total in kcal/mol into
+// the MC matrix; otherwise raw LJ/DH totals (+ optional matrix).
+// ---------------------------------------------------------------------------
+template <bool SCALED>
+__global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
+{
+    __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
+    __shared__ float4 jrec[kJChunk];
+    load_table(p.table, tab);
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int per_rep = p.n_active * p.ib_max * p.jc_count;
+    for (ItemIter it(p, per_rep); it.valid(); it.next()) {
+        const int r = it.replica();
+        int q = it.item();
+        const int jc = q % p.jc_count;
+        q /= p.jc_count;
+        const int ib = q % p.ib_max;
+        const int I = q / p.ib_max;
+        const int i_beg = p.mol_off[I], i_end = p.mol_off[I + 1];
+        const int j_lo = i_end, j_hi = p.mol_off[p.n_active];
+        const int j0 = max(jc * kJChunk, j_lo), j1 = min((jc + 1) * kJChunk, j_hi);
+        if (i_beg + ib * kIBlock >= i_end || j0 >= j1) continue;  // uniform
+
+        const float4 *pos = p.pos + (size_t)r * p.stride;
+        __syncthreads();  // previous item finished reading jrec
+        {
+            const int j = jc * kJChunk + threadIdx.x;
+            if (j < p.n_res) jrec[threadIdx.x] = pos[j];
+        }
+        const int i = i_beg + ib * kIBlock + lane;
+        const bool valid = i < i_end;
+        const float4 pi = pos[valid ? i : i_beg];
+        const uint32_t lane_off = (__float_as_uint(pi.w) >> kRowShift & 0x1Fu) << kColShift;
+        __syncthreads();
+
+        int js = max(j0, jc * kJChunk + wave * kWave);
+        const int je = min(j1, jc * kJChunk + (wave + 1) * kWave);
+        while (js < je) {
+            const int J = p.mol_of[js];
+            const int se = min(je, p.mol_off[J + 1]);
+            float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+            pair_segment<!SCALED>(jrec, js - jc * kJChunk, se - jc * kJChunk, pi.x, pi.y, pi.z, lane_off, tab, p.box,
+                                  p.inv_box, s0, c0, s1, c1);
+            double v0 = valid ? (double)s0 - (double)c0 : 0.0;
+            double v1 = valid ? (double)s1 - (double)c1 : 0.0;
+            v0 = wave_sum(v0);
+            if (!SCALED) v1 = wave_sum(v1);
+            if (lane == 0) {
+                const size_t mm = (size_t)p.n_mol * p.n_mol;
+                if (SCALED) {
+                    atomicAdd(p.emat + r * mm + (size_t)I * p.n_mol + J, v0);
+                } else {
+                    atomicAdd(p.out_lj + r, v0);
+                    atomicAdd(p.out_dh + r, v1);
+                    if (p.out_mat)
+                        atomicAdd(p.out_mat + r * mm + (size_t)I * p.n_mol + J,
+                                  v0 * (double)p.rt_to_kcal + v1 * (double)p.dh_factor);
+                }
+            }
+            js = se;
+        }
+    }
+}
+
+
+// Completes the MC step of replica r once all its items are in: Metropolis on
+// the delta-E row (src/Replica.cpp:385-410, delta and exponent in FP64),
+// apply the move (src/Molecule.cpp:468-504) and update the matrix row and
+// column and the running total.  Runs on one whole workgroup.
+__device__ void finish_mc_step(const PairParams &p, int r, const Proposal &pr, const Move &mv, int64_t step,
+                               int i_beg, int i_end, double *red, int *accept_flag)
+{
+    const int M = p.n_mol, m = pr.mol;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    double *row = p.rowbuf + (size_t)r * M;
+    double *mat = p.emat + (size_t)r * M * M;
+    double d = 0.0;
+    for (int J = threadIdx.x; J < M; J += kWG)
+        if (J != m) d += row[J] - mat[(size_t)m * M + J];
+    d = wave_sum(d);
+    if (lane == 0) red[wave] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double delta = 0.0;
+        for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
+        long long *cnt = p.counters + 3 * (size_t)r;
+        int acc = 0;
+        if (delta < 0.0) {
+            acc = 1;
+            cnt[0] += 1;
+        } else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r]))) {
+            acc = 1;
+            cnt[1] += 1;
+        } else {
+            cnt[2] += 1;
+        }
+        if (acc) p.energy[r] += delta;
+        *accept_flag = acc;
+    }
+    __syncthreads();
+    if (*accept_flag) {
+        for (int J = threadIdx.x; J < M; J += kWG) {
+            if (J == m) continue;
+            const double v = row[J];
+            mat[(size_t)m * M + J] = v;
+            mat[(size_t)J * M + m] = v;
+        }
+        float4 *pos = p.pos + (size_t)r * p.stride;
+        float4 *rel = p.rel + (size_t)r * p.stride;
+        for (int i = i_beg + threadIdx.x; i < i_end; i += kWG) {
+            float4 rn, pn;
+            move_residue(mv, rel[i], rn, pn);
+            rel[i] = rn;
+            pos[i] = pn;
+        }
+        if (threadIdx.x == 0) p.center[(size_t)r * M + m] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+    }
+    for (int J = threadIdx.x; J < M; J += kWG) row[J] = 0.0;
+    if (threadIdx.x == 0) {
+        p.done[r] = 0u;
+        p.steps[r] = step + 1;
+        const unsigned long long n_m = (unsigned long long)(i_end - i_beg);
+        atomicAdd(p.pair_evals, n_m * ((unsigned long long)p.n_res - n_m));
+    }
+    __syncthreads();
+}
+
+// One MC step of every replica.  Items: (replica, i-block of the moved
+// molecule, j-chunk); inactive i-blocks (beyond the moved molecule) are
+// skipped without counting.
+__global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
+{
+    __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
+    __shared__ float4 jrec[kJChunk];
+    __shared__ double red[kWavesPerWG];
+    __shared__ int last_flag, accept_flag;
+    load_table(p.table, tab);
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int per_rep = p.ib_max * p.jc_count;
+    for (ItemIter it(p, per_rep); it.valid(); it.next()) {
+        const int r = it.replica();
+        const int q = it.item();
+        const int jc = q % p.jc_count, ib = q / p.jc_count;
+        const int64_t step = p.steps[r];
+        const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
+        const int m = pr.mol;
+        const int i_beg = p.mol_off[m], i_end = p.mol_off[m + 1];
+        const int n_ib = (i_end - i_beg + kIBlock - 1) / kIBlock;
+        if (ib >= n_ib) continue;  // uniform
+
+        const float4 *pos = p.pos + (size_t)r * p.stride;
+        const float4 *rel = p.rel + (size_t)r * p.stride;
+        const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + m], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
+        const int jbase = jc * kJChunk;
+        __syncthreads();  // previous item finished reading jrec
+        {
+            const int j = jbase + threadIdx.x;
+            if (j < p.n_res) jrec[threadIdx.x] = pos[j];
+        }
+        const int i = i_beg + ib * kIBlock + lane;
+        const bool valid = i < i_end;
+        float4 rn, pn;
+        move_residue(mv, rel[valid ? i : i_beg], rn, pn);
+        const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
+        __syncthreads();
+
+        int js = jbase + wave * kWave;
+        const int je = min(p.n_res, jbase + (wave + 1) * kWave);
+        while (js < je) {
+            const int J = p.mol_of[js];
+            const int se = min(je, p.mol_off[J + 1]);
+            if (J != m) {
+                float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+                pair_segment<false>(jrec, js - jbase, se - jbase, pn.x, pn.y, pn.z, lane_off, tab, p.box, p.inv_box,
+                                    s0, c0, s1, c1);
+                double v = valid ? (double)s0 - (double)c0 : 0.0;
+                v = wave_sum(v);
+                if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
+            }
+            js = se;
+        }
+        // publish this item, then let the workgroup that completes the
+        // replica's last item finish the step
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned old = atomicAdd(p.done + r, 1u);
+            last_flag = old == (unsigned)(n_ib * p.jc_count - 1);
+        }
+        __syncthreads();
+        if (last_flag) {
+            __threadfence();
+            finish_mc_step(p, r, pr, mv, step, i_beg, i_end, red, &accept_flag);
+        }
+    }
+}
+
+// After the scaled full-energy pass: mirror the upper triangle of each
+// replica's matrix and set the running total to the sum over I < J.
+__global__ __launch_bounds__(kWG) void prepare_finish_kernel(PairParams p)
+{
+    __shared__ double red[kWavesPerWG];
+    const int r = blockIdx.x;
+    const int M = p.n_mol;
+    double *mat = p.emat + (size_t)r * M * M;
+    double e = 0.0;
+    for (int k = threadIdx.x; k < M * M; k += kWG) {
+        const int I = k / M, J = k % M;
+        if (I < J) {
+            const double v = mat[k];
+            mat[(size_t)J * M + I] = v;
+            e += v;
+        } else if (I == J) {
+            mat[k] = 0.0;
+        }
+    }
+    e = wave_sum(e);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kWavesPerWG; ++w) t += red[w];
+        p.energy[r] = t;
+        p.done[r] = 0u;
+    }
+    for (int J = threadIdx.x; J < M; J += kWG) p.rowbuf[(size_t)r * M + J] = 0.0;
+}
+
+struct ExchangeParams {
+    int n_rep;
+    uint64_t seed;
+    double *temperature;
+    const double *energy;
+    int *label;
+    int *rank_of;
+    int *rep_at_rank;
+    long long *counters;
+    long long *round;
+    unsigned long long *stats;  // tests, accepted
+};
+
+// One exchange round: src/Simulation.cpp:770-795 (pairs of adjacent
+// temperature ranks starting at the round's parity, delta and min(1, e^delta)
+// in FP64) and src/Replica.cpp:188-210 (temperature, label and statistics
+// swap, coordinates stay).  The pairs of one round are disjoint, so lanes
+// take them in parallel.
+__global__ __launch_bounds__(kWave) void exchange_kernel(ExchangeParams e)
+{
+    const long long k = *e.round;
+    const int offset = (int)(k & 1);
+    const float factor = kKcal / kRgas;
+    unsigned long long tests = 0, accepted = 0;
+    for (int ti = offset + 2 * (int)threadIdx.x; ti < e.n_rep - 1; ti += 2 * kWave) {
+        const int i = e.rep_at_rank[ti], j = e.rep_at_rank[ti + 1];
+        const double delta =
+            (1.0 / e.temperature[i] - 1.0 / e.temperature[j]) * (e.energy[i] - e.energy[j]) * (double)factor;
+        uint32_t w[4];
+        philox((uint32_t)k, (uint32_t)((unsigned long long)k >> 32), (uint32_t)ti, kTagEX, (uint32_t)e.seed,
+               (uint32_t)(e.seed >> 32), w);
+        const double u = unit53(w[0], w[1]);
+        double pacc = exp(delta);
+        if (pacc > 1.0) pacc = 1.0;
+        ++tests;
+        if (u < pacc) {
+            const double t = e.temperature[i];
+            e.temperature[i] = e.temperature[j];
+            e.temperature[j] = t;
+            const int l = e.label[i];
+            e.label[i] = e.label[j];
+            e.label[j] = l;
+            for (int c = 0; c < 3; ++c) {
+                const long long a = e.counters[3 * i + c];
+                e.counters[3 * i + c] = e.counters[3 * j + c];
+                e.counters[3 * j + c] = a;
+            }
+            e.rank_of[i] = ti + 1;
+            e.rank_of[j] = ti;
+            e.rep_at_rank[ti] = j;
+            e.rep_at_rank[ti + 1] = i;
+            ++accepted;
+        }
+    }
+    if (tests) {
+        atomicAdd(e.stats, tests);
+        atomicAdd(e.stats + 1, accepted);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *e.round = k + 1;
+}
+
+__global__ void fill_zero_kernel(double *p, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        p[k] = 0.0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct cgppd_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    bool have_ff = false, have_box = false, have_sys = false, have_rep = false, mc_ready = false;
+    float eps[400], sigma[20], charge[20];
+    float box = 0.0f;
+    float rt_to_kcal = 0.0f, dh_factor = 0.0f, r0f = 0.0f;
+
+    int M = 0, N = 0, stride = 0, n_nc = 0, max_n = 0;
+    std::vector<int> mol_off, type, code;
+    int R = 0;
+    uint64_t seed = 0;
+
+    Coef *d_table_raw = nullptr, *d_table_scaled = nullptr;
+    int *d_mol_off = nullptr, *d_mol_of = nullptr;
+    float4 *d_pos = nullptr, *d_rel = nullptr, *d_center = nullptr;
+    double *d_rowbuf = nullptr, *d_emat = nullptr, *d_energy = nullptr, *d_temp = nullptr;
+    double *d_rcos = nullptr, *d_rsin = nullptr;
+    float *d_tstep = nullptr;
+    unsigned *d_done = nullptr;
+    long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
+    unsigned long long *d_ex_stats = nullptr;
+    unsigned long long *d_pair_evals = nullptr;
+    int *d_label = nullptr, *d_rank_of = nullptr, *d_rep_at_rank = nullptr;
+    double *d_out = nullptr;      // 2R: lj, dh
+    double *d_out_mat = nullptr;  // R*M*M
+    double *h_out = nullptr;      // pinned 2R
+
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+
+    int wg_per_cu = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    double t_mean_ms = 0.0;
+    long long t_launches = 0;
+};
+
+namespace {
+
+int fail(cgppd_ctx *ctx, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define CG_HIP(ctx, expr)                                                                               \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? CGPPD_E_NOMEM : CGPPD_E_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                              \
+    } while (0)
+
+template <typename T>
+int dev_alloc(cgppd_ctx *ctx, T **p, size_t n)
+{
+    if (*p) {
+        CG_HIP(ctx, hipFree(*p));
+        *p = nullptr;
+    }
+    if (n == 0) n = 1;
+    CG_HIP(ctx, hipMalloc((void **)p, n * sizeof(T)));
+    return CGPPD_OK;
+}
+
+template <typename T>
+void dev_free(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void model_constants(float *rt, float *dhc, float *r0)
+{
+    // src/constants.h:8-26, evaluated in float as the reference does
+    const float e_charge = 1.602176487e-19f, n_a = 6.02214129e23f, eps0 = 8.85418782e-12f;
+    const float ang = 1e-10f, d_water = 80.0f, t_room = 298.0f;
+    const float pi_f = (float)3.14159265358979323846264338327950288L;
+    *rt = t_room * kRgas / kKcal;
+    const float e2 = e_charge * e_charge;
+    *dhc = e2 * n_a / (4.0f * pi_f * eps0 * kKcal * ang * d_water);
+    *r0 = (float)std::pow(2.0, (double)(1.0f / 6.0f));
+}
+
+void destroy_graph(cgppd_ctx *ctx)
+{
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_steps = 0;
+}
+
+void free_replicas(cgppd_ctx *ctx)
+{
+    destroy_graph(ctx);
+    dev_free(ctx->d_pos);
+    dev_free(ctx->d_rel);
+    dev_free(ctx->d_center);
+    dev_free(ctx->d_rowbuf);
+    dev_free(ctx->d_emat);
+    dev_free(ctx->d_energy);
+    dev_free(ctx->d_temp);
+    dev_free(ctx->d_rcos);
+    dev_free(ctx->d_rsin);
+    dev_free(ctx->d_tstep);
+    dev_free(ctx->d_done);
+    dev_free(ctx->d_counters);
+    dev_free(ctx->d_steps);
+    dev_free(ctx->d_ex_round);
+    dev_free(ctx->d_ex_stats);
+    dev_free(ctx->d_pair_evals);
+    dev_free(ctx->d_label);
+    dev_free(ctx->d_rank_of);
+    dev_free(ctx->d_rep_at_rank);
+    dev_free(ctx->d_out);
+    dev_free(ctx->d_out_mat);
+    if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+    ctx->h_out = nullptr;
+    ctx->have_rep = false;
+    ctx->mc_ready = false;
+    ctx->R = 0;
+}
+
+int build_tables(cgppd_ctx *ctx)
+{
+    std::vector<Coef> raw(400), scaled(400);
+    for (int tj = 0; tj < 20; ++tj)
+        for (int ti = 0; ti < 20; ++ti) {
+            // src/Potential.cpp:14-17, float arithmetic as in the reference
+            const float Eij = kLambda * (ctx->eps[ti * 20 + tj] - kE0);
+            const float sij = 0.5f * (ctx->sigma[ti] + ctx->sigma[tj]);
+            Coef c;
+            c.sig2 = sij * sij;
+            c.A = -4.0f * Eij;
+            const float rc = sij * ctx->r0f;
+            c.rc2 = Eij > 0.0f ? rc * rc : -1.0f;
+            c.qq = ctx->charge[ti] * ctx->charge[tj];
+            raw[tj * 20 + ti] = c;
+            Coef s = c;
+            s.A = c.A * ctx->rt_to_kcal;
+            s.qq = c.qq * ctx->dh_factor;
+            scaled[tj * 20 + ti] = s;
+        }
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_table_raw, 400))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_table_scaled, 400))) return rc;
+    CG_HIP(ctx, hipMemcpy(ctx->d_table_raw, raw.data(), sizeof(Coef) * 400, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_table_scaled, scaled.data(), sizeof(Coef) * 400, hipMemcpyHostToDevice));
+    return CGPPD_OK;
+}
+
+uint32_t residue_code(const cgppd_ctx *ctx, int t)
+{
+    return ((uint32_t)t << kRowShift) | (ctx->charge[t] != 0.0f ? kChargedBit : 0u);
+}
+
+// The residue code travels in the w lane of the float4 coordinates.
+float code_bits(uint32_t c)
+{
+    float f;
+    std::memcpy(&f, &c, sizeof(f));
+    return f;
+}
+
+PairParams base_params(cgppd_ctx *ctx)
+{
+    PairParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.n_mol = ctx->M;
+    p.n_res = ctx->N;
+    p.stride = ctx->stride;
+    p.n_active = ctx->M;
+    p.ib_max = (ctx->max_n + kIBlock - 1) / kIBlock;
+    p.jc_count = (ctx->N + kJChunk - 1) / kJChunk;
+    p.mol_off = ctx->d_mol_off;
+    p.mol_of = ctx->d_mol_of;
+    p.rep_begin = 0;
+    p.rep_count = ctx->R;
+    p.xcd_map = 0;
+    p.table = ctx->d_table_raw;
+    p.box = ctx->box;
+    p.inv_box = 1.0f / ctx->box;
+    p.pos = ctx->d_pos;
+    p.rel = ctx->d_rel;
+    p.center = ctx->d_center;
+    p.out_lj = ctx->d_out;
+    p.out_dh = ctx->d_out + ctx->R;
+    p.out_mat = nullptr;
+    p.rt_to_kcal = ctx->rt_to_kcal;
+    p.dh_factor = ctx->dh_factor;
+    p.seed = ctx->seed;
+    p.rowbuf = ctx->d_rowbuf;
+    p.done = ctx->d_done;
+    p.emat = ctx->d_emat;
+    p.energy = ctx->d_energy;
+    p.temperature = ctx->d_temp;
+    p.tstep = ctx->d_tstep;
+    p.rcos = ctx->d_rcos;
+    p.rsin = ctx->d_rsin;
+    p.counters = ctx->d_counters;
+    p.steps = ctx->d_steps;
+    p.pair_evals = ctx->d_pair_evals;
+    return p;
+}
+
+int grid_for(const cgppd_ctx *ctx, long long items, bool xcd)
+{
+    const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 4;
+    long long g = (long long)ctx->n_cu * per_cu;
+    if (xcd) {
+        // each XCD takes items / 8
+        long long per = (items + 7) / 8;
+        g = std::min(g, per * 8);
+        g = std::max(8LL, g / 8 * 8);
+    } else {
+        g = std::min(g, items);
+        g = std::max(1LL, g);
+    }
+    return (int)g;
+}
+
+int check_ready(cgppd_ctx *ctx, bool need_rep)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (!ctx->have_ff) return fail(ctx, CGPPD_E_STATE, "force field not set");
+    if (!ctx->have_box) return fail(ctx, CGPPD_E_STATE, "box not set");
+    if (!ctx->have_sys) return fail(ctx, CGPPD_E_STATE, "system not set");
+    if (need_rep && !ctx->have_rep) return fail(ctx, CGPPD_E_STATE, "replicas not set");
+    return CGPPD_OK;
+}
+
+int launch_full(cgppd_ctx *ctx, int r_begin, int r_count, int n_active, bool scaled, double *mat)
+{
+    PairParams p = base_params(ctx);
+    p.rep_begin = r_begin;
+    p.rep_count = r_count;
+    p.n_active = n_active;
+    p.out_mat = mat;
+    p.table = scaled ? ctx->d_table_scaled : ctx->d_table_raw;
+    p.xcd_map = (r_begin == 0 && r_count % 8 == 0) ? 1 : 0;
+    const long long items = (long long)r_count * n_active * p.ib_max * p.jc_count;
+    if (items == 0) return CGPPD_OK;
+    const int g = grid_for(ctx, items, p.xcd_map);
+    if (scaled)
+        hipLaunchKernelGGL(full_energy_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    else
+        hipLaunchKernelGGL(full_energy_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
+{
+    if (n == 0) return CGPPD_OK;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, ctx->stream, p, n);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+int launch_mc_step(cgppd_ctx *ctx)
+{
+    PairParams p = base_params(ctx);
+    p.table = ctx->d_table_scaled;
+    p.xcd_map = (ctx->R % 8 == 0) ? 1 : 0;
+    const long long items = (long long)ctx->R * p.ib_max * p.jc_count;
+    const int g = grid_for(ctx, items, p.xcd_map);
+    hipLaunchKernelGGL(mc_step_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+int launch_exchange(cgppd_ctx *ctx)
+{
+    ExchangeParams e;
+    e.n_rep = ctx->R;
+    e.seed = ctx->seed;
+    e.temperature = ctx->d_temp;
+    e.energy = ctx->d_energy;
+    e.label = ctx->d_label;
+    e.rank_of = ctx->d_rank_of;
+    e.rep_at_rank = ctx->d_rep_at_rank;
+    e.counters = ctx->d_counters;
+    e.round = ctx->d_ex_round;
+    e.stats = ctx->d_ex_stats;
+    hipLaunchKernelGGL(exchange_kernel, dim3(1), dim3(kWave), 0, ctx->stream, e);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+int energy_outputs(cgppd_ctx *ctx, int r, double *lj, double *dh, double *total)
+{
+    CG_HIP(ctx, hipMemcpyAsync(ctx->h_out, ctx->d_out, sizeof(double) * 2 * ctx->R, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double l = ctx->h_out[r], d = ctx->h_out[ctx->R + r];
+    if (lj) *lj = l;
+    if (dh) *dh = d;
+    // src/Potential.cpp:219
+    if (total) *total = d * (double)ctx->dh_factor + l * (double)ctx->rt_to_kcal;
+    return CGPPD_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int cgppd_device_count(int *count)
+{
+    if (!count) return CGPPD_E_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return CGPPD_OK;
+}
+
+int cgppd_create(cgppd_ctx **out, int device)
+{
+    if (!out) return CGPPD_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CGPPD_E_HIP;
+    if (device < 0 || device >= n) return CGPPD_E_ARG;
+    cgppd_ctx *ctx = new cgppd_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return CGPPD_E_HIP;
+    }
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+        ctx->n_cu = cu;
+    model_constants(&ctx->rt_to_kcal, &ctx->dh_factor, &ctx->r0f);
+    *out = ctx;
+    return CGPPD_OK;
+}
+
+int cgppd_destroy(cgppd_ctx *ctx)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_replicas(ctx);
+    dev_free(ctx->d_table_raw);
+    dev_free(ctx->d_table_scaled);
+    dev_free(ctx->d_mol_off);
+    dev_free(ctx->d_mol_of);
+    for (hipEvent_t ev : ctx->events) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CGPPD_OK;
+}
+
+const char *cgppd_last_error(const cgppd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cgppd_model_constants(float *rt_to_kcal, float *dh_factor, float *r0_factor)
+{
+    float a, b, c;
+    model_constants(&a, &b, &c);
+    if (rt_to_kcal) *rt_to_kcal = a;
+    if (dh_factor) *dh_factor = b;
+    if (r0_factor) *r0_factor = c;
+    return CGPPD_OK;
+}
+
+int cgppd_set_force_field(cgppd_ctx *ctx, const float *eps, const float *sigma, const float *charge)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (!eps || !sigma || !charge) return fail(ctx, CGPPD_E_ARG, "null force-field array");
+    std::memcpy(ctx->eps, eps, sizeof(ctx->eps));
+    std::memcpy(ctx->sigma, sigma, sizeof(ctx->sigma));
+    std::memcpy(ctx->charge, charge, sizeof(ctx->charge));
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = build_tables(ctx);
+    if (rc) return rc;
+    ctx->have_ff = true;
+    // residue codes carry the charged flag: replicas must be set again
+    if (ctx->have_rep) free_replicas(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_set_box(cgppd_ctx *ctx, float box)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (!(box > 0.0f) || !std::isfinite(box)) return fail(ctx, CGPPD_E_ARG, "box must be a positive finite edge");
+    ctx->box = box;
+    ctx->have_box = true;
+    ctx->mc_ready = false;
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_set_system(cgppd_ctx *ctx, int n_mol, const int *mol_off, const int *type, int n_noncrowder)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (n_mol < 1 || !mol_off || !type) return fail(ctx, CGPPD_E_ARG, "need n_mol >= 1 and non-null arrays");
+    if (mol_off[0] != 0) return fail(ctx, CGPPD_E_ARG, "mol_off[0] must be 0");
+    int max_n = 0;
+    for (int m = 0; m < n_mol; ++m) {
+        const int n = mol_off[m + 1] - mol_off[m];
+        if (n < 1) return fail(ctx, CGPPD_E_ARG, "molecule %d has %d residues", m, n);
+        max_n = std::max(max_n, n);
+    }
+    const int N = mol_off[n_mol];
+    for (int i = 0; i < N; ++i)
+        if (type[i] < 0 || type[i] >= CGPPD_AA_COUNT)
+            return fail(ctx, CGPPD_E_ARG, "residue %d has type %d outside 0..19", i, type[i]);
+    if (n_noncrowder < 0 || n_noncrowder > n_mol) return fail(ctx, CGPPD_E_ARG, "n_noncrowder out of range");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    free_replicas(ctx);
+    ctx->M = n_mol;
+    ctx->N = N;
+    ctx->stride = (N + kJChunk - 1) / kJChunk * kJChunk;
+    ctx->n_nc = n_noncrowder;
+    ctx->max_n = max_n;
+    ctx->mol_off.assign(mol_off, mol_off + n_mol + 1);
+    ctx->type.assign(type, type + N);
+    std::vector<int> mol_of(N);
+    for (int m = 0; m < n_mol; ++m)
+        for (int i = mol_off[m]; i < mol_off[m + 1]; ++i) mol_of[i] = m;
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_mol_off, n_mol + 1))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_mol_of, N))) return rc;
+    CG_HIP(ctx, hipMemcpy(ctx->d_mol_off, mol_off, sizeof(int) * (n_mol + 1), hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_mol_of, mol_of.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    ctx->have_sys = true;
+    return CGPPD_OK;
+}
+
+int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const float *rel, const double *temperature,
+                       const float *translate_step, const float *rotate_step, const int *label, uint64_t seed)
+{
+    int rc = check_ready(ctx, false);
+    if (rc) return rc;
+    if (n_rep < 1 || !center || !rel) return fail(ctx, CGPPD_E_ARG, "need n_rep >= 1, centres and positions");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    free_replicas(ctx);
+    const int M = ctx->M, N = ctx->N, S = ctx->stride, R = n_rep;
+    ctx->R = R;
+    ctx->seed = seed;
+    std::vector<float4> hpos((size_t)R * S), hrel((size_t)R * S), hcen((size_t)R * M);
+    for (int r = 0; r < R; ++r) {
+        for (int m = 0; m < M; ++m) {
+            const float *c = center + ((size_t)r * M + m) * 3;
+            hcen[(size_t)r * M + m] = make_float4(c[0], c[1], c[2], 0.0f);
+            for (int i = ctx->mol_off[m]; i < ctx->mol_off[m + 1]; ++i) {
+                const float *v = rel + ((size_t)r * N + i) * 3;
+                const float w = code_bits(residue_code(ctx, ctx->type[i]));
+                hrel[(size_t)r * S + i] = make_float4(v[0], v[1], v[2], w);
+                // src/Molecule.cpp:183-191: position = relative + centre
+                hpos[(size_t)r * S + i] = make_float4(v[0] + c[0], v[1] + c[1], v[2] + c[2], w);
+            }
+        }
+        for (int i = N; i < S; ++i) {
+            hpos[(size_t)r * S + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            hrel[(size_t)r * S + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    std::vector<double> temp(R), rcos(R), rsin(R);
+    std::vector<float> tstep(R);
+    std::vector<int> lab(R), rank_of(R), at_rank(R);
+    for (int r = 0; r < R; ++r) {
+        temp[r] = temperature ? temperature[r] : 300.0;
+        tstep[r] = translate_step ? translate_step[r] : 0.5f;
+        const float rs = rotate_step ? rotate_step[r] : 0.2f;
+        // src/Quaternion.cpp:14-15 with angle = rotate step (double)
+        rcos[r] = std::cos((double)rs / 2.0);
+        rsin[r] = std::sin((double)rs / 2.0);
+        lab[r] = label ? label[r] : r;
+        at_rank[r] = r;
+    }
+    std::stable_sort(at_rank.begin(), at_rank.end(), [&](int a, int b) { return temp[a] < temp[b]; });
+    for (int k = 0; k < R; ++k) rank_of[at_rank[k]] = k;
+
+    if ((rc = dev_alloc(ctx, &ctx->d_pos, (size_t)R * S))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_rel, (size_t)R * S))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_center, (size_t)R * M))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_rowbuf, (size_t)R * M))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_emat, (size_t)R * M * M))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_energy, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_temp, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_rcos, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_rsin, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_tstep, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_done, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_counters, (size_t)3 * R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_steps, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_ex_round, 1))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_ex_stats, 2))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_pair_evals, 1))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_label, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_rank_of, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_rep_at_rank, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_out, (size_t)2 * R))) return rc;
+    CG_HIP(ctx, hipHostMalloc((void **)&ctx->h_out, sizeof(double) * 2 * R, hipHostMallocDefault));
+
+    CG_HIP(ctx, hipMemcpy(ctx->d_pos, hpos.data(), sizeof(float4) * hpos.size(), hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_rel, hrel.data(), sizeof(float4) * hrel.size(), hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_center, hcen.data(), sizeof(float4) * hcen.size(), hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_temp, temp.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_rcos, rcos.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_rsin, rsin.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_tstep, tstep.data(), sizeof(float) * R, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_label, lab.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_rank_of, rank_of.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_rep_at_rank, at_rank.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemset(ctx->d_rowbuf, 0, sizeof(double) * R * M));
+    CG_HIP(ctx, hipMemset(ctx->d_emat, 0, sizeof(double) * R * M * M));
+    CG_HIP(ctx, hipMemset(ctx->d_energy, 0, sizeof(double) * R));
+    CG_HIP(ctx, hipMemset(ctx->d_done, 0, sizeof(unsigned) * R));
+    CG_HIP(ctx, hipMemset(ctx->d_counters, 0, sizeof(long long) * 3 * R));
+    CG_HIP(ctx, hipMemset(ctx->d_steps, 0, sizeof(long long) * R));
+    CG_HIP(ctx, hipMemset(ctx->d_ex_round, 0, sizeof(long long)));
+    CG_HIP(ctx, hipMemset(ctx->d_ex_stats, 0, sizeof(unsigned long long) * 2));
+    CG_HIP(ctx, hipMemset(ctx->d_pair_evals, 0, sizeof(unsigned long long)));
+    CG_HIP(ctx, hipMemset(ctx->d_out, 0, sizeof(double) * 2 * R));
+    ctx->have_rep = true;
+    ctx->mc_ready = false;
+    return CGPPD_OK;
+}
+
+int cgppd_upload_positions(cgppd_ctx *ctx, int replica, const float *xyz)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R || !xyz) return fail(ctx, CGPPD_E_ARG, "bad replica or null coordinates");
+    std::vector<float4> h(ctx->N);
+    for (int i = 0; i < ctx->N; ++i)
+        h[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
+                           code_bits(residue_code(ctx, ctx->type[i])));
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipMemcpyAsync(ctx->d_pos + (size_t)replica * ctx->stride, h.data(), sizeof(float4) * ctx->N,
+                               hipMemcpyHostToDevice, ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mc_ready = false;
+    return CGPPD_OK;
+}
+
+int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float *xyz)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R || molecule < 0 || molecule >= ctx->M || !xyz)
+        return fail(ctx, CGPPD_E_ARG, "bad replica/molecule or null coordinates");
+    const int i0 = ctx->mol_off[molecule], n = ctx->mol_off[molecule + 1] - i0;
+    std::vector<float4> h(n);
+    for (int k = 0; k < n; ++k)
+        h[k] = make_float4(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2],
+                           code_bits(residue_code(ctx, ctx->type[i0 + k])));
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipMemcpyAsync(ctx->d_pos + (size_t)replica * ctx->stride + i0, h.data(), sizeof(float4) * n,
+                               hipMemcpyHostToDevice, ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mc_ready = false;
+    return CGPPD_OK;
+}
+
+static int energy_common(cgppd_ctx *ctx, int replica, int n_active)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R) return fail(ctx, CGPPD_E_ARG, "replica %d out of range", replica);
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipMemsetAsync(ctx->d_out + replica, 0, sizeof(double), ctx->stream));
+    CG_HIP(ctx, hipMemsetAsync(ctx->d_out + ctx->R + replica, 0, sizeof(double), ctx->stream));
+    return launch_full(ctx, replica, 1, n_active, false, nullptr);
+}
+
+int cgppd_energy(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total)
+{
+    int rc = energy_common(ctx, replica, ctx ? ctx->M : 0);
+    if (rc) return rc;
+    return energy_outputs(ctx, replica, lj, dh, total);
+}
+
+int cgppd_energy_nc(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total)
+{
+    int rc = energy_common(ctx, replica, ctx ? ctx->n_nc : 0);
+    if (rc) return rc;
+    return energy_outputs(ctx, replica, lj, dh, total);
+}
+
+int cgppd_energy_async(cgppd_ctx *ctx, int replica) { return energy_common(ctx, replica, ctx ? ctx->M : 0); }
+
+int cgppd_energy_wait(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R) return fail(ctx, CGPPD_E_ARG, "replica %d out of range", replica);
+    return energy_outputs(ctx, replica, lj, dh, total);
+}
+
+int cgppd_energy_all(cgppd_ctx *ctx, double *lj, double *dh, double *total, double *mat)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    const int R = ctx->R, M = ctx->M;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if (mat && !ctx->d_out_mat)
+        if ((rc = dev_alloc(ctx, &ctx->d_out_mat, (size_t)R * M * M))) return rc;
+    if ((rc = zero_dev(ctx, ctx->d_out, (size_t)2 * R))) return rc;
+    if (mat && (rc = zero_dev(ctx, ctx->d_out_mat, (size_t)R * M * M))) return rc;
+    if ((rc = launch_full(ctx, 0, R, M, false, mat ? ctx->d_out_mat : nullptr))) return rc;
+    CG_HIP(ctx, hipMemcpyAsync(ctx->h_out, ctx->d_out, sizeof(double) * 2 * R, hipMemcpyDeviceToHost, ctx->stream));
+    if (mat)
+        CG_HIP(ctx, hipMemcpyAsync(mat, ctx->d_out_mat, sizeof(double) * R * M * M, hipMemcpyDeviceToHost, ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < R; ++r) {
+        const double l = ctx->h_out[r], d = ctx->h_out[R + r];
+        if (lj) lj[r] = l;
+        if (dh) dh[r] = d;
+        if (total) total[r] = d * (double)ctx->dh_factor + l * (double)ctx->rt_to_kcal;
+    }
+    if (mat)
+        for (int r = 0; r < R; ++r) {
+            double *m = mat + (size_t)r * M * M;
+            for (int I = 0; I < M; ++I) {
+                m[(size_t)I * M + I] = 0.0;
+                for (int J = I + 1; J < M; ++J) m[(size_t)J * M + I] = m[(size_t)I * M + J];
+            }
+        }
+    return CGPPD_OK;
+}
+
+int cgppd_mc_prepare(cgppd_ctx *ctx)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = zero_dev(ctx, ctx->d_emat, (size_t)ctx->R * ctx->M * ctx->M))) return rc;
+    if ((rc = launch_full(ctx, 0, ctx->R, ctx->M, true, nullptr))) return rc;
+    PairParams p = base_params(ctx);
+    hipLaunchKernelGGL(prepare_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p);
+    CG_HIP(ctx, hipGetLastError());
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mc_ready = true;
+    return CGPPD_OK;
+}
+
+static int ensure_events(cgppd_ctx *ctx, size_t n)
+{
+    while (ctx->events.size() < n) {
+        hipEvent_t ev;
+        CG_HIP(ctx, hipEventCreate(&ev));
+        ctx->events.push_back(ev);
+    }
+    return CGPPD_OK;
+}
+
+int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before MC steps");
+    if (n_steps < 0) return fail(ctx, CGPPD_E_ARG, "negative step count");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if (ctx->timing && n_steps > 0) {
+        if ((rc = ensure_events(ctx, (size_t)2 * n_steps))) return rc;
+        for (int k = 0; k < n_steps; ++k) {
+            CG_HIP(ctx, hipEventRecord(ctx->events[2 * k], ctx->stream));
+            if ((rc = launch_mc_step(ctx))) return rc;
+            CG_HIP(ctx, hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
+        }
+        CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double sum = 0.0;
+        for (int k = 0; k < n_steps; ++k) {
+            float ms = 0.f;
+            CG_HIP(ctx, hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]));
+            sum += ms;
+        }
+        ctx->t_mean_ms = sum / n_steps;
+        ctx->t_launches = n_steps;
+        return CGPPD_OK;
+    }
+    for (int k = 0; k < n_steps; ++k)
+        if ((rc = launch_mc_step(ctx))) return rc;
+    return CGPPD_OK;
+}
+
+int cgppd_exchange(cgppd_ctx *ctx)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before exchanges");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    return launch_exchange(ctx);
+}
+
+int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_graph)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before running");
+    if (n_intervals < 0 || steps_per_interval < 0) return fail(ctx, CGPPD_E_ARG, "negative counts");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if (!use_graph) {
+        for (int k = 0; k < n_intervals; ++k) {
+            if ((rc = cgppd_mc_steps(ctx, steps_per_interval))) return rc;
+            if ((rc = launch_exchange(ctx))) return rc;
+        }
+        return CGPPD_OK;
+    }
+    if (!ctx->graph_exec || ctx->graph_steps != steps_per_interval) {
+        destroy_graph(ctx);
+        CG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int lrc = CGPPD_OK;
+        for (int k = 0; k < steps_per_interval && !lrc; ++k) lrc = launch_mc_step(ctx);
+        if (!lrc) lrc = launch_exchange(ctx);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (lrc) {
+            if (g) (void)hipGraphDestroy(g);
+            return lrc;
+        }
+        if (ce != hipSuccess) return fail(ctx, CGPPD_E_HIP, "graph capture: %s", hipGetErrorString(ce));
+        ctx->graph = g;
+        CG_HIP(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+        ctx->graph_steps = steps_per_interval;
+    }
+    for (int k = 0; k < n_intervals; ++k) CG_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    return CGPPD_OK;
+}
+
+int cgppd_synchronize(cgppd_ctx *ctx)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CGPPD_OK;
+}
+
+int cgppd_get_replica_stats(cgppd_ctx *ctx, double *energy, double *temperature, int *label, int64_t *counters,
+                            int64_t *steps, int64_t *exchange)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    const int R = ctx->R;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (energy) CG_HIP(ctx, hipMemcpy(energy, ctx->d_energy, sizeof(double) * R, hipMemcpyDeviceToHost));
+    if (temperature) CG_HIP(ctx, hipMemcpy(temperature, ctx->d_temp, sizeof(double) * R, hipMemcpyDeviceToHost));
+    if (label) CG_HIP(ctx, hipMemcpy(label, ctx->d_label, sizeof(int) * R, hipMemcpyDeviceToHost));
+    if (counters) CG_HIP(ctx, hipMemcpy(counters, ctx->d_counters, sizeof(long long) * 3 * R, hipMemcpyDeviceToHost));
+    if (steps) CG_HIP(ctx, hipMemcpy(steps, ctx->d_steps, sizeof(long long) * R, hipMemcpyDeviceToHost));
+    if (exchange) CG_HIP(ctx, hipMemcpy(exchange, ctx->d_ex_stats, sizeof(long long) * 2, hipMemcpyDeviceToHost));
+    return CGPPD_OK;
+}
+
+int cgppd_get_pair_evaluations(cgppd_ctx *ctx, int64_t *pairs)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!pairs) return fail(ctx, CGPPD_E_ARG, "null output");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CG_HIP(ctx, hipMemcpy(pairs, ctx->d_pair_evals, sizeof(int64_t), hipMemcpyDeviceToHost));
+    return CGPPD_OK;
+}
+
+int cgppd_get_coordinates(cgppd_ctx *ctx, int replica, float *pos, float *rel, float *center)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R) return fail(ctx, CGPPD_E_ARG, "replica %d out of range", replica);
+    const int N = ctx->N, M = ctx->M;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<float4> h(std::max(N, M));
+    if (pos) {
+        CG_HIP(ctx, hipMemcpy(h.data(), ctx->d_pos + (size_t)replica * ctx->stride, sizeof(float4) * N,
+                              hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i) {
+            pos[3 * i] = h[i].x;
+            pos[3 * i + 1] = h[i].y;
+            pos[3 * i + 2] = h[i].z;
+        }
+    }
+    if (rel) {
+        CG_HIP(ctx, hipMemcpy(h.data(), ctx->d_rel + (size_t)replica * ctx->stride, sizeof(float4) * N,
+                              hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i) {
+            rel[3 * i] = h[i].x;
+            rel[3 * i + 1] = h[i].y;
+            rel[3 * i + 2] = h[i].z;
+        }
+    }
+    if (center) {
+        CG_HIP(ctx, hipMemcpy(h.data(), ctx->d_center + (size_t)replica * M, sizeof(float4) * M,
+                              hipMemcpyDeviceToHost));
+        for (int m = 0; m < M; ++m) {
+            center[3 * m] = h[m].x;
+            center[3 * m + 1] = h[m].y;
+            center[3 * m + 2] = h[m].z;
+        }
+    }
+    return CGPPD_OK;
+}
+
+int cgppd_get_energy_matrix(cgppd_ctx *ctx, int replica, double *mat)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R || !mat) return fail(ctx, CGPPD_E_ARG, "bad replica or null output");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CG_HIP(ctx, hipMemcpy(mat, ctx->d_emat + (size_t)replica * ctx->M * ctx->M,
+                          sizeof(double) * ctx->M * ctx->M, hipMemcpyDeviceToHost));
+    return CGPPD_OK;
+}
+
+int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu)
+{
+    if (!ctx || wg_per_cu < 0 || wg_per_cu > 64) return CGPPD_E_ARG;
+    ctx->wg_per_cu = wg_per_cu;
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_set_timing(cgppd_ctx *ctx, int enable)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    ctx->timing = enable != 0;
+    return CGPPD_OK;
+}
+
+int cgppd_last_kernel_ms(cgppd_ctx *ctx, double *mean_ms, int64_t *launches)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (mean_ms) *mean_ms = ctx->t_mean_ms;
+    if (launches) *launches = ctx->t_launches;
+    return CGPPD_OK;
+}
+
+int cgppd_get_sizes(const cgppd_ctx *ctx, int *n_mol, int *n_res, int *n_rep)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (n_mol) *n_mol = ctx->M;
+    if (n_res) *n_res = ctx->N;
+    if (n_rep) *n_rep = ctx->R;
+    return CGPPD_OK;
+}
+
+}  // extern "C"

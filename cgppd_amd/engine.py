@@ -1,0 +1,197 @@
+"""Python handle over the C ABI (include/cgppd_hip.h), for the test and
+benchmark harness.  All compute happens in libcgppd_hip.so on the GPU."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import f32, f64, i32, ptr
+
+
+class CgppdError(RuntimeError):
+    pass
+
+
+class Engine:
+    """One device context: force field, box, system, R resident replicas."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        rc = self.lib.cgppd_create(C.byref(h), int(device))
+        if rc != 0:
+            raise CgppdError(f"cgppd_create(device={device}) failed with status {rc}")
+        self.h = h
+        self.n_mol = self.n_res = self.n_rep = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cgppd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.cgppd_last_error(self.h)
+            raise CgppdError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # set-up -------------------------------------------------------------
+    def set_force_field(self, eps, sigma, charge):
+        e, s, q = f32(eps).reshape(400), f32(sigma), f32(charge)
+        self._check(self.lib.cgppd_set_force_field(self.h, ptr(e, C.c_float), ptr(s, C.c_float), ptr(q, C.c_float)),
+                    "set_force_field")
+
+    def set_box(self, box):
+        self._check(self.lib.cgppd_set_box(self.h, float(box)), "set_box")
+
+    def set_system(self, mol_off, types, n_noncrowder=None):
+        mo, ty = i32(mol_off), i32(types)
+        n_mol = len(mo) - 1
+        nc = n_mol if n_noncrowder is None else int(n_noncrowder)
+        self._check(self.lib.cgppd_set_system(self.h, n_mol, ptr(mo, C.c_int), ptr(ty, C.c_int), nc), "set_system")
+        self.n_mol, self.n_res = n_mol, int(mo[-1])
+
+    def set_replicas(self, centers, rel, temperature=None, translate_step=None, rotate_step=None, label=None,
+                     seed=0):
+        c, r = f32(centers), f32(rel)
+        R = c.shape[0] if c.ndim == 3 else 1
+        t = None if temperature is None else f64(temperature)
+        ts = None if translate_step is None else f32(translate_step)
+        rs = None if rotate_step is None else f32(rotate_step)
+        lb = None if label is None else i32(label)
+        self._check(self.lib.cgppd_set_replicas(self.h, R, ptr(c, C.c_float), ptr(r, C.c_float), ptr(t, C.c_double),
+                                                ptr(ts, C.c_float), ptr(rs, C.c_float), ptr(lb, C.c_int),
+                                                C.c_uint64(int(seed))), "set_replicas")
+        self.n_rep = R
+
+    def load_spec(self, spec, n_rep=None, seed=0):
+        """Force field, box, topology and R identical replicas of a
+        cgppd_amd.synth.SystemSpec (temperatures and steps from the spec)."""
+        R = spec.n_rep if n_rep is None else n_rep
+        self.set_force_field(spec.ff.eps, spec.ff.sigma, spec.ff.charge)
+        self.set_box(spec.box)
+        self.set_system(spec.mol_off, spec.types)
+        temps = spec.temperatures() if R == spec.n_rep else np.linspace(spec.t_min, spec.t_max, R)
+        ts, rs = spec.step_sizes()
+        if R != spec.n_rep:
+            ts, rs = np.resize(ts, R), np.resize(rs, R)
+        self.set_replicas(np.broadcast_to(spec.centers, (R,) + spec.centers.shape),
+                          np.broadcast_to(spec.rel, (R,) + spec.rel.shape), temps, ts, rs, np.arange(R), seed)
+
+    def upload_positions(self, r, xyz):
+        x = f32(xyz)
+        self._check(self.lib.cgppd_upload_positions(self.h, int(r), ptr(x, C.c_float)), "upload_positions")
+
+    def update_molecule(self, r, m, xyz):
+        x = f32(xyz)
+        self._check(self.lib.cgppd_update_molecule(self.h, int(r), int(m), ptr(x, C.c_float)), "update_molecule")
+
+    # energies -----------------------------------------------------------
+    def _energy3(self, fn, r, what):
+        lj, dh, tot = C.c_double(), C.c_double(), C.c_double()
+        self._check(fn(self.h, int(r), C.byref(lj), C.byref(dh), C.byref(tot)), what)
+        return lj.value, dh.value, tot.value
+
+    def energy(self, r=0):
+        return self._energy3(self.lib.cgppd_energy, r, "energy")
+
+    def energy_nc(self, r=0):
+        return self._energy3(self.lib.cgppd_energy_nc, r, "energy_nc")
+
+    def energy_async(self, r=0):
+        self._check(self.lib.cgppd_energy_async(self.h, int(r)), "energy_async")
+
+    def energy_wait(self, r=0):
+        return self._energy3(self.lib.cgppd_energy_wait, r, "energy_wait")
+
+    def energy_all(self, matrix=False):
+        R, M = self.n_rep, self.n_mol
+        lj, dh, tot = np.zeros(R), np.zeros(R), np.zeros(R)
+        mat = np.zeros((R, M, M)) if matrix else None
+        self._check(self.lib.cgppd_energy_all(self.h, ptr(lj, C.c_double), ptr(dh, C.c_double),
+                                              ptr(tot, C.c_double), ptr(mat, C.c_double)), "energy_all")
+        return (lj, dh, tot, mat) if matrix else (lj, dh, tot)
+
+    # MC -----------------------------------------------------------------
+    def mc_prepare(self):
+        self._check(self.lib.cgppd_mc_prepare(self.h), "mc_prepare")
+
+    def mc_steps(self, n):
+        self._check(self.lib.cgppd_mc_steps(self.h, int(n)), "mc_steps")
+
+    def exchange(self):
+        self._check(self.lib.cgppd_exchange(self.h), "exchange")
+
+    def run(self, n_intervals, steps_per_interval, use_graph=True):
+        self._check(self.lib.cgppd_run(self.h, int(n_intervals), int(steps_per_interval), int(bool(use_graph))), "run")
+
+    def synchronize(self):
+        self._check(self.lib.cgppd_synchronize(self.h), "synchronize")
+
+    def stats(self):
+        R = self.n_rep
+        e, t = np.zeros(R), np.zeros(R)
+        lab = np.zeros(R, dtype=np.int32)
+        cnt = np.zeros((R, 3), dtype=np.int64)
+        st = np.zeros(R, dtype=np.int64)
+        ex = np.zeros(2, dtype=np.int64)
+        self._check(self.lib.cgppd_get_replica_stats(self.h, ptr(e, C.c_double), ptr(t, C.c_double),
+                                                     ptr(lab, C.c_int), ptr(cnt, C.c_int64), ptr(st, C.c_int64),
+                                                     ptr(ex, C.c_int64)), "get_replica_stats")
+        return {"energy": e, "temperature": t, "label": lab, "counters": cnt, "steps": st, "exchange": ex}
+
+    def pair_evaluations(self):
+        n = C.c_int64()
+        self._check(self.lib.cgppd_get_pair_evaluations(self.h, C.byref(n)), "get_pair_evaluations")
+        return n.value
+
+    def coordinates(self, r):
+        N, M = self.n_res, self.n_mol
+        pos, rel, cen = np.zeros((N, 3), np.float32), np.zeros((N, 3), np.float32), np.zeros((M, 3), np.float32)
+        self._check(self.lib.cgppd_get_coordinates(self.h, int(r), ptr(pos, C.c_float), ptr(rel, C.c_float),
+                                                   ptr(cen, C.c_float)), "get_coordinates")
+        return pos, rel, cen
+
+    def energy_matrix(self, r):
+        M = self.n_mol
+        mat = np.zeros((M, M))
+        self._check(self.lib.cgppd_get_energy_matrix(self.h, int(r), ptr(mat, C.c_double)), "get_energy_matrix")
+        return mat
+
+    def set_launch(self, wg_per_cu):
+        self._check(self.lib.cgppd_set_launch(self.h, int(wg_per_cu)), "set_launch")
+
+    def set_timing(self, on):
+        self._check(self.lib.cgppd_set_timing(self.h, int(bool(on))), "set_timing")
+
+    def last_kernel_ms(self):
+        ms, n = C.c_double(), C.c_int64()
+        self._check(self.lib.cgppd_last_kernel_ms(self.h, C.byref(ms), C.byref(n)), "last_kernel_ms")
+        return ms.value, n.value
+
+
+def model_constants():
+    lib = _lib.load()
+    a, b, c = C.c_float(), C.c_float(), C.c_float()
+    lib.cgppd_model_constants(C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def device_count() -> int:
+    lib = _lib.load()
+    n = C.c_int()
+    lib.cgppd_device_count(C.byref(n))
+    return n.value

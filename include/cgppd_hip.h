@@ -1,0 +1,179 @@
+/*
+ * cgppd_hip.h -- C ABI of the gfx950 REMC energy library (libcgppd_hip.so).
+ *
+ * Plain C types only: opaque context handle, pointers, sizes, int status.
+ * Every entry point returns CGPPD_OK (0) or a negative CGPPD_E* code; the
+ * message of the last failure is available from cgppd_last_error().
+ *
+ * One context = one GPU, one HIP stream, one force field, one system
+ * topology (molecules and residue types, shared by every replica) and R
+ * replicas whose coordinates, temperatures and energies stay resident in
+ * HBM.  A context is driven by one host thread at a time.
+ *
+ * Each declaration names the reference interface it replaces.  Reference
+ * paths are relative to the cgppd source tree.
+ */
+#ifndef CGPPD_HIP_H
+#define CGPPD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGPPD_OK 0
+#define CGPPD_E_ARG (-1)      /* invalid argument or call order   */
+#define CGPPD_E_HIP (-2)      /* HIP runtime error                */
+#define CGPPD_E_NOMEM (-3)    /* device allocation failed         */
+#define CGPPD_E_STATE (-4)    /* context not ready for this call  */
+
+#define CGPPD_AA_COUNT 20
+
+typedef struct cgppd_ctx cgppd_ctx;
+
+/* Library / device set-up.  Replaces the per-thread part of setup_CUDA
+ * (src/Simulation.cpp:892-926: cudaSetDevice + stream creation) and
+ * teardown_CUDA (src/Simulation.cpp:928-948). */
+int cgppd_create(cgppd_ctx **out, int device);
+int cgppd_destroy(cgppd_ctx *ctx);
+const char *cgppd_last_error(const cgppd_ctx *ctx);
+int cgppd_device_count(int *count);
+
+/* Force field.  Replaces LJPotentialDataToDevice (src/cudaExterns.h:19),
+ * copyLJPotentialDataToDevice (src/cudaExterns.h:20) and bindLJTexture /
+ * unbindLJTexture (src/cudaExterns.h:53-54); the table is owned by the
+ * context.  eps[20*20] are the raw contact energies e_ij [RT] of
+ * AminoAcids::LJpotentials (src/AminoAcids.h:13); sigma[20] and charge[20]
+ * are the per-type van der Waals radius and charge (src/AminoAcids.cpp:29-73),
+ * i.e. what the reference packs per residue into meta.y/meta.z
+ * (src/Replica.cpp:763-765). */
+int cgppd_set_force_field(cgppd_ctx *ctx, const float *eps, const float *sigma, const float *charge);
+
+/* Periodic box edge.  Replaces CUDA_setBoxDimension / CUDA_freeBoxDimension
+ * (src/cudaExterns.h:47-48).  box > 0. */
+int cgppd_set_box(cgppd_ctx *ctx, float box);
+
+/* System topology, shared by all replicas.  Replaces the residueCount /
+ * moleculeCount / moleculePositions device arrays and the metadata float4s
+ * built in Replica::ReplicaDataToDevice (src/Replica.cpp:705-830).
+ *   n_mol, mol_off[n_mol+1] (mol_off[0] = 0, mol_off[n_mol] = N),
+ *   type[N] in 0..19, n_noncrowder = number of leading non-crowder molecules
+ *   (the reference requires non-crowders first, src/Replica.cpp:924).
+ * Resets all replica state. */
+int cgppd_set_system(cgppd_ctx *ctx, int n_mol, const int *mol_off, const int *type, int n_noncrowder);
+
+/* Replica state.  Replaces ReplicaDataToDevice (src/Replica.cpp:705-830) for
+ * R replicas at once plus Replica::init_child_replica's ladders
+ * (src/Replica.cpp:79-98).  Arrays are row-major by replica:
+ *   center[R*M*3], rel[R*N*3]   molecule centres and residue positions
+ *                               relative to them (position = rel + center),
+ *   temperature[R] [K], translate_step[R] [A], rotate_step[R] [rad],
+ *   label[R].  Temperature ranks are assigned by ascending temperature.
+ * seed keys the counter-based Philox streams of every move and exchange. */
+int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const float *rel,
+                       const double *temperature, const float *translate_step,
+                       const float *rotate_step, const int *label, uint64_t seed);
+
+/* Overwrite the absolute coordinates of one replica (xyz[N*3]), as
+ * Replica::ReplicaDataToDevice copies positions (src/Replica.cpp:739-760).
+ * Centres and relative positions are left unchanged: use this for energy
+ * evaluation of arbitrary conformations, not before MC steps. */
+int cgppd_upload_positions(cgppd_ctx *ctx, int replica, const float *xyz);
+
+/* Overwrite one molecule's coordinates (xyz[n_m*3]).  Replaces
+ * Replica::MoleculeDataToDevice (src/Replica.cpp:833-881). */
+int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float *xyz);
+
+/* Full inter-molecular energy of one replica, synchronous.  Replaces
+ * CUDA_EonDevice (src/cudaExterns.h:29) behind Replica::EonDevice
+ * (src/Replica.cpp:903-918).  lj and dh are the raw sums (Potential::LJ,
+ * Potential::DH, src/Potential.h:34-35); total is kcal/mol as
+ * Potential::total (src/Potential.cpp:214-221).  Any output may be NULL. */
+int cgppd_energy(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total);
+
+/* Same over the first n_noncrowder molecules only.  Replaces
+ * CUDA_EonDeviceNC (src/cudaExterns.h:30) behind Replica::EonDeviceNC
+ * (src/Replica.cpp:920-935). */
+int cgppd_energy_nc(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total);
+
+/* Asynchronous variant: enqueue on the context stream, then wait.  Replaces
+ * CUDA_EonDevice_async (src/cudaExterns.h:42) + the D2H copy in
+ * Replica::EonDeviceAsync (src/Replica.cpp:883-901) and
+ * Replica::SumGridResults (src/Replica.cpp:627-638). */
+int cgppd_energy_async(cgppd_ctx *ctx, int replica);
+int cgppd_energy_wait(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total);
+
+/* Full energies of all replicas in one launch (lj, dh, total: R each; any
+ * may be NULL) and, if mat != NULL, the molecule-pair matrix of totals
+ * (R*M*M, kcal/mol).  New: the reference evaluates one replica per call. */
+int cgppd_energy_all(cgppd_ctx *ctx, double *lj, double *dh, double *total, double *mat);
+
+/* Recompute the device molecule-pair energy matrix and running totals of
+ * every replica from their coordinates.  Required once after
+ * cgppd_set_replicas and before cgppd_mc_steps; may be called at any time
+ * to resynchronise the running totals. */
+int cgppd_mc_prepare(cgppd_ctx *ctx);
+
+/* n_steps MC steps on every replica, all on the device: molecule pick,
+ * rigid translate/rotate (src/Molecule.cpp:146-266,468-504), delta-E of the
+ * moved molecule against all others, Metropolis (src/Replica.cpp:385-410)
+ * and state update.  Replaces Replica::MCSearch (src/Replica.cpp:350-414)
+ * and MCSearchMutate/Evaluate/AcceptReject (src/Replica.cpp:643-701). */
+int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps);
+
+/* One replica-exchange round over adjacent temperature ranks, alternating
+ * even/odd pairs.  Replaces the exchange loop of Simulation::run
+ * (src/Simulation.cpp:770-795) and Replica::exchangeReplicas
+ * (src/Replica.cpp:188-210). */
+int cgppd_exchange(cgppd_ctx *ctx);
+
+/* n_intervals x (steps_per_interval MC steps + one exchange).  With
+ * use_graph != 0 one interval is captured once in a hipGraph and replayed.
+ * Replaces the MC/exchange rendezvous of MCthreadableFunction
+ * (src/Simulation.cpp:1042-1094) and Simulation::run (src/Simulation.cpp:737-816). */
+int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_graph);
+
+/* Block until all work queued on the context stream has finished. */
+int cgppd_synchronize(cgppd_ctx *ctx);
+
+/* State read-back (any pointer may be NULL):
+ *   energy[R] running totals, temperature[R], label[R], counters[R*3]
+ *   (accept, acceptA, reject as src/Replica.h accept/acceptA/reject),
+ *   steps[R], exchange[2] (tests, accepted). */
+int cgppd_get_replica_stats(cgppd_ctx *ctx, double *energy, double *temperature, int *label,
+                            int64_t *counters, int64_t *steps, int64_t *exchange);
+
+/* Inter-molecular residue pairs evaluated by MC steps since
+ * cgppd_set_replicas: the sum over steps of n_m * (N - n_m) for the moved
+ * molecule m (the delta-E row).  New: the reference counts none. */
+int cgppd_get_pair_evaluations(cgppd_ctx *ctx, int64_t *pairs);
+
+/* Coordinates of replica r: pos[N*3], rel[N*3], center[M*3] (any NULL). */
+int cgppd_get_coordinates(cgppd_ctx *ctx, int replica, float *pos, float *rel, float *center);
+
+/* Device molecule-pair matrix of replica r (M*M, kcal/mol). */
+int cgppd_get_energy_matrix(cgppd_ctx *ctx, int replica, double *mat);
+
+/* Timing and launch shape of the dominant kernel.
+ * cgppd_set_launch: workgroups per CU of the persistent pair kernel
+ * (0 = automatic).  cgppd_last_kernel_ms: mean duration in ms of the pair
+ * kernel launches of the last cgppd_mc_steps / cgppd_run call, measured with
+ * HIP events on the context stream (0 when not measured);
+ * cgppd_set_timing(ctx, 1) enables the measurement. */
+int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
+int cgppd_set_timing(cgppd_ctx *ctx, int enable);
+int cgppd_last_kernel_ms(cgppd_ctx *ctx, double *mean_ms, int64_t *launches);
+
+/* The model's float constants as this library evaluates them
+ * (src/constants.h:24-25 RT_to_kcalmol and DH_CONVERSION_FACTOR,
+ * src/constants.h:16 r0_constant); no context needed. */
+int cgppd_model_constants(float *rt_to_kcal, float *dh_factor, float *r0_factor);
+
+/* Sizes of the current system: n_mol, n_res, n_rep (any NULL). */
+int cgppd_get_sizes(const cgppd_ctx *ctx, int *n_mol, int *n_res, int *n_rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py -- as the checker and as the timed CPU
+baseline -- and by nothing in the product (cgppd_amd/).
+
+Parity pins (see tests/test_oracle.py):
+  orc_energy_ref       ten reference conformations (tests/testReferenceConformations.cpp:83-94),
+                       Kim 2008 comparison (:119-189), bit-exact vs oracle/_ref
+  orc_energy_fp64      closed-form two-residue cases, agreement with orc_energy_ref
+  orc_philox4x32_10    Random123 known-answer vectors
+  moves                bit-exact vs oracle/_ref (reference Quaternion/Vector3f),
+                       rigid-body invariants
+  Metropolis/exchange  closed form (acceptance frequencies), invariants
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_DRIVER = HERE / "_ref" / "ref_driver"
+
+_i32p = C.POINTER(C.c_int)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+class RemcState(C.Structure):
+    _fields_ = [
+        ("n_mol", C.c_int), ("n_res", C.c_int), ("n_rep", C.c_int),
+        ("mol_off", _i32p), ("type", _i32p),
+        ("charge", _f32p), ("sigma", _f32p), ("eps", _f32p),
+        ("box", C.c_float),
+        ("seed", C.c_uint64),
+        ("center", _f32p), ("rel", _f32p), ("pos", _f32p),
+        ("temperature", _f64p), ("translate_step", _f32p), ("rotate_step", _f32p),
+        ("energy", _f64p), ("emat", _f64p), ("counters", _i64p), ("step", _i64p),
+        ("label", _i32p), ("rank_of", _i32p), ("rep_at_rank", _i32p),
+        ("exchange_round", _i64p), ("exchange_stats", _i64p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        sysargs = [C.c_int, _i32p, _i32p, _f32p, _f32p, _f32p, _f32p, C.c_float]
+        for name in ("orc_energy_ref", "orc_energy_fp64"):
+            getattr(L, name).argtypes = sysargs + [_f64p, _f64p]
+            getattr(L, name).restype = None
+        L.orc_energy_nc_ref.argtypes = sysargs + [_f64p, _f64p]
+        L.orc_energy_pair_ref.argtypes = [_i32p, C.c_int, C.c_int, _i32p, _f32p, _f32p, _f32p, _f32p, C.c_float,
+                                          _f64p, _f64p]
+        L.orc_energy_matrix_ref.argtypes = sysargs + [_f64p]
+        for name in ("orc_rt_to_kcal", "orc_dh_factor", "orc_r0_factor"):
+            getattr(L, name).restype = C.c_float
+            getattr(L, name).argtypes = []
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_mc_proposal.argtypes = [C.c_uint64, C.c_int, C.c_int64, C.c_int, _i32p, _i32p, _f64p, _f64p]
+        L.orc_apply_move.argtypes = [C.c_int, _f64p, C.c_float, C.c_double, C.c_double, C.c_float, _f32p, _f32p,
+                                     C.c_int, _f32p, _f32p, _f32p]
+        L.orc_remc_init.argtypes = [C.POINTER(RemcState)]
+        L.orc_remc_mc_steps.argtypes = [C.POINTER(RemcState), C.c_int]
+        L.orc_remc_exchange.argtypes = [C.POINTER(RemcState)]
+        L.orc_remc_reference_run.argtypes = [C.POINTER(RemcState), C.c_int, C.c_int]
+        L.orc_remc_reference_run.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _sys(mol_off, types, charge, sigma, eps, pos, box):
+    mo = np.ascontiguousarray(mol_off, dtype=np.int32)
+    ty = np.ascontiguousarray(types, dtype=np.int32)
+    q = np.ascontiguousarray(charge, dtype=np.float32)
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    e = np.ascontiguousarray(eps, dtype=np.float32).reshape(400)
+    x = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    return mo, ty, q, s, e, x, float(box)
+
+
+def constants():
+    L = lib()
+    return L.orc_rt_to_kcal(), L.orc_dh_factor(), L.orc_r0_factor()
+
+
+def total(lj, dh):
+    rt, dhc, _ = constants()
+    return dh * float(np.float32(dhc)) + lj * float(np.float32(rt))
+
+
+def energy_ref(mol_off, types, charge, sigma, eps, pos, box):
+    """Reference semantics, raw (LJ, DH)."""
+    mo, ty, q, s, e, x, L = _sys(mol_off, types, charge, sigma, eps, pos, box)
+    lj, dh = C.c_double(), C.c_double()
+    lib().orc_energy_ref(len(mo) - 1, _p(mo, C.c_int), _p(ty, C.c_int), _p(q, C.c_float), _p(s, C.c_float),
+                         _p(e, C.c_float), _p(x, C.c_float), L, C.byref(lj), C.byref(dh))
+    return lj.value, dh.value
+
+
+def energy_nc_ref(n_nc, mol_off, types, charge, sigma, eps, pos, box):
+    mo, ty, q, s, e, x, L = _sys(mol_off, types, charge, sigma, eps, pos, box)
+    lj, dh = C.c_double(), C.c_double()
+    lib().orc_energy_nc_ref(int(n_nc), _p(mo, C.c_int), _p(ty, C.c_int), _p(q, C.c_float), _p(s, C.c_float),
+                            _p(e, C.c_float), _p(x, C.c_float), L, C.byref(lj), C.byref(dh))
+    return lj.value, dh.value
+
+
+def energy_pair_ref(mol_off, a, b, types, charge, sigma, eps, pos, box):
+    mo, ty, q, s, e, x, L = _sys(mol_off, types, charge, sigma, eps, pos, box)
+    lj, dh = C.c_double(), C.c_double()
+    lib().orc_energy_pair_ref(_p(mo, C.c_int), int(a), int(b), _p(ty, C.c_int), _p(q, C.c_float),
+                              _p(s, C.c_float), _p(e, C.c_float), _p(x, C.c_float), L, C.byref(lj), C.byref(dh))
+    return lj.value, dh.value
+
+
+def energy_fp64(mol_off, types, charge, sigma, eps, pos, box):
+    mo, ty, q, s, e, x, L = _sys(mol_off, types, charge, sigma, eps, pos, box)
+    lj, dh = C.c_double(), C.c_double()
+    lib().orc_energy_fp64(len(mo) - 1, _p(mo, C.c_int), _p(ty, C.c_int), _p(q, C.c_float), _p(s, C.c_float),
+                          _p(e, C.c_float), _p(x, C.c_float), L, C.byref(lj), C.byref(dh))
+    return lj.value, dh.value
+
+
+def energy_matrix_ref(mol_off, types, charge, sigma, eps, pos, box):
+    mo, ty, q, s, e, x, L = _sys(mol_off, types, charge, sigma, eps, pos, box)
+    M = len(mo) - 1
+    mat = np.zeros((M, M))
+    lib().orc_energy_matrix_ref(M, _p(mo, C.c_int), _p(ty, C.c_int), _p(q, C.c_float), _p(s, C.c_float),
+                                _p(e, C.c_float), _p(x, C.c_float), L, _p(mat, C.c_double))
+    return mat
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, out)
+    return list(out)
+
+
+def proposal(seed, r, step, n_mol):
+    m, rot = C.c_int(), C.c_int()
+    axis = (C.c_double * 3)()
+    u = C.c_double()
+    lib().orc_mc_proposal(C.c_uint64(seed), int(r), int(step), int(n_mol), C.byref(m), C.byref(rot), axis,
+                          C.byref(u))
+    return m.value, rot.value, list(axis), u.value
+
+
+def apply_move(is_rotate, axis_raw, tstep, rcos, rsin, box, c_in, rel_in):
+    """One rigid move of n residues; returns (c_out, rel_out, pos_out)."""
+    ax = np.ascontiguousarray(axis_raw, dtype=np.float64)
+    ci = np.ascontiguousarray(c_in, dtype=np.float32)
+    ri = np.ascontiguousarray(rel_in, dtype=np.float32).reshape(-1, 3)
+    co = np.zeros(3, dtype=np.float32)
+    ro = np.zeros_like(ri)
+    po = np.zeros_like(ri)
+    lib().orc_apply_move(int(is_rotate), _p(ax, C.c_double), float(tstep), float(rcos), float(rsin), float(box),
+                         _p(ci, C.c_float), _p(co, C.c_float), len(ri), _p(ri, C.c_float), _p(ro, C.c_float),
+                         _p(po, C.c_float))
+    return co, ro, po
+
+
+class Remc:
+    """Owns numpy arrays for an orc_remc state (all replicas)."""
+
+    def __init__(self, spec, n_rep=None, seed=0, temperatures=None, steps=None):
+        R = spec.n_rep if n_rep is None else n_rep
+        M, N = spec.n_mol, spec.n_res
+        self.mol_off = np.ascontiguousarray(spec.mol_off, dtype=np.int32)
+        self.types = np.ascontiguousarray(spec.types, dtype=np.int32)
+        self.charge = np.ascontiguousarray(spec.ff.charge, dtype=np.float32)
+        self.sigma = np.ascontiguousarray(spec.ff.sigma, dtype=np.float32)
+        self.eps = np.ascontiguousarray(spec.ff.eps, dtype=np.float32).reshape(400)
+        self.center = np.ascontiguousarray(np.broadcast_to(spec.centers, (R, M, 3)), dtype=np.float32).copy()
+        self.rel = np.ascontiguousarray(np.broadcast_to(spec.rel, (R, N, 3)), dtype=np.float32).copy()
+        self.pos = np.zeros((R, N, 3), dtype=np.float32)
+        if temperatures is None:
+            temperatures = spec.temperatures() if R == spec.n_rep else np.linspace(spec.t_min, spec.t_max, R)
+        self.temperature = np.ascontiguousarray(temperatures, dtype=np.float64).copy()
+        if steps is None:
+            ts, rs = spec.step_sizes()
+            if R != spec.n_rep:
+                ts, rs = np.resize(ts, R), np.resize(rs, R)
+        else:
+            ts, rs = steps
+        self.translate_step = np.ascontiguousarray(ts, dtype=np.float32).copy()
+        self.rotate_step = np.ascontiguousarray(rs, dtype=np.float32).copy()
+        self.energy = np.zeros(R)
+        self.emat = np.zeros((R, M, M))
+        self.counters = np.zeros((R, 3), dtype=np.int64)
+        self.step = np.zeros(R, dtype=np.int64)
+        self.label = np.arange(R, dtype=np.int32)
+        order = np.argsort(self.temperature, kind="stable").astype(np.int32)
+        self.rep_at_rank = np.ascontiguousarray(order)
+        self.rank_of = np.zeros(R, dtype=np.int32)
+        self.rank_of[order] = np.arange(R, dtype=np.int32)
+        self.exchange_round = np.zeros(1, dtype=np.int64)
+        self.exchange_stats = np.zeros(2, dtype=np.int64)
+        self.st = RemcState(M, N, R, _p(self.mol_off, C.c_int), _p(self.types, C.c_int),
+                            _p(self.charge, C.c_float), _p(self.sigma, C.c_float), _p(self.eps, C.c_float),
+                            float(spec.box), C.c_uint64(int(seed)),
+                            _p(self.center, C.c_float), _p(self.rel, C.c_float), _p(self.pos, C.c_float),
+                            _p(self.temperature, C.c_double), _p(self.translate_step, C.c_float),
+                            _p(self.rotate_step, C.c_float), _p(self.energy, C.c_double), _p(self.emat, C.c_double),
+                            _p(self.counters, C.c_int64), _p(self.step, C.c_int64), _p(self.label, C.c_int),
+                            _p(self.rank_of, C.c_int), _p(self.rep_at_rank, C.c_int),
+                            _p(self.exchange_round, C.c_int64), _p(self.exchange_stats, C.c_int64))
+        lib().orc_remc_init(C.byref(self.st))
+
+    def mc_steps(self, n):
+        lib().orc_remc_mc_steps(C.byref(self.st), int(n))
+
+    def exchange(self):
+        lib().orc_remc_exchange(C.byref(self.st))
+
+    def reference_run(self, n, threads):
+        return lib().orc_remc_reference_run(C.byref(self.st), int(n), int(threads))

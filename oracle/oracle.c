@@ -1,0 +1,508 @@
+/*
+ * oracle.c -- CPU restatement of cgppd's REMC energy hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py, as the checker and as the CPU baseline.  The
+ * product (cgppd_amd/) never links or loads it.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, so every float
+ * expression rounds the way the reference's x86-64 build rounds it).
+ *
+ * Parity status of each function is stated in oracle.h.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* src/constants.h:8-26 and src/definitions.h:10 */
+#define ORC_E_CHARGE 1.602176487e-19f
+#define ORC_ANG 1e-10f
+#define ORC_D_WATER 80.0f
+#define ORC_RGAS 8.3144621f
+#define ORC_T_ROOM 298.0f
+#define ORC_KCAL 4184.0f
+#define ORC_N_A 6.02214129e23f
+#define ORC_EPS0 8.85418782e-12f
+#define ORC_E0 (-2.27f)
+#define ORC_LAMBDA 0.159f
+#define ORC_XI 10.0f
+#define ORC_EPS 1e-38f
+
+float orc_rt_to_kcal(void) { return ORC_T_ROOM * ORC_RGAS / ORC_KCAL; }
+
+float orc_dh_factor(void)
+{
+    /* float pow and float arithmetic, left to right, as the macro expands */
+    return powf(ORC_E_CHARGE, 2.0f) * ORC_N_A /
+           (4.0f * (float)3.14159265358979323846264338327950288L * ORC_EPS0 * ORC_KCAL * ORC_ANG * ORC_D_WATER);
+}
+
+float orc_r0_factor(void) { return powf(2.0f, 1.0f / 6.0f); }
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10                                                        */
+/* ------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* ------------------------------------------------------------------ */
+/* Reference-semantics pair term                                       */
+/* ------------------------------------------------------------------ */
+
+/* src/Residue.cpp:62-76: float difference, float minimum image with
+ * round(), sqrtf; src/Replica.cpp:479 adds EPS in double. */
+static inline double ref_distance(const float *a, const float *b, float box)
+{
+    float X = a[0] - b[0];
+    float Y = a[1] - b[1];
+    float Z = a[2] - b[2];
+    if (box != 0.0f) {
+        X = X - box * roundf(X / box);
+        Y = Y - box * roundf(Y / box);
+        Z = Z - box * roundf(Z / box);
+    }
+    return (double)sqrtf(X * X + Y * Y + Z * Z) + (double)ORC_EPS;
+}
+
+/* src/Potential.cpp:5-35 */
+static inline double ref_lj(int ti, int tj, const float *sigma, const float *eps, double r, float r0f)
+{
+    double Eij = (double)(ORC_LAMBDA * (eps[ti * 20 + tj] - ORC_E0));
+    double sij = (double)(0.5f * (sigma[ti] + sigma[tj]));
+    double t = pow(sij / r, 6.0);
+    double lj = -4.0 * Eij * t * (t - 1.0);
+    if (Eij > 0.0 && r < sij * (double)r0f)
+        lj = -lj + 2.0 * Eij;
+    return lj;
+}
+
+/* src/Potential.cpp:37-42 */
+static inline double ref_dh(int ti, int tj, const float *charge, double r)
+{
+    float e = expf((float)(-r / (double)ORC_XI));
+    return (double)(charge[ti] * charge[tj] * e) / r;
+}
+
+static void ref_block(int i0, int i1, int j0, int j1, const int *type, const float *charge,
+                      const float *sigma, const float *eps, const float *pos, float box,
+                      double *lj, double *dh)
+{
+    const float r0f = orc_r0_factor();
+    double sl = *lj, sd = *dh;
+    for (int i = i0; i < i1; ++i) {
+        for (int j = j0; j < j1; ++j) {
+            double r = ref_distance(pos + 3 * i, pos + 3 * j, box);
+            sl += ref_lj(type[i], type[j], sigma, eps, r, r0f);
+            sd += ref_dh(type[i], type[j], charge, r);
+        }
+    }
+    *lj = sl;
+    *dh = sd;
+}
+
+/* src/Replica.cpp:421-500 (non-crowding branch, LJ on) */
+void orc_energy_ref(int n_mol, const int *mol_off, const int *type, const float *charge,
+                    const float *sigma, const float *eps, const float *pos, float box,
+                    double *lj, double *dh)
+{
+    double sl = 0.0, sd = 0.0;
+    for (int I = 0; I < n_mol; ++I)
+        for (int J = I + 1; J < n_mol; ++J)
+            ref_block(mol_off[I], mol_off[I + 1], mol_off[J], mol_off[J + 1], type, charge, sigma,
+                      eps, pos, box, &sl, &sd);
+    *lj = sl;
+    *dh = sd;
+}
+
+/* src/Replica.cpp:502-528 */
+void orc_energy_pair_ref(const int *mol_off, int a, int b, const int *type, const float *charge,
+                         const float *sigma, const float *eps, const float *pos, float box,
+                         double *lj, double *dh)
+{
+    double sl = 0.0, sd = 0.0;
+    ref_block(mol_off[a], mol_off[a + 1], mol_off[b], mol_off[b + 1], type, charge, sigma, eps,
+              pos, box, &sl, &sd);
+    *lj = sl;
+    *dh = sd;
+}
+
+/* src/Replica.cpp:1062-1068: sum of E(a,b) over non-crowder pairs.  The
+ * reference adds totals per pair; the raw sums here are the same pairs. */
+void orc_energy_nc_ref(int n_nc, const int *mol_off, const int *type, const float *charge,
+                       const float *sigma, const float *eps, const float *pos, float box,
+                       double *lj, double *dh)
+{
+    orc_energy_ref(n_nc, mol_off, type, charge, sigma, eps, pos, box, lj, dh);
+}
+
+static inline double pair_total(double lj, double dh)
+{
+    /* src/Potential.cpp:219 */
+    return dh * (double)orc_dh_factor() + lj * (double)orc_rt_to_kcal();
+}
+
+void orc_energy_matrix_ref(int n_mol, const int *mol_off, const int *type, const float *charge,
+                           const float *sigma, const float *eps, const float *pos, float box,
+                           double *mat)
+{
+    for (int I = 0; I < n_mol; ++I) {
+        mat[I * n_mol + I] = 0.0;
+        for (int J = I + 1; J < n_mol; ++J) {
+            double lj, dh;
+            orc_energy_pair_ref(mol_off, I, J, type, charge, sigma, eps, pos, box, &lj, &dh);
+            mat[I * n_mol + J] = mat[J * n_mol + I] = pair_total(lj, dh);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* FP64 oracle                                                          */
+/* ------------------------------------------------------------------ */
+typedef struct { double s, c; } nsum;
+static inline void nsum_add(nsum *a, double v)
+{
+    double t = a->s + v;
+    if (fabs(a->s) >= fabs(v)) a->c += (a->s - t) + v;
+    else a->c += (v - t) + a->s;
+    a->s = t;
+}
+
+void orc_energy_fp64(int n_mol, const int *mol_off, const int *type, const float *charge,
+                     const float *sigma, const float *eps, const float *pos, float box,
+                     double *lj, double *dh)
+{
+    const double L = (double)box;
+    const double r0 = pow(2.0, 1.0 / 6.0);
+    nsum sl = {0, 0}, sd = {0, 0};
+    for (int I = 0; I < n_mol; ++I)
+        for (int J = I + 1; J < n_mol; ++J)
+            for (int i = mol_off[I]; i < mol_off[I + 1]; ++i)
+                for (int j = mol_off[J]; j < mol_off[J + 1]; ++j) {
+                    double d[3];
+                    for (int k = 0; k < 3; ++k) {
+                        d[k] = (double)pos[3 * i + k] - (double)pos[3 * j + k];
+                        if (L != 0.0) d[k] -= L * round(d[k] / L);
+                    }
+                    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + 1e-38;
+                    int ti = type[i], tj = type[j];
+                    double Eij = (double)ORC_LAMBDA * ((double)eps[ti * 20 + tj] - (double)ORC_E0);
+                    double sij = 0.5 * ((double)sigma[ti] + (double)sigma[tj]);
+                    double t = pow(sij / r, 6.0);
+                    double u = -4.0 * Eij * t * (t - 1.0);
+                    if (Eij > 0.0 && r < sij * r0) u = -u + 2.0 * Eij;
+                    nsum_add(&sl, u);
+                    nsum_add(&sd, (double)charge[ti] * (double)charge[tj] * exp(-r / (double)ORC_XI) / r);
+                }
+    *lj = sl.s + sl.c;
+    *dh = sd.s + sd.c;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rigid-body moves                                                    */
+/* ------------------------------------------------------------------ */
+#define ORC_TAG_MC 0x4D430000u
+#define ORC_TAG_EX 0x45580000u
+
+static inline double u32_to_unit(uint32_t w) { return (double)w * (1.0 / 4294967296.0); }
+static inline double u53(uint32_t hi, uint32_t lo)
+{
+    return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+void orc_mc_proposal(uint64_t seed, int r, int64_t step, int n_mol, int *mol, int *is_rotate,
+                     double axis[3], double *u_accept)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t c0[4] = {(uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, ORC_TAG_MC};
+    uint32_t c1[4] = {(uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, ORC_TAG_MC | 1u};
+    uint32_t w[4], v[4];
+    orc_philox4x32_10(c0, key, w);
+    orc_philox4x32_10(c1, key, v);
+    /* src/Replica.cpp:355 picks the molecule uniformly */
+    *mol = (int)(((uint64_t)w[0] * (uint64_t)n_mol) >> 32);
+    /* src/Molecule.cpp:461: Bernoulli(0.5), 1 = MC_ROTATE */
+    *is_rotate = w[1] < 0x80000000u ? 1 : 0;
+    /* src/Molecule.cpp:213,218: (u-0.5, 0.5-u, u-0.5) before normalising */
+    axis[0] = u32_to_unit(v[0]) - 0.5;
+    axis[1] = 0.5 - u32_to_unit(v[1]);
+    axis[2] = u32_to_unit(v[2]) - 0.5;
+    *u_accept = u53(w[2], w[3]);
+}
+
+/* Apply the proposal to molecule residues [i0,i1) of one replica; writes
+ * the new centre, relative and absolute positions into the output arrays
+ * (which may alias the inputs). */
+static void apply_move(int is_rotate, const double axis_raw[3], float tstep, double rcos,
+                       double rsin, float box, const float *c_in, float *c_out, int i0, int i1,
+                       const float *rel_in, float *rel_out, float *pos_out)
+{
+    if (!is_rotate) {
+        /* src/Molecule.cpp:262-266, 210-214: float vector, float normalise */
+        float n[3] = {(float)axis_raw[0], (float)axis_raw[1], (float)axis_raw[2]};
+        float mag = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (mag != 0.0f) {
+            n[0] = n[0] / mag; n[1] = n[1] / mag; n[2] = n[2] / mag;
+        }
+        float c[3];
+        for (int k = 0; k < 3; ++k) {
+            float v = (float)((double)n[k] * (double)tstep);
+            /* src/Molecule.cpp:146-159 + wrapped_centre :234-241 (fmod) */
+            c[k] = fmodf(c_in[k] + v, box);
+        }
+        for (int k = 0; k < 3; ++k) c_out[k] = c[k];
+        for (int i = i0; i < i1; ++i)
+            for (int k = 0; k < 3; ++k) {
+                rel_out[3 * i + k] = rel_in[3 * i + k];
+                /* src/Molecule.cpp:183-191 */
+                pos_out[3 * i + k] = rel_in[3 * i + k] + c[k];
+            }
+    } else {
+        /* src/Molecule.cpp:216-221, 193-208; src/Quaternion.cpp:12-38,95-112 */
+        double ax = axis_raw[0], ay = axis_raw[1], az = axis_raw[2];
+        double m = sqrt(ax * ax + ay * ay + az * az);
+        ax /= m; ay /= m; az /= m;
+        double w = rcos, x = rsin * ax, y = rsin * ay, z = rsin * az;
+        double qm = sqrt(w * w + x * x + y * y + z * z);
+        w /= qm; x /= qm; y /= qm; z /= qm;
+        double w2 = w * w, x2 = x * x, y2 = y * y, z2 = z * z;
+        double wx = w * x, wy = w * y, wz = w * z, xy = x * y, xz = x * z, yz = y * z;
+        float c[3] = {c_in[0], c_in[1], c_in[2]};
+        for (int k = 0; k < 3; ++k) c_out[k] = c[k];
+        for (int i = i0; i < i1; ++i) {
+            double vx = rel_in[3 * i], vy = rel_in[3 * i + 1], vz = rel_in[3 * i + 2];
+            float nx = (float)(vx * (w2 + x2 - y2 - z2) + (vy * (xy - wz) + vz * (wy + xz)) * 2.0);
+            float ny = (float)((vx * (wz + xy) + vz * (yz - wx)) * 2.0 + vy * (w2 - x2 + y2 - z2));
+            float nz = (float)((vx * (xz - wy) + vy * (yz + wx)) * 2.0 + vz * (w2 - x2 - y2 + z2));
+            rel_out[3 * i] = nx; rel_out[3 * i + 1] = ny; rel_out[3 * i + 2] = nz;
+            /* src/Residue.cpp:93-99 */
+            pos_out[3 * i] = c[0] + nx;
+            pos_out[3 * i + 1] = c[1] + ny;
+            pos_out[3 * i + 2] = c[2] + nz;
+        }
+    }
+}
+
+void orc_apply_move(int is_rotate, const double axis_raw[3], float tstep, double rcos, double rsin,
+                    float box, const float c_in[3], float c_out[3], int n, const float *rel_in,
+                    float *rel_out, float *pos_out)
+{
+    apply_move(is_rotate, axis_raw, tstep, rcos, rsin, box, c_in, c_out, 0, n, rel_in, rel_out, pos_out);
+}
+
+void orc_remc_init(orc_remc *s)
+{
+    const int M = s->n_mol, N = s->n_res;
+    for (int r = 0; r < s->n_rep; ++r) {
+        float *pos = s->pos + (size_t)r * N * 3;
+        const float *rel = s->rel + (size_t)r * N * 3;
+        const float *c = s->center + (size_t)r * M * 3;
+        for (int I = 0; I < M; ++I)
+            for (int i = s->mol_off[I]; i < s->mol_off[I + 1]; ++i)
+                for (int k = 0; k < 3; ++k) pos[3 * i + k] = rel[3 * i + k] + c[3 * I + k];
+        double *mat = s->emat + (size_t)r * M * M;
+        orc_energy_matrix_ref(M, s->mol_off, s->type, s->charge, s->sigma, s->eps, pos, s->box, mat);
+        double e = 0.0;
+        for (int I = 0; I < M; ++I)
+            for (int J = I + 1; J < M; ++J) e += mat[I * M + J];
+        s->energy[r] = e;
+    }
+}
+
+static void mc_replica_steps(orc_remc *s, int r, int nsteps, float *scratch_rel, float *scratch_pos,
+                             double *row)
+{
+    const int M = s->n_mol, N = s->n_res;
+    float *center = s->center + (size_t)r * M * 3;
+    float *rel = s->rel + (size_t)r * N * 3;
+    float *pos = s->pos + (size_t)r * N * 3;
+    double *mat = s->emat + (size_t)r * M * M;
+    const double rs = (double)s->rotate_step[r];
+    const double rcos = cos(rs / 2.0), rsin = sin(rs / 2.0);
+    for (int k = 0; k < nsteps; ++k) {
+        int64_t step = s->step[r];
+        int m, rot;
+        double axis[3], u;
+        orc_mc_proposal(s->seed, r, step, M, &m, &rot, axis, &u);
+        const int i0 = s->mol_off[m], i1 = s->mol_off[m + 1];
+        float cnew[3];
+        /* proposal positions live in a full-size scratch so that pair
+         * indexing stays global */
+        memcpy(scratch_pos, pos, sizeof(float) * 3 * N);
+        apply_move(rot, axis, s->translate_step[r], rcos, rsin, s->box, center + 3 * m, cnew, i0, i1,
+                   rel, scratch_rel, scratch_pos);
+        double delta = 0.0;
+        for (int J = 0; J < M; ++J) {
+            if (J == m) { row[J] = 0.0; continue; }
+            double lj = 0.0, dh = 0.0;
+            ref_block(i0, i1, s->mol_off[J], s->mol_off[J + 1], s->type, s->charge, s->sigma, s->eps,
+                      scratch_pos, s->box, &lj, &dh);
+            row[J] = pair_total(lj, dh);
+            delta += row[J] - mat[m * M + J];
+        }
+        int64_t *cnt = s->counters + 3 * r;
+        int accept = 0;
+        if (delta < 0.0) { accept = 1; cnt[0]++; }
+        else if (u < exp(-delta * 4184.0 / ((double)ORC_RGAS * s->temperature[r]))) { accept = 1; cnt[1]++; }
+        else cnt[2]++;
+        if (accept) {
+            for (int q = 0; q < 3; ++q) center[3 * m + q] = cnew[q];
+            memcpy(rel + 3 * i0, scratch_rel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+            memcpy(pos + 3 * i0, scratch_pos + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+            for (int J = 0; J < M; ++J) {
+                if (J == m) continue;
+                mat[m * M + J] = mat[J * M + m] = row[J];
+            }
+            s->energy[r] += delta;
+        }
+        s->step[r] = step + 1;
+    }
+}
+
+void orc_remc_mc_steps(orc_remc *s, int nsteps)
+{
+    const int N = s->n_res, M = s->n_mol;
+    float *srel = (float *)malloc(sizeof(float) * 3 * N);
+    float *spos = (float *)malloc(sizeof(float) * 3 * N);
+    double *row = (double *)malloc(sizeof(double) * M);
+    for (int r = 0; r < s->n_rep; ++r) mc_replica_steps(s, r, nsteps, srel, spos, row);
+    free(srel); free(spos); free(row);
+}
+
+void orc_remc_exchange(orc_remc *s)
+{
+    const int R = s->n_rep;
+    int64_t k = *s->exchange_round;
+    int offset = (int)(k & 1);
+    const uint32_t key[2] = {(uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
+    for (int ti = offset; ti < R - 1; ti += 2) {
+        int i = s->rep_at_rank[ti], j = s->rep_at_rank[ti + 1];
+        /* src/Simulation.cpp:780 */
+        double delta = (1.0 / s->temperature[i] - 1.0 / s->temperature[j]) *
+                       (s->energy[i] - s->energy[j]) * (double)(4184.0f / ORC_RGAS);
+        uint32_t c[4] = {(uint32_t)k, (uint32_t)((uint64_t)k >> 32), (uint32_t)ti, ORC_TAG_EX};
+        uint32_t w[4];
+        orc_philox4x32_10(c, key, w);
+        double u = u53(w[0], w[1]);
+        double p = exp(delta);
+        if (p > 1.0) p = 1.0;
+        s->exchange_stats[0]++;
+        if (u < p) {
+            /* src/Replica.cpp:188-210: temperature, label and statistics move */
+            double t = s->temperature[i]; s->temperature[i] = s->temperature[j]; s->temperature[j] = t;
+            int l = s->label[i]; s->label[i] = s->label[j]; s->label[j] = l;
+            for (int q = 0; q < 3; ++q) {
+                int64_t a = s->counters[3 * i + q];
+                s->counters[3 * i + q] = s->counters[3 * j + q];
+                s->counters[3 * j + q] = a;
+            }
+            s->rank_of[i] = ti + 1;
+            s->rank_of[j] = ti;
+            s->rep_at_rank[ti] = j;
+            s->rep_at_rank[ti + 1] = i;
+            s->exchange_stats[1]++;
+        }
+    }
+    *s->exchange_round = k + 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* CPU baseline: the reference's CPU loop                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    orc_remc *s;
+    int r0, r1, nsteps;
+} ref_job;
+
+static void *ref_worker(void *arg)
+{
+    ref_job *job = (ref_job *)arg;
+    orc_remc *s = job->s;
+    const int M = s->n_mol, N = s->n_res;
+    float *brel = (float *)malloc(sizeof(float) * 3 * N);
+    float *bpos = (float *)malloc(sizeof(float) * 3 * N);
+    for (int r = job->r0; r < job->r1; ++r) {
+        float *center = s->center + (size_t)r * M * 3;
+        float *rel = s->rel + (size_t)r * N * 3;
+        float *pos = s->pos + (size_t)r * N * 3;
+        const double rs = (double)s->rotate_step[r];
+        const double rcos = cos(rs / 2.0), rsin = sin(rs / 2.0);
+        for (int k = 0; k < job->nsteps; ++k) {
+            int64_t step = s->step[r];
+            int m, rot;
+            double axis[3], u;
+            orc_mc_proposal(s->seed, r, step, M, &m, &rot, axis, &u);
+            const int i0 = s->mol_off[m], i1 = s->mol_off[m + 1];
+            /* src/Replica.cpp:357 MC_backup_restore */
+            float bc[3] = {center[3 * m], center[3 * m + 1], center[3 * m + 2]};
+            memcpy(brel + 3 * i0, rel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+            memcpy(bpos + 3 * i0, pos + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+            apply_move(rot, axis, s->translate_step[r], rcos, rsin, s->box, center + 3 * m,
+                       center + 3 * m, i0, i1, rel, rel, pos);
+            /* src/Replica.cpp:373-380 full recompute */
+            double lj, dh;
+            orc_energy_ref(M, s->mol_off, s->type, s->charge, s->sigma, s->eps, pos, s->box, &lj, &dh);
+            double enew = pair_total(lj, dh);
+            /* src/Replica.cpp:385-410 with float delta and float exponent */
+            float delta = (float)(enew - s->energy[r]);
+            float T = (float)s->temperature[r];
+            int64_t *cnt = s->counters + 3 * r;
+            if (delta < 0.0f) { s->energy[r] = enew; cnt[0]++; }
+            else if (u < (double)expf(-(delta * ORC_KCAL) / (ORC_RGAS * T))) { s->energy[r] = enew; cnt[1]++; }
+            else {
+                cnt[2]++;
+                for (int q = 0; q < 3; ++q) center[3 * m + q] = bc[q];
+                memcpy(rel + 3 * i0, brel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+                memcpy(pos + 3 * i0, bpos + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+            }
+            s->step[r] = step + 1;
+        }
+    }
+    free(brel); free(bpos);
+    return NULL;
+}
+
+double orc_remc_reference_run(orc_remc *s, int nsteps, int threads)
+{
+    if (threads < 1) threads = 1;
+    if (threads > s->n_rep) threads = s->n_rep;
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    ref_job *jobs = (ref_job *)malloc(sizeof(ref_job) * threads);
+    /* src/Simulation.cpp:479-487,684-688: contiguous blocks of replicas */
+    int per = (s->n_rep + threads - 1) / threads;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    int used = 0;
+    for (int t = 0; t < threads; ++t) {
+        int r0 = t * per, r1 = r0 + per;
+        if (r0 >= s->n_rep) break;
+        if (r1 > s->n_rep) r1 = s->n_rep;
+        jobs[t].s = s; jobs[t].r0 = r0; jobs[t].r1 = r1; jobs[t].nsteps = nsteps;
+        pthread_create(&tid[t], NULL, ref_worker, &jobs[t]);
+        used++;
+    }
+    for (int t = 0; t < used; ++t) pthread_join(tid[t], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    free(tid); free(jobs);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

@@ -1,0 +1,142 @@
+/*
+ * oracle.h -- CPU restatement of cgppd's REMC energy hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (cgppd_amd/) links,
+ * loads or calls this library; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, as the checker and as the timed CPU
+ * baseline.
+ *
+ * Every function cites the reference file:line it restates.  Pinning (what
+ * each function is checked against) is listed beside it; functions with no
+ * independent check say "parity unpinned".
+ *
+ * Layout conventions shared by all entry points:
+ *   n_mol            molecules in the system
+ *   mol_off[n_mol+1] first residue of each molecule (mol_off[n_mol] = N)
+ *   type[N]          amino-acid index 0..19 (reference order A R N D C E Q G L K M F P S T W Y V H I)
+ *   charge[20], sigma[20]  per-type charge and van der Waals radius
+ *   eps[400]         raw contact energies e_ij [RT] (AminoAcids::LJpotentials)
+ *   pos[N*3]         xyz of every residue, row-major
+ *   box              periodic box edge (0 = no periodic boundary)
+ */
+#ifndef CGPPD_ORACLE_H
+#define CGPPD_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Constants as the reference evaluates them (src/constants.h:8-26), float. */
+float orc_rt_to_kcal(void);      /* T_room*Rgas/kcal                       */
+float orc_dh_factor(void);       /* e^2 N_A / (4 pi eps0 kcal Ang D_water)  */
+float orc_r0_factor(void);       /* 2^(1/6), float pow                     */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Pinned by the
+ * Random123 known-answer vectors in tests/test_oracle.py. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Reference-semantics inter-molecular energy: src/Replica.cpp:421-500 with
+ * src/Residue.cpp:62-76 (FP32 minimum image + sqrtf), src/Potential.cpp:5-42
+ * (double LJ, float expf DH), double accumulation.  Raw LJ [RT] and raw DH
+ * are returned; total = dh*orc_dh_factor() + lj*orc_rt_to_kcal()
+ * (src/Potential.cpp:214-221).  Pinned by the ten reference conformations
+ * (tests/testReferenceConformations.cpp:83-94) and bit-exact against
+ * oracle/_ref built from the reference sources. */
+void orc_energy_ref(int n_mol, const int *mol_off, const int *type,
+                    const float *charge, const float *sigma, const float *eps,
+                    const float *pos, float box, double *lj, double *dh);
+
+/* Same, restricted to molecules a and b: src/Replica.cpp:502-528. */
+void orc_energy_pair_ref(const int *mol_off, int a, int b, const int *type,
+                         const float *charge, const float *sigma, const float *eps,
+                         const float *pos, float box, double *lj, double *dh);
+
+/* Reference semantics over the first n_nc molecules only (the non-crowders):
+ * the CPU branch of src/Replica.cpp:1062-1068. */
+void orc_energy_nc_ref(int n_nc, const int *mol_off, const int *type,
+                       const float *charge, const float *sigma, const float *eps,
+                       const float *pos, float box, double *lj, double *dh);
+
+/* Molecule-pair matrix in kcal/mol, mat[I*n_mol+J] = mat[J*n_mol+I],
+ * diagonal 0, each entry with reference-semantics pair terms. */
+void orc_energy_matrix_ref(int n_mol, const int *mol_off, const int *type,
+                           const float *charge, const float *sigma, const float *eps,
+                           const float *pos, float box, double *mat);
+
+/* FP64 oracle of the same model: every operation in double (positions and
+ * table values are the float inputs), Neumaier-compensated sums.  Pinned by
+ * closed-form two-residue cases and agreement with orc_energy_ref. */
+void orc_energy_fp64(int n_mol, const int *mol_off, const int *type,
+                     const float *charge, const float *sigma, const float *eps,
+                     const float *pos, float box, double *lj, double *dh);
+
+/* ---------------------------------------------------------------------
+ * Rigid-body MC (device algorithm restated on the CPU).
+ *
+ * The device keeps replicas resident and proposes moves from counter-based
+ * Philox streams, so this CPU restatement reproduces the same trajectory.
+ * Moves follow src/Molecule.cpp:146-266,468-504 and src/Quaternion.cpp:12-112;
+ * Metropolis follows src/Replica.cpp:385-410 (delta and exponent in FP64);
+ * exchange follows src/Simulation.cpp:770-795 and src/Replica.cpp:188-210.
+ * --------------------------------------------------------------------- */
+typedef struct {
+    int n_mol, n_res, n_rep;
+    const int *mol_off;       /* n_mol+1 */
+    const int *type;          /* n_res   */
+    const float *charge, *sigma, *eps;
+    float box;
+    uint64_t seed;
+    /* per-replica state, row-major by replica */
+    float *center;            /* R*M*3 */
+    float *rel;               /* R*N*3 */
+    float *pos;               /* R*N*3 */
+    double *temperature;      /* R */
+    float *translate_step;    /* R */
+    float *rotate_step;       /* R */
+    double *energy;           /* R, kcal/mol */
+    double *emat;             /* R*M*M, kcal/mol */
+    int64_t *counters;        /* R*3: accept (dE<0), acceptA (Boltzmann), reject */
+    int64_t *step;            /* R   MC steps taken by each replica */
+    int *label;               /* R */
+    int *rank_of;             /* R   temperature rank of replica r */
+    int *rep_at_rank;         /* R   replica holding temperature rank k */
+    int64_t *exchange_round;  /* 1 */
+    int64_t *exchange_stats;  /* 2: tests, accepted swaps */
+} orc_remc;
+
+/* Initialise positions (pos = rel + center), the molecule-pair matrix and
+ * the per-replica totals. */
+void orc_remc_init(orc_remc *s);
+
+/* The proposal of step `step` of replica r, exactly as the device draws it.
+ * Outputs: molecule, move kind (1 = rotate, 0 = translate), translation
+ * vector (float) or unit axis (double), and the Metropolis uniform. */
+void orc_mc_proposal(uint64_t seed, int r, int64_t step, int n_mol,
+                     int *mol, int *is_rotate, double axis[3], double *u_accept);
+
+/* Applies one rigid move to n residues (translation when is_rotate == 0):
+ * the move code of orc_remc_mc_steps, exposed for the move tests.  rcos,
+ * rsin = cos(angle/2), sin(angle/2). */
+void orc_apply_move(int is_rotate, const double axis_raw[3], float tstep, double rcos, double rsin,
+                    float box, const float c_in[3], float c_out[3], int n, const float *rel_in,
+                    float *rel_out, float *pos_out);
+
+/* nsteps MC steps on every replica (delta-E rows against the matrix). */
+void orc_remc_mc_steps(orc_remc *s, int nsteps);
+
+/* One exchange round over adjacent temperature ranks. */
+void orc_remc_exchange(orc_remc *s);
+
+/* ---------------------------------------------------------------------
+ * CPU baseline: the reference's CPU REMC loop (src/Replica.cpp:350-414,
+ * src/Simulation.cpp:996-1039): full Replica::E() recompute every MC step,
+ * float Metropolis delta, restore on reject; replicas split over pthreads
+ * as in src/Simulation.cpp:479-487.  Moves come from the same Philox
+ * streams.  Returns wall seconds.
+ * --------------------------------------------------------------------- */
+double orc_remc_reference_run(orc_remc *s, int nsteps, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,170 @@
+"""GPU energy parity: libcgppd_hip.so through the C ABI against the oracle.
+
+Bar (north star): every energy within 1e-5 relative of the FP64 oracle and
+of the reference-semantics CPU path (Replica::E), plus the reference's own
+GPU tolerance (Catch Approx, tests/testReferenceConformations.cpp:100-103)
+on the ten reference conformations.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN_SYSTEMS, REFERENCE_CONFORMATIONS, golden_system
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def close(got, want, rel=REL, floor=1e-9):
+    return abs(got - want) <= rel * abs(want) + floor
+
+
+def catch_approx(got, want):
+    """Catch 1.x Approx default: |a-b| < 100*FLT_EPSILON*(1 + max(|a|,|b|))."""
+    return abs(got - want) < 100 * np.finfo(np.float32).eps * (1 + max(abs(got), abs(want)))
+
+
+def engine_for(factory, eps, sigma, charge, box, mol_off, types, pos_list, n_nc=None):
+    e = factory()
+    e.set_force_field(eps, sigma, charge)
+    e.set_box(box)
+    e.set_system(mol_off, types, n_nc)
+    R = len(pos_list)
+    M = len(mol_off) - 1
+    rel = np.stack([np.asarray(p, np.float32) for p in pos_list])
+    e.set_replicas(np.zeros((R, M, 3), np.float32), rel, seed=1)
+    return e
+
+
+def test_reference_conformations(gpu_engine_factory, conformations, mj_ff):
+    rt, dhc, _ = oracle.constants()
+    for (off, types, pos), (elj, edh) in zip(conformations, REFERENCE_CONFORMATIONS):
+        e = engine_for(gpu_engine_factory, mj_ff.eps, mj_ff.sigma, mj_ff.charge, 1000.0, off, types, [pos])
+        lj, dh, tot = e.energy(0)
+        want = elj + edh
+        print(f"conformation total {tot:.9g} expected {want:.9g}")
+        assert catch_approx(tot, want)                         # EonDevice
+        assert catch_approx(e.energy_nc(0)[2], want)            # EonDeviceNC
+        e.energy_async(0)
+        assert catch_approx(e.energy_wait(0)[2], want)          # EonDeviceAsync + SumGridResults
+        rlj, rdh = oracle.energy_ref(off, types, mj_ff.charge, mj_ff.sigma, mj_ff.eps, pos, 1000.0)
+        flj, fdh = oracle.energy_fp64(off, types, mj_ff.charge, mj_ff.sigma, mj_ff.eps, pos, 1000.0)
+        assert close(lj, rlj) and close(dh, rdh)
+        assert close(lj, flj) and close(dh, fdh)
+        assert close(tot, oracle.total(flj, fdh))
+        e.close()
+
+
+@pytest.mark.parametrize("name", GOLDEN_SYSTEMS)
+def test_golden_systems(gpu_engine_factory, golden, name):
+    s = golden_system(golden, name)
+    e = engine_for(gpu_engine_factory, s["eps"], s["sigma"], s["charge"], s["box"], s["mol_off"], s["types"],
+                   [s["pos"]])
+    lj, dh, tot = e.energy(0)
+    print(f"{name}: gpu lj {lj:.10g} dh {dh:.10g}; reference build {s['energy'][0]:.10g} {s['energy'][1]:.10g}")
+    assert close(lj, s["energy"][0]) and close(dh, s["energy"][1]) and close(tot, s["energy"][2])
+    flj, fdh = oracle.energy_fp64(s["mol_off"], s["types"], s["charge"], s["sigma"], s["eps"], s["pos"], s["box"])
+    assert close(lj, flj) and close(dh, fdh)
+    _, _, tots, mat = e.energy_all(matrix=True)
+    assert close(tots[0], tot)
+    ref = s["matrix"]
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(mat[0], ref, rtol=REL, atol=REL * 1e-3 * scale + 1e-12)
+    e.close()
+
+
+def test_many_replicas_one_launch(gpu_engine_factory):
+    """energy_all over R replicas of different conformations (R % 8 == 0 uses
+    the XCD-aware item mapping, R = 5 the linear one)."""
+    from cgppd_amd import synth
+    spec = synth.make_system("multi", [40, 260, 75, 18, 130], 160.7, 1, 300.0, 300.0, 11)
+    rng = np.random.default_rng(2)
+    for R in (8, 5):
+        pos_list = []
+        for r in range(R):
+            p = spec.positions().copy()
+            for m in range(spec.n_mol):
+                a, b = spec.mol_off[m], spec.mol_off[m + 1]
+                p[a:b] += rng.uniform(-20, 20, 3).astype(np.float32)
+            pos_list.append(p)
+        e = engine_for(gpu_engine_factory, spec.ff.eps, spec.ff.sigma, spec.ff.charge, spec.box, spec.mol_off,
+                       spec.types, pos_list)
+        lj, dh, tot = e.energy_all()
+        for r in range(R):
+            flj, fdh = oracle.energy_fp64(spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps,
+                                          pos_list[r], spec.box)
+            assert close(lj[r], flj) and close(dh[r], fdh), (R, r)
+            one = e.energy(r)
+            assert close(one[2], tot[r], rel=1e-12, floor=1e-12)
+        e.close()
+
+
+def test_noncrowder_subset(gpu_engine_factory):
+    """EonDeviceNC: only the leading non-crowder molecules interact."""
+    from cgppd_amd import synth
+    spec = synth.make_system("nc", [50, 30, 80, 20, 60], 150.0, 1, 300.0, 300.0, 13)
+    pos = spec.positions()
+    for n_nc in (2, 3):
+        e = engine_for(gpu_engine_factory, spec.ff.eps, spec.ff.sigma, spec.ff.charge, spec.box, spec.mol_off,
+                       spec.types, [pos], n_nc=n_nc)
+        lj, dh, tot = e.energy_nc(0)
+        rlj, rdh = oracle.energy_nc_ref(n_nc, spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps,
+                                        pos, spec.box)
+        assert close(lj, rlj) and close(dh, rdh)
+        e.close()
+
+
+def test_update_molecule_and_upload(gpu_engine_factory):
+    """MoleculeDataToDevice / ReplicaDataToDevice equivalents."""
+    from cgppd_amd import synth
+    spec = synth.make_system("upd", [33, 71, 12], 100.0, 1, 300.0, 300.0, 17)
+    pos = spec.positions()
+    e = engine_for(gpu_engine_factory, spec.ff.eps, spec.ff.sigma, spec.ff.charge, spec.box, spec.mol_off,
+                   spec.types, [pos])
+    moved = pos.copy()
+    moved[33:104] += np.float32([3.0, -2.0, 1.5])
+    e.update_molecule(0, 1, moved[33:104])
+    rlj, rdh = oracle.energy_ref(spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps, moved, spec.box)
+    lj, dh, _ = e.energy(0)
+    assert close(lj, rlj) and close(dh, rdh)
+    e.upload_positions(0, pos)
+    rlj, rdh = oracle.energy_ref(spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos, spec.box)
+    lj, dh, _ = e.energy(0)
+    assert close(lj, rlj) and close(dh, rdh)
+    e.close()
+
+
+def test_single_residue_molecules_and_ragged_sizes(gpu_engine_factory):
+    from cgppd_amd import synth
+    spec = synth.make_system("rag", [1, 1, 2, 63, 64, 65, 129, 1], 90.0, 1, 300.0, 300.0, 19)
+    pos = spec.positions()
+    e = engine_for(gpu_engine_factory, spec.ff.eps, spec.ff.sigma, spec.ff.charge, spec.box, spec.mol_off,
+                   spec.types, [pos])
+    lj, dh, _ = e.energy(0)
+    flj, fdh = oracle.energy_fp64(spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos, spec.box)
+    assert close(lj, flj) and close(dh, fdh)
+    e.close()
+
+
+def test_one_molecule_has_zero_inter_energy(gpu_engine_factory):
+    from cgppd_amd import synth
+    spec = synth.make_system("one", [40], 90.0, 1, 300.0, 300.0, 23)
+    e = engine_for(gpu_engine_factory, spec.ff.eps, spec.ff.sigma, spec.ff.charge, spec.box, spec.mol_off,
+                   spec.types, [spec.positions()])
+    assert e.energy(0) == (0.0, 0.0, 0.0)
+    e.close()
+
+
+def test_config1_full_size_against_fp64(gpu_engine_factory):
+    """The 40-chain crowded box (about 6.4k residues) against the FP64 oracle."""
+    from cgppd_amd import synth
+    spec = synth.config(1)
+    pos = spec.positions()
+    e = engine_for(gpu_engine_factory, spec.ff.eps, spec.ff.sigma, spec.ff.charge, spec.box, spec.mol_off,
+                   spec.types, [pos])
+    lj, dh, tot = e.energy(0)
+    flj, fdh = oracle.energy_fp64(spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos, spec.box)
+    print(f"cfg1 gpu {lj:.12g} {dh:.12g} fp64 {flj:.12g} {fdh:.12g}")
+    assert close(lj, flj) and close(dh, fdh)
+    e.close()
