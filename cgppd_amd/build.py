@@ -61,15 +61,20 @@ def build_hip(verbose=False):
     return HIP_LIB
 
 
+CLI = PKG / "cgppd"
+
+
 def build_host(verbose=False):
-    srcs = sorted((CSRC / "host").glob("*.cpp"))
-    if not srcs:
-        return None
+    """C++ host library (Replica/Simulation API) and the cgppd command line."""
+    srcs = [CSRC / "host" / "cgppd_host.cpp", CSRC / "host" / "host_capi.cpp"]
+    common = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC / 'host'}"]
     tmp = HOST_LIB.with_suffix(".so.tmp")
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INCLUDE}", f"-I{CSRC / 'host'}",
-           "-o", tmp, *srcs, f"-L{PKG}", "-lcgppd_hip", "-Wl,-rpath,$ORIGIN", "-lpthread"]
-    _run(cmd, verbose)
+    _run([*common, "-shared", "-o", tmp, *srcs, f"-L{PKG}", "-lcgppd_hip", "-Wl,-rpath,$ORIGIN", "-lpthread"], verbose)
     os.replace(tmp, HOST_LIB)
+    tmp = CLI.with_suffix(".tmp")
+    _run([*common, "-o", tmp, CSRC / "host" / "main.cpp", f"-L{PKG}", "-lcgppd_host", "-lcgppd_hip",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, CLI)
     return HOST_LIB
 
 

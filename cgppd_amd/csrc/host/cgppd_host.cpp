@@ -1,0 +1,937 @@
+// cgppd_host.cpp -- C++ host side (see cgppd_host.h).
+#include "cgppd_host.h"
+
+#include <getopt.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+
+namespace cgppd {
+
+// ---------------------------------------------------------------------------
+// constants (float arithmetic as src/constants.h:8-26 evaluates it)
+// ---------------------------------------------------------------------------
+float rt_to_kcalmol()
+{
+    const float t_room = 298.0f, rgas = 8.3144621f, kcal = 4184.0f;
+    return t_room * rgas / kcal;
+}
+
+float dh_conversion_factor()
+{
+    const float e = 1.602176487e-19f, n_a = 6.02214129e23f, eps0 = 8.85418782e-12f;
+    const float kcal = 4184.0f, ang = 1e-10f, d_water = 80.0f;
+    const float pi_f = (float)3.14159265358979323846264338327950288L;
+    const float e2 = e * e;
+    return e2 * n_a / (4.0f * pi_f * eps0 * kcal * ang * d_water);
+}
+
+float r0_constant() { return (float)std::pow(2.0, (double)(1.0f / 6.0f)); }
+
+static std::runtime_error error(const std::string &what) { return std::runtime_error(what); }
+
+// ---------------------------------------------------------------------------
+// AminoAcids (src/AminoAcids.cpp:19-194)
+// ---------------------------------------------------------------------------
+void AminoAcids::init(const char *aminoAcidFile, const char *pairPotentialFile)
+{
+    clean = loadAminoAcidData(aminoAcidFile) && loadLJPotentialData(pairPotentialFile);
+}
+
+static bool getline_c(std::ifstream &in, std::string &line)
+{
+    if (!std::getline(in, line)) return false;
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    return true;
+}
+
+bool AminoAcids::loadAminoAcidData(const char *filename)
+{
+    std::ifstream in(filename);
+    if (!in.good()) throw error(std::string("Failed to open amino acid data file: ") + filename);
+    std::string line;
+    int i = 0;
+    while (getline_c(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        if (i >= AA_COUNT) throw error("more than 20 amino acids in " + std::string(filename));
+        AminoAcid &a = data[i];
+        a.name = line.substr(0, 19);
+        std::string s;
+        if (!getline_c(in, s)) throw error("truncated amino acid entry");
+        a.sname = s.substr(0, 3);
+        if (!getline_c(in, s)) throw error("truncated amino acid entry");
+        a.shortSymbol = s.empty() ? ' ' : s[0];
+        if (!getline_c(in, s)) throw error("truncated amino acid entry");
+        a.vanderWaalRadius = (float)std::atof(s.c_str());
+        if (!getline_c(in, s)) throw error("truncated amino acid entry");
+        float q = 0.f;
+        std::sscanf(s.c_str(), "%f", &q);
+        a.electrostaticCharge = q;
+        a.index = i;
+        ++i;
+    }
+    if (i != AA_COUNT) throw error("expected 20 amino acids in " + std::string(filename));
+    return true;
+}
+
+bool AminoAcids::loadLJPotentialData(const char *filename)
+{
+    std::ifstream in(filename);
+    if (!in.good()) throw error(std::string("Failed to open LJ potential data file: ") + filename);
+    std::string line;
+    int labels[AA_COUNT];
+    bool labelline = true;
+    int start = 0;
+    while (getline_c(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        std::string tok;
+        if (labelline) {
+            int n = 0;
+            while (ss >> tok) {
+                if (n >= AA_COUNT) throw error("too many labels in pair table");
+                labels[n] = getAminoAcidIndex(tok.c_str());
+                if (labels[n] < 0) throw error("unknown residue label in pair table: " + tok);
+                ++n;
+            }
+            if (n != AA_COUNT) throw error("pair table label row needs 20 labels");
+            labelline = false;
+            continue;
+        }
+        if (start >= AA_COUNT) break;
+        ss >> tok;  // row label
+        for (int j = start; j < AA_COUNT; ++j) {
+            if (!(ss >> tok)) throw error("pair table row too short");
+            float e = 0.f;
+            std::sscanf(tok.c_str(), "%f", &e);
+            LJpotentials[labels[start]][labels[j]] = e;
+            LJpotentials[labels[j]][labels[start]] = e;
+        }
+        ++start;
+    }
+    if (start != AA_COUNT) throw error("incomplete pair table " + std::string(filename));
+    return true;
+}
+
+int AminoAcids::getAminoAcidIndex(const char *NAME) const
+{
+    const size_t n = std::strlen(NAME);
+    if (n < 1) return -1;
+    if (n == 3) {
+        std::string key{(char)std::toupper(NAME[0]), (char)std::tolower(NAME[1]), (char)std::tolower(NAME[2])};
+        for (int i = AA_COUNT - 1; i >= 0; --i)
+            if (data[i].sname == key) return i;
+        return -1;
+    }
+    if (n == 1) {
+        const char c = (char)std::toupper(NAME[0]);
+        for (int i = AA_COUNT - 1; i >= 0; --i)
+            if (data[i].shortSymbol == c) return i;
+        return -1;
+    }
+    std::string key(NAME);
+    key[0] = (char)std::toupper(key[0]);
+    for (int i = AA_COUNT - 1; i >= 0; --i)
+        if (data[i].name == key) return i;
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Potential (src/Potential.cpp:5-42,116-230)
+// ---------------------------------------------------------------------------
+Potential::Potential(double LJ_kcal, double DH_kcal)
+{
+    LJ = LJ_kcal / rt_to_kcalmol();
+    DH = DH_kcal / dh_conversion_factor();
+}
+double Potential::total_LJ() const { return LJ * rt_to_kcalmol(); }
+double Potential::total_DH() const { return DH * dh_conversion_factor(); }
+double Potential::total() const { return DH * dh_conversion_factor() + LJ * rt_to_kcalmol(); }
+bool Potential::almost_equal(const Potential &p, float eps) const
+{
+    return std::fabs(LJ - p.LJ) < eps && std::fabs(DH - p.DH) < eps;
+}
+
+double calculate_LJ(int ti, int tj, double r, const AminoAcids &aa)
+{
+    const double Eij = (double)(0.159f * (aa.LJpotentials[ti][tj] - (-2.27f)));
+    const double sij = (double)(0.5f * (aa.data[ti].vanderWaalRadius + aa.data[tj].vanderWaalRadius));
+    const double t = std::pow(sij / r, 6.0);
+    double lj = -4.0 * Eij * t * (t - 1.0);
+    if (Eij > 0.0 && r < sij * (double)r0_constant()) lj = -lj + 2.0 * Eij;
+    return lj;
+}
+
+double calculate_DH(float qi, float qj, double r)
+{
+    const float e = std::exp((float)(-r / 10.0));
+    return (double)(qi * qj * e) / r;
+}
+
+double Residue::distance(const Residue &o, float L) const
+{
+    float X = position.x - o.position.x, Y = position.y - o.position.y, Z = position.z - o.position.z;
+    if (L != 0.0f) {
+        X = X - L * std::round(X / L);
+        Y = Y - L * std::round(Y / L);
+        Z = Z - L * std::round(Z / L);
+    }
+    return (double)std::sqrt(X * X + Y * Y + Z * Z);
+}
+
+// ---------------------------------------------------------------------------
+// Molecule (src/Molecule.cpp:25-64,146-208,506-612)
+// ---------------------------------------------------------------------------
+static float scan_float(const std::string &line, size_t col)
+{
+    float v = 0.f;
+    if (col < line.size()) std::sscanf(line.c_str() + col, "%f", &v);
+    return v;
+}
+
+void Molecule::initFromPDB(const char *pdbfilename, const AminoAcids &aa)
+{
+    filename = pdbfilename;
+    std::ifstream in(pdbfilename);
+    if (!in.good()) throw error(std::string("Failed to open PDB file: ") + pdbfilename);
+    Residues.clear();
+    chainCount = 0;
+    std::string line;
+    while (getline_c(in, line)) {
+        if (line.compare(0, 4, "ATOM") == 0) {
+            char name[16] = {0}, resName[16] = {0};
+            if (line.size() > 12) std::sscanf(line.c_str() + 12, "%15s", name);
+            if (std::strcmp(name, "CA") != 0) continue;
+            if (line.size() > 17) std::sscanf(line.c_str() + 17, "%15s", resName);
+            Residue R;
+            R.aminoAcidIndex = aa.getAminoAcidIndex(resName);
+            if (R.aminoAcidIndex < 0) throw error(std::string("unknown residue ") + resName + " in " + pdbfilename);
+            R.electrostaticCharge = aa.get(R.aminoAcidIndex).electrostaticCharge;
+            R.vanderWaalRadius = aa.get(R.aminoAcidIndex).vanderWaalRadius;
+            R.chainId = line.size() > 21 ? line[21] : ' ';
+            R.chain_int_id = chainCount;
+            if (line.size() > 22) std::sscanf(line.c_str() + 22, "%d", &R.resSeq);
+            R.position.x = scan_float(line, 30);
+            R.position.y = scan_float(line, 38);
+            R.position.z = scan_float(line, 46);
+            R.moleculeId = index;
+            Residues.push_back(R);
+        } else if (line.compare(0, 3, "TER") == 0) {
+            ++chainCount;
+        } else if (line.compare(0, 3, "END") == 0) {
+            break;
+        }
+    }
+    if (!Residues.empty() && chainCount == 0) chainCount = 1;
+    residueCount = (int)Residues.size();
+    if (residueCount == 0) throw error(std::string("no C-alpha atoms in ") + pdbfilename);
+    // src/Molecule.cpp:590-609: float accumulation of the centre
+    Vector3f c;
+    for (const Residue &r : Residues) {
+        c.x += r.position.x;
+        c.y += r.position.y;
+        c.z += r.position.z;
+    }
+    c.x /= residueCount;
+    c.y /= residueCount;
+    c.z /= residueCount;
+    center = c;
+    for (Residue &r : Residues) {
+        r.relativePosition.x = r.position.x - c.x;
+        r.relativePosition.y = r.position.y - c.y;
+        r.relativePosition.z = r.position.z - c.z;
+    }
+}
+
+void Molecule::setPosition(Vector3f c)
+{
+    center = c;
+    for (Residue &r : Residues) {
+        r.position.x = r.relativePosition.x + c.x;
+        r.position.y = r.relativePosition.y + c.y;
+        r.position.z = r.relativePosition.z + c.z;
+    }
+}
+
+void Molecule::translate(Vector3f v)
+{
+    Vector3f c{center.x + v.x, center.y + v.y, center.z + v.z};
+    if (bounding_value != 0.0f) {
+        c.x = std::fmod(c.x, bounding_value);
+        c.y = std::fmod(c.y, bounding_value);
+        c.z = std::fmod(c.z, bounding_value);
+    }
+    setPosition(c);
+}
+
+void Molecule::rotate(const double axis[3], double angle)
+{
+    // src/Quaternion.cpp:12-38,95-112
+    double w = std::cos(angle / 2.0), s = std::sin(angle / 2.0);
+    double x = s * axis[0], y = s * axis[1], z = s * axis[2];
+    const double m = std::sqrt(w * w + x * x + y * y + z * z);
+    w /= m;
+    x /= m;
+    y /= m;
+    z /= m;
+    const double w2 = w * w, x2 = x * x, y2 = y * y, z2 = z * z;
+    const double wx = w * x, wy = w * y, wz = w * z, xy = x * y, xz = x * z, yz = y * z;
+    for (Residue &r : Residues) {
+        const double vx = r.relativePosition.x, vy = r.relativePosition.y, vz = r.relativePosition.z;
+        r.relativePosition.x = (float)(vx * (w2 + x2 - y2 - z2) + (vy * (xy - wz) + vz * (wy + xz)) * 2.0);
+        r.relativePosition.y = (float)((vx * (wz + xy) + vz * (yz - wx)) * 2.0 + vy * (w2 - x2 + y2 - z2));
+        r.relativePosition.z = (float)((vx * (xz - wy) + vy * (yz + wx)) * 2.0 + vz * (w2 - x2 - y2 + z2));
+        r.position.x = center.x + r.relativePosition.x;
+        r.position.y = center.y + r.relativePosition.y;
+        r.position.z = center.z + r.relativePosition.z;
+    }
+}
+
+void Molecule::init(const moldata &mol, const AminoAcids &aa, int idx, float bv)
+{
+    index = idx;
+    bounding_value = bv;
+    name = mol.name;
+    initFromPDB(mol.pdbfilename.c_str(), aa);
+    if (mol.translate) {
+        if (mol.px || mol.py || mol.pz) translate(Vector3f{mol.px, mol.py, mol.pz});
+    } else {
+        setPosition(Vector3f{mol.px, mol.py, mol.pz});
+    }
+    if (mol.ra) {
+        double v[3] = {mol.rx, mol.ry, mol.rz};
+        const double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (n > 0) {
+            v[0] /= n;
+            v[1] /= n;
+            v[2] /= n;
+            rotate(v, mol.ra);
+        }
+    }
+    crowder = mol.crowder;
+}
+
+// ---------------------------------------------------------------------------
+// DeviceContext (replaces setup_CUDA / teardown_CUDA)
+// ---------------------------------------------------------------------------
+DeviceContext::DeviceContext(int device, float box_dimension, const AminoAcids &aa)
+{
+    int rc = cgppd_create(&ctx_, device);
+    if (rc) throw error("cgppd_create failed on device " + std::to_string(device) + " (status " + std::to_string(rc) + ")");
+    float eps[AA_COUNT * AA_COUNT], sigma[AA_COUNT], charge[AA_COUNT];
+    for (int i = 0; i < AA_COUNT; ++i) {
+        sigma[i] = aa.data[i].vanderWaalRadius;
+        charge[i] = aa.data[i].electrostaticCharge;
+        for (int j = 0; j < AA_COUNT; ++j) eps[i * AA_COUNT + j] = aa.LJpotentials[i][j];
+    }
+    check(cgppd_set_force_field(ctx_, eps, sigma, charge), "cgppd_set_force_field");
+    check(cgppd_set_box(ctx_, box_dimension), "cgppd_set_box");
+}
+
+DeviceContext::~DeviceContext()
+{
+    if (ctx_) cgppd_destroy(ctx_);
+}
+
+void DeviceContext::check(int rc, const char *what) const
+{
+    if (rc != CGPPD_OK) throw error(std::string(what) + ": " + cgppd_last_error(ctx_));
+}
+
+// ---------------------------------------------------------------------------
+// Replica
+// ---------------------------------------------------------------------------
+void Replica::loadMolecule(const moldata &mol)
+{
+    Molecule m;
+    m.init(mol, aminoAcids, moleculeCount, boundingValue);
+    residueCount += m.residueCount;
+    max_residue_count = std::max(max_residue_count, m.residueCount);
+    if (!mol.crowder) {
+        ++nonCrowderCount;
+        nonCrowderResidues += m.residueCount;
+    }
+    molecules.push_back(std::move(m));
+    ++moleculeCount;
+}
+
+void Replica::init_first_replica(const argdata &parameters, const AminoAcids &amino_acid_data, int)
+{
+    aminoAcids = amino_acid_data;
+    boundingValue = parameters.bound;
+    molecules.clear();
+    moleculeCount = residueCount = nonCrowderCount = nonCrowderResidues = max_residue_count = 0;
+    for (const moldata &m : parameters.mdata) loadMolecule(m);
+    // the device requires non-crowders first (src/Replica.cpp:924)
+    bool seen_crowder = false;
+    for (const Molecule &m : molecules) {
+        if (m.crowder) seen_crowder = true;
+        else if (seen_crowder) throw error("non-crowder molecules must precede crowders");
+    }
+}
+
+void Replica::copy(const Replica &r)
+{
+    aminoAcids = r.aminoAcids;
+    molecules = r.molecules;
+    moleculeCount = r.moleculeCount;
+    residueCount = r.residueCount;
+    nonCrowderCount = r.nonCrowderCount;
+    nonCrowderResidues = r.nonCrowderResidues;
+    max_residue_count = r.max_residue_count;
+    boundingValue = r.boundingValue;
+    label = r.label;
+    temperature = r.temperature;
+    potential = r.potential;
+}
+
+void Replica::init_child_replica(const Replica &ir, int index, double geometricTemperature, double geometricRotation,
+                                 double geometricTranslate, const argdata &parameters)
+{
+    copy(ir);
+    label = (short)index;
+    temperature = (float)(parameters.temperatureMin * std::pow(geometricTemperature, index));
+    rotateStep = MIN_ROTATION * std::pow(geometricRotation, index);
+    translateStep = (float)(MIN_TRANSLATION * std::pow(geometricTranslate, index));
+}
+
+void Replica::exchangeReplicas(Replica &r)
+{
+    std::swap(label, r.label);
+    std::swap(temperature, r.temperature);
+    std::swap(accept, r.accept);
+    std::swap(acceptA, r.acceptA);
+    std::swap(reject, r.reject);
+    std::swap(totalAcceptReject, r.totalAcceptReject);
+    std::swap(totalAccept, r.totalAccept);
+    std::swap(acceptanceRatio, r.acceptanceRatio);
+    std::swap(accumulativeAcceptanceRatio, r.accumulativeAcceptanceRatio);
+    std::swap(boundSamples, r.boundSamples);
+    std::swap(samplesSinceLastExchange, r.samplesSinceLastExchange);
+    std::swap(totalBoundSamples, r.totalBoundSamples);
+    std::swap(totalSamples, r.totalSamples);
+    std::swap(fractionBound, r.fractionBound);
+    std::swap(accumulativeFractionBound, r.accumulativeFractionBound);
+}
+
+Potential Replica::E(const Molecule *a, const Molecule *b) const
+{
+    Potential p;
+    for (const Residue &ri : a->Residues)
+        for (const Residue &rj : b->Residues) {
+            const double r = ri.distance(rj, boundingValue) + (double)EPS;
+            p.increment_LJ(calculate_LJ(ri.aminoAcidIndex, rj.aminoAcidIndex, r, aminoAcids));
+            p.increment_DH(calculate_DH(ri.electrostaticCharge, rj.electrostaticCharge, r));
+        }
+    return p;
+}
+
+Potential Replica::E() const
+{
+    // src/Replica.cpp:432-493 (one running Potential over all pairs)
+    Potential p;
+    for (int I = 0; I < moleculeCount; ++I)
+        for (int J = I + 1; J < moleculeCount; ++J)
+            for (const Residue &ri : molecules[I].Residues)
+                for (const Residue &rj : molecules[J].Residues) {
+                    const double r = ri.distance(rj, boundingValue) + (double)EPS;
+                    p.increment_LJ(calculate_LJ(ri.aminoAcidIndex, rj.aminoAcidIndex, r, aminoAcids));
+                    p.increment_DH(calculate_DH(ri.electrostaticCharge, rj.electrostaticCharge, r));
+                }
+    return p;
+}
+
+std::vector<int> Replica::molecule_offsets() const
+{
+    std::vector<int> off(moleculeCount + 1, 0);
+    for (int m = 0; m < moleculeCount; ++m) off[m + 1] = off[m] + molecules[m].residueCount;
+    return off;
+}
+
+std::vector<int> Replica::residue_types() const
+{
+    std::vector<int> t;
+    for (const Molecule &m : molecules)
+        for (const Residue &r : m.Residues) t.push_back(r.aminoAcidIndex);
+    return t;
+}
+
+std::vector<float> Replica::positions() const
+{
+    std::vector<float> x;
+    for (const Molecule &m : molecules)
+        for (const Residue &r : m.Residues) x.insert(x.end(), {r.position.x, r.position.y, r.position.z});
+    return x;
+}
+
+std::vector<float> Replica::relative_positions() const
+{
+    std::vector<float> x;
+    for (const Molecule &m : molecules)
+        for (const Residue &r : m.Residues)
+            x.insert(x.end(), {r.relativePosition.x, r.relativePosition.y, r.relativePosition.z});
+    return x;
+}
+
+std::vector<float> Replica::centres() const
+{
+    std::vector<float> c;
+    for (const Molecule &m : molecules) c.insert(c.end(), {m.center.x, m.center.y, m.center.z});
+    return c;
+}
+
+void Replica::setup_device(DeviceContext *dev, int slot)
+{
+    dev_ = dev;
+    slot_ = slot;
+}
+
+void Replica::ReplicaDataToDevice()
+{
+    if (!dev_) throw error("Replica::ReplicaDataToDevice: no device context (call setup_device)");
+    const std::vector<float> x = positions();
+    dev_->check(cgppd_upload_positions(dev_->ctx(), slot_, x.data()), "cgppd_upload_positions");
+}
+
+void Replica::MoleculeDataToDevice(int moleculeId)
+{
+    if (!dev_) throw error("Replica::MoleculeDataToDevice: no device context");
+    std::vector<float> x;
+    for (const Residue &r : molecules.at(moleculeId).Residues)
+        x.insert(x.end(), {r.position.x, r.position.y, r.position.z});
+    dev_->check(cgppd_update_molecule(dev_->ctx(), slot_, moleculeId, x.data()), "cgppd_update_molecule");
+}
+
+double Replica::EonDevice()
+{
+    if (!dev_) throw error("Replica::EonDevice: no device context");
+    double t = 0.0;
+    dev_->check(cgppd_energy(dev_->ctx(), slot_, nullptr, nullptr, &t), "cgppd_energy");
+    return t;
+}
+
+double Replica::EonDeviceNC()
+{
+    if (!dev_) throw error("Replica::EonDeviceNC: no device context");
+    double t = 0.0;
+    dev_->check(cgppd_energy_nc(dev_->ctx(), slot_, nullptr, nullptr, &t), "cgppd_energy_nc");
+    return t;
+}
+
+void Replica::EonDeviceAsync()
+{
+    if (!dev_) throw error("Replica::EonDeviceAsync: no device context");
+    dev_->check(cgppd_energy_async(dev_->ctx(), slot_), "cgppd_energy_async");
+}
+
+double Replica::SumGridResults()
+{
+    if (!dev_) throw error("Replica::SumGridResults: no device context");
+    double t = 0.0;
+    dev_->check(cgppd_energy_wait(dev_->ctx(), slot_, nullptr, nullptr, &t), "cgppd_energy_wait");
+    return t;
+}
+
+void Replica::fraction_bound(FILE *f)
+{
+    totalBoundSamples += boundSamples;
+    totalSamples += samplesSinceLastExchange;
+    fractionBound = float(boundSamples) / std::max(1.0f, float(samplesSinceLastExchange));
+    accumulativeFractionBound = float(totalBoundSamples) / std::max(1.0f, float(totalSamples));
+    if (f) std::fprintf(f, "| %6.4f %6.4f ", fractionBound, accumulativeFractionBound);
+    boundSamples = 0;
+    samplesSinceLastExchange = 0;
+}
+
+void Replica::acceptance_ratio(FILE *f)
+{
+    totalAccept += acceptA + accept;
+    totalAcceptReject += acceptA + accept + reject;
+    acceptanceRatio = float(acceptA + accept) / std::max(1.0f, float(acceptA + accept + reject));
+    accumulativeAcceptanceRatio = float(totalAccept) / std::max(1.0f, float(totalAcceptReject));
+    if (f) std::fprintf(f, "| %6.4f %6.4f ", acceptanceRatio, accumulativeAcceptanceRatio);
+    acceptA = accept = reject = 0;
+}
+
+void Replica::saveAsSinglePDB(const char *filename, const char *prefix, int sample) const
+{
+    // src/Replica.cpp:1124-1168
+    FILE *out = std::fopen(filename, "w");
+    if (!out) throw error(std::string("cannot write ") + filename);
+    std::fprintf(out, "REMARK simulation: %s \n", prefix);
+    std::fprintf(out, "REMARK temperature: %5.1f \n", temperature);
+    std::fprintf(out, "REMARK sample: %d \n", sample);
+    std::fprintf(out, "REMARK potential: %0.10f \n", float(potential));
+    int item = 0;
+    for (int m = 0; m < nonCrowderCount; ++m) {
+        const Molecule &mol = molecules[m];
+        const char chain = (char)('A' + m);
+        for (const Residue &r : mol.Residues) {
+            ++item;
+            std::fprintf(out, "ATOM  %5d CA   %3s %c%4d    %8.3f%8.3f%8.3f%6.2f%6.2f\n", item,
+                         aminoAcids.get(r.aminoAcidIndex).sname.c_str(), chain, r.resSeq, r.position.x, r.position.y,
+                         r.position.z, 0.0f, 0.0f);
+        }
+        std::fprintf(out, "TER   %5d      %3s %c%4d \n", item,
+                     aminoAcids.get(mol.Residues.back().aminoAcidIndex).sname.c_str(), chain, mol.Residues.back().resSeq);
+    }
+    std::fprintf(out, "END \n");
+    std::fclose(out);
+}
+
+// ---------------------------------------------------------------------------
+// Simulation
+// ---------------------------------------------------------------------------
+Simulation::~Simulation() { closeSamplingFiles(); }
+
+void Simulation::printHelp() const
+{
+    std::printf(
+        "Usage: cgppd -f <config file> [options]\n"
+        "\t-h|--help\n\t-f|--file <config>\n\t-v|--verbosity <0-4>\n\t-q|--nosim\n"
+        "\t-t|--threads n (accepted, unused: replicas run on the device)\n"
+        "\t-s|--streams n (accepted, unused)\n\t-g|--gpus n (one GPU)\n\t-z|--gpuoffset n (device id)\n"
+        "\t-m|--mcsteps n\n\t-a|--sampleafter n\n\t-e|--resteps n\n\t-r|--replicas n\n\t-o|--output prefix\n"
+        "\t-b|--boundary L\n\t-x|--tmax T\n\t-n|--tmin T\n\t-d|--blockdim n (accepted, unused)\n"
+        "\t-S|--seed n (device Philox seed)\n\t-G|--nograph (launch intervals without a hipGraph)\n");
+}
+
+void Simulation::getArgs(int argc, char **argv, bool first_pass)
+{
+    // src/Simulation.cpp:78-263: two passes, the file first
+    static const struct option long_options[] = {
+        {"file", required_argument, 0, 'f'},      {"help", no_argument, 0, 'h'},
+        {"verbosity", required_argument, 0, 'v'}, {"preview", no_argument, 0, 'p'},
+        {"nosim", no_argument, 0, 'q'},           {"threads", required_argument, 0, 't'},
+        {"streams", required_argument, 0, 's'},   {"gpus", required_argument, 0, 'g'},
+        {"gpuoffset", required_argument, 0, 'z'}, {"mcsteps", required_argument, 0, 'm'},
+        {"sampleafter", required_argument, 0, 'a'}, {"resteps", required_argument, 0, 'e'},
+        {"replicas", required_argument, 0, 'r'},  {"output", required_argument, 0, 'o'},
+        {"boundary", required_argument, 0, 'b'},  {"tmax", required_argument, 0, 'x'},
+        {"tmin", required_argument, 0, 'n'},      {"blockdim", required_argument, 0, 'd'},
+        {"seed", required_argument, 0, 'S'},      {"nograph", no_argument, 0, 'G'},
+        {0, 0, 0, 0},
+    };
+    const char short_options[] = "hf:pqt:s:g:z:m:a:e:r:o:b:x:n:d:v:w:S:G";
+    int opt_index = 0;
+    optind = 1;
+    if (first_pass) {
+        while (true) {
+            const int opt = getopt_long(argc, argv, short_options, long_options, &opt_index);
+            if (opt == -1) break;
+            if (opt == 'h') {
+                printHelp();
+                throw error("help");
+            }
+            if (opt == 'f') {
+                parameters.file = optarg;
+                parameters.inputFile = true;
+            }
+        }
+        if (!parameters.inputFile) {
+            printHelp();
+            throw error("No configuration file provided.");
+        }
+        return;
+    }
+    while (true) {
+        const int opt = getopt_long(argc, argv, short_options, long_options, &opt_index);
+        if (opt == -1) break;
+        switch (opt) {
+        case 'q': parameters.skipsimulation = true; break;
+        case 't': parameters.threads = std::atoi(optarg); break;
+        case 's': parameters.streams = std::atoi(optarg); break;
+        case 'g': parameters.gpus = std::atoi(optarg); break;
+        case 'z': parameters.gpuoffset = std::atoi(optarg); break;
+        case 'm': parameters.MCsteps = std::atoi(optarg); break;
+        case 'a': parameters.sampleStartsAfter = std::atoi(optarg); break;
+        case 'e': parameters.REsteps = std::atoi(optarg); break;
+        case 'r': parameters.replicas = std::atoi(optarg); break;
+        case 'o': parameters.prefix = std::string(optarg) + "_" + std::to_string(parameters.pid); break;
+        case 'b': parameters.bound = (float)std::atof(optarg); break;
+        case 'x': parameters.temperatureMax = (float)std::atof(optarg); break;
+        case 'n': parameters.temperatureMin = (float)std::atof(optarg); break;
+        case 'd':
+            parameters.cuda_blockSize = std::atoi(optarg);
+            parameters.auto_blockdim = false;
+            break;
+        case 'S': parameters.seed = std::strtoull(optarg, nullptr, 10); break;
+        case 'G': parameters.use_graph = false; break;
+        default: break;
+        }
+    }
+}
+
+void Simulation::loadArgsFromFile()
+{
+    // src/Simulation.cpp:265-453
+    std::ifstream in(parameters.file);
+    if (!in.good()) throw error("Failed to open config file: " + parameters.file);
+    enum { PARAMS, MOLECULES, CROWDERS, SEGMENTS } section = PARAMS;
+    std::string line;
+    while (getline_c(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        if (line == "files") {
+            section = MOLECULES;
+        } else if (line == "crowders") {
+            section = CROWDERS;
+        } else if (line == "segments") {
+            section = SEGMENTS;
+        } else if (section == MOLECULES || section == CROWDERS) {
+            moldata m;
+            char flag = 0, pdb[512] = {0}, name[512] = {0};
+            const int n = std::sscanf(line.c_str(), "%c(%f,%f,%f) r(%f,%f,%f,%f) %511s %511s", &flag, &m.px, &m.py,
+                                      &m.pz, &m.rx, &m.ry, &m.rz, &m.ra, pdb, name);
+            if (n < 8) {
+                std::fprintf(stderr, "warning: unable to parse molecule: %s\n", line.c_str());
+                continue;
+            }
+            m.translate = flag == 't';
+            m.pdbfilename = pdb;
+            m.name = name;
+            m.crowder = section == CROWDERS;
+            if (section == MOLECULES) ++parameters.nonCrowders;
+            parameters.mdata.push_back(m);
+            if (!m.name.empty()) parameters.mdata_map[m.name] = (int)parameters.mdata.size() - 1;
+        } else if (section == SEGMENTS) {
+            std::istringstream ss(line);
+            std::string molname, word;
+            ss >> molname >> word;
+            auto it = parameters.mdata_map.find(molname);
+            if (it == parameters.mdata_map.end()) throw error("segment for unknown molecule " + molname);
+            if (word == "all") {
+                parameters.mdata[it->second].all_flexible = true;
+            } else {
+                std::vector<int> seg;
+                do {
+                    seg.push_back(std::atoi(word.c_str()) - 1);
+                } while (ss >> word);
+                parameters.mdata[it->second].segments.push_back(seg);
+            }
+        } else {
+            std::istringstream ss(line);
+            std::string key, value;
+            ss >> key >> value;
+            if (key == "gpus") parameters.gpus = std::atoi(value.c_str());
+            else if (key == "gpuoffset") parameters.gpuoffset = std::atoi(value.c_str());
+            else if (key == "streams") parameters.streams = std::atoi(value.c_str());
+            else if (key == "threads") parameters.threads = std::atoi(value.c_str());
+            else if (key == "mcsteps") parameters.MCsteps = std::atoi(value.c_str());
+            else if (key == "resteps") parameters.REsteps = std::atoi(value.c_str());
+            else if (key == "temperaturemax") parameters.temperatureMax = (float)std::atof(value.c_str());
+            else if (key == "temperaturemin") parameters.temperatureMin = (float)std::atof(value.c_str());
+            else if (key == "boundary") parameters.bound = (float)std::atof(value.c_str());
+            else if (key == "replicas") parameters.replicas = std::atoi(value.c_str());
+            else if (key == "samplefrequency") parameters.sampleFrequency = std::atoi(value.c_str());
+            else if (key == "sampleafter") parameters.sampleStartsAfter = std::atoi(value.c_str());
+            else if (key == "prefix") parameters.prefix = value + "_" + std::to_string(parameters.pid);
+            else if (key == "title") {
+                std::string rest, w;
+                rest = value;
+                while (ss >> w) rest += " " + w;
+                parameters.title = rest;
+            }
+            else if (key == "seed") parameters.seed = std::strtoull(value.c_str(), nullptr, 10);
+            else if (key == "aminoacids") parameters.aminoAcidFile = value;
+            else if (key == "pairpotentials") parameters.pairPotentialFile = value;
+            else std::fprintf(stderr, "warning: unknown parameter: %s\n", key.c_str());
+        }
+    }
+}
+
+void Simulation::check_and_modify_parameters()
+{
+    // src/Simulation.cpp:455-514 (thread/stream checks no longer apply)
+    if (parameters.bound <= 0) parameters.bound = 150.0f;
+    if (parameters.temperatureMax < parameters.temperatureMin)
+        std::swap(parameters.temperatureMax, parameters.temperatureMin);
+    if (parameters.replicas < 1) throw error("need at least one replica");
+    if (parameters.REsteps < 1) parameters.REsteps = 1;
+    if (parameters.gpus != 1) {
+        std::fprintf(stderr, "warning: this build runs on one GPU; using device %d\n", parameters.gpuoffset);
+        parameters.gpus = 1;
+    }
+}
+
+void Simulation::init(int argc, char **argv, int pid)
+{
+    parameters.pid = pid;
+    getArgs(argc, argv, true);
+    loadArgsFromFile();
+    getArgs(argc, argv, false);
+    check_and_modify_parameters();
+    aminoAcidData.init(parameters.aminoAcidFile.c_str(), parameters.pairPotentialFile.c_str());
+    initialReplica.init_first_replica(parameters, aminoAcidData, 30);
+    if (initialReplica.moleculeCount == 0) throw error("No molecules loaded.");
+    initialReplica.potential = initialReplica.E().total();
+    const int R = parameters.replicas;
+    // src/Simulation.cpp:590-592
+    geometricTemperature = std::pow(double(parameters.temperatureMax / parameters.temperatureMin), 1.0 / double(std::max(R - 1, 1)));
+    geometricTranslate = std::pow(double(MAX_TRANSLATION / MIN_TRANSLATION), 1.0 / double(std::max(R - 1, 1)));
+    geometricRotation = std::pow(double(MAX_ROTATION / MIN_ROTATION), 1.0 / double(std::max(R - 1, 1)));
+    replica.assign(R, Replica());
+    for (int i = 0; i < R; ++i)
+        replica[i].init_child_replica(initialReplica, i, geometricTemperature, geometricRotation, geometricTranslate,
+                                      parameters);
+    position_of_temperature.clear();
+    temperature.clear();
+    for (int i = 0; i < R; ++i) {
+        position_of_temperature[replica[i].temperature] = i;
+        temperature[i] = replica[i].temperature;
+    }
+    near_300k = temperature[0];
+    for (auto &p : position_of_temperature)
+        if (std::fabs(p.first - 300.0f) < std::fabs(near_300k - 300.0f)) near_300k = p.first;
+}
+
+void Simulation::setup_device(int device)
+{
+    dev_.reset(new DeviceContext(device, parameters.bound, aminoAcidData));
+    const std::vector<int> off = initialReplica.molecule_offsets();
+    const std::vector<int> types = initialReplica.residue_types();
+    dev_->check(cgppd_set_system(dev_->ctx(), initialReplica.moleculeCount, off.data(), types.data(),
+                                 initialReplica.nonCrowderCount),
+                "cgppd_set_system");
+    const int R = (int)replica.size(), M = initialReplica.moleculeCount, N = initialReplica.residueCount;
+    std::vector<float> centres, rel;
+    std::vector<double> temps(R);
+    std::vector<float> tsteps(R), rsteps(R);
+    std::vector<int> labels(R);
+    centres.reserve((size_t)R * M * 3);
+    rel.reserve((size_t)R * N * 3);
+    for (int r = 0; r < R; ++r) {
+        const std::vector<float> c = replica[r].centres(), x = replica[r].relative_positions();
+        centres.insert(centres.end(), c.begin(), c.end());
+        rel.insert(rel.end(), x.begin(), x.end());
+        temps[r] = replica[r].temperature;
+        tsteps[r] = replica[r].translateStep;
+        rsteps[r] = (float)replica[r].rotateStep;
+        labels[r] = replica[r].label;
+    }
+    dev_->check(cgppd_set_replicas(dev_->ctx(), R, centres.data(), rel.data(), temps.data(), tsteps.data(),
+                                   rsteps.data(), labels.data(), parameters.seed),
+                "cgppd_set_replicas");
+    for (int r = 0; r < R; ++r) replica[r].setup_device(dev_.get(), r);
+}
+
+void Simulation::initSamplingFiles()
+{
+    if (parameters.prefix.empty()) parameters.prefix = "cgppd_" + std::to_string(parameters.pid);
+    const std::string dir = "output/" + parameters.prefix;
+    ::mkdir("output", 0755);
+    ::mkdir(dir.c_str(), 0755);
+    fractionBoundFile = std::fopen((dir + "/fractionBound").c_str(), "w");
+    acceptanceRatioFile = std::fopen((dir + "/acceptance_ratios").c_str(), "w");
+    exchangeFrequencyFile = std::fopen((dir + "/exchange_freq").c_str(), "w");
+    boundConformationsFile = std::fopen((dir + "/boundconformations").c_str(), "w");
+    if (!fractionBoundFile || !acceptanceRatioFile || !exchangeFrequencyFile || !boundConformationsFile)
+        throw error("cannot create output files under " + dir);
+    // src/Simulation.cpp:1114-1194 headers
+    for (FILE *f : {fractionBoundFile, acceptanceRatioFile}) {
+        std::fprintf(f, "# Temperature: ");
+        for (auto &t : temperature) std::fprintf(f, "%0.1fKi ", t.second);
+        std::fprintf(f, "\n");
+    }
+    std::fprintf(exchangeFrequencyFile, "# step exchange_fraction accumulative_fraction\n");
+}
+
+void Simulation::closeSamplingFiles()
+{
+    for (FILE **f : {&fractionBoundFile, &acceptanceRatioFile, &exchangeFrequencyFile, &boundConformationsFile}) {
+        if (*f) std::fclose(*f);
+        *f = nullptr;
+    }
+}
+
+void Simulation::sync_from_device()
+{
+    // copy energies, temperatures, labels, move statistics and coordinates
+    // back into the Replica objects
+    const int R = (int)replica.size(), M = initialReplica.moleculeCount, N = initialReplica.residueCount;
+    std::vector<double> e(R), t(R);
+    std::vector<int> lab(R);
+    std::vector<int64_t> cnt((size_t)3 * R), st(R), ex(2);
+    dev_->check(cgppd_get_replica_stats(dev_->ctx(), e.data(), t.data(), lab.data(), cnt.data(), st.data(), ex.data()),
+                "cgppd_get_replica_stats");
+    std::vector<float> pos((size_t)N * 3), rel((size_t)N * 3), cen((size_t)M * 3);
+    for (int r = 0; r < R; ++r) {
+        Replica &rp = replica[r];
+        rp.potential = e[r];
+        rp.temperature = (float)t[r];
+        rp.label = (short)lab[r];
+        rp.accept = (long)cnt[3 * r];
+        rp.acceptA = (long)cnt[3 * r + 1];
+        rp.reject = (long)cnt[3 * r + 2];
+        dev_->check(cgppd_get_coordinates(dev_->ctx(), r, pos.data(), rel.data(), cen.data()), "cgppd_get_coordinates");
+        int k = 0;
+        for (int m = 0; m < M; ++m) {
+            Molecule &mol = rp.molecules[m];
+            mol.center = Vector3f{cen[3 * m], cen[3 * m + 1], cen[3 * m + 2]};
+            for (Residue &res : mol.Residues) {
+                res.position = Vector3f{pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
+                res.relativePosition = Vector3f{rel[3 * k], rel[3 * k + 1], rel[3 * k + 2]};
+                ++k;
+            }
+        }
+    }
+    totalTests = (int)ex[0];
+    totalExchanges = (int)ex[1];
+}
+
+void Simulation::run()
+{
+    if (!dev_) setup_device(parameters.gpuoffset);
+    initSamplingFiles();
+    dev_->check(cgppd_mc_prepare(dev_->ctx()), "cgppd_mc_prepare");
+    const int per = std::max(1, parameters.MCsteps / parameters.REsteps);
+    const int R = (int)replica.size();
+    int prev_tests = 0, prev_ex = 0;
+    // src/Simulation.cpp:737: REsteps - 1 exchange rounds, then the final
+    // MC interval of the threads
+    for (steps = 1; steps <= parameters.REsteps; ++steps) {
+        const bool exchange = steps < parameters.REsteps;
+        if (exchange) {
+            dev_->check(cgppd_run(dev_->ctx(), 1, per, parameters.use_graph ? 1 : 0), "cgppd_run");
+        } else {
+            dev_->check(cgppd_mc_steps(dev_->ctx(), per), "cgppd_mc_steps");
+        }
+        // interval statistics (src/Simulation.cpp:749-768, 876-890)
+        std::vector<double> e(R), t(R);
+        std::vector<int> lab(R);
+        std::vector<int64_t> cnt((size_t)3 * R), st(R), ex(2);
+        dev_->check(cgppd_get_replica_stats(dev_->ctx(), e.data(), t.data(), lab.data(), cnt.data(), st.data(), ex.data()),
+                    "cgppd_get_replica_stats");
+        const int step_no = per * steps;
+        std::fprintf(acceptanceRatioFile, "%9d: ", step_no);
+        std::vector<int> order(R);
+        for (int r = 0; r < R; ++r) order[r] = r;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return t[a] < t[b]; });
+        for (int r : order) {
+            const int64_t a = cnt[3 * r] + cnt[3 * r + 1], all = a + cnt[3 * r + 2];
+            std::fprintf(acceptanceRatioFile, "| %6.4f ", all ? double(a) / double(all) : 0.0);
+            std::fprintf(boundConformationsFile, "%d; %0.5f (%0.5f); %0.1f\n", step_no, e[r], e[r], t[r]);
+        }
+        std::fprintf(acceptanceRatioFile, "\n");
+        exchanges = (int)ex[1] - prev_ex;
+        tests = (int)ex[0] - prev_tests;
+        prev_ex = (int)ex[1];
+        prev_tests = (int)ex[0];
+        exchangeFraction = tests ? float(exchanges) / float(tests) : 0.f;
+        accumulativeExchangeFraction = ex[0] ? float(ex[1]) / float(ex[0]) : 0.f;
+        if (exchange)
+            std::fprintf(exchangeFrequencyFile, "%10d %6.4f %6.4f\n", step_no, exchangeFraction,
+                         accumulativeExchangeFraction);
+    }
+    sync_from_device();
+    closeSamplingFiles();
+}
+
+}  // namespace cgppd
