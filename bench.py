@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 4])
     ap.add_argument("--exchange-every", type=int, default=1000)
     ap.add_argument("--wg-per-cu", type=int, default=0)
+    ap.add_argument("--no-image-shift", action="store_true", help="per-pair minimum image everywhere")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
@@ -98,6 +99,8 @@ def main():
     e.load_spec(spec, seed=args.seed)
     if args.wg_per_cu:
         e.set_launch(args.wg_per_cu)
+    if args.no_image_shift:
+        e.set_image_shift(False)
     e.mc_prepare()
     R = spec.n_rep
 
@@ -152,7 +155,8 @@ def main():
             "workload": spec.name,
             "n_mol": spec.n_mol, "n_res": spec.n_res, "replicas": R, "box_A": round(spec.box, 2),
             "exchange_every": per, "moves": "rigid translate 2 A / rotate 0.2 rad",
-            "pair_terms": "fp32 + per-lane Kahan, fp64 row reduction",
+            "pair_terms": "fp32, Kahan over 4-term groups per lane, fp64 row reduction",
+            "image_shift": not args.no_image_shift,
         },
         "pairs_per_s": pairs / wall,
         "acceptance": float((acc[0] + acc[1]) / max(1, acc.sum())),

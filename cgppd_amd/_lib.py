@@ -46,6 +46,7 @@ SIGNATURES = {
     "cgppd_get_coordinates": (C.c_int, [_ctxp, C.c_int, _f32p, _f32p, _f32p]),
     "cgppd_get_energy_matrix": (C.c_int, [_ctxp, C.c_int, _f64p]),
     "cgppd_set_launch": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_set_image_shift": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_set_timing": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_last_kernel_ms": (C.c_int, [_ctxp, _f64p, _i64p]),
     "cgppd_model_constants": (C.c_int, [_f32p, _f32p, _f32p]),

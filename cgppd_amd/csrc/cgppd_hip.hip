@@ -217,6 +217,8 @@ struct PairParams {
     float4 *pos;         // R * stride
     float4 *rel;         // R * stride
     float4 *center;      // R * n_mol
+    const float *radius; // R * n_mol, max |relative position| of each molecule
+    int fast_image;      // use the per-segment image shift where it is exact
     // full-energy outputs (raw LJ/DH)
     double *out_lj, *out_dh;  // R
     double *out_mat;          // R * M * M (may be null)
@@ -260,47 +262,114 @@ __device__ __forceinline__ void load_table(const Coef *__restrict__ g, char *lds
     }
 }
 
+// One pair term for this lane's residue i against the wave-uniform j record.
+// MINIMG: apply the per-axis minimum image; otherwise the segment's single
+// image shift is already folded into (xi, yi, zi).  Returns LJ in `lj` and
+// DH in `dh` (DH only computed for charged j, a wave-uniform branch).
+template <bool MINIMG>
+__device__ __forceinline__ void pair_term(const float4 rj, uint32_t code, float xi, float yi, float zi,
+                                          uint32_t lane_off, const char *__restrict__ tab, float box, float inv_box,
+                                          float &lj, float &dh)
+{
+    float dx = xi - rj.x;
+    float dy = yi - rj.y;
+    float dz = zi - rj.z;
+    if (MINIMG) {
+        dx = fmaf(-box, rintf(dx * inv_box), dx);
+        dy = fmaf(-box, rintf(dy * inv_box), dy);
+        dz = fmaf(-box, rintf(dz * inv_box), dz);
+    }
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const Coef cf = *reinterpret_cast<const Coef *>(tab + ((code & kRowMask) | lane_off));
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float s2 = cf.sig2 * (rinv * rinv);
+    const float s6 = s2 * s2 * s2;
+    const float att = fmaf(cf.A, s6, -cf.A) * s6;
+    const float rep = fmaf(cf.A, -0.5f, -att);
+    lj = r2 < cf.rc2 ? rep : att;
+    dh = 0.0f;
+    if (code & kChargedBit) {
+        const float r = r2 * rinv;
+        dh = cf.qq * __builtin_amdgcn_exp2f(r * kDHExp2) * rinv;
+    }
+}
+
 // Inner loop over j in [js, je) (wave-uniform) for this lane's residue i.
-// SEPARATE: LJ and DH accumulated apart (raw table); otherwise one total.
-template <bool SEPARATE>
+// Terms are summed plainly in groups of four and the group sums enter a
+// Kahan-compensated accumulator.  SEPARATE: LJ and DH accumulated apart
+// (raw table); otherwise one total (scaled table).
+template <bool SEPARATE, bool MINIMG>
 __device__ __forceinline__ void pair_segment(const float4 *__restrict__ jrec, int js, int je, float xi, float yi,
                                              float zi, uint32_t lane_off, const char *__restrict__ tab, float box,
                                              float inv_box, float &s0, float &c0, float &s1, float &c1)
 {
-    for (int j = js; j < je; ++j) {
+    int j = js;
+    for (; j + 4 <= je; j += 4) {
+        float g0 = 0.0f, g1 = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float4 rj = jrec[j + u];
+            const uint32_t code = __builtin_amdgcn_readfirstlane(__float_as_uint(rj.w));
+            float lj, dh;
+            pair_term<MINIMG>(rj, code, xi, yi, zi, lane_off, tab, box, inv_box, lj, dh);
+            if (SEPARATE) {
+                g0 += lj;
+                g1 += dh;
+            } else {
+                g0 += lj + dh;
+            }
+        }
+        kahan_add(s0, c0, g0);
+        if (SEPARATE) kahan_add(s1, c1, g1);
+    }
+    for (; j < je; ++j) {
         const float4 rj = jrec[j];
-        const uint32_t code = __float_as_uint(rj.w);
-        float dx = xi - rj.x;
-        float dy = yi - rj.y;
-        float dz = zi - rj.z;
-        dx = fmaf(-box, rintf(dx * inv_box), dx);
-        dy = fmaf(-box, rintf(dy * inv_box), dy);
-        dz = fmaf(-box, rintf(dz * inv_box), dz);
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const Coef cf = *reinterpret_cast<const Coef *>(tab + ((code & kRowMask) | lane_off));
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float rinv2 = rinv * rinv;
-        const float s2 = cf.sig2 * rinv2;
-        const float s6 = s2 * s2 * s2;
-        float lj = fmaf(cf.A, s6, -cf.A) * s6;
-        const float rep = fmaf(cf.A, -0.5f, -lj);
-        lj = r2 < cf.rc2 ? rep : lj;
+        const uint32_t code = __builtin_amdgcn_readfirstlane(__float_as_uint(rj.w));
+        float lj, dh;
+        pair_term<MINIMG>(rj, code, xi, yi, zi, lane_off, tab, box, inv_box, lj, dh);
         if (SEPARATE) {
             kahan_add(s0, c0, lj);
-            if (code & kChargedBit) {
-                const float r = r2 * rinv;
-                const float dh = cf.qq * __builtin_amdgcn_exp2f(r * kDHExp2) * rinv;
-                kahan_add(s1, c1, dh);
-            }
+            kahan_add(s1, c1, dh);
         } else {
-            float e = lj;
-            if (code & kChargedBit) {
-                const float r = r2 * rinv;
-                e += cf.qq * __builtin_amdgcn_exp2f(r * kDHExp2) * rinv;
-            }
-            kahan_add(s0, c0, e);
+            kahan_add(s0, c0, lj + dh);
         }
     }
+}
+
+// Minimum-image shift shared by a whole segment.  With the two molecules'
+// centres ci, cj and bounding radii (max |relative position|) summing to
+// rsum, if every axis satisfies |d - L k| + rsum < L/2 (d = ci - cj,
+// k = rint(d/L)), then every residue pair of the two molecules has the same
+// minimum image k and the per-pair rint can be skipped.  Returns the shift
+// L k to subtract from the i coordinates, or false when the segment needs the
+// per-pair minimum image.
+__device__ __forceinline__ bool segment_shift(float4 ci, float4 cj, float rsum, float box, float inv_box, float &sx,
+                                              float &sy, float &sz)
+{
+    const float lim = 0.5f * box - (rsum * 1.0001f + 0.01f);
+    const float kx = rintf((ci.x - cj.x) * inv_box), ky = rintf((ci.y - cj.y) * inv_box),
+                kz = rintf((ci.z - cj.z) * inv_box);
+    const float dx = fmaf(-box, kx, ci.x - cj.x), dy = fmaf(-box, ky, ci.y - cj.y), dz = fmaf(-box, kz, ci.z - cj.z);
+    sx = box * kx;
+    sy = box * ky;
+    sz = box * kz;
+    return fabsf(dx) < lim && fabsf(dy) < lim && fabsf(dz) < lim;
+}
+
+// A segment (i-block of one molecule against the j residues [js, je) of
+// molecule J): the shifted fast path when the bounding spheres allow it.
+template <bool SEPARATE>
+__device__ __forceinline__ void segment(const float4 *__restrict__ jrec, int js, int je, float4 pi, uint32_t lane_off,
+                                        const char *__restrict__ tab, const PairParams &p, float4 ci, float ri,
+                                        float4 cj, float rj, float &s0, float &c0, float &s1, float &c1)
+{
+    float sx, sy, sz;
+    if (p.fast_image && segment_shift(ci, cj, ri + rj, p.box, p.inv_box, sx, sy, sz))
+        pair_segment<SEPARATE, false>(jrec, js, je, pi.x - sx, pi.y - sy, pi.z - sz, lane_off, tab, p.box,
+                                      p.inv_box, s0, c0, s1, c1);
+    else
+        pair_segment<SEPARATE, true>(jrec, js, je, pi.x, pi.y, pi.z, lane_off, tab, p.box, p.inv_box, s0, c0, s1,
+                                     c1);
 }
 
 // Maps a persistent workgroup's k-th item to (replica, item-in-replica).
@@ -368,6 +437,8 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
         const bool valid = i < i_end;
         const float4 pi = pos[valid ? i : i_beg];
         const uint32_t lane_off = (__float_as_uint(pi.w) >> kRowShift & 0x1Fu) << kColShift;
+        const float4 ci = p.center[(size_t)r * p.n_mol + I];
+        const float ri = p.radius[(size_t)r * p.n_mol + I];
         __syncthreads();
 
         int js = max(j0, jc * kJChunk + wave * kWave);
@@ -376,8 +447,8 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
             const int J = p.mol_of[js];
             const int se = min(je, p.mol_off[J + 1]);
             float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-            pair_segment<!SCALED>(jrec, js - jc * kJChunk, se - jc * kJChunk, pi.x, pi.y, pi.z, lane_off, tab, p.box,
-                                  p.inv_box, s0, c0, s1, c1);
+            segment<!SCALED>(jrec, js - jc * kJChunk, se - jc * kJChunk, pi, lane_off, tab, p, ci, ri,
+                             p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
             double v0 = valid ? (double)s0 - (double)c0 : 0.0;
             double v1 = valid ? (double)s1 - (double)c1 : 0.0;
             v0 = wave_sum(v0);
@@ -501,6 +572,8 @@ __global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
         float4 rn, pn;
         move_residue(mv, rel[valid ? i : i_beg], rn, pn);
         const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
+        const float4 ci = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+        const float ri = p.radius[(size_t)r * p.n_mol + m];
         __syncthreads();
 
         int js = jbase + wave * kWave;
@@ -510,8 +583,8 @@ __global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
             const int se = min(je, p.mol_off[J + 1]);
             if (J != m) {
                 float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-                pair_segment<false>(jrec, js - jbase, se - jbase, pn.x, pn.y, pn.z, lane_off, tab, p.box, p.inv_box,
-                                    s0, c0, s1, c1);
+                segment<false>(jrec, js - jbase, se - jbase, pn, lane_off, tab, p, ci, ri,
+                               p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
                 double v = valid ? (double)s0 - (double)c0 : 0.0;
                 v = wave_sum(v);
                 if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
@@ -657,6 +730,8 @@ struct cgppd_ctx {
     Coef *d_table_raw = nullptr, *d_table_scaled = nullptr;
     int *d_mol_off = nullptr, *d_mol_of = nullptr;
     float4 *d_pos = nullptr, *d_rel = nullptr, *d_center = nullptr;
+    float *d_radius = nullptr;
+    bool fast_image = true;
     double *d_rowbuf = nullptr, *d_emat = nullptr, *d_energy = nullptr, *d_temp = nullptr;
     double *d_rcos = nullptr, *d_rsin = nullptr;
     float *d_tstep = nullptr;
@@ -747,6 +822,7 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_pos);
     dev_free(ctx->d_rel);
     dev_free(ctx->d_center);
+    dev_free(ctx->d_radius);
     dev_free(ctx->d_rowbuf);
     dev_free(ctx->d_emat);
     dev_free(ctx->d_energy);
@@ -813,6 +889,40 @@ float code_bits(uint32_t c)
     return f;
 }
 
+// Bounding radius of a molecule: max |relative position|.
+float molecule_radius(const float4 *rel, int n)
+{
+    double r2 = 0.0;
+    for (int k = 0; k < n; ++k)
+        r2 = std::max(r2, (double)rel[k].x * rel[k].x + (double)rel[k].y * rel[k].y + (double)rel[k].z * rel[k].z);
+    return (float)std::sqrt(r2);
+}
+
+// Centre and relative positions from absolute coordinates, with the float
+// accumulation of src/Molecule.cpp:590-609; writes pos, rel (w = code),
+// and returns the centre and radius.
+void split_molecule(const cgppd_ctx *ctx, int i0, int n, const float *xyz, float4 *pos, float4 *rel, float4 &centre,
+                    float &radius)
+{
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    for (int k = 0; k < n; ++k) {
+        cx += xyz[3 * k];
+        cy += xyz[3 * k + 1];
+        cz += xyz[3 * k + 2];
+    }
+    const float fn = (float)n;
+    cx /= fn;
+    cy /= fn;
+    cz /= fn;
+    for (int k = 0; k < n; ++k) {
+        const float w = code_bits(residue_code(ctx, ctx->type[i0 + k]));
+        pos[k] = make_float4(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], w);
+        rel[k] = make_float4(xyz[3 * k] - cx, xyz[3 * k + 1] - cy, xyz[3 * k + 2] - cz, w);
+    }
+    centre = make_float4(cx, cy, cz, 0.0f);
+    radius = molecule_radius(rel, n);
+}
+
 PairParams base_params(cgppd_ctx *ctx)
 {
     PairParams p;
@@ -834,6 +944,8 @@ PairParams base_params(cgppd_ctx *ctx)
     p.pos = ctx->d_pos;
     p.rel = ctx->d_rel;
     p.center = ctx->d_center;
+    p.radius = ctx->d_radius;
+    p.fast_image = ctx->fast_image ? 1 : 0;
     p.out_lj = ctx->d_out;
     p.out_dh = ctx->d_out + ctx->R;
     p.out_mat = nullptr;
@@ -1110,6 +1222,11 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
             hrel[(size_t)r * S + i] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
+    std::vector<float> hrad((size_t)R * M);
+    for (int r = 0; r < R; ++r)
+        for (int m = 0; m < M; ++m)
+            hrad[(size_t)r * M + m] = molecule_radius(&hrel[(size_t)r * S + ctx->mol_off[m]],
+                                                      ctx->mol_off[m + 1] - ctx->mol_off[m]);
     std::vector<double> temp(R), rcos(R), rsin(R);
     std::vector<float> tstep(R);
     std::vector<int> lab(R), rank_of(R), at_rank(R);
@@ -1129,6 +1246,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_pos, (size_t)R * S))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_rel, (size_t)R * S))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_center, (size_t)R * M))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_radius, (size_t)R * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_rowbuf, (size_t)R * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_emat, (size_t)R * M * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_energy, R))) return rc;
@@ -1151,6 +1269,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     CG_HIP(ctx, hipMemcpy(ctx->d_pos, hpos.data(), sizeof(float4) * hpos.size(), hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_rel, hrel.data(), sizeof(float4) * hrel.size(), hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_center, hcen.data(), sizeof(float4) * hcen.size(), hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_radius, hrad.data(), sizeof(float) * hrad.size(), hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_temp, temp.data(), sizeof(double) * R, hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_rcos, rcos.data(), sizeof(double) * R, hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_rsin, rsin.data(), sizeof(double) * R, hipMemcpyHostToDevice));
@@ -1173,21 +1292,34 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     return CGPPD_OK;
 }
 
+static int upload_molecules(cgppd_ctx *ctx, int replica, int m0, int m1, const float *xyz)
+{
+    const int i0 = ctx->mol_off[m0], n = ctx->mol_off[m1] - i0;
+    std::vector<float4> hp(n), hr(n), hc(m1 - m0);
+    std::vector<float> hrad(m1 - m0);
+    for (int m = m0; m < m1; ++m) {
+        const int a = ctx->mol_off[m] - i0, b = ctx->mol_off[m + 1] - i0;
+        split_molecule(ctx, i0 + a, b - a, xyz + 3 * a, &hp[a], &hr[a], hc[m - m0], hrad[m - m0]);
+    }
+    const size_t rs = (size_t)replica * ctx->stride + i0, rm = (size_t)replica * ctx->M + m0;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipMemcpyAsync(ctx->d_pos + rs, hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
+    CG_HIP(ctx, hipMemcpyAsync(ctx->d_rel + rs, hr.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream));
+    CG_HIP(ctx, hipMemcpyAsync(ctx->d_center + rm, hc.data(), sizeof(float4) * (m1 - m0), hipMemcpyHostToDevice,
+                               ctx->stream));
+    CG_HIP(ctx, hipMemcpyAsync(ctx->d_radius + rm, hrad.data(), sizeof(float) * (m1 - m0), hipMemcpyHostToDevice,
+                               ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mc_ready = false;
+    return CGPPD_OK;
+}
+
 int cgppd_upload_positions(cgppd_ctx *ctx, int replica, const float *xyz)
 {
     int rc = check_ready(ctx, true);
     if (rc) return rc;
     if (replica < 0 || replica >= ctx->R || !xyz) return fail(ctx, CGPPD_E_ARG, "bad replica or null coordinates");
-    std::vector<float4> h(ctx->N);
-    for (int i = 0; i < ctx->N; ++i)
-        h[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
-                           code_bits(residue_code(ctx, ctx->type[i])));
-    CG_HIP(ctx, hipSetDevice(ctx->device));
-    CG_HIP(ctx, hipMemcpyAsync(ctx->d_pos + (size_t)replica * ctx->stride, h.data(), sizeof(float4) * ctx->N,
-                               hipMemcpyHostToDevice, ctx->stream));
-    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->mc_ready = false;
-    return CGPPD_OK;
+    return upload_molecules(ctx, replica, 0, ctx->M, xyz);
 }
 
 int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float *xyz)
@@ -1196,16 +1328,14 @@ int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float
     if (rc) return rc;
     if (replica < 0 || replica >= ctx->R || molecule < 0 || molecule >= ctx->M || !xyz)
         return fail(ctx, CGPPD_E_ARG, "bad replica/molecule or null coordinates");
-    const int i0 = ctx->mol_off[molecule], n = ctx->mol_off[molecule + 1] - i0;
-    std::vector<float4> h(n);
-    for (int k = 0; k < n; ++k)
-        h[k] = make_float4(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2],
-                           code_bits(residue_code(ctx, ctx->type[i0 + k])));
-    CG_HIP(ctx, hipSetDevice(ctx->device));
-    CG_HIP(ctx, hipMemcpyAsync(ctx->d_pos + (size_t)replica * ctx->stride + i0, h.data(), sizeof(float4) * n,
-                               hipMemcpyHostToDevice, ctx->stream));
-    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->mc_ready = false;
+    return upload_molecules(ctx, replica, molecule, molecule + 1, xyz);
+}
+
+int cgppd_set_image_shift(cgppd_ctx *ctx, int enable)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    ctx->fast_image = enable != 0;
+    destroy_graph(ctx);
     return CGPPD_OK;
 }
 

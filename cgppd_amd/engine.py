@@ -174,6 +174,9 @@ class Engine:
     def set_launch(self, wg_per_cu):
         self._check(self.lib.cgppd_set_launch(self.h, int(wg_per_cu)), "set_launch")
 
+    def set_image_shift(self, on):
+        self._check(self.lib.cgppd_set_image_shift(self.h, int(bool(on))), "set_image_shift")
+
     def set_timing(self, on):
         self._check(self.lib.cgppd_set_timing(self.h, int(bool(on))), "set_timing")
 

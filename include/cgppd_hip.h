@@ -77,13 +77,21 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
 
 /* Overwrite the absolute coordinates of one replica (xyz[N*3]), as
  * Replica::ReplicaDataToDevice copies positions (src/Replica.cpp:739-760).
- * Centres and relative positions are left unchanged: use this for energy
- * evaluation of arbitrary conformations, not before MC steps. */
+ * Each molecule's centre becomes the float mean of its residues and the
+ * relative positions follow (src/Molecule.cpp:590-609), so MC may continue
+ * from the uploaded conformation after cgppd_mc_prepare. */
 int cgppd_upload_positions(cgppd_ctx *ctx, int replica, const float *xyz);
 
-/* Overwrite one molecule's coordinates (xyz[n_m*3]).  Replaces
- * Replica::MoleculeDataToDevice (src/Replica.cpp:833-881). */
+/* Overwrite one molecule's coordinates (xyz[n_m*3]), centre and relative
+ * positions recomputed as above.  Replaces Replica::MoleculeDataToDevice
+ * (src/Replica.cpp:833-881). */
 int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float *xyz);
+
+/* Segment image shift (default on): when two molecules' bounding spheres
+ * cannot straddle half a box, one minimum-image shift serves all their
+ * residue pairs and the per-pair rint is skipped.  0 forces the per-pair
+ * minimum image everywhere (as src/Residue.cpp:68-73). */
+int cgppd_set_image_shift(cgppd_ctx *ctx, int enable);
 
 /* Full inter-molecular energy of one replica, synchronous.  Replaces
  * CUDA_EonDevice (src/cudaExterns.h:29) behind Replica::EonDevice

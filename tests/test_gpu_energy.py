@@ -56,11 +56,13 @@ def test_reference_conformations(gpu_engine_factory, conformations, mj_ff):
         e.close()
 
 
+@pytest.mark.parametrize("image_shift", [True, False])
 @pytest.mark.parametrize("name", GOLDEN_SYSTEMS)
-def test_golden_systems(gpu_engine_factory, golden, name):
+def test_golden_systems(gpu_engine_factory, golden, name, image_shift):
     s = golden_system(golden, name)
     e = engine_for(gpu_engine_factory, s["eps"], s["sigma"], s["charge"], s["box"], s["mol_off"], s["types"],
                    [s["pos"]])
+    e.set_image_shift(image_shift)
     lj, dh, tot = e.energy(0)
     print(f"{name}: gpu lj {lj:.10g} dh {dh:.10g}; reference build {s['energy'][0]:.10g} {s['energy'][1]:.10g}")
     assert close(lj, s["energy"][0]) and close(dh, s["energy"][1]) and close(tot, s["energy"][2])
@@ -154,6 +156,29 @@ def test_one_molecule_has_zero_inter_energy(gpu_engine_factory):
                    spec.types, [spec.positions()])
     assert e.energy(0) == (0.0, 0.0, 0.0)
     e.close()
+
+
+def test_image_shift_boundary_cases(gpu_engine_factory):
+    """Molecules whose bounding spheres straddle half a box (per-pair minimum
+    image) next to well separated ones (one shift per segment), in a box
+    barely larger than two molecules."""
+    from cgppd_amd import synth
+    spec = synth.make_system("straddle", [76, 76, 30], 100.0, 1, 300.0, 300.0, 29)
+    rng = np.random.default_rng(4)
+    for trial in range(6):
+        pos = spec.positions().copy()
+        for m in range(spec.n_mol):
+            a, b = spec.mol_off[m], spec.mol_off[m + 1]
+            pos[a:b] += rng.uniform(-60, 60, 3).astype(np.float32)
+        flj, fdh = oracle.energy_fp64(spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos,
+                                      spec.box)
+        for shift in (True, False):
+            e = engine_for(gpu_engine_factory, spec.ff.eps, spec.ff.sigma, spec.ff.charge, spec.box, spec.mol_off,
+                           spec.types, [pos])
+            e.set_image_shift(shift)
+            lj, dh, _ = e.energy(0)
+            assert close(lj, flj) and close(dh, fdh), (trial, shift, lj, flj, dh, fdh)
+            e.close()
 
 
 def test_config1_full_size_against_fp64(gpu_engine_factory):
