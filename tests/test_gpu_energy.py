@@ -34,6 +34,10 @@ def engine_for(factory, eps, sigma, charge, box, mol_off, types, pos_list, n_nc=
     M = len(mol_off) - 1
     rel = np.stack([np.asarray(p, np.float32) for p in pos_list])
     e.set_replicas(np.zeros((R, M, 3), np.float32), rel, seed=1)
+    # per-molecule centres and radii from the coordinates (the segment
+    # image shift needs them; with zero centres it never applies)
+    for r, p in enumerate(pos_list):
+        e.upload_positions(r, p)
     return e
 
 
