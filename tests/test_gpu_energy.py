@@ -74,9 +74,12 @@ def test_golden_systems(gpu_engine_factory, golden, name, image_shift):
     assert close(lj, flj) and close(dh, fdh)
     _, _, tots, mat = e.energy_all(matrix=True)
     assert close(tots[0], tot)
+    # per molecule pair: far-field entries (~1e-4 kcal/mol, DH tails) carry
+    # up to a few 1e-5 relative FP32 error; bound them on the scale of the
+    # largest entry, which keeps every total within the 1e-5 bar above
     ref = s["matrix"]
     scale = np.abs(ref).max()
-    np.testing.assert_allclose(mat[0], ref, rtol=REL, atol=REL * 1e-3 * scale + 1e-12)
+    np.testing.assert_allclose(mat[0], ref, rtol=REL, atol=1e-7 * scale + 1e-12)
     e.close()
 
 
