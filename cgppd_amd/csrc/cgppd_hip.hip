@@ -47,6 +47,7 @@ constexpr uint32_t kRowMask = 0x1Fu << kRowShift;
 constexpr uint32_t kChargedBit = 0x80000000u;
 constexpr uint32_t kTagMC = 0x4D430000u;
 constexpr uint32_t kTagEX = 0x45580000u;
+#define CGPPD_MAX_MOLECULES 1024  // molecules per system (LDS row buffer of the MC finish)
 
 // Model constants, as the reference evaluates them in float
 // (src/constants.h:8-26): T_room*Rgas/kcal and e^2 N_A/(4 pi eps0 kcal Ang D).
@@ -474,17 +475,22 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
 // Completes the MC step of replica r once all its items are in: Metropolis on
 // the delta-E row (src/Replica.cpp:385-410, delta and exponent in FP64),
 // apply the move (src/Molecule.cpp:468-504) and update the matrix row and
-// column and the running total.  Runs on one whole workgroup.
+// column and the running total.  Runs on one whole workgroup.  The row is
+// read and cleared with atomic exchanges, at the same coherence point as
+// the atomic additions that built it, so no cache flush is needed.
 __device__ void finish_mc_step(const PairParams &p, int r, const Proposal &pr, const Move &mv, int64_t step,
-                               int i_beg, int i_end, double *red, int *accept_flag)
+                               int i_beg, int i_end, double *rowv, double *red, int *accept_flag)
 {
     const int M = p.n_mol, m = pr.mol;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    double *row = p.rowbuf + (size_t)r * M;
+    unsigned long long *row = reinterpret_cast<unsigned long long *>(p.rowbuf + (size_t)r * M);
     double *mat = p.emat + (size_t)r * M * M;
     double d = 0.0;
-    for (int J = threadIdx.x; J < M; J += kWG)
-        if (J != m) d += row[J] - mat[(size_t)m * M + J];
+    for (int J = threadIdx.x; J < M; J += kWG) {
+        const double v = __longlong_as_double((long long)atomicExch(row + J, 0ull));
+        rowv[J] = v;
+        if (J != m) d += v - mat[(size_t)m * M + J];
+    }
     d = wave_sum(d);
     if (lane == 0) red[wave] = d;
     __syncthreads();
@@ -504,12 +510,16 @@ __device__ void finish_mc_step(const PairParams &p, int r, const Proposal &pr, c
         }
         if (acc) p.energy[r] += delta;
         *accept_flag = acc;
+        p.done[r] = 0u;
+        p.steps[r] = step + 1;
+        const unsigned long long n_m = (unsigned long long)(i_end - i_beg);
+        atomicAdd(p.pair_evals, n_m * ((unsigned long long)p.n_res - n_m));
     }
     __syncthreads();
     if (*accept_flag) {
         for (int J = threadIdx.x; J < M; J += kWG) {
             if (J == m) continue;
-            const double v = row[J];
+            const double v = rowv[J];
             mat[(size_t)m * M + J] = v;
             mat[(size_t)J * M + m] = v;
         }
@@ -523,23 +533,20 @@ __device__ void finish_mc_step(const PairParams &p, int r, const Proposal &pr, c
         }
         if (threadIdx.x == 0) p.center[(size_t)r * M + m] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
-    for (int J = threadIdx.x; J < M; J += kWG) row[J] = 0.0;
-    if (threadIdx.x == 0) {
-        p.done[r] = 0u;
-        p.steps[r] = step + 1;
-        const unsigned long long n_m = (unsigned long long)(i_end - i_beg);
-        atomicAdd(p.pair_evals, n_m * ((unsigned long long)p.n_res - n_m));
-    }
     __syncthreads();
 }
 
-// One MC step of every replica.  Items: (replica, i-block of the moved
-// molecule, j-chunk); inactive i-blocks (beyond the moved molecule) are
-// skipped without counting.
+// One MC step of every replica.  Items: (replica, i-block, j-chunk) over the
+// largest molecule's i-blocks; items beyond the moved molecule do no pair
+// work but are still counted, so a replica's step completes only after every
+// item of this launch has read the replica's step counter.  Each wave waits
+// for its row additions to be acknowledged (s_waitcnt) before the barrier
+// that precedes the completion count; no cache-flushing fence is used.
 __global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ float4 jrec[kJChunk];
+    __shared__ double rowv[CGPPD_MAX_MOLECULES];
     __shared__ double red[kWavesPerWG];
     __shared__ int last_flag, accept_flag;
     load_table(p.table, tab);
@@ -556,54 +563,52 @@ __global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
         const int m = pr.mol;
         const int i_beg = p.mol_off[m], i_end = p.mol_off[m + 1];
         const int n_ib = (i_end - i_beg + kIBlock - 1) / kIBlock;
-        if (ib >= n_ib) continue;  // uniform
-
-        const float4 *pos = p.pos + (size_t)r * p.stride;
-        const float4 *rel = p.rel + (size_t)r * p.stride;
         const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + m], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
-        const int jbase = jc * kJChunk;
-        __syncthreads();  // previous item finished reading jrec
-        {
-            const int j = jbase + threadIdx.x;
-            if (j < p.n_res) jrec[threadIdx.x] = pos[j];
-        }
-        const int i = i_beg + ib * kIBlock + lane;
-        const bool valid = i < i_end;
-        float4 rn, pn;
-        move_residue(mv, rel[valid ? i : i_beg], rn, pn);
-        const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
-        const float4 ci = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-        const float ri = p.radius[(size_t)r * p.n_mol + m];
-        __syncthreads();
-
-        int js = jbase + wave * kWave;
-        const int je = min(p.n_res, jbase + (wave + 1) * kWave);
-        while (js < je) {
-            const int J = p.mol_of[js];
-            const int se = min(je, p.mol_off[J + 1]);
-            if (J != m) {
-                float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-                segment<false>(jrec, js - jbase, se - jbase, pn, lane_off, tab, p, ci, ri,
-                               p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
-                double v = valid ? (double)s0 - (double)c0 : 0.0;
-                v = wave_sum(v);
-                if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
+        __syncthreads();  // previous item finished with jrec and the flags
+        if (ib < n_ib) {  // uniform
+            const float4 *pos = p.pos + (size_t)r * p.stride;
+            const float4 *rel = p.rel + (size_t)r * p.stride;
+            const int jbase = jc * kJChunk;
+            {
+                const int j = jbase + threadIdx.x;
+                if (j < p.n_res) jrec[threadIdx.x] = pos[j];
             }
-            js = se;
+            const int i = i_beg + ib * kIBlock + lane;
+            const bool valid = i < i_end;
+            float4 rn, pn;
+            move_residue(mv, rel[valid ? i : i_beg], rn, pn);
+            const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
+            const float4 ci = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+            const float ri = p.radius[(size_t)r * p.n_mol + m];
+            __syncthreads();
+
+            int js = jbase + wave * kWave;
+            const int je = min(p.n_res, jbase + (wave + 1) * kWave);
+            while (js < je) {
+                const int J = p.mol_of[js];
+                const int se = min(je, p.mol_off[J + 1]);
+                if (J != m) {
+                    float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+                    segment<false>(jrec, js - jbase, se - jbase, pn, lane_off, tab, p, ci, ri,
+                                   p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1,
+                                   c1);
+                    double v = valid ? (double)s0 - (double)c0 : 0.0;
+                    v = wave_sum(v);
+                    if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
+                }
+                js = se;
+            }
+            // wait until this wave's row additions are acknowledged by the
+            // memory system before the barrier that precedes the count
+            __builtin_amdgcn_s_waitcnt(0);
         }
-        // publish this item, then let the workgroup that completes the
-        // replica's last item finish the step
-        __threadfence();
         __syncthreads();
         if (threadIdx.x == 0) {
             const unsigned old = atomicAdd(p.done + r, 1u);
-            last_flag = old == (unsigned)(n_ib * p.jc_count - 1);
+            last_flag = old == (unsigned)(per_rep - 1);
         }
         __syncthreads();
-        if (last_flag) {
-            __threadfence();
-            finish_mc_step(p, r, pr, mv, step, i_beg, i_end, red, &accept_flag);
-        }
+        if (last_flag) finish_mc_step(p, r, pr, mv, step, i_beg, i_end, rowv, red, &accept_flag);
     }
 }
 
@@ -1160,6 +1165,8 @@ int cgppd_set_system(cgppd_ctx *ctx, int n_mol, const int *mol_off, const int *t
 {
     if (!ctx) return CGPPD_E_ARG;
     if (n_mol < 1 || !mol_off || !type) return fail(ctx, CGPPD_E_ARG, "need n_mol >= 1 and non-null arrays");
+    if (n_mol > CGPPD_MAX_MOLECULES)
+        return fail(ctx, CGPPD_E_ARG, "at most %d molecules per system (got %d)", CGPPD_MAX_MOLECULES, n_mol);
     if (mol_off[0] != 0) return fail(ctx, CGPPD_E_ARG, "mol_off[0] must be 0");
     int max_n = 0;
     for (int m = 0; m < n_mol; ++m) {
