@@ -6,6 +6,7 @@ raises.  Build it with ``python -m cgppd_amd.build``.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -61,7 +62,7 @@ def load(path: Path | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = Path(path) if path else HIP_LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("CGPPD_HIP_LIB", HIP_LIB_PATH))
     if not p.exists():
         raise RuntimeError(f"{p} is not built; run `python -m cgppd_amd.build` (no CPU fallback exists)")
     lib = C.CDLL(str(p))

@@ -47,8 +47,7 @@ constexpr uint32_t kRowMask = 0x1Fu << kRowShift;
 constexpr uint32_t kChargedBit = 0x80000000u;
 constexpr uint32_t kTagMC = 0x4D430000u;
 constexpr uint32_t kTagEX = 0x45580000u;
-#define CGPPD_MAX_MOLECULES 1024  // molecules per system
-#define CGPPD_MAX_REPLICAS 1024   // replicas per context (LDS item offsets of the MC step)
+#define CGPPD_MAX_MOLECULES 1024  // molecules per system (LDS row buffer of the MC finish)
 
 // Model constants, as the reference evaluates them in float
 // (src/constants.h:8-26): T_room*Rgas/kcal and e^2 N_A/(4 pi eps0 kcal Ang D).
@@ -237,10 +236,6 @@ struct PairParams {
     long long *counters;  // R * 3
     long long *steps;     // R
     unsigned long long *pair_evals;  // 1: inter-molecular residue pairs evaluated by MC steps
-    double *rowval;       // R * M: the finished row of the step being completed
-    const long long *step_base;  // MC step of launch 0 of the current batch of launches
-    int launch_k;         // this launch's index in the batch: step = *step_base + launch_k
-    int jw_count;         // 64-residue j-chunks over n_res
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -477,199 +472,145 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
 }
 
 
-// Completes the MC step of replica r once all its items are in, on one
-// wave: Metropolis on the delta-E row (src/Replica.cpp:385-410, delta and
-// exponent in FP64), apply the move (src/Molecule.cpp:468-504), update the
-// matrix row and column and the running total.  The row is read and cleared
-// with atomic exchanges, at the coherence point of the atomic additions that
-// built it, so no cache flush is needed.
-__device__ __attribute__((noinline)) void finish_mc_step(const PairParams &p, int r, int64_t step)
+// Completes the MC step of replica r once all its items are in: Metropolis on
+// the delta-E row (src/Replica.cpp:385-410, delta and exponent in FP64),
+// apply the move (src/Molecule.cpp:468-504) and update the matrix row and
+// column and the running total.  Runs on one whole workgroup.  The row is
+// read and cleared with atomic exchanges, at the same coherence point as
+// the atomic additions that built it, so no cache flush is needed.
+__device__ void finish_mc_step(const PairParams &p, int r, const Proposal &pr, const Move &mv, int64_t step,
+                               int i_beg, int i_end, double *rowv, double *red, int *accept_flag)
 {
-    // the proposal and move are re-derived from the counter-based stream
-    const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
-    const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + pr.mol], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
     const int M = p.n_mol, m = pr.mol;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int i_beg = p.mol_off[m], i_end = p.mol_off[m + 1];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     unsigned long long *row = reinterpret_cast<unsigned long long *>(p.rowbuf + (size_t)r * M);
-    double *val = p.rowval + (size_t)r * M;
     double *mat = p.emat + (size_t)r * M * M;
     double d = 0.0;
-    for (int J = lane; J < M; J += kWave) {
+    for (int J = threadIdx.x; J < M; J += kWG) {
         const double v = __longlong_as_double((long long)atomicExch(row + J, 0ull));
-        val[J] = v;
+        rowv[J] = v;
         if (J != m) d += v - mat[(size_t)m * M + J];
     }
-    // one lane's sum, broadcast, so every lane takes the same decision
-    const double delta = __shfl(wave_sum(d), 0, kWave);
-    int acc;
-    if (delta < 0.0)
-        acc = 1;
-    else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r])))
-        acc = 2;
-    else
-        acc = 0;
-    if (lane == 0) {
+    d = wave_sum(d);
+    if (lane == 0) red[wave] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double delta = 0.0;
+        for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
         long long *cnt = p.counters + 3 * (size_t)r;
-        cnt[acc == 1 ? 0 : (acc == 2 ? 1 : 2)] += 1;
+        int acc = 0;
+        if (delta < 0.0) {
+            acc = 1;
+            cnt[0] += 1;
+        } else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r]))) {
+            acc = 1;
+            cnt[1] += 1;
+        } else {
+            cnt[2] += 1;
+        }
         if (acc) p.energy[r] += delta;
+        *accept_flag = acc;
         p.done[r] = 0u;
         p.steps[r] = step + 1;
         const unsigned long long n_m = (unsigned long long)(i_end - i_beg);
         atomicAdd(p.pair_evals, n_m * ((unsigned long long)p.n_res - n_m));
     }
-    if (acc) {
-        for (int J = lane; J < M; J += kWave) {
+    __syncthreads();
+    if (*accept_flag) {
+        for (int J = threadIdx.x; J < M; J += kWG) {
             if (J == m) continue;
-            const double v = val[J];
+            const double v = rowv[J];
             mat[(size_t)m * M + J] = v;
             mat[(size_t)J * M + m] = v;
         }
         float4 *pos = p.pos + (size_t)r * p.stride;
         float4 *rel = p.rel + (size_t)r * p.stride;
-        for (int i = i_beg + lane; i < i_end; i += kWave) {
+        for (int i = i_beg + threadIdx.x; i < i_end; i += kWG) {
             float4 rn, pn;
             move_residue(mv, rel[i], rn, pn);
             rel[i] = rn;
             pos[i] = pn;
         }
-        if (lane == 0) p.center[(size_t)r * M + m] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+        if (threadIdx.x == 0) p.center[(size_t)r * M + m] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
+    __syncthreads();
 }
 
-// This lane's proposed residue for i-block ib of replica r's moved molecule,
-// with the moved molecule's new centre and radius (kept out of line: the
-// double-precision move code would otherwise raise the pair loop's register
-// pressure).
-struct IBlock {
-    float4 pn, ci;
-    float ri;
-    int m;
-    uint32_t lane_off;
-    int valid;
-};
-
-__device__ __attribute__((noinline)) IBlock prepare_iblock(const PairParams &p, int r, int ib, int64_t step)
-{
-    const int lane = threadIdx.x & (kWave - 1);
-    const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
-    const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + pr.mol], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
-    IBlock b;
-    b.m = pr.mol;
-    b.ci = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-    b.ri = p.radius[(size_t)r * p.n_mol + b.m];
-    const int i_beg = p.mol_off[b.m], i_end = p.mol_off[b.m + 1];
-    const int i = i_beg + ib * kIBlock + lane;
-    b.valid = i < i_end;
-    float4 rn;
-    move_residue(mv, p.rel[(size_t)r * p.stride + (b.valid ? i : i_beg)], rn, b.pn);
-    b.lane_off = (__float_as_uint(b.pn.w) >> kRowShift & 0x1Fu) << kColShift;
-    return b;
-}
-
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// One MC step of every replica (step = *step_base + launch_k for all of
-// them, fixed for the whole launch).  Every workgroup derives all replicas'
-// proposals, lists the active wave-items (replica, i-block of the moved
-// molecule, 64-residue j-chunk) by prefix sum, and each wave takes one
-// contiguous range of that list -- with workgroups ordered by XCD, so the
-// replicas of one range stay in one L2.  Waves stage their own j-chunk in
-// LDS (no workgroup barrier in the loop), add each (i-block, molecule J)
-// segment to the replica's row buffer, and the wave completing a replica's
-// last item finishes the step.
+// One MC step of every replica.  Items: (replica, i-block, j-chunk) over the
+// largest molecule's i-blocks; items beyond the moved molecule do no pair
+// work but are still counted, so a replica's step completes only after every
+// item of this launch has read the replica's step counter.  Each wave waits
+// for its row additions to be acknowledged (s_waitcnt) before the barrier
+// that precedes the completion count; no cache-flushing fence is used.
 __global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
-    __shared__ float4 jrec_all[kWavesPerWG][kWave];
-    __shared__ int item_off[CGPPD_MAX_REPLICAS + 1];
+    __shared__ float4 jrec[kJChunk];
+    __shared__ double rowv[CGPPD_MAX_MOLECULES];
+    __shared__ double red[kWavesPerWG];
+    __shared__ int last_flag, accept_flag;
     load_table(p.table, tab);
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    float4 *jrec = jrec_all[wave];
-    const int R = p.rep_count;
-    const int64_t step = *p.step_base + p.launch_k;
-    const int JW = p.jw_count;
-    for (int r = threadIdx.x; r < R; r += kWG) {
+    const int per_rep = p.ib_max * p.jc_count;
+    for (ItemIter it(p, per_rep); it.valid(); it.next()) {
+        const int r = it.replica();
+        const int q = it.item();
+        const int jc = q % p.jc_count, ib = q / p.jc_count;
+        const int64_t step = p.steps[r];
         const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
-        const int n = p.mol_off[pr.mol + 1] - p.mol_off[pr.mol];
-        item_off[r + 1] = ((n + kIBlock - 1) / kIBlock) * JW;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        item_off[0] = 0;
-        for (int r = 0; r < R; ++r) item_off[r + 1] += item_off[r];
-    }
-    __syncthreads();
-    const int total = item_off[R];
-
-    // logical workgroup order: all workgroups of XCD 0 first, then XCD 1, ...
-    const int G = gridDim.x, b = blockIdx.x;
-    const int lb = (p.xcd_map && (G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
-    const long long W = (long long)G * kWavesPerWG, gw = (long long)lb * kWavesPerWG + wave;
-    const int k0 = (int)(gw * total / W), k1 = (int)((gw + 1) * total / W);
-
-    // state kept across items: the moved molecule of the current replica
-    // and this lane's proposed residue of the current i-block
-    int cur_r = -1, cur_ib = -1, m = 0;
-    float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 ci = pn;
-    float ri = 0.f;
-    uint32_t lane_off = 0;
-    bool valid = false;
-    int r = 0;
-    for (int k = k0; k < k1; ++k) {
-        while (k >= item_off[r + 1]) ++r;
-        const int local = k - item_off[r];
-        const int ib = local / JW, jw = local % JW;
-        if (r != cur_r || ib != cur_ib) {
-            const IBlock blk = prepare_iblock(p, r, ib, step);
-            m = blk.m;
-            ci = blk.ci;
-            ri = blk.ri;
-            pn = blk.pn;
-            lane_off = blk.lane_off;
-            valid = blk.valid != 0;
-            cur_r = r;
-            cur_ib = ib;
-        }
-        // stage this wave's 64 j residues
-        const int jbase = jw * kWave;
-        const int je = min(p.n_res, jbase + kWave);
-        wave_sync();
-        if (jbase + lane < p.n_res) jrec[lane] = p.pos[(size_t)r * p.stride + jbase + lane];
-        wave_sync();
-        int js = jbase;
-        while (js < je) {
-            const int J = p.mol_of[js];
-            const int se = min(je, p.mol_off[J + 1]);
-            if (J != m) {
-                float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-                segment<false>(jrec, js - jbase, se - jbase, pn, lane_off, tab, p, ci, ri,
-                               p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
-                double v = valid ? (double)s0 - (double)c0 : 0.0;
-                v = wave_sum(v);
-                if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
+        const int m = pr.mol;
+        const int i_beg = p.mol_off[m], i_end = p.mol_off[m + 1];
+        const int n_ib = (i_end - i_beg + kIBlock - 1) / kIBlock;
+        const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + m], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
+        __syncthreads();  // previous item finished with jrec and the flags
+        if (ib < n_ib) {  // uniform
+            const float4 *pos = p.pos + (size_t)r * p.stride;
+            const float4 *rel = p.rel + (size_t)r * p.stride;
+            const int jbase = jc * kJChunk;
+            {
+                const int j = jbase + threadIdx.x;
+                if (j < p.n_res) jrec[threadIdx.x] = pos[j];
             }
-            js = se;
+            const int i = i_beg + ib * kIBlock + lane;
+            const bool valid = i < i_end;
+            float4 rn, pn;
+            move_residue(mv, rel[valid ? i : i_beg], rn, pn);
+            const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
+            const float4 ci = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+            const float ri = p.radius[(size_t)r * p.n_mol + m];
+            __syncthreads();
+
+            int js = jbase + wave * kWave;
+            const int je = min(p.n_res, jbase + (wave + 1) * kWave);
+            while (js < je) {
+                const int J = p.mol_of[js];
+                const int se = min(je, p.mol_off[J + 1]);
+                if (J != m) {
+                    float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+                    segment<false>(jrec, js - jbase, se - jbase, pn, lane_off, tab, p, ci, ri,
+                                   p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1,
+                                   c1);
+                    double v = valid ? (double)s0 - (double)c0 : 0.0;
+                    v = wave_sum(v);
+                    if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
+                }
+                js = se;
+            }
+            // wait until this wave's row additions are acknowledged by the
+            // memory system before the barrier that precedes the count
+            __builtin_amdgcn_s_waitcnt(0);
         }
-        // the row additions are acknowledged before the item is counted
-        __builtin_amdgcn_s_waitcnt(0);
-        unsigned old = 0;
-        if (lane == 0) old = atomicAdd(p.done + r, 1u);
-        old = __shfl(old, 0, kWave);
-        if (old == (unsigned)(item_off[r + 1] - item_off[r] - 1)) finish_mc_step(p, r, step);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned old = atomicAdd(p.done + r, 1u);
+            last_flag = old == (unsigned)(per_rep - 1);
+        }
+        __syncthreads();
+        if (last_flag) finish_mc_step(p, r, pr, mv, step, i_beg, i_end, rowv, red, &accept_flag);
     }
 }
-
-// Advances the step base by the number of MC launches just enqueued.
-__global__ void advance_step_base(long long *step_base, int n) { *step_base += n; }
 
 // After the scaled full-energy pass: mirror the upper triangle of each
 // replica's matrix and set the running total to the sum over I < J.
@@ -797,8 +738,6 @@ struct cgppd_ctx {
     float *d_radius = nullptr;
     bool fast_image = true;
     double *d_rowbuf = nullptr, *d_emat = nullptr, *d_energy = nullptr, *d_temp = nullptr;
-    double *d_rowval = nullptr;
-    long long *d_step_base = nullptr;
     double *d_rcos = nullptr, *d_rsin = nullptr;
     float *d_tstep = nullptr;
     unsigned *d_done = nullptr;
@@ -890,8 +829,6 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_center);
     dev_free(ctx->d_radius);
     dev_free(ctx->d_rowbuf);
-    dev_free(ctx->d_rowval);
-    dev_free(ctx->d_step_base);
     dev_free(ctx->d_emat);
     dev_free(ctx->d_energy);
     dev_free(ctx->d_temp);
@@ -1021,10 +958,6 @@ PairParams base_params(cgppd_ctx *ctx)
     p.dh_factor = ctx->dh_factor;
     p.seed = ctx->seed;
     p.rowbuf = ctx->d_rowbuf;
-    p.rowval = ctx->d_rowval;
-    p.step_base = ctx->d_step_base;
-    p.launch_k = 0;
-    p.jw_count = (ctx->N + kWave - 1) / kWave;
     p.done = ctx->d_done;
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
@@ -1093,23 +1026,14 @@ int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
     return CGPPD_OK;
 }
 
-int launch_mc_step(cgppd_ctx *ctx, int k)
+int launch_mc_step(cgppd_ctx *ctx)
 {
     PairParams p = base_params(ctx);
     p.table = ctx->d_table_scaled;
-    p.launch_k = k;
-    const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 4;
-    int g = ctx->n_cu * per_cu;
-    g = std::max(8, g / 8 * 8);
-    p.xcd_map = 1;
+    p.xcd_map = (ctx->R % 8 == 0) ? 1 : 0;
+    const long long items = (long long)ctx->R * p.ib_max * p.jc_count;
+    const int g = grid_for(ctx, items, p.xcd_map);
     hipLaunchKernelGGL(mc_step_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
-    CG_HIP(ctx, hipGetLastError());
-    return CGPPD_OK;
-}
-
-int launch_advance(cgppd_ctx *ctx, int n)
-{
-    hipLaunchKernelGGL(advance_step_base, dim3(1), dim3(1), 0, ctx->stream, ctx->d_step_base, n);
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
@@ -1282,8 +1206,6 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     int rc = check_ready(ctx, false);
     if (rc) return rc;
     if (n_rep < 1 || !center || !rel) return fail(ctx, CGPPD_E_ARG, "need n_rep >= 1, centres and positions");
-    if (n_rep > CGPPD_MAX_REPLICAS)
-        return fail(ctx, CGPPD_E_ARG, "at most %d replicas per context (got %d)", CGPPD_MAX_REPLICAS, n_rep);
     CG_HIP(ctx, hipSetDevice(ctx->device));
     free_replicas(ctx);
     const int M = ctx->M, N = ctx->N, S = ctx->stride, R = n_rep;
@@ -1333,8 +1255,6 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_center, (size_t)R * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_radius, (size_t)R * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_rowbuf, (size_t)R * M))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_rowval, (size_t)R * M))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_step_base, 1))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_emat, (size_t)R * M * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_energy, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_temp, R))) return rc;
@@ -1365,7 +1285,6 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     CG_HIP(ctx, hipMemcpy(ctx->d_rank_of, rank_of.data(), sizeof(int) * R, hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_rep_at_rank, at_rank.data(), sizeof(int) * R, hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemset(ctx->d_rowbuf, 0, sizeof(double) * R * M));
-    CG_HIP(ctx, hipMemset(ctx->d_step_base, 0, sizeof(long long)));
     CG_HIP(ctx, hipMemset(ctx->d_emat, 0, sizeof(double) * R * M * M));
     CG_HIP(ctx, hipMemset(ctx->d_energy, 0, sizeof(double) * R));
     CG_HIP(ctx, hipMemset(ctx->d_done, 0, sizeof(unsigned) * R));
@@ -1530,10 +1449,9 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
         if ((rc = ensure_events(ctx, (size_t)2 * n_steps))) return rc;
         for (int k = 0; k < n_steps; ++k) {
             CG_HIP(ctx, hipEventRecord(ctx->events[2 * k], ctx->stream));
-            if ((rc = launch_mc_step(ctx, k))) return rc;
+            if ((rc = launch_mc_step(ctx))) return rc;
             CG_HIP(ctx, hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
         }
-        if ((rc = launch_advance(ctx, n_steps))) return rc;
         CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         double sum = 0.0;
         for (int k = 0; k < n_steps; ++k) {
@@ -1546,8 +1464,7 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
         return CGPPD_OK;
     }
     for (int k = 0; k < n_steps; ++k)
-        if ((rc = launch_mc_step(ctx, k))) return rc;
-    if (n_steps > 0 && (rc = launch_advance(ctx, n_steps))) return rc;
+        if ((rc = launch_mc_step(ctx))) return rc;
     return CGPPD_OK;
 }
 
@@ -1578,8 +1495,7 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
         destroy_graph(ctx);
         CG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int lrc = CGPPD_OK;
-        for (int k = 0; k < steps_per_interval && !lrc; ++k) lrc = launch_mc_step(ctx, k);
-        if (!lrc && steps_per_interval > 0) lrc = launch_advance(ctx, steps_per_interval);
+        for (int k = 0; k < steps_per_interval && !lrc; ++k) lrc = launch_mc_step(ctx);
         if (!lrc) lrc = launch_exchange(ctx);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
