@@ -210,6 +210,9 @@ struct PairParams {
     int jc_count;       // j-chunks over n_res
     const int *mol_off;  // n_mol + 1
     const int *mol_of;   // n_res
+    const int *run_of;       // n_res: run (constant type within a molecule) of each residue
+    const int *run_start;    // n_runs + 1: first residue of each run
+    const unsigned *run_code;  // n_runs: the run's residue code (table row | charged flag)
     // replicas
     int rep_begin, rep_count;
     int xcd_map;  // items of replica r stay on XCD r % 8
@@ -263,14 +266,14 @@ __device__ __forceinline__ void load_table(const Coef *__restrict__ g, char *lds
     }
 }
 
-// One pair term for this lane's residue i against the wave-uniform j record.
+// One pair term for this lane's residue i against the wave-uniform j record
+// rj, with this lane's table entry cf for the (i type, j type) pair.
 // MINIMG: apply the per-axis minimum image; otherwise the segment's single
-// image shift is already folded into (xi, yi, zi).  Returns LJ in `lj` and
-// DH in `dh` (DH only computed for charged j, a wave-uniform branch).
-template <bool MINIMG>
-__device__ __forceinline__ void pair_term(const float4 rj, uint32_t code, float xi, float yi, float zi,
-                                          uint32_t lane_off, const char *__restrict__ tab, float box, float inv_box,
-                                          float &lj, float &dh)
+// image shift is already folded into (xi, yi, zi).  CHARGED: the j residue
+// carries charge, so the Debye-Hueckel term is evaluated.
+template <bool MINIMG, bool CHARGED>
+__device__ __forceinline__ void pair_term(const float4 rj, float xi, float yi, float zi, const Coef &cf, float box,
+                                          float inv_box, float &lj, float &dh)
 {
     float dx = xi - rj.x;
     float dy = yi - rj.y;
@@ -281,59 +284,91 @@ __device__ __forceinline__ void pair_term(const float4 rj, uint32_t code, float 
         dz = fmaf(-box, rintf(dz * inv_box), dz);
     }
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const Coef cf = *reinterpret_cast<const Coef *>(tab + ((code & kRowMask) | lane_off));
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float s2 = cf.sig2 * (rinv * rinv);
     const float s6 = s2 * s2 * s2;
     const float att = fmaf(cf.A, s6, -cf.A) * s6;
     const float rep = fmaf(cf.A, -0.5f, -att);
     lj = r2 < cf.rc2 ? rep : att;
-    dh = 0.0f;
-    if (code & kChargedBit) {
+    if (CHARGED) {
         const float r = r2 * rinv;
         dh = cf.qq * __builtin_amdgcn_exp2f(r * kDHExp2) * rinv;
+    } else {
+        dh = 0.0f;
     }
 }
 
-// Inner loop over j in [js, je) (wave-uniform) for this lane's residue i.
-// Terms are summed plainly in groups of four and the group sums enter a
-// Kahan-compensated accumulator.  SEPARATE: LJ and DH accumulated apart
-// (raw table); otherwise one total (scaled table).
-template <bool SEPARATE, bool MINIMG>
-__device__ __forceinline__ void pair_segment(const float4 *__restrict__ jrec, int js, int je, float xi, float yi,
-                                             float zi, uint32_t lane_off, const char *__restrict__ tab, float box,
-                                             float inv_box, float &s0, float &c0, float &s1, float &c1)
+// A run [a, b) of j residues that share one type (wave-uniform), against
+// this lane's residue i.  Terms are summed plainly in groups of four and the
+// group sums enter a Kahan-compensated accumulator.  SEPARATE: LJ and DH
+// accumulated apart (raw table); otherwise one total (scaled table).
+template <bool SEPARATE, bool MINIMG, bool CHARGED>
+__device__ __forceinline__ void pair_run(const float4 *__restrict__ jrec, int a, int b, float xi, float yi, float zi,
+                                         const Coef &cf, float box, float inv_box, float &s0, float &c0, float &s1,
+                                         float &c1)
 {
-    int j = js;
-    for (; j + 4 <= je; j += 4) {
+    int j = a;
+    for (; j + 4 <= b; j += 4) {
         float g0 = 0.0f, g1 = 0.0f;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const float4 rj = jrec[j + u];
-            const uint32_t code = __builtin_amdgcn_readfirstlane(__float_as_uint(rj.w));
             float lj, dh;
-            pair_term<MINIMG>(rj, code, xi, yi, zi, lane_off, tab, box, inv_box, lj, dh);
+            pair_term<MINIMG, CHARGED>(jrec[j + u], xi, yi, zi, cf, box, inv_box, lj, dh);
             if (SEPARATE) {
                 g0 += lj;
-                g1 += dh;
+                if (CHARGED) g1 += dh;
             } else {
-                g0 += lj + dh;
+                g0 += CHARGED ? lj + dh : lj;
             }
         }
         kahan_add(s0, c0, g0);
-        if (SEPARATE) kahan_add(s1, c1, g1);
+        if (SEPARATE && CHARGED) kahan_add(s1, c1, g1);
     }
-    for (; j < je; ++j) {
-        const float4 rj = jrec[j];
-        const uint32_t code = __builtin_amdgcn_readfirstlane(__float_as_uint(rj.w));
-        float lj, dh;
-        pair_term<MINIMG>(rj, code, xi, yi, zi, lane_off, tab, box, inv_box, lj, dh);
-        if (SEPARATE) {
-            kahan_add(s0, c0, lj);
-            kahan_add(s1, c1, dh);
-        } else {
-            kahan_add(s0, c0, lj + dh);
+    // the remaining 0-3 terms form one more group
+    const int n = b - j;
+    if (n > 0) {
+        float g0 = 0.0f, g1 = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (u < n) {
+                float lj, dh;
+                pair_term<MINIMG, CHARGED>(jrec[j + u], xi, yi, zi, cf, box, inv_box, lj, dh);
+                if (SEPARATE) {
+                    g0 += lj;
+                    if (CHARGED) g1 += dh;
+                } else {
+                    g0 += CHARGED ? lj + dh : lj;
+                }
+            }
         }
+        kahan_add(s0, c0, g0);
+        if (SEPARATE && CHARGED) kahan_add(s1, c1, g1);
+    }
+}
+
+// Inner loop over the j residues [js, je) of one molecule (wave-uniform,
+// indices relative to the staged chunk starting at jbase).  On the device
+// each molecule's residues are ordered charged-first, then by type, so the
+// range splits into constant-type runs: the lane's table entry is loaded
+// once per run and uncharged runs skip the Debye-Hueckel term.
+template <bool SEPARATE, bool MINIMG>
+__device__ __forceinline__ void pair_segment(const float4 *__restrict__ jrec, int js, int je, int jbase,
+                                             const PairParams &p, float xi, float yi, float zi, uint32_t lane_off,
+                                             const char *__restrict__ tab, float &s0, float &c0, float &s1,
+                                             float &c1)
+{
+    int run = p.run_of[jbase + js];
+    int a = js;
+    while (a < je) {
+        const int b = min(je, p.run_start[run + 1] - jbase);
+        const uint32_t code = p.run_code[run];
+        const Coef cf = *reinterpret_cast<const Coef *>(tab + ((code & kRowMask) | lane_off));
+        if (code & kChargedBit)
+            pair_run<SEPARATE, MINIMG, true>(jrec, a, b, xi, yi, zi, cf, p.box, p.inv_box, s0, c0, s1, c1);
+        else
+            pair_run<SEPARATE, MINIMG, false>(jrec, a, b, xi, yi, zi, cf, p.box, p.inv_box, s0, c0, s1, c1);
+        a = b;
+        ++run;
     }
 }
 
@@ -360,17 +395,17 @@ __device__ __forceinline__ bool segment_shift(float4 ci, float4 cj, float rsum, 
 // A segment (i-block of one molecule against the j residues [js, je) of
 // molecule J): the shifted fast path when the bounding spheres allow it.
 template <bool SEPARATE>
-__device__ __forceinline__ void segment(const float4 *__restrict__ jrec, int js, int je, float4 pi, uint32_t lane_off,
-                                        const char *__restrict__ tab, const PairParams &p, float4 ci, float ri,
-                                        float4 cj, float rj, float &s0, float &c0, float &s1, float &c1)
+__device__ __forceinline__ void segment(const float4 *__restrict__ jrec, int js, int je, int jbase, float4 pi,
+                                        uint32_t lane_off, const char *__restrict__ tab, const PairParams &p,
+                                        float4 ci, float ri, float4 cj, float rj, float &s0, float &c0, float &s1,
+                                        float &c1)
 {
     float sx, sy, sz;
     if (p.fast_image && segment_shift(ci, cj, ri + rj, p.box, p.inv_box, sx, sy, sz))
-        pair_segment<SEPARATE, false>(jrec, js, je, pi.x - sx, pi.y - sy, pi.z - sz, lane_off, tab, p.box,
-                                      p.inv_box, s0, c0, s1, c1);
+        pair_segment<SEPARATE, false>(jrec, js, je, jbase, p, pi.x - sx, pi.y - sy, pi.z - sz, lane_off, tab, s0, c0,
+                                      s1, c1);
     else
-        pair_segment<SEPARATE, true>(jrec, js, je, pi.x, pi.y, pi.z, lane_off, tab, p.box, p.inv_box, s0, c0, s1,
-                                     c1);
+        pair_segment<SEPARATE, true>(jrec, js, je, jbase, p, pi.x, pi.y, pi.z, lane_off, tab, s0, c0, s1, c1);
 }
 
 // Maps a persistent workgroup's k-th item to (replica, item-in-replica).
@@ -448,7 +483,7 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
             const int J = p.mol_of[js];
             const int se = min(je, p.mol_off[J + 1]);
             float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-            segment<!SCALED>(jrec, js - jc * kJChunk, se - jc * kJChunk, pi, lane_off, tab, p, ci, ri,
+            segment<!SCALED>(jrec, js - jc * kJChunk, se - jc * kJChunk, jc * kJChunk, pi, lane_off, tab, p, ci, ri,
                              p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
             double v0 = valid ? (double)s0 - (double)c0 : 0.0;
             double v1 = valid ? (double)s1 - (double)c1 : 0.0;
@@ -589,7 +624,7 @@ __global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
                 const int se = min(je, p.mol_off[J + 1]);
                 if (J != m) {
                     float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-                    segment<false>(jrec, js - jbase, se - jbase, pn, lane_off, tab, p, ci, ri,
+                    segment<false>(jrec, js - jbase, se - jbase, jbase, pn, lane_off, tab, p, ci, ri,
                                    p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1,
                                    c1);
                     double v = valid ? (double)s0 - (double)c0 : 0.0;
@@ -729,6 +764,11 @@ struct cgppd_ctx {
 
     int M = 0, N = 0, stride = 0, n_nc = 0, max_n = 0;
     std::vector<int> mol_off, type, code;
+    // device residue order: within each molecule charged residues first, then
+    // by type (perm[device index] = caller's index, inv the reverse)
+    std::vector<int> perm, inv, type_dev;
+    int *d_run_of = nullptr, *d_run_start = nullptr;
+    unsigned *d_run_code = nullptr;
     int R = 0;
     uint64_t seed = 0;
 
@@ -894,6 +934,46 @@ float code_bits(uint32_t c)
     return f;
 }
 
+// Device residue order and the constant-type runs of each molecule.  Needs
+// the topology and the charges of the force field.
+int build_order(cgppd_ctx *ctx)
+{
+    const int M = ctx->M, N = ctx->N;
+    ctx->perm.resize(N);
+    ctx->inv.resize(N);
+    ctx->type_dev.resize(N);
+    std::vector<int> run_of(N), run_start;
+    std::vector<unsigned> run_code;
+    for (int m = 0; m < M; ++m) {
+        const int a = ctx->mol_off[m], b = ctx->mol_off[m + 1];
+        for (int i = a; i < b; ++i) ctx->perm[i] = i;
+        std::stable_sort(ctx->perm.begin() + a, ctx->perm.begin() + b, [&](int x, int y) {
+            const bool cx = ctx->charge[ctx->type[x]] != 0.0f, cy = ctx->charge[ctx->type[y]] != 0.0f;
+            if (cx != cy) return cx;
+            return ctx->type[x] < ctx->type[y];
+        });
+        for (int i = a; i < b; ++i) {
+            ctx->inv[ctx->perm[i]] = i;
+            ctx->type_dev[i] = ctx->type[ctx->perm[i]];
+            if (i == a || ctx->type_dev[i] != ctx->type_dev[i - 1]) {
+                run_start.push_back(i);
+                run_code.push_back(residue_code(ctx, ctx->type_dev[i]));
+            }
+            run_of[i] = (int)run_start.size() - 1;
+        }
+    }
+    run_start.push_back(N);
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_run_of, N))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_run_start, run_start.size()))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_run_code, run_code.size()))) return rc;
+    CG_HIP(ctx, hipMemcpy(ctx->d_run_of, run_of.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_run_start, run_start.data(), sizeof(int) * run_start.size(), hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_run_code, run_code.data(), sizeof(unsigned) * run_code.size(),
+                          hipMemcpyHostToDevice));
+    return CGPPD_OK;
+}
+
 // Bounding radius of a molecule: max |relative position|.
 float molecule_radius(const float4 *rel, int n)
 {
@@ -904,8 +984,9 @@ float molecule_radius(const float4 *rel, int n)
 }
 
 // Centre and relative positions from absolute coordinates, with the float
-// accumulation of src/Molecule.cpp:590-609; writes pos, rel (w = code),
-// and returns the centre and radius.
+// accumulation of src/Molecule.cpp:590-609 in the caller's residue order;
+// writes pos and rel (w = code) in device order for the molecule starting
+// at residue i0, and returns the centre and radius.
 void split_molecule(const cgppd_ctx *ctx, int i0, int n, const float *xyz, float4 *pos, float4 *rel, float4 &centre,
                     float &radius)
 {
@@ -920,9 +1001,10 @@ void split_molecule(const cgppd_ctx *ctx, int i0, int n, const float *xyz, float
     cy /= fn;
     cz /= fn;
     for (int k = 0; k < n; ++k) {
+        const int d = ctx->inv[i0 + k] - i0;
         const float w = code_bits(residue_code(ctx, ctx->type[i0 + k]));
-        pos[k] = make_float4(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], w);
-        rel[k] = make_float4(xyz[3 * k] - cx, xyz[3 * k + 1] - cy, xyz[3 * k + 2] - cz, w);
+        pos[d] = make_float4(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], w);
+        rel[d] = make_float4(xyz[3 * k] - cx, xyz[3 * k + 1] - cy, xyz[3 * k + 2] - cz, w);
     }
     centre = make_float4(cx, cy, cz, 0.0f);
     radius = molecule_radius(rel, n);
@@ -940,6 +1022,9 @@ PairParams base_params(cgppd_ctx *ctx)
     p.jc_count = (ctx->N + kJChunk - 1) / kJChunk;
     p.mol_off = ctx->d_mol_off;
     p.mol_of = ctx->d_mol_of;
+    p.run_of = ctx->d_run_of;
+    p.run_start = ctx->d_run_start;
+    p.run_code = ctx->d_run_code;
     p.rep_begin = 0;
     p.rep_count = ctx->R;
     p.xcd_map = 0;
@@ -1116,6 +1201,9 @@ int cgppd_destroy(cgppd_ctx *ctx)
     dev_free(ctx->d_table_scaled);
     dev_free(ctx->d_mol_off);
     dev_free(ctx->d_mol_of);
+    dev_free(ctx->d_run_of);
+    dev_free(ctx->d_run_start);
+    dev_free(ctx->d_run_code);
     for (hipEvent_t ev : ctx->events) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1145,8 +1233,10 @@ int cgppd_set_force_field(cgppd_ctx *ctx, const float *eps, const float *sigma, 
     int rc = build_tables(ctx);
     if (rc) return rc;
     ctx->have_ff = true;
-    // residue codes carry the charged flag: replicas must be set again
+    // residue codes carry the charged flag, and charges set the device
+    // residue order: replicas must be set again
     if (ctx->have_rep) free_replicas(ctx);
+    if (ctx->have_sys && (rc = build_order(ctx))) return rc;
     return CGPPD_OK;
 }
 
@@ -1197,6 +1287,7 @@ int cgppd_set_system(cgppd_ctx *ctx, int n_mol, const int *mol_off, const int *t
     CG_HIP(ctx, hipMemcpy(ctx->d_mol_off, mol_off, sizeof(int) * (n_mol + 1), hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_mol_of, mol_of.data(), sizeof(int) * N, hipMemcpyHostToDevice));
     ctx->have_sys = true;
+    if (ctx->have_ff && (rc = build_order(ctx))) return rc;
     return CGPPD_OK;
 }
 
@@ -1217,8 +1308,8 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
             const float *c = center + ((size_t)r * M + m) * 3;
             hcen[(size_t)r * M + m] = make_float4(c[0], c[1], c[2], 0.0f);
             for (int i = ctx->mol_off[m]; i < ctx->mol_off[m + 1]; ++i) {
-                const float *v = rel + ((size_t)r * N + i) * 3;
-                const float w = code_bits(residue_code(ctx, ctx->type[i]));
+                const float *v = rel + ((size_t)r * N + ctx->perm[i]) * 3;
+                const float w = code_bits(residue_code(ctx, ctx->type_dev[i]));
                 hrel[(size_t)r * S + i] = make_float4(v[0], v[1], v[2], w);
                 // src/Molecule.cpp:183-191: position = relative + centre
                 hpos[(size_t)r * S + i] = make_float4(v[0] + c[0], v[1] + c[1], v[2] + c[2], w);
@@ -1561,18 +1652,20 @@ int cgppd_get_coordinates(cgppd_ctx *ctx, int replica, float *pos, float *rel, f
         CG_HIP(ctx, hipMemcpy(h.data(), ctx->d_pos + (size_t)replica * ctx->stride, sizeof(float4) * N,
                               hipMemcpyDeviceToHost));
         for (int i = 0; i < N; ++i) {
-            pos[3 * i] = h[i].x;
-            pos[3 * i + 1] = h[i].y;
-            pos[3 * i + 2] = h[i].z;
+            const int u = ctx->perm[i];
+            pos[3 * u] = h[i].x;
+            pos[3 * u + 1] = h[i].y;
+            pos[3 * u + 2] = h[i].z;
         }
     }
     if (rel) {
         CG_HIP(ctx, hipMemcpy(h.data(), ctx->d_rel + (size_t)replica * ctx->stride, sizeof(float4) * N,
                               hipMemcpyDeviceToHost));
         for (int i = 0; i < N; ++i) {
-            rel[3 * i] = h[i].x;
-            rel[3 * i + 1] = h[i].y;
-            rel[3 * i + 2] = h[i].z;
+            const int u = ctx->perm[i];
+            rel[3 * u] = h[i].x;
+            rel[3 * u + 1] = h[i].y;
+            rel[3 * u + 2] = h[i].z;
         }
     }
     if (center) {
