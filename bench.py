@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 4])
     ap.add_argument("--exchange-every", type=int, default=1000)
+    ap.add_argument("--replicas", type=int, default=0, help="override the configuration's replica count")
     ap.add_argument("--wg-per-cu", type=int, default=0)
     ap.add_argument("--no-image-shift", action="store_true", help="per-pair minimum image everywhere")
     ap.add_argument("--seed", type=int, default=20261203)
@@ -95,6 +96,8 @@ def main():
     from cgppd_amd import engine, synth
 
     spec = synth.config(args.config)
+    if args.replicas:
+        spec.n_rep = args.replicas
     e = engine.Engine(0)
     e.load_spec(spec, seed=args.seed)
     if args.wg_per_cu:

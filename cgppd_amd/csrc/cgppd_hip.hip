@@ -47,7 +47,8 @@ constexpr uint32_t kRowMask = 0x1Fu << kRowShift;
 constexpr uint32_t kChargedBit = 0x80000000u;
 constexpr uint32_t kTagMC = 0x4D430000u;
 constexpr uint32_t kTagEX = 0x45580000u;
-#define CGPPD_MAX_MOLECULES 1024  // molecules per system (LDS row buffer of the MC finish)
+#define CGPPD_MAX_MOLECULES 1024  // molecules per system
+#define CGPPD_MAX_REPLICAS 1024   // replicas per context (LDS item offsets of the pair kernel)
 
 // Model constants, as the reference evaluates them in float
 // (src/constants.h:8-26): T_room*Rgas/kcal and e^2 N_A/(4 pi eps0 kcal Ang D).
@@ -239,6 +240,13 @@ struct PairParams {
     long long *counters;  // R * 3
     long long *steps;     // R
     unsigned long long *pair_evals;  // 1: inter-molecular residue pairs evaluated by MC steps
+    // the pending proposal of every replica (laid out by mc_finish_kernel)
+    int max_n;            // residues of the largest molecule (prop_* row stride)
+    float4 *prop_pos;     // R * max_n: proposed positions, device order, w = code
+    float4 *prop_rel;     // R * max_n: proposed relative positions
+    float4 *prop_center;  // R
+    int *prop_mol;        // R
+    int *prop_nib;        // R: i-blocks of the moved molecule
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -507,144 +515,162 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
 }
 
 
-// Completes the MC step of replica r once all its items are in: Metropolis on
-// the delta-E row (src/Replica.cpp:385-410, delta and exponent in FP64),
-// apply the move (src/Molecule.cpp:468-504) and update the matrix row and
-// column and the running total.  Runs on one whole workgroup.  The row is
-// read and cleared with atomic exchanges, at the same coherence point as
-// the atomic additions that built it, so no cache flush is needed.
-__device__ void finish_mc_step(const PairParams &p, int r, const Proposal &pr, const Move &mv, int64_t step,
-                               int i_beg, int i_end, double *rowv, double *red, int *accept_flag)
-{
-    const int M = p.n_mol, m = pr.mol;
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    unsigned long long *row = reinterpret_cast<unsigned long long *>(p.rowbuf + (size_t)r * M);
-    double *mat = p.emat + (size_t)r * M * M;
-    double d = 0.0;
-    for (int J = threadIdx.x; J < M; J += kWG) {
-        const double v = __longlong_as_double((long long)atomicExch(row + J, 0ull));
-        rowv[J] = v;
-        if (J != m) d += v - mat[(size_t)m * M + J];
-    }
-    d = wave_sum(d);
-    if (lane == 0) red[wave] = d;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double delta = 0.0;
-        for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
-        long long *cnt = p.counters + 3 * (size_t)r;
-        int acc = 0;
-        if (delta < 0.0) {
-            acc = 1;
-            cnt[0] += 1;
-        } else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r]))) {
-            acc = 1;
-            cnt[1] += 1;
-        } else {
-            cnt[2] += 1;
-        }
-        if (acc) p.energy[r] += delta;
-        *accept_flag = acc;
-        p.done[r] = 0u;
-        p.steps[r] = step + 1;
-        const unsigned long long n_m = (unsigned long long)(i_end - i_beg);
-        atomicAdd(p.pair_evals, n_m * ((unsigned long long)p.n_res - n_m));
-    }
-    __syncthreads();
-    if (*accept_flag) {
-        for (int J = threadIdx.x; J < M; J += kWG) {
-            if (J == m) continue;
-            const double v = rowv[J];
-            mat[(size_t)m * M + J] = v;
-            mat[(size_t)J * M + m] = v;
-        }
-        float4 *pos = p.pos + (size_t)r * p.stride;
-        float4 *rel = p.rel + (size_t)r * p.stride;
-        for (int i = i_beg + threadIdx.x; i < i_end; i += kWG) {
-            float4 rn, pn;
-            move_residue(mv, rel[i], rn, pn);
-            rel[i] = rn;
-            pos[i] = pn;
-        }
-        if (threadIdx.x == 0) p.center[(size_t)r * M + m] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-    }
-    __syncthreads();
-}
-
-// One MC step of every replica.  Items: (replica, i-block, j-chunk) over the
-// largest molecule's i-blocks; items beyond the moved molecule do no pair
-// work but are still counted, so a replica's step completes only after every
-// item of this launch has read the replica's step counter.  Each wave waits
-// for its row additions to be acknowledged (s_waitcnt) before the barrier
-// that precedes the completion count; no cache-flushing fence is used.
-__global__ __launch_bounds__(kWG) void mc_step_kernel(PairParams p)
+// MC step, phase 1 (pair kernel): the delta-E row of every replica's moved
+// molecule against all other molecules, FP32 pair terms only.  The proposed
+// residue positions of the moved molecule were laid out by the previous
+// phase 2 (prop_pos, device order, w = residue code), so this kernel carries
+// no double-precision move code.  Work items are the active (replica,
+// i-block, 256-residue j-chunk) triples, listed by prefix sum over the
+// replicas' i-block counts and split into equal contiguous ranges -- one per
+// workgroup, workgroups ordered by XCD so a replica's items share an L2.
+__global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ float4 jrec[kJChunk];
-    __shared__ double rowv[CGPPD_MAX_MOLECULES];
-    __shared__ double red[kWavesPerWG];
-    __shared__ int last_flag, accept_flag;
+    __shared__ int item_off[CGPPD_MAX_REPLICAS + 1];
     load_table(p.table, tab);
+    const int R = p.rep_count, JC = p.jc_count;
+    for (int r = threadIdx.x; r < R; r += kWG) item_off[r + 1] = p.prop_nib[r] * JC;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        item_off[0] = 0;
+        for (int r = 0; r < R; ++r) item_off[r + 1] += item_off[r];
+    }
+    __syncthreads();
+    const int total = item_off[R];
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lb = (p.xcd_map && (G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
+    const int k0 = (int)((long long)lb * total / G), k1 = (int)((long long)(lb + 1) * total / G);
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int per_rep = p.ib_max * p.jc_count;
-    for (ItemIter it(p, per_rep); it.valid(); it.next()) {
-        const int r = it.replica();
-        const int q = it.item();
-        const int jc = q % p.jc_count, ib = q / p.jc_count;
-        const int64_t step = p.steps[r];
-        const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
-        const int m = pr.mol;
-        const int i_beg = p.mol_off[m], i_end = p.mol_off[m + 1];
-        const int n_ib = (i_end - i_beg + kIBlock - 1) / kIBlock;
-        const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + m], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
-        __syncthreads();  // previous item finished with jrec and the flags
-        if (ib < n_ib) {  // uniform
-            const float4 *pos = p.pos + (size_t)r * p.stride;
-            const float4 *rel = p.rel + (size_t)r * p.stride;
-            const int jbase = jc * kJChunk;
-            {
-                const int j = jbase + threadIdx.x;
-                if (j < p.n_res) jrec[threadIdx.x] = pos[j];
-            }
-            const int i = i_beg + ib * kIBlock + lane;
-            const bool valid = i < i_end;
-            float4 rn, pn;
-            move_residue(mv, rel[valid ? i : i_beg], rn, pn);
-            const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
-            const float4 ci = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-            const float ri = p.radius[(size_t)r * p.n_mol + m];
-            __syncthreads();
-
-            int js = jbase + wave * kWave;
-            const int je = min(p.n_res, jbase + (wave + 1) * kWave);
-            while (js < je) {
-                const int J = p.mol_of[js];
-                const int se = min(je, p.mol_off[J + 1]);
-                if (J != m) {
-                    float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-                    segment<false>(jrec, js - jbase, se - jbase, jbase, pn, lane_off, tab, p, ci, ri,
-                                   p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1,
-                                   c1);
-                    double v = valid ? (double)s0 - (double)c0 : 0.0;
-                    v = wave_sum(v);
-                    if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
-                }
-                js = se;
-            }
-            // wait until this wave's row additions are acknowledged by the
-            // memory system before the barrier that precedes the count
-            __builtin_amdgcn_s_waitcnt(0);
+    int r = 0;
+    for (int k = k0; k < k1; ++k) {
+        while (k >= item_off[r + 1]) ++r;
+        const int local = k - item_off[r];
+        const int ib = local / JC, jc = local % JC;
+        const int m = p.prop_mol[r];
+        const int n_m = p.mol_off[m + 1] - p.mol_off[m];
+        const int jbase = jc * kJChunk;
+        __syncthreads();  // previous item finished reading jrec
+        {
+            const int j = jbase + threadIdx.x;
+            if (j < p.n_res) jrec[threadIdx.x] = p.pos[(size_t)r * p.stride + j];
         }
+        const int il = ib * kIBlock + lane;
+        const bool valid = il < n_m;
+        const float4 pn = p.prop_pos[(size_t)r * p.max_n + (valid ? il : 0)];
+        const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
+        const float4 ci = p.prop_center[r];
+        const float ri = p.radius[(size_t)r * p.n_mol + m];
+        __syncthreads();
+        int js = jbase + wave * kWave;
+        const int je = min(p.n_res, jbase + (wave + 1) * kWave);
+        while (js < je) {
+            const int J = p.mol_of[js];
+            const int se = min(je, p.mol_off[J + 1]);
+            if (J != m) {
+                float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+                segment<false>(jrec, js - jbase, se - jbase, jbase, pn, lane_off, tab, p, ci, ri,
+                               p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
+                double v = valid ? (double)s0 - (double)c0 : 0.0;
+                v = wave_sum(v);
+                if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
+            }
+            js = se;
+        }
+    }
+}
+
+// Lays out the proposal of MC step `step` for replica r (whole workgroup):
+// molecule, new centre, and the proposed positions of its residues in
+// device order (src/Molecule.cpp:146-266, src/Quaternion.cpp:12-112).
+__device__ void propose(const PairParams &p, int r, int64_t step)
+{
+    const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
+    const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + pr.mol], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
+    const int i_beg = p.mol_off[pr.mol], i_end = p.mol_off[pr.mol + 1];
+    const float4 *rel = p.rel + (size_t)r * p.stride;
+    for (int i = i_beg + threadIdx.x; i < i_end; i += kWG) {
+        float4 rn, pn;
+        move_residue(mv, rel[i], rn, pn);
+        p.prop_pos[(size_t)r * p.max_n + (i - i_beg)] = pn;
+        p.prop_rel[(size_t)r * p.max_n + (i - i_beg)] = rn;
+    }
+    if (threadIdx.x == 0) {
+        p.prop_mol[r] = pr.mol;
+        p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
+        p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+    }
+}
+
+// MC step, phase 2 (one workgroup per replica): Metropolis on the finished
+// delta-E row (src/Replica.cpp:385-410, delta and exponent in FP64), apply
+// the proposal (src/Molecule.cpp:468-504), update the matrix row and column
+// and the running total, clear the row, then lay out the proposal of the
+// next step.  With propose_only the first proposal is laid out.
+__global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propose_only)
+{
+    __shared__ double red[kWavesPerWG];
+    __shared__ int accept_flag;
+    const int r = blockIdx.x;
+    const int M = p.n_mol;
+    const int64_t step = p.steps[r];
+    if (!propose_only) {
+        const int m = p.prop_mol[r];
+        const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+        double *row = p.rowbuf + (size_t)r * M;
+        double *mat = p.emat + (size_t)r * M * M;
+        double d = 0.0;
+        for (int J = threadIdx.x; J < M; J += kWG)
+            if (J != m) d += row[J] - mat[(size_t)m * M + J];
+        d = wave_sum(d);
+        if (lane == 0) red[wave] = d;
         __syncthreads();
         if (threadIdx.x == 0) {
-            const unsigned old = atomicAdd(p.done + r, 1u);
-            last_flag = old == (unsigned)(per_rep - 1);
+            double delta = 0.0;
+            for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
+            const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
+            long long *cnt = p.counters + 3 * (size_t)r;
+            int acc = 0;
+            if (delta < 0.0) {
+                acc = 1;
+                cnt[0] += 1;
+            } else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r]))) {
+                acc = 1;
+                cnt[1] += 1;
+            } else {
+                cnt[2] += 1;
+            }
+            if (acc) p.energy[r] += delta;
+            accept_flag = acc;
+            const unsigned long long n_m = (unsigned long long)(p.mol_off[m + 1] - p.mol_off[m]);
+            atomicAdd(p.pair_evals, n_m * ((unsigned long long)p.n_res - n_m));
         }
         __syncthreads();
-        if (last_flag) finish_mc_step(p, r, pr, mv, step, i_beg, i_end, rowv, red, &accept_flag);
+        if (accept_flag) {
+            for (int J = threadIdx.x; J < M; J += kWG) {
+                if (J == m) continue;
+                const double v = row[J];
+                mat[(size_t)m * M + J] = v;
+                mat[(size_t)J * M + m] = v;
+            }
+            const int i_beg = p.mol_off[m], n_m = p.mol_off[m + 1] - i_beg;
+            float4 *pos = p.pos + (size_t)r * p.stride + i_beg;
+            float4 *rel = p.rel + (size_t)r * p.stride + i_beg;
+            for (int k = threadIdx.x; k < n_m; k += kWG) {
+                pos[k] = p.prop_pos[(size_t)r * p.max_n + k];
+                rel[k] = p.prop_rel[(size_t)r * p.max_n + k];
+            }
+            if (threadIdx.x == 0) p.center[(size_t)r * M + m] = p.prop_center[r];
+        }
+        __syncthreads();  // every thread read the row before it is cleared
+        for (int J = threadIdx.x; J < M; J += kWG) row[J] = 0.0;
+        __syncthreads();  // the new centre and relative positions are in place
     }
+    const int64_t next = propose_only ? step : step + 1;
+    if (!propose_only && threadIdx.x == 0) p.steps[r] = next;
+    propose(p, r, next);
 }
 
 // After the scaled full-energy pass: mirror the upper triangle of each
@@ -781,6 +807,8 @@ struct cgppd_ctx {
     double *d_rcos = nullptr, *d_rsin = nullptr;
     float *d_tstep = nullptr;
     unsigned *d_done = nullptr;
+    float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr;
+    int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
     unsigned long long *d_ex_stats = nullptr;
     unsigned long long *d_pair_evals = nullptr;
@@ -876,6 +904,11 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_rsin);
     dev_free(ctx->d_tstep);
     dev_free(ctx->d_done);
+    dev_free(ctx->d_prop_pos);
+    dev_free(ctx->d_prop_rel);
+    dev_free(ctx->d_prop_center);
+    dev_free(ctx->d_prop_mol);
+    dev_free(ctx->d_prop_nib);
     dev_free(ctx->d_counters);
     dev_free(ctx->d_steps);
     dev_free(ctx->d_ex_round);
@@ -1044,6 +1077,12 @@ PairParams base_params(cgppd_ctx *ctx)
     p.seed = ctx->seed;
     p.rowbuf = ctx->d_rowbuf;
     p.done = ctx->d_done;
+    p.max_n = ctx->max_n;
+    p.prop_pos = ctx->d_prop_pos;
+    p.prop_rel = ctx->d_prop_rel;
+    p.prop_center = ctx->d_prop_center;
+    p.prop_mol = ctx->d_prop_mol;
+    p.prop_nib = ctx->d_prop_nib;
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
     p.temperature = ctx->d_temp;
@@ -1111,16 +1150,31 @@ int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
     return CGPPD_OK;
 }
 
-int launch_mc_step(cgppd_ctx *ctx)
+int launch_pair(cgppd_ctx *ctx)
 {
     PairParams p = base_params(ctx);
     p.table = ctx->d_table_scaled;
-    p.xcd_map = (ctx->R % 8 == 0) ? 1 : 0;
-    const long long items = (long long)ctx->R * p.ib_max * p.jc_count;
-    const int g = grid_for(ctx, items, p.xcd_map);
-    hipLaunchKernelGGL(mc_step_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    p.xcd_map = 1;
+    const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 4;
+    const int g = std::max(8, ctx->n_cu * per_cu / 8 * 8);
+    hipLaunchKernelGGL(mc_pair_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
+}
+
+int launch_finish(cgppd_ctx *ctx, int propose_only)
+{
+    PairParams p = base_params(ctx);
+    hipLaunchKernelGGL(mc_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, propose_only);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+int launch_mc_step(cgppd_ctx *ctx)
+{
+    int rc = launch_pair(ctx);
+    if (rc) return rc;
+    return launch_finish(ctx, 0);
 }
 
 int launch_exchange(cgppd_ctx *ctx)
@@ -1297,6 +1351,8 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     int rc = check_ready(ctx, false);
     if (rc) return rc;
     if (n_rep < 1 || !center || !rel) return fail(ctx, CGPPD_E_ARG, "need n_rep >= 1, centres and positions");
+    if (n_rep > CGPPD_MAX_REPLICAS)
+        return fail(ctx, CGPPD_E_ARG, "at most %d replicas per context (got %d)", CGPPD_MAX_REPLICAS, n_rep);
     CG_HIP(ctx, hipSetDevice(ctx->device));
     free_replicas(ctx);
     const int M = ctx->M, N = ctx->N, S = ctx->stride, R = n_rep;
@@ -1353,6 +1409,11 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_rsin, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_tstep, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_done, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_pos, (size_t)R * ctx->max_n))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_rel, (size_t)R * ctx->max_n))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_center, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_mol, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_counters, (size_t)3 * R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_steps, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_ex_round, 1))) return rc;
@@ -1514,6 +1575,7 @@ int cgppd_mc_prepare(cgppd_ctx *ctx)
     PairParams p = base_params(ctx);
     hipLaunchKernelGGL(prepare_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p);
     CG_HIP(ctx, hipGetLastError());
+    if ((rc = launch_finish(ctx, 1))) return rc;
     CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->mc_ready = true;
     return CGPPD_OK;
@@ -1540,8 +1602,9 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
         if ((rc = ensure_events(ctx, (size_t)2 * n_steps))) return rc;
         for (int k = 0; k < n_steps; ++k) {
             CG_HIP(ctx, hipEventRecord(ctx->events[2 * k], ctx->stream));
-            if ((rc = launch_mc_step(ctx))) return rc;
+            if ((rc = launch_pair(ctx))) return rc;
             CG_HIP(ctx, hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
+            if ((rc = launch_finish(ctx, 0))) return rc;
         }
         CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         double sum = 0.0;
