@@ -247,7 +247,6 @@ struct PairParams {
     float4 *prop_center;  // R
     int *prop_mol;        // R
     int *prop_nib;        // R: i-blocks of the moved molecule
-    unsigned *work;       // 8 item counters of the pair kernel (reset by the finish kernel)
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -522,15 +521,13 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
 // phase 2 (prop_pos, device order, w = residue code), so this kernel carries
 // no double-precision move code.  Work items are the active (replica,
 // i-block, 256-residue j-chunk) triples, listed by prefix sum over the
-// replicas' i-block counts; workgroups claim them dynamically from one
-// counter per XCD part, so a replica's items share an L2 and no workgroup
-// idles while another holds two items.
+// replicas' i-block counts and split into equal contiguous ranges -- one per
+// workgroup, workgroups ordered by XCD so a replica's items share an L2.
 __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ float4 jrec[kJChunk];
     __shared__ int item_off[CGPPD_MAX_REPLICAS + 1];
-    __shared__ int next_item;
     load_table(p.table, tab);
     const int R = p.rep_count, JC = p.jc_count;
     for (int r = threadIdx.x; r < R; r += kWG) item_off[r + 1] = p.prop_nib[r] * JC;
@@ -541,30 +538,21 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     }
     __syncthreads();
     const int total = item_off[R];
-    // the item list is cut into 8 equal contiguous parts, one per XCD
-    // (workgroup b runs on XCD b % 8); a part's workgroups claim items one
-    // at a time from its counter, which the finish kernel resets
-    const int x = p.xcd_map ? (blockIdx.x & 7) : 0;
-    const int nparts = p.xcd_map ? 8 : 1;
-    const int part0 = (int)((long long)x * total / nparts), part1 = (int)((long long)(x + 1) * total / nparts);
-    if (threadIdx.x == 0) next_item = part0 + (int)atomicAdd(p.work + x, 1u);
-    __syncthreads();
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lb = (p.xcd_map && (G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
+    const int k0 = (int)((long long)lb * total / G), k1 = (int)((long long)(lb + 1) * total / G);
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     int r = 0;
-    while (true) {
-        const int k = next_item;
-        if (k >= part1) break;  // uniform
-        __syncthreads();        // everyone read next_item; jrec is free
-        if (threadIdx.x == 0) next_item = part0 + (int)atomicAdd(p.work + x, 1u);  // claim ahead
+    for (int k = k0; k < k1; ++k) {
         while (k >= item_off[r + 1]) ++r;
-        while (k < item_off[r]) --r;
         const int local = k - item_off[r];
         const int ib = local / JC, jc = local % JC;
         const int m = p.prop_mol[r];
         const int n_m = p.mol_off[m + 1] - p.mol_off[m];
         const int jbase = jc * kJChunk;
+        __syncthreads();  // previous item finished reading jrec
         {
             const int j = jbase + threadIdx.x;
             if (j < p.n_res) jrec[threadIdx.x] = p.pos[(size_t)r * p.stride + j];
@@ -682,7 +670,6 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
     }
     const int64_t next = propose_only ? step : step + 1;
     if (!propose_only && threadIdx.x == 0) p.steps[r] = next;
-    if (r == 0 && threadIdx.x < 8) p.work[threadIdx.x] = 0u;
     propose(p, r, next);
 }
 
@@ -822,7 +809,6 @@ struct cgppd_ctx {
     unsigned *d_done = nullptr;
     float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr;
     int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
-    unsigned *d_work = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
     unsigned long long *d_ex_stats = nullptr;
     unsigned long long *d_pair_evals = nullptr;
@@ -923,7 +909,6 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_prop_center);
     dev_free(ctx->d_prop_mol);
     dev_free(ctx->d_prop_nib);
-    dev_free(ctx->d_work);
     dev_free(ctx->d_counters);
     dev_free(ctx->d_steps);
     dev_free(ctx->d_ex_round);
@@ -1098,7 +1083,6 @@ PairParams base_params(cgppd_ctx *ctx)
     p.prop_center = ctx->d_prop_center;
     p.prop_mol = ctx->d_prop_mol;
     p.prop_nib = ctx->d_prop_nib;
-    p.work = ctx->d_work;
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
     p.temperature = ctx->d_temp;
@@ -1171,7 +1155,8 @@ int launch_pair(cgppd_ctx *ctx)
     PairParams p = base_params(ctx);
     p.table = ctx->d_table_scaled;
     p.xcd_map = 1;
-    const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 4;
+    // 6 workgroups per CU measured best for the pair kernel (4: -9 %, 8: -6 %)
+    const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 6;
     const int g = std::max(8, ctx->n_cu * per_cu / 8 * 8);
     hipLaunchKernelGGL(mc_pair_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
     CG_HIP(ctx, hipGetLastError());
@@ -1430,7 +1415,6 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_prop_center, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_mol, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_work, 8))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_counters, (size_t)3 * R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_steps, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_ex_round, 1))) return rc;
