@@ -11,9 +11,13 @@ Exchange intervals are replayed from a captured hipGraph.
 
 value = MC steps/s summed over all replicas (replicas x steps / wall), with
 the replicas already resident in HBM.  The roofline object refers to the
-dominant kernel (mc_step_kernel): algorithmic FP32 flops = 35 per inter-
+dominant kernel (mc_pair_kernel): algorithmic FP32 flops = 35 per inter-
 molecular residue pair, pairs counted on the device, divided by the kernel's
-mean duration measured with HIP events on the context stream.
+mean duration measured with HIP events on the context stream.  Its HBM
+traffic per launch comes from two rocprofv3 --pmc passes (FETCH_SIZE,
+WRITE_SIZE), scaled by what the same counters read for a plain device copy
+of known size (--copy-probe-mib), and is read from the committed summary in
+profiles/ when present.
 
 The cpu_baseline leg times the reference's CPU loop (full Replica::E per MC
 step, pthreads over replicas) as restated in oracle/, on a bounded sample.
@@ -51,7 +55,40 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run this many direct MC launches (for rocprofv3), no JSON line")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this size (counter calibration)")
     return ap.parse_args()
+
+
+TRAFFIC_SUMMARY = ROOT / "profiles" / "mc_pair_kernel_hbm_traffic_crowded40.txt"
+
+
+def committed_traffic(workload):
+    """HBM bytes per pair-kernel launch from the committed PMC summary
+    (line 'bytes_per_launch <workload> <value>'), or None."""
+    try:
+        for line in TRAFFIC_SUMMARY.read_text().splitlines():
+            f = line.split()
+            if len(f) == 3 and f[0] == "bytes_per_launch" and f[1] == workload:
+                return float(f[2])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib):
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
+    calibrate the counters (each should read mib MiB)."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB x 3", flush=True)
+    return 0
 
 
 def cpu_baseline(spec, seed, target_s):
@@ -93,6 +130,8 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         return 2
+    if args.copy_probe_mib:
+        return copy_probe(args.copy_probe_mib)
     from cgppd_amd import engine, synth
 
     spec = synth.config(args.config)
@@ -166,12 +205,12 @@ def main():
         "exchange_acceptance": float(s["exchange"][1] / max(1, s["exchange"][0])),
         "roofline": {
             "bound": "valu",
-            "kernel": "mc_step_kernel",
+            "kernel": "mc_pair_kernel",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": committed_traffic(spec.name),
             "kernel_ms": kern_ms,
             "pairs_per_launch": pairs_timed / n_timed,
             "flops_per_pair": FLOPS_PER_PAIR,

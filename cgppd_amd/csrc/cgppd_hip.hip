@@ -22,6 +22,7 @@
 //     MC step; an exchange interval is captured once in a hipGraph.
 //   * No MFMA: the pair sum is a table-lookup n-body reduction.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -1150,7 +1151,9 @@ int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
     return CGPPD_OK;
 }
 
-int launch_pair(cgppd_ctx *ctx)
+// ev_start/ev_stop (optional) are recorded by the dispatch itself, so their
+// elapsed time is the kernel's execution time without separate event packets.
+int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr)
 {
     PairParams p = base_params(ctx);
     p.table = ctx->d_table_scaled;
@@ -1158,7 +1161,10 @@ int launch_pair(cgppd_ctx *ctx)
     // 6 workgroups per CU measured best for the pair kernel (4: -9 %, 8: -6 %)
     const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 6;
     const int g = std::max(8, ctx->n_cu * per_cu / 8 * 8);
-    hipLaunchKernelGGL(mc_pair_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    if (ev_start)
+        hipExtLaunchKernelGGL(mc_pair_kernel, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+    else
+        hipLaunchKernelGGL(mc_pair_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
@@ -1602,9 +1608,7 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
     if (ctx->timing && n_steps > 0) {
         if ((rc = ensure_events(ctx, (size_t)2 * n_steps))) return rc;
         for (int k = 0; k < n_steps; ++k) {
-            CG_HIP(ctx, hipEventRecord(ctx->events[2 * k], ctx->stream));
-            if ((rc = launch_pair(ctx))) return rc;
-            CG_HIP(ctx, hipEventRecord(ctx->events[2 * k + 1], ctx->stream));
+            if ((rc = launch_pair(ctx, ctx->events[2 * k], ctx->events[2 * k + 1]))) return rc;
             if ((rc = launch_finish(ctx, 0))) return rc;
         }
         CG_HIP(ctx, hipStreamSynchronize(ctx->stream));

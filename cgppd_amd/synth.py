@@ -204,17 +204,17 @@ def config(k: int, seed: int | None = None) -> SystemSpec:
     """The synthetic system of benchmark configuration k (0, 1 or 4)."""
     if k == 0:
         s = 1000 if seed is None else seed
-        return make_system("cfg0_2x76", [76, 76], 150.0, 20, 288.0, 450.0, s, 2.0, 0.2, 10000, 1000)
+        return make_system("pair_2x76", [76, 76], 150.0, 20, 288.0, 450.0, s, 2.0, 0.2, 10000, 1000)
     if k == 1:
         s = 1001 if seed is None else seed
         rng = np.random.default_rng(s + 7)
         lengths = rng.integers(20, 301, size=40)
-        return make_system("cfg1_40chains", lengths, box_for_concentration(40, 400.0), 32, 288.0, 450.0, s,
+        return make_system("crowded40", lengths, box_for_concentration(40, 400.0), 32, 288.0, 450.0, s,
                            2.0, 0.2, 50000, 1000)
     if k == 4:
         s = 1004 if seed is None else seed
         rng = np.random.default_rng(s + 7)
         lengths = np.exp(rng.uniform(np.log(20.0), np.log(2000.0), size=256)).astype(np.int64)
-        return make_system("cfg4_256chains", lengths, box_for_concentration(256, 800.0), 64, 288.0, 450.0, s,
+        return make_system("crowded256", lengths, box_for_concentration(256, 800.0), 64, 288.0, 450.0, s,
                            2.0, 0.2, 50000, 1000)
     raise ValueError(f"no synthetic rigid system for configuration {k}")
