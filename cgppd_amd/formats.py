@@ -168,6 +168,16 @@ class PDBMolecule:
     chain_ids: list = field(default_factory=list)
     res_seq: list = field(default_factory=list)
     chain_count: int = 1
+    chain_index: list = field(default_factory=list)  # TER records seen before each residue
+
+    def centred(self):
+        """Centre (float sum / count) and relative positions as
+        src/Molecule.cpp:590-611 computes them."""
+        c = np.zeros(3, dtype=np.float32)
+        for p in self.xyz:
+            c = (c + p).astype(np.float32)
+        c = (c / np.float32(len(self.xyz))).astype(np.float32)
+        return c, (self.xyz - c).astype(np.float32)
 
 
 def _field_float(line, col):
@@ -177,7 +187,7 @@ def _field_float(line, col):
 
 def read_pdb(path, acids) -> PDBMolecule:
     """src/Molecule.cpp:506-612: CA atoms only, TER counts chains, END stops."""
-    types, xyz, chains, seqs = [], [], [], []
+    types, xyz, chains, seqs, chain_index = [], [], [], [], []
     chain_count = 0
     with open(path, "r", errors="replace") as fh:
         for line in fh:
@@ -194,6 +204,7 @@ def read_pdb(path, acids) -> PDBMolecule:
                 xyz.append((_field_float(line, 30), _field_float(line, 38), _field_float(line, 46)))
                 chains.append(chain)
                 seqs.append(seq)
+                chain_index.append(chain_count)
             elif line.startswith("TER"):
                 chain_count += 1
             elif line.startswith("END"):
@@ -201,7 +212,36 @@ def read_pdb(path, acids) -> PDBMolecule:
     if types and chain_count == 0:
         chain_count = 1
     return PDBMolecule(np.array(types, dtype=np.int32), np.array(xyz, dtype=np.float32).reshape(-1, 3),
-                       chains, seqs, chain_count)
+                       chains, seqs, chain_count, chain_index)
+
+
+def read_torsions(path, acids):
+    """Torsion table (src/TorsionalLookupMatrix.cpp:27-52): lines
+    'AAA BBB V n sigma_degrees'; returns V and the phase in radians, each
+    (20, 20, 4) float64, indexed [type_a, type_b, n - 1]."""
+    V = np.zeros((AA_COUNT, AA_COUNT, 4))
+    D = np.zeros((AA_COUNT, AA_COUNT, 4))
+    for line in _data_lines(path):
+        if line == "" or line.startswith("#"):
+            continue
+        f = line.split()
+        a, b = amino_acid_index(acids, f[0][:3]), amino_acid_index(acids, f[1][:3])
+        n = int(f[3])
+        V[a, b, n - 1] = float(f[2])
+        D[a, b, n - 1] = float(f[4]) / 57.29577951308232
+    return V, D
+
+
+def write_torsions(path, acids, V, delta):
+    """Writes a torsion table in the layout read_torsions reads (phase in
+    degrees, as data/torsional_pair_potentials stores it)."""
+    with open(path, "w") as fh:
+        fh.write("# NAME NAME V N SIGMA\n")
+        for a in range(AA_COUNT):
+            for b in range(AA_COUNT):
+                for n in range(4):
+                    fh.write("%s %s %12.6f %d %12.6f\n" % (acids[a].sname.upper(), acids[b].sname.upper(), V[a, b, n],
+                                                           n + 1, delta[a, b, n] * 57.29577951308232))
 
 
 def write_pdb(path, acids, types, xyz, chain="A", res_seq=None):

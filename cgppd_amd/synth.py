@@ -10,6 +10,10 @@ Configurations (rigid chains, one C-alpha bead per residue):
   cfg1  40 chains, lengths ~ U[20,300], 400 uM total molecule concentration
         (box 549.6 A), 32 replicas 288-450 K
   cfg4  256 chains, lengths ~ log-U[20,2000], 800 uM (box 810 A), 64 replicas
+  cfg2  flexible linkers: one molecule of four 76-residue domains joined by
+        6-residue linkers (322 residues); 20 % of each domain's residues
+        (its tails next to the linkers) and the linkers are flexible
+        segments; seeded 20x20x4 torsion table; 64 replicas 288-500 K
 
 Chains are self-avoiding random walks with 3.8 A bonds grown inside a sphere
 so that the radius of gyration is about 12 A at 76 residues (scaled as
@@ -116,6 +120,23 @@ class SystemSpec:
     rotate_step: float | None
     mc_steps: int
     exchange_every: int
+    # flexible topology (src/structures.h:49-75 moldata/segdata): per residue
+    # chain index (TERs before it), chain code and resSeq; per molecule the
+    # all-flexible flag and segments (0-based residue indices in the molecule)
+    chain_index: np.ndarray | None = None
+    chain_code: np.ndarray | None = None
+    res_seq: np.ndarray | None = None
+    all_flexible: list | None = None
+    segments: list | None = None
+    torsion_v: np.ndarray | None = None      # (20, 20, 4) kcal/mol
+    torsion_delta: np.ndarray | None = None  # (20, 20, 4) radians
+
+    @property
+    def flexible(self) -> bool:
+        return bool(self.all_flexible and any(self.all_flexible)) or bool(self.segments and any(self.segments))
+
+    def molecule_is_flexible(self, m) -> bool:
+        return bool((self.all_flexible and self.all_flexible[m]) or (self.segments and self.segments[m]))
 
     @property
     def n_mol(self):
@@ -200,8 +221,124 @@ def make_system(name: str, lengths, box: float, n_rep: int, t_min: float, t_max:
                       translate_step, rotate_step, mc_steps, exchange_every)
 
 
+def synthetic_torsions(seed: int):
+    """Seeded 20x20x4 torsion table (V ~ U[0, 1.6] kcal/mol, phase ~
+    U[0, 360) degrees), rounded as the text format stores it and converted
+    to radians as src/TorsionalLookupMatrix.cpp:48 does."""
+    rng = np.random.default_rng(seed)
+    V = np.round(rng.uniform(0.0, 1.6, size=(20, 20, 4)), 6)
+    deg = np.round(rng.uniform(0.0, 360.0, size=(20, 20, 4)), 6)
+    return V, deg / 57.29577951308232
+
+
+def _min_dist_ok(pts, others, lim=4.0):
+    if len(others) == 0 or len(pts) == 0:
+        return True
+    d = np.linalg.norm(pts[:, None, :] - others[None, :, :], axis=2)
+    return bool((d >= lim).all())
+
+
+def _linker(last, u, n, rng, bond=3.8):
+    """n linker residues after `last` along direction u: a zig-zag whose
+    every other residue is offset perpendicular to u (bond length `bond`,
+    bond angles about 120-140 degrees, never collinear)."""
+    d = bond * np.sqrt(0.75)          # axial advance per bond
+    h = bond * 0.5                    # perpendicular offset
+    a = np.cross(u, rng.normal(size=3))
+    a /= np.linalg.norm(a)
+    b = np.cross(u, a)
+    pts = []
+    for t in range(n + 1):            # the last point is the next domain's first residue
+        off = np.zeros(3)
+        if t % 2 == 0:
+            th = 0.9 * t
+            off = h * (np.cos(th) * a + np.sin(th) * b)
+        pts.append(last + d * (t + 1) * u + off)
+    return np.array(pts)
+
+
+def _angles_ok(pts, lim=0.995):
+    v1 = pts[:-2] - pts[1:-1]
+    v2 = pts[2:] - pts[1:-1]
+    c = (v1 * v2).sum(1) / (np.linalg.norm(v1, axis=1) * np.linalg.norm(v2, axis=1))
+    return bool((np.abs(c) < lim).all())
+
+
+def linked_domains(n_domains, domain_len, linker_len, rng, bond=3.8):
+    """One chain of compact domains joined by zig-zag linkers with bond
+    length `bond`; returns (n, 3) float64 positions centred at 0."""
+    pts = compact_chain(domain_len, rng) @ _random_rotation(rng).T
+    chain = [pts]
+    for _k in range(1, n_domains):
+        done = False
+        for _attempt in range(4000):
+            prev = np.concatenate(chain)
+            last = prev[-1]
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            lk = _linker(last, u, linker_len, rng, bond)
+            linker, first = lk[:-1], lk[-1]
+            dom = compact_chain(domain_len, rng) @ _random_rotation(rng).T
+            dom = dom - dom[0] + first
+            cand = np.concatenate([prev, linker, dom])
+            if (_min_dist_ok(linker, prev[:-1]) and _min_dist_ok(dom, np.concatenate([prev, linker[:-1]]))
+                    and _angles_ok(cand[len(prev) - 3:len(prev) + linker_len + 3])):
+                chain += [linker, dom]
+                done = True
+                break
+        if not done:
+            raise RuntimeError("could not place a linked domain")
+    out = np.concatenate(chain)
+    if not _angles_ok(out, 0.99999):
+        raise RuntimeError("collinear residues in a linked chain")
+    return out - out.mean(axis=0)
+
+
+def make_linked_system(name, n_domains, domain_len, linker_len, flex_frac, box, n_rep, t_min, t_max, seed,
+                       translate_step=2.0, rotate_step=0.2, mc_steps=10000, exchange_every=1000,
+                       extra_rigid=()) -> SystemSpec:
+    """One flexible molecule of linked domains (plus optional rigid chains
+    of the given lengths).  Each domain's flexible residues are its tails
+    next to the linkers; every linker with the flanking flexible tails forms
+    one segment."""
+    rng = np.random.default_rng(seed)
+    ff = synthetic_force_field(seed + 1)
+    tv, td = synthetic_torsions(seed + 2)
+    n_flex = int(round(flex_frac * domain_len))
+    pts = linked_domains(n_domains, domain_len, linker_len, rng)
+    segments = []
+    stride = domain_len + linker_len
+    for k in range(n_domains - 1):
+        tail = n_flex if k == 0 else n_flex - n_flex // 2
+        head = n_flex if k == n_domains - 2 else n_flex // 2
+        a = k * stride + domain_len - tail
+        b = (k + 1) * stride + head - 1
+        segments.append(list(range(a, b + 1)))
+    lengths = [len(pts)] + [int(n) for n in extra_rigid]
+    mol_off = np.zeros(len(lengths) + 1, dtype=np.int32)
+    mol_off[1:] = np.cumsum(lengths)
+    N = int(mol_off[-1])
+    types = rng.integers(0, 20, size=N).astype(np.int32)
+    rel = np.zeros((N, 3), dtype=np.float32)
+    radii = np.zeros(len(lengths))
+    rel[: len(pts)] = pts.astype(np.float32)
+    radii[0] = np.linalg.norm(pts, axis=1).max()
+    for m, n in enumerate(extra_rigid, start=1):
+        q = compact_chain(int(n), rng) @ _random_rotation(rng).T
+        rel[mol_off[m]:mol_off[m + 1]] = q.astype(np.float32)
+        radii[m] = np.linalg.norm(q, axis=1).max()
+    centers = _place(lengths, radii, box, rng).astype(np.float32)
+    res_seq = np.concatenate([np.arange(1, n + 1) for n in lengths]).astype(np.int32)
+    spec = SystemSpec(name, ff, float(np.float32(box)), np.asarray(lengths, dtype=np.int64), mol_off, types, rel,
+                      centers, n_rep, t_min, t_max, translate_step, rotate_step, mc_steps, exchange_every,
+                      chain_index=np.zeros(N, dtype=np.int32), chain_code=np.full(N, ord("A"), dtype=np.int32),
+                      res_seq=res_seq, all_flexible=[False] * len(lengths),
+                      segments=[segments] + [[] for _ in extra_rigid], torsion_v=tv, torsion_delta=td)
+    return spec
+
+
 def config(k: int, seed: int | None = None) -> SystemSpec:
-    """The synthetic system of benchmark configuration k (0, 1 or 4)."""
+    """The synthetic system of benchmark configuration k (0, 1, 2 or 4)."""
     if k == 0:
         s = 1000 if seed is None else seed
         return make_system("pair_2x76", [76, 76], 150.0, 20, 288.0, 450.0, s, 2.0, 0.2, 10000, 1000)
@@ -211,10 +348,13 @@ def config(k: int, seed: int | None = None) -> SystemSpec:
         lengths = rng.integers(20, 301, size=40)
         return make_system("crowded40", lengths, box_for_concentration(40, 400.0), 32, 288.0, 450.0, s,
                            2.0, 0.2, 50000, 1000)
+    if k == 2:
+        s = 1002 if seed is None else seed
+        return make_linked_system("linked4x76", 4, 76, 6, 0.2, 300.0, 64, 288.0, 500.0, s, 2.0, 0.2, 10000, 1000)
     if k == 4:
         s = 1004 if seed is None else seed
         rng = np.random.default_rng(s + 7)
         lengths = np.exp(rng.uniform(np.log(20.0), np.log(2000.0), size=256)).astype(np.int64)
         return make_system("crowded256", lengths, box_for_concentration(256, 800.0), 64, 288.0, 450.0, s,
                            2.0, 0.2, 50000, 1000)
-    raise ValueError(f"no synthetic rigid system for configuration {k}")
+    raise ValueError(f"no synthetic system for configuration {k}")

@@ -12,6 +12,10 @@ Parity pins (see tests/test_oracle.py):
   moves                bit-exact vs oracle/_ref (reference Quaternion/Vector3f),
                        rigid-body invariants
   Metropolis/exchange  closed form (acceptance frequencies), invariants
+  flexible linkers     topology (oracle/topology.py), bonded and intra-molecular
+                       energies against oracle/_ref/ref_flex_driver and
+                       tests/testMolecule.cpp / tests/testReplica.cpp values;
+                       flexible moves against tests/testMolecule.cpp:146-283
 """
 from __future__ import annotations
 
@@ -30,6 +34,13 @@ _f64p = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 
 
+class FlexState(C.Structure):
+    _fields_ = [(name, _i32p) for name in (
+        "mol_flexible", "tag", "bond_off", "bond", "angle_off", "angle", "tors_off", "tors", "ind_off", "ind",
+        "local_off", "local", "crank_off", "crank", "flexr_off", "flexr", "nb_off", "nb", "br_off", "br")] + [
+        ("tor_v", _f64p), ("tor_d", _f64p)]
+
+
 class RemcState(C.Structure):
     _fields_ = [
         ("n_mol", C.c_int), ("n_res", C.c_int), ("n_rep", C.c_int),
@@ -42,6 +53,7 @@ class RemcState(C.Structure):
         ("energy", _f64p), ("emat", _f64p), ("counters", _i64p), ("step", _i64p),
         ("label", _i32p), ("rank_of", _i32p), ("rep_at_rank", _i32p),
         ("exchange_round", _i64p), ("exchange_stats", _i64p),
+        ("flex", C.POINTER(FlexState)),
     ]
 
 
@@ -74,6 +86,25 @@ def lib():
         L.orc_remc_exchange.argtypes = [C.POINTER(RemcState)]
         L.orc_remc_reference_run.argtypes = [C.POINTER(RemcState), C.c_int, C.c_int]
         L.orc_remc_reference_run.restype = C.c_double
+        for name in ("orc_internal_ref", "orc_internal_fp64"):
+            getattr(L, name).argtypes = [C.POINTER(FlexState), C.c_int, _i32p, _i32p, _f32p, _f32p, _f32p, _f32p,
+                                         C.c_float, C.c_int, _f64p]
+            getattr(L, name).restype = None
+        L.orc_internal_total.argtypes = [_f64p]
+        L.orc_internal_total.restype = C.c_double
+        L.orc_bond_term.argtypes = [_f32p, _f32p, C.c_float]
+        L.orc_bond_term.restype = C.c_double
+        L.orc_angle_term.argtypes = [_f32p, _f32p, _f32p]
+        L.orc_angle_term.restype = C.c_double
+        L.orc_torsion_term.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, _f64p, _f64p]
+        L.orc_torsion_term.restype = C.c_double
+        L.orc_local_translate.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int, _f32p, C.c_float]
+        L.orc_crankshaft.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.c_float]
+        L.orc_flex_move.argtypes = [_f32p, _f32p, _f32p, C.c_int, C.c_int, _f64p, C.c_double, C.c_double, C.c_int,
+                                    _i32p, C.c_float]
+        L.orc_mc_proposal_flex.argtypes = [C.c_uint64, C.c_int, C.c_int64, C.c_int, _i32p, _i32p, _i32p, _f64p,
+                                           _f64p, C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
 
@@ -175,8 +206,117 @@ def apply_move(is_rotate, axis_raw, tstep, rcos, rsin, box, c_in, rel_in):
     return co, ro, po
 
 
+class Flex:
+    """The flexible topology of a system as the C oracle takes it (global
+    residue indices, caller order), built by oracle/topology.py.  `system`
+    is a SystemSpec or a dict with mol_off, types, chain_code, chain_index,
+    res_seq, all_flexible, segments; tor_v/tor_d the (20, 20, 4) table."""
+
+    def __init__(self, system, tor_v, tor_d):
+        from . import topology
+
+        d = topology._get(system)
+        mol_off = np.asarray(d["mol_off"], dtype=np.int64)
+        M = len(mol_off) - 1
+        self.graphs = topology.molecule_graphs(d)
+        uids = topology.assign_uids(d, self.graphs)
+        N = int(mol_off[-1])
+        tag = np.zeros((N, 4), dtype=np.int32)
+        tag[:, :3] = uids
+        tag[:, 3] = np.asarray(d["res_seq"], dtype=np.int32)
+        self.uids = uids
+        self.mol_flexible = np.array([1 if (d["all_flexible"][m] or d["segments"][m]) else 0 for m in range(M)],
+                                     dtype=np.int32)
+        lists = {k: ([0], []) for k in ("bond", "angle", "tors", "ind", "local", "crank", "flexr")}
+        nb_off, nb, br_off, br = [0], [], [0], []
+        for m, g in enumerate(self.graphs):
+            a = int(mol_off[m])
+            flexible = bool(self.mol_flexible[m])
+            src = {"bond": g.bonds, "angle": g.angles, "tors": g.torsions, "ind": g.indirect, "local": g.local,
+                   "crank": g.crankshaft, "flexr": g.flex}
+            for k, items in src.items():
+                off, flat = lists[k]
+                if flexible:
+                    for it in items:
+                        flat.extend([a + v for v in it] if isinstance(it, tuple) else [a + it])
+                off.append(len(flat) // {"bond": 2, "angle": 3, "tors": 4, "ind": 2}.get(k, 1))
+            if flexible:
+                for ri in g.flex:
+                    for n_ in g.neighbours(ri):
+                        nb.append(a + n_)
+                        br.extend(a + v for v in g.flex_branch(ri, n_))
+                        br_off.append(len(br))
+                    nb_off.append(len(nb))
+        arr = lambda x: np.ascontiguousarray(np.asarray(x, dtype=np.int32).reshape(-1))  # noqa: E731
+        self.arrays = {"mol_flexible": arr(self.mol_flexible), "tag": arr(tag)}
+        for k, (off, flat) in lists.items():
+            self.arrays[f"{k}_off"] = arr(off)
+            self.arrays[k] = arr(flat if flat else [0])
+        self.arrays.update(nb_off=arr(nb_off), nb=arr(nb if nb else [0]), br_off=arr(br_off),
+                           br=arr(br if br else [0]))
+        self.tor_v = np.ascontiguousarray(tor_v, dtype=np.float64).reshape(-1)
+        self.tor_d = np.ascontiguousarray(tor_d, dtype=np.float64).reshape(-1)
+        a_ = self.arrays
+        self.st = FlexState(*[_p(a_[n], C.c_int) for n, _t in FlexState._fields_[:-2]],
+                            _p(self.tor_v, C.c_double), _p(self.tor_d, C.c_double))
+
+
+def internal(flex, m, mol_off, types, charge, sigma, eps, pos, box, mode, fp64=False):
+    """Internal energy of molecule m: (raw LJ, raw DH, raw bond, angle
+    product, torsion).  mode 0 Molecule::E(true), 1 Molecule::E(false),
+    2 the GPU kernel's intra-molecular pairs."""
+    mo, ty, q, s, e, x, L = _sys(mol_off, types, charge, sigma, eps, pos, box)
+    out = np.zeros(5)
+    fn = lib().orc_internal_fp64 if fp64 else lib().orc_internal_ref
+    fn(C.byref(flex.st), int(m), _p(mo, C.c_int), _p(ty, C.c_int), _p(q, C.c_float), _p(s, C.c_float),
+       _p(e, C.c_float), _p(x, C.c_float), L, int(mode), _p(out, C.c_double))
+    return out
+
+
+def internal_total(v):
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_internal_total(_p(a, C.c_double))
+
+
+def local_translate(pos, rel, c, ri, v, box):
+    pos, rel, c = (np.ascontiguousarray(a, dtype=np.float32).copy() for a in (pos, rel, c))
+    vv = np.ascontiguousarray(v, dtype=np.float32)
+    lib().orc_local_translate(_p(pos, C.c_float), _p(rel, C.c_float), _p(c, C.c_float), len(pos), int(ri),
+                              _p(vv, C.c_float), float(box))
+    return pos, rel, c
+
+
+def crankshaft(pos, rel, c, ri, angle, flip, box):
+    pos, rel, c = (np.ascontiguousarray(a, dtype=np.float32).copy() for a in (pos, rel, c))
+    lib().orc_crankshaft(_p(pos, C.c_float), _p(rel, C.c_float), _p(c, C.c_float), len(pos), int(ri),
+                         float(np.cos(angle / 2.0)), float(np.sin(angle / 2.0)), int(flip), float(box))
+    return pos, rel, c
+
+
+def flex_move(pos, rel, c, ri, axis, angle, branch, box):
+    pos, rel, c = (np.ascontiguousarray(a, dtype=np.float32).copy() for a in (pos, rel, c))
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    b = np.ascontiguousarray(branch, dtype=np.int32)
+    lib().orc_flex_move(_p(pos, C.c_float), _p(rel, C.c_float), _p(c, C.c_float), len(pos), int(ri),
+                        _p(ax, C.c_double), float(np.cos(angle / 2.0)), float(np.sin(angle / 2.0)), len(b),
+                        _p(b, C.c_int), float(box))
+    return pos, rel, c
+
+
+def proposal_flex(seed, r, step, mol_flexible):
+    mf = np.ascontiguousarray(mol_flexible, dtype=np.int32)
+    m, kind = C.c_int(), C.c_int()
+    axis = (C.c_double * 3)()
+    u = C.c_double()
+    extra = (C.c_uint32 * 4)()
+    lib().orc_mc_proposal_flex(C.c_uint64(seed), int(r), int(step), len(mf), _p(mf, C.c_int), C.byref(m),
+                               C.byref(kind), axis, C.byref(u), extra)
+    return m.value, kind.value, list(axis), u.value, list(extra)
+
+
 class Remc:
-    """Owns numpy arrays for an orc_remc state (all replicas)."""
+    """Owns numpy arrays for an orc_remc state (all replicas).  Flexible
+    systems (spec.flexible) carry their topology (Flex)."""
 
     def __init__(self, spec, n_rep=None, seed=0, temperatures=None, steps=None):
         R = spec.n_rep if n_rep is None else n_rep
@@ -219,7 +359,11 @@ class Remc:
                             _p(self.rotate_step, C.c_float), _p(self.energy, C.c_double), _p(self.emat, C.c_double),
                             _p(self.counters, C.c_int64), _p(self.step, C.c_int64), _p(self.label, C.c_int),
                             _p(self.rank_of, C.c_int), _p(self.rep_at_rank, C.c_int),
-                            _p(self.exchange_round, C.c_int64), _p(self.exchange_stats, C.c_int64))
+                            _p(self.exchange_round, C.c_int64), _p(self.exchange_stats, C.c_int64), None)
+        self.flex = None
+        if getattr(spec, "flexible", False):
+            self.flex = Flex(spec, spec.torsion_v, spec.torsion_delta)
+            self.st.flex = C.pointer(self.flex.st)
         lib().orc_remc_init(C.byref(self.st))
 
     def mc_steps(self, n):

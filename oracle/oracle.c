@@ -219,6 +219,345 @@ void orc_energy_fp64(int n_mol, const int *mol_off, const int *type, const float
 }
 
 /* ------------------------------------------------------------------ */
+/* Flexible linkers: bonded terms and intra-molecular pairs            */
+/* ------------------------------------------------------------------ */
+/* src/constants.h:34-46 */
+#define ORC_K_SPRING 378
+#define ORC_R0 3.81f
+#define ORC_GAMMA_ANGLE 0.1f
+#define ORC_GAMMA_ANGLE_RECIP 10.0f
+#define ORC_EPSILON_ALPHA 4.3f
+#define ORC_THETA_ALPHA 1.6f
+#define ORC_THETA_BETA 2.27f
+#define ORC_K_ALPHA 106.4f
+#define ORC_K_BETA 26.3f
+
+/* src/vector3f.h:59-61: float magnitude */
+static inline float vmag(const float v[3]) { return sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+static inline float vdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void vcross(const float a[3], const float b[3], float out[3])
+{
+    /* src/vector3f.cpp:152-155 */
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+double orc_bond_term(const float *pi, const float *pj, float box)
+{
+    /* src/Potential.cpp:45-55: rj.distance(ri) without EPS */
+    float X = pj[0] - pi[0], Y = pj[1] - pi[1], Z = pj[2] - pi[2];
+    if (box != 0.0f) {
+        X = X - box * roundf(X / box);
+        Y = Y - box * roundf(Y / box);
+        Z = Z - box * roundf(Z / box);
+    }
+    double r = (double)sqrtf(X * X + Y * Y + Z * Z);
+    return (r - (double)ORC_R0) * (r - (double)ORC_R0);
+}
+
+double orc_angle_term(const float *pi, const float *pj, const float *pk)
+{
+    /* src/Potential.cpp:57-71 with Vector3f::angle (src/vector3f.cpp:137-143) */
+    float ab[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+    float cb[3] = {pk[0] - pj[0], pk[1] - pj[1], pk[2] - pj[2]};
+    double cmag = (double)(vmag(cb) * vmag(ab));
+    float thf;
+    if (cmag < 1e-13 * 1e-13) thf = 0.0f;
+    else thf = (float)acos((double)vdot(ab, cb) / cmag);
+    double theta = (double)thf;
+    double a = theta - (double)ORC_THETA_ALPHA, b = theta - (double)ORC_THETA_BETA;
+    return exp(-(double)ORC_GAMMA_ANGLE * ((double)ORC_K_ALPHA * a * a + (double)ORC_EPSILON_ALPHA)) +
+           exp(-(double)ORC_GAMMA_ANGLE * ((double)ORC_K_BETA * b * b));
+}
+
+double orc_torsion_term(const float *pi, const float *pj, const float *pk, const float *pl, int type_j,
+                        int type_k, const double *tor_v, const double *tor_d)
+{
+    /* src/Potential.cpp:73-98: float vectors, float atan2 (the arguments
+     * are floats), double cosines */
+    float b1[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+    float b2[3] = {pk[0] - pj[0], pk[1] - pj[1], pk[2] - pj[2]};
+    float b3[3] = {pl[0] - pk[0], pl[1] - pk[1], pl[2] - pk[2]};
+    float b2xb3[3], b1xb2[3];
+    vcross(b2, b3, b2xb3);
+    vcross(b1, b2, b1xb2);
+    double phi = (double)atan2f(vmag(b2) * vdot(b1, b2xb3), vdot(b1xb2, b2xb3));
+    const double *V = tor_v + (type_j * 20 + type_k) * 4;
+    const double *D = tor_d + (type_j * 20 + type_k) * 4;
+    return (1 + cos(phi - D[0])) * V[0] + (1 + cos(2 * phi - D[1])) * V[1] + (1 + cos(3 * phi - D[2])) * V[2] +
+           (1 + cos(4 * phi - D[3])) * V[3];
+}
+
+static int is_indirect(const orc_flex *f, int m, int i, int j)
+{
+    for (int k = f->ind_off[m]; k < f->ind_off[m + 1]; ++k)
+        if (f->ind[2 * k] == i && f->ind[2 * k + 1] == j) return 1;
+    return 0;
+}
+
+/* src/Residue.cpp:116-129 (CPU) and src/CudaFunctions.cu:447-476 (GPU) */
+static int excluded(const orc_flex *f, int m, int i, int j, int gpu_rule)
+{
+    const int *ti = f->tag + 4 * i, *tj = f->tag + 4 * j;
+    if (ti[1] && ti[1] == tj[1]) return 1;
+    if (ti[2] && ti[2] == tj[2]) return 1;
+    if (gpu_rule) return ti[0] == tj[0] && (j - i) < 4;
+    if (ti[0] == tj[0] && abs(ti[3] - tj[3]) < 4) return 1;
+    return is_indirect(f, m, i, j);
+}
+
+void orc_internal_ref(const orc_flex *f, int m, const int *mol_off, const int *type, const float *charge,
+                      const float *sigma, const float *eps, const float *pos, float box, int mode, double out[5])
+{
+    out[0] = out[1] = out[2] = 0.0;
+    out[3] = 1.0;
+    out[4] = 0.0;
+    if (!f->mol_flexible[m]) return;
+    const float r0f = orc_r0_factor();
+    const int i0 = mol_off[m], i1 = mol_off[m + 1];
+    if (mode == 0 || mode == 2) {
+        for (int i = i0; i < i1; ++i)
+            for (int j = i + 1; j < i1; ++j) {
+                if (excluded(f, m, i, j, mode == 2)) continue;
+                double r = ref_distance(pos + 3 * i, pos + 3 * j, box);
+                out[0] += ref_lj(type[i], type[j], sigma, eps, r, r0f);
+                out[1] += ref_dh(type[i], type[j], charge, r);
+            }
+    }
+    if (mode == 1) {
+        for (int k = f->ind_off[m]; k < f->ind_off[m + 1]; ++k) {
+            const int i = f->ind[2 * k], j = f->ind[2 * k + 1];
+            double r = ref_distance(pos + 3 * i, pos + 3 * j, box);
+            out[0] += -ref_lj(type[i], type[j], sigma, eps, r, r0f);
+            out[1] += -ref_dh(type[i], type[j], charge, r);
+        }
+    }
+    if (mode == 2) return;
+    for (int k = f->bond_off[m]; k < f->bond_off[m + 1]; ++k)
+        out[2] += orc_bond_term(pos + 3 * f->bond[2 * k], pos + 3 * f->bond[2 * k + 1], box);
+    for (int k = f->angle_off[m]; k < f->angle_off[m + 1]; ++k) {
+        const int *a = f->angle + 3 * k;
+        out[3] *= orc_angle_term(pos + 3 * a[0], pos + 3 * a[1], pos + 3 * a[2]);
+    }
+    for (int k = f->tors_off[m]; k < f->tors_off[m + 1]; ++k) {
+        const int *t = f->tors + 4 * k;
+        out[4] += orc_torsion_term(pos + 3 * t[0], pos + 3 * t[1], pos + 3 * t[2], pos + 3 * t[3], type[t[1]],
+                                   type[t[2]], f->tor_v, f->tor_d);
+    }
+}
+
+/* FP64 pair term and bonded terms */
+static void fp64_pair(const float *a, const float *b, int ti, int tj, const float *charge, const float *sigma,
+                      const float *eps, float box, double *lj, double *dh)
+{
+    const double L = (double)box;
+    double d[3];
+    for (int k = 0; k < 3; ++k) {
+        d[k] = (double)a[k] - (double)b[k];
+        if (L != 0.0) d[k] -= L * round(d[k] / L);
+    }
+    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + 1e-38;
+    double Eij = (double)ORC_LAMBDA * ((double)eps[ti * 20 + tj] - (double)ORC_E0);
+    double sij = 0.5 * ((double)sigma[ti] + (double)sigma[tj]);
+    double t = pow(sij / r, 6.0);
+    double u = -4.0 * Eij * t * (t - 1.0);
+    if (Eij > 0.0 && r < sij * pow(2.0, 1.0 / 6.0)) u = -u + 2.0 * Eij;
+    *lj = u;
+    *dh = (double)charge[ti] * (double)charge[tj] * exp(-r / (double)ORC_XI) / r;
+}
+
+void orc_internal_fp64(const orc_flex *f, int m, const int *mol_off, const int *type, const float *charge,
+                       const float *sigma, const float *eps, const float *pos, float box, int mode,
+                       double out[5])
+{
+    out[0] = out[1] = out[2] = 0.0;
+    out[3] = 1.0;
+    out[4] = 0.0;
+    if (!f->mol_flexible[m]) return;
+    const int i0 = mol_off[m], i1 = mol_off[m + 1];
+    nsum sl = {0, 0}, sd = {0, 0};
+    if (mode == 0 || mode == 2) {
+        for (int i = i0; i < i1; ++i)
+            for (int j = i + 1; j < i1; ++j) {
+                if (excluded(f, m, i, j, mode == 2)) continue;
+                double lj, dh;
+                fp64_pair(pos + 3 * i, pos + 3 * j, type[i], type[j], charge, sigma, eps, box, &lj, &dh);
+                nsum_add(&sl, lj);
+                nsum_add(&sd, dh);
+            }
+    }
+    if (mode == 1) {
+        for (int k = f->ind_off[m]; k < f->ind_off[m + 1]; ++k) {
+            const int i = f->ind[2 * k], j = f->ind[2 * k + 1];
+            double lj, dh;
+            fp64_pair(pos + 3 * i, pos + 3 * j, type[i], type[j], charge, sigma, eps, box, &lj, &dh);
+            nsum_add(&sl, -lj);
+            nsum_add(&sd, -dh);
+        }
+    }
+    out[0] = sl.s + sl.c;
+    out[1] = sd.s + sd.c;
+    if (mode == 2) return;
+    const double L = (double)box;
+    nsum sb = {0, 0}, st = {0, 0};
+    double logang = 0.0;
+    for (int k = f->bond_off[m]; k < f->bond_off[m + 1]; ++k) {
+        const float *a = pos + 3 * f->bond[2 * k], *b = pos + 3 * f->bond[2 * k + 1];
+        double d[3];
+        for (int q = 0; q < 3; ++q) {
+            d[q] = (double)b[q] - (double)a[q];
+            if (L != 0.0) d[q] -= L * round(d[q] / L);
+        }
+        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        nsum_add(&sb, (r - 3.81) * (r - 3.81));
+    }
+    for (int k = f->angle_off[m]; k < f->angle_off[m + 1]; ++k) {
+        const int *t = f->angle + 3 * k;
+        double ab[3], cb[3];
+        for (int q = 0; q < 3; ++q) {
+            ab[q] = (double)pos[3 * t[0] + q] - (double)pos[3 * t[1] + q];
+            cb[q] = (double)pos[3 * t[2] + q] - (double)pos[3 * t[1] + q];
+        }
+        double c = (ab[0] * cb[0] + ab[1] * cb[1] + ab[2] * cb[2]) /
+                   (sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]) * sqrt(cb[0] * cb[0] + cb[1] * cb[1] + cb[2] * cb[2]));
+        double th = acos(c);
+        double a = th - 1.6, b = th - 2.27;
+        logang += log(exp(-0.1 * (106.4 * a * a + 4.3)) + exp(-0.1 * (26.3 * b * b)));
+    }
+    for (int k = f->tors_off[m]; k < f->tors_off[m + 1]; ++k) {
+        const int *t = f->tors + 4 * k;
+        double b1[3], b2[3], b3[3], x23[3], x12[3];
+        for (int q = 0; q < 3; ++q) {
+            b1[q] = (double)pos[3 * t[1] + q] - (double)pos[3 * t[0] + q];
+            b2[q] = (double)pos[3 * t[2] + q] - (double)pos[3 * t[1] + q];
+            b3[q] = (double)pos[3 * t[3] + q] - (double)pos[3 * t[2] + q];
+        }
+        x23[0] = b2[1] * b3[2] - b2[2] * b3[1];
+        x23[1] = b2[2] * b3[0] - b2[0] * b3[2];
+        x23[2] = b2[0] * b3[1] - b2[1] * b3[0];
+        x12[0] = b1[1] * b2[2] - b1[2] * b2[1];
+        x12[1] = b1[2] * b2[0] - b1[0] * b2[2];
+        x12[2] = b1[0] * b2[1] - b1[1] * b2[0];
+        double m2 = sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+        double phi = atan2(m2 * (b1[0] * x23[0] + b1[1] * x23[1] + b1[2] * x23[2]),
+                           x12[0] * x23[0] + x12[1] * x23[1] + x12[2] * x23[2]);
+        const double *V = f->tor_v + (type[t[1]] * 20 + type[t[2]]) * 4;
+        const double *D = f->tor_d + (type[t[1]] * 20 + type[t[2]]) * 4;
+        for (int n = 0; n < 4; ++n) nsum_add(&st, (1.0 + cos((n + 1) * phi - D[n])) * V[n]);
+    }
+    out[2] = sb.s + sb.c;
+    out[3] = exp(logang);
+    out[4] = st.s + st.c;
+}
+
+double orc_internal_total(const double v[5])
+{
+    /* src/Potential.cpp:203-210 */
+    return v[1] * (double)orc_dh_factor() + v[0] * (double)orc_rt_to_kcal() + v[2] * 0.5 * ORC_K_SPRING +
+           log(v[3]) * -(double)ORC_GAMMA_ANGLE_RECIP + v[4];
+}
+
+/* src/Quaternion.cpp:95-112 */
+static inline void qrot(const double q[4], const float v[3], float out[3])
+{
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    const double w2 = w * w, x2 = x * x, y2 = y * y, z2 = z * z;
+    const double wx = w * x, wy = w * y, wz = w * z, xy = x * y, xz = x * z, yz = y * z;
+    const double vx = v[0], vy = v[1], vz = v[2];
+    out[0] = (float)(vx * (w2 + x2 - y2 - z2) + (vy * (xy - wz) + vz * (wy + xz)) * 2.0);
+    out[1] = (float)((vx * (wz + xy) + vz * (yz - wx)) * 2.0 + vy * (w2 - x2 + y2 - z2));
+    out[2] = (float)((vx * (xz - wy) + vy * (yz + wx)) * 2.0 + vz * (w2 - x2 - y2 + z2));
+}
+
+/* src/Molecule.cpp:242-252 wrap_if_necessary with wrapped_centre :234-241 */
+static void wrap_molecule(float *pos, const float *rel, float c[3], int n, float box)
+{
+    float w[3] = {fmodf(c[0], box), fmodf(c[1], box), fmodf(c[2], box)};
+    float d[3] = {w[0] - c[0], w[1] - c[1], w[2] - c[2]};
+    if (vmag(d) != 0.0f) {
+        c[0] = w[0]; c[1] = w[1]; c[2] = w[2];
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) pos[3 * i + k] = rel[3 * i + k] + c[k];
+    }
+}
+
+void orc_local_translate(float *pos, float *rel, float c[3], int n, int ri, const float v[3], float box)
+{
+    const float nf = (float)n;
+    float vf[3], nc[3];
+    for (int k = 0; k < 3; ++k) {
+        vf[k] = v[k] / nf;
+        nc[k] = c[k] + vf[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        pos[3 * ri + k] = pos[3 * ri + k] + v[k];
+        rel[3 * ri + k] = rel[3 * ri + k] + v[k];
+    }
+    c[0] = nc[0]; c[1] = nc[1]; c[2] = nc[2];
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) rel[3 * i + k] = rel[3 * i + k] - vf[k];
+    wrap_molecule(pos, rel, c, n, box);
+}
+
+void orc_crankshaft(float *pos, float *rel, float c[3], int n, int ri, double rcos, double rsin, int flip,
+                    float box)
+{
+    /* axis from the neighbouring residues, float difference, double normalise */
+    double ax[3];
+    for (int k = 0; k < 3; ++k) ax[k] = (double)(pos[3 * (ri + 1) + k] - pos[3 * (ri - 1) + k]);
+    double m = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    ax[0] /= m; ax[1] /= m; ax[2] /= m;
+    const double s = flip ? -rsin : rsin;
+    double q[4] = {rcos, s * ax[0], s * ax[1], s * ax[2]};
+    double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    q[0] /= qm; q[1] /= qm; q[2] /= qm; q[3] /= qm;
+    float rp[3], rr[3], v[3];
+    for (int k = 0; k < 3; ++k) rp[k] = pos[3 * ri + k] - pos[3 * (ri - 1) + k];
+    qrot(q, rp, rr);
+    for (int k = 0; k < 3; ++k) v[k] = rr[k] + pos[3 * (ri - 1) + k] - pos[3 * ri + k];
+    orc_local_translate(pos, rel, c, n, ri, v, box);
+}
+
+void orc_flex_move(float *pos, float *rel, float c[3], int n, int ri, const double axis[3], double rcos,
+                   double rsin, int n_branch, const int *branch, float box)
+{
+    /* Quaternion(angle, axis) without normalisation (src/Molecule.cpp:333) */
+    const double q[4] = {rcos, rsin * axis[0], rsin * axis[1], rsin * axis[2]};
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    float *diff = (float *)malloc(sizeof(float) * 3 * (n_branch > 0 ? n_branch : 1));
+    const float pr[3] = {pos[3 * ri], pos[3 * ri + 1], pos[3 * ri + 2]};
+    for (int b = 0; b < n_branch; ++b) {
+        const int i = branch[b];
+        float rp[3], rr[3];
+        for (int k = 0; k < 3; ++k) rp[k] = pos[3 * i + k] - pr[k];
+        qrot(q, rp, rr);
+        for (int k = 0; k < 3; ++k) {
+            diff[3 * b + k] = rr[k] + pr[k] - pos[3 * i + k];
+            acc[k] = acc[k] + diff[3 * b + k];
+        }
+    }
+    const float nf = (float)n;
+    float vf[3], nc[3];
+    for (int k = 0; k < 3; ++k) {
+        vf[k] = acc[k] / nf;
+        nc[k] = c[k] + vf[k];
+    }
+    for (int b = 0; b < n_branch; ++b) {
+        const int i = branch[b];
+        for (int k = 0; k < 3; ++k) {
+            pos[3 * i + k] = pos[3 * i + k] + diff[3 * b + k];
+            rel[3 * i + k] = rel[3 * i + k] + diff[3 * b + k];
+        }
+    }
+    free(diff);
+    c[0] = nc[0]; c[1] = nc[1]; c[2] = nc[2];
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) rel[3 * i + k] = rel[3 * i + k] - vf[k];
+    wrap_molecule(pos, rel, c, n, box);
+}
+
+/* ------------------------------------------------------------------ */
 /* Rigid-body moves                                                    */
 /* ------------------------------------------------------------------ */
 #define ORC_TAG_MC 0x4D430000u
@@ -248,6 +587,27 @@ void orc_mc_proposal(uint64_t seed, int r, int64_t step, int n_mol, int *mol, in
     axis[1] = 0.5 - u32_to_unit(v[1]);
     axis[2] = u32_to_unit(v[2]) - 0.5;
     *u_accept = u53(w[2], w[3]);
+}
+
+static inline int mulhi_n(uint32_t w, int n) { return (int)(((uint64_t)w * (uint64_t)n) >> 32); }
+
+void orc_mc_proposal_flex(uint64_t seed, int r, int64_t step, int n_mol, const int *mol_flexible, int *mol,
+                          int *kind, double axis[3], double *u_accept, uint32_t extra[4])
+{
+    int rot;
+    orc_mc_proposal(seed, r, step, n_mol, mol, &rot, axis, u_accept);
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t c0[4] = {(uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, ORC_TAG_MC};
+    uint32_t c2[4] = {(uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, ORC_TAG_MC | 2u};
+    uint32_t w[4];
+    orc_philox4x32_10(c0, key, w);
+    orc_philox4x32_10(c2, key, extra);
+    if (mol_flexible && mol_flexible[*mol]) {
+        /* src/Molecule.cpp:451-466 with the equal weights of src/Replica.cpp:309-334 */
+        *kind = n_mol > 1 ? mulhi_n(w[1], 4) : 1 + mulhi_n(w[1], 3);
+    } else {
+        *kind = rot;
+    }
 }
 
 /* Apply the proposal to molecule residues [i0,i1) of one replica; writes
@@ -310,6 +670,68 @@ void orc_apply_move(int is_rotate, const double axis_raw[3], float tstep, double
     apply_move(is_rotate, axis_raw, tstep, rcos, rsin, box, c_in, c_out, 0, n, rel_in, rel_out, pos_out);
 }
 
+/* The MC diagonal of a flexible molecule: the GPU kernel's intra-molecular
+ * pairs plus Molecule::E(false) (src/Replica.cpp:362-371), kcal/mol. */
+static double mc_internal(const orc_remc *s, int m, const float *pos)
+{
+    double a[5], b[5];
+    orc_internal_ref(s->flex, m, s->mol_off, s->type, s->charge, s->sigma, s->eps, pos, s->box, 2, a);
+    orc_internal_ref(s->flex, m, s->mol_off, s->type, s->charge, s->sigma, s->eps, pos, s->box, 1, b);
+    double v[5] = {a[0] + b[0], a[1] + b[1], b[2], b[3], b[4]};
+    return orc_internal_total(v);
+}
+
+/* Applies the proposal (kind, axis, extra) to molecule m of one replica:
+ * pos/rel are the replica's arrays, c the molecule's centre (in place). */
+static void apply_proposal(const orc_remc *s, int r, int m, int kind, const double axis_raw[3],
+                           const uint32_t extra[4], double rcos, double rsin, float *pos, float *rel, float *c)
+{
+    const int i0 = s->mol_off[m], i1 = s->mol_off[m + 1], n = i1 - i0;
+    if (kind <= 1) {
+        apply_move(kind, axis_raw, s->translate_step[r], rcos, rsin, s->box, c, c, i0, i1, rel, rel, pos);
+        return;
+    }
+    const orc_flex *f = s->flex;
+    float *mp = pos + 3 * i0, *mr = rel + 3 * i0;
+    if (kind == 2) {
+        /* src/Molecule.cpp:397-409 */
+        const int nf = f->flexr_off[m + 1] - f->flexr_off[m];
+        if (nf == 0) return;
+        const int e = f->flexr_off[m] + mulhi_n(extra[0], nf);
+        const int deg = f->nb_off[e + 1] - f->nb_off[e];
+        const int k = f->nb_off[e] + mulhi_n(extra[1], deg);
+        double ax = axis_raw[0], ay = axis_raw[1], az = axis_raw[2];
+        double mag = sqrt(ax * ax + ay * ay + az * az);
+        const double unit[3] = {ax / mag, ay / mag, az / mag};
+        const int nb = f->br_off[k + 1] - f->br_off[k];
+        int *loc = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+        for (int b = 0; b < nb; ++b) loc[b] = f->br[f->br_off[k] + b] - i0;
+        orc_flex_move(mp, mr, c, n, f->flexr[e] - i0, unit, rcos, rsin, nb, loc, s->box);
+        free(loc);
+        return;
+    }
+    /* kind 3, src/Molecule.cpp:415-448: one local move */
+    if (extra[2] < 0x80000000u) {
+        const int nc = f->crank_off[m + 1] - f->crank_off[m];
+        if (nc == 0) return;
+        const int ri = f->crank[f->crank_off[m] + mulhi_n(extra[0], nc)] - i0;
+        orc_crankshaft(mp, mr, c, n, ri, rcos, rsin, extra[3] < 0x80000000u, s->box);
+    } else {
+        const int nl = f->local_off[m + 1] - f->local_off[m];
+        if (nl == 0) return;
+        const int ri = f->local[f->local_off[m] + mulhi_n(extra[0], nl)] - i0;
+        float v3[3] = {(float)axis_raw[0], (float)axis_raw[1], (float)axis_raw[2]};
+        float mag = sqrtf(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+        if (mag != 0.0f) {
+            v3[0] = v3[0] / mag; v3[1] = v3[1] / mag; v3[2] = v3[2] / mag;
+        }
+        const double sc = (double)0.1f * (double)s->translate_step[r];
+        float v[3];
+        for (int q = 0; q < 3; ++q) v[q] = (float)((double)v3[q] * sc);
+        orc_local_translate(mp, mr, c, n, ri, v, s->box);
+    }
+}
+
 void orc_remc_init(orc_remc *s)
 {
     const int M = s->n_mol, N = s->n_res;
@@ -322,9 +744,12 @@ void orc_remc_init(orc_remc *s)
                 for (int k = 0; k < 3; ++k) pos[3 * i + k] = rel[3 * i + k] + c[3 * I + k];
         double *mat = s->emat + (size_t)r * M * M;
         orc_energy_matrix_ref(M, s->mol_off, s->type, s->charge, s->sigma, s->eps, pos, s->box, mat);
+        if (s->flex)
+            for (int I = 0; I < M; ++I)
+                if (s->flex->mol_flexible[I]) mat[I * M + I] = mc_internal(s, I, pos);
         double e = 0.0;
         for (int I = 0; I < M; ++I)
-            for (int J = I + 1; J < M; ++J) e += mat[I * M + J];
+            for (int J = I; J < M; ++J) e += mat[I * M + J];
         s->energy[r] = e;
     }
 }
@@ -341,17 +766,25 @@ static void mc_replica_steps(orc_remc *s, int r, int nsteps, float *scratch_rel,
     const double rcos = cos(rs / 2.0), rsin = sin(rs / 2.0);
     for (int k = 0; k < nsteps; ++k) {
         int64_t step = s->step[r];
-        int m, rot;
+        int m, kind;
         double axis[3], u;
-        orc_mc_proposal(s->seed, r, step, M, &m, &rot, axis, &u);
+        uint32_t extra[4];
+        orc_mc_proposal_flex(s->seed, r, step, M, s->flex ? s->flex->mol_flexible : NULL, &m, &kind, axis, &u,
+                             extra);
         const int i0 = s->mol_off[m], i1 = s->mol_off[m + 1];
-        float cnew[3];
+        float cnew[3] = {center[3 * m], center[3 * m + 1], center[3 * m + 2]};
         /* proposal positions live in a full-size scratch so that pair
          * indexing stays global */
         memcpy(scratch_pos, pos, sizeof(float) * 3 * N);
-        apply_move(rot, axis, s->translate_step[r], rcos, rsin, s->box, center + 3 * m, cnew, i0, i1,
-                   rel, scratch_rel, scratch_pos);
+        memcpy(scratch_rel + 3 * i0, rel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+        apply_proposal(s, r, m, kind, axis, extra, rcos, rsin, scratch_pos, scratch_rel, cnew);
+        const int flexible = s->flex && s->flex->mol_flexible[m];
+        double diag = 0.0;
         double delta = 0.0;
+        if (flexible) {
+            diag = mc_internal(s, m, scratch_pos);
+            delta += diag - mat[m * M + m];
+        }
         for (int J = 0; J < M; ++J) {
             if (J == m) { row[J] = 0.0; continue; }
             double lj = 0.0, dh = 0.0;
@@ -373,6 +806,7 @@ static void mc_replica_steps(orc_remc *s, int r, int nsteps, float *scratch_rel,
                 if (J == m) continue;
                 mat[m * M + J] = mat[J * M + m] = row[J];
             }
+            if (flexible) mat[m * M + m] = diag;
             s->energy[r] += delta;
         }
         s->step[r] = step + 1;
@@ -449,20 +883,29 @@ static void *ref_worker(void *arg)
         const double rcos = cos(rs / 2.0), rsin = sin(rs / 2.0);
         for (int k = 0; k < job->nsteps; ++k) {
             int64_t step = s->step[r];
-            int m, rot;
+            int m, kind;
             double axis[3], u;
-            orc_mc_proposal(s->seed, r, step, M, &m, &rot, axis, &u);
+            uint32_t extra[4];
+            orc_mc_proposal_flex(s->seed, r, step, M, s->flex ? s->flex->mol_flexible : NULL, &m, &kind, axis,
+                                 &u, extra);
             const int i0 = s->mol_off[m], i1 = s->mol_off[m + 1];
             /* src/Replica.cpp:357 MC_backup_restore */
             float bc[3] = {center[3 * m], center[3 * m + 1], center[3 * m + 2]};
             memcpy(brel + 3 * i0, rel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
             memcpy(bpos + 3 * i0, pos + 3 * i0, sizeof(float) * 3 * (i1 - i0));
-            apply_move(rot, axis, s->translate_step[r], rcos, rsin, s->box, center + 3 * m,
-                       center + 3 * m, i0, i1, rel, rel, pos);
-            /* src/Replica.cpp:373-380 full recompute */
+            apply_proposal(s, r, m, kind, axis, extra, rcos, rsin, pos, rel, center + 3 * m);
+            /* src/Replica.cpp:373-380 full recompute, E() + internal_molecule_E(true) */
             double lj, dh;
             orc_energy_ref(M, s->mol_off, s->type, s->charge, s->sigma, s->eps, pos, s->box, &lj, &dh);
             double enew = pair_total(lj, dh);
+            if (s->flex)
+                for (int I = 0; I < M; ++I)
+                    if (s->flex->mol_flexible[I]) {
+                        double v[5];
+                        orc_internal_ref(s->flex, I, s->mol_off, s->type, s->charge, s->sigma, s->eps, pos,
+                                         s->box, 0, v);
+                        enew += orc_internal_total(v);
+                    }
             /* src/Replica.cpp:385-410 with float delta and float exponent */
             float delta = (float)(enew - s->energy[r]);
             float T = (float)s->temperature[r];

@@ -71,6 +71,72 @@ void orc_energy_fp64(int n_mol, const int *mol_off, const int *type,
                      const float *pos, float box, double *lj, double *dh);
 
 /* ---------------------------------------------------------------------
+ * Flexible linkers (the reference's FLEXIBLE_LINKS build).
+ *
+ * Topology of the whole system, in caller order (global residue indices):
+ * lists are CSR per molecule unless stated.  Built by oracle/topology.py
+ * (a restatement of src/Geometry.cpp:34-302).
+ * --------------------------------------------------------------------- */
+typedef struct {
+    const int *mol_flexible;       /* M: 1 when the molecule has a segment or is all flexible */
+    const int *tag;                /* N*4: chain UID, rigid-domain UID, segment-bond UID, resSeq */
+    const int *bond_off, *bond;    /* M+1; 2 ints per bond (Graph::bonds order) */
+    const int *angle_off, *angle;  /* M+1; 3 ints per angle */
+    const int *tors_off, *tors;    /* M+1; 4 ints per torsion */
+    const int *ind_off, *ind;      /* M+1; 2 ints per indirect-neighbour pair */
+    const int *local_off, *local;  /* M+1; MC_local_residues */
+    const int *crank_off, *crank;  /* M+1; MC_crankshaft_residues */
+    const int *flexr_off, *flexr;  /* M+1; MC_flex_residues */
+    const int *nb_off, *nb;        /* per flex-residue entry: its neighbours (Graph::neighbours order) */
+    const int *br_off, *br;        /* per neighbour entry: the residues flex(ri, nb) rotates, ascending */
+    const double *tor_v, *tor_d;   /* 20*20*4 each, [a][b][n-1]; phase in radians */
+} orc_flex;
+
+/* Bonded terms with the reference's float vectors and double potentials:
+ * calculate_bond src/Potential.cpp:45-55 ((r - R0)^2, minimum image),
+ * calculate_angle :57-71 (src/vector3f.cpp:137-143 for the angle),
+ * calculate_torsion :73-98.  Pinned against oracle/_ref/ref_flex_driver
+ * (the reference's own functions) and tests/testMolecule.cpp:287-303. */
+double orc_bond_term(const float *pi, const float *pj, float box);
+double orc_angle_term(const float *pi, const float *pj, const float *pk);
+double orc_torsion_term(const float *pi, const float *pj, const float *pk, const float *pl, int type_j,
+                        int type_k, const double *tor_v, const double *tor_d);
+
+/* Internal energy of molecule m, reference semantics, out[5] = raw LJ,
+ * raw DH, raw bond sum, angle product, torsion (src/Potential.h layout):
+ *   mode 0  Molecule::E(true)  src/Molecule.cpp:636-699 (CPU exclusions)
+ *   mode 1  Molecule::E(false) (minus the indirect neighbours, + bonded)
+ *   mode 2  the GPU kernel's intra-molecular pairs only
+ *           (src/CudaFunctions.cu:447-476: same rigid domain, same segment
+ *           bond, |index difference| < 4 on one chain excluded)
+ * Rigid molecules give (0, 0, 0, 1, 0).  Pinned against ref_flex_driver. */
+void orc_internal_ref(const orc_flex *f, int m, const int *mol_off, const int *type, const float *charge,
+                      const float *sigma, const float *eps, const float *pos, float box, int mode, double out[5]);
+
+/* The same three modes with every operation in double (FP64 oracle). */
+void orc_internal_fp64(const orc_flex *f, int m, const int *mol_off, const int *type, const float *charge,
+                       const float *sigma, const float *eps, const float *pos, float box, int mode,
+                       double out[5]);
+
+/* Potential::total() of such a vector (src/Potential.cpp:203-210). */
+double orc_internal_total(const double v[5]);
+
+/* Flexible moves of one molecule, residues [0, n) in caller order, positions
+ * pos/rel (n*3) and centre c[3] updated in place, then wrapped
+ * (src/Molecule.cpp:242-252):
+ *   orc_local_translate  translate(v, ri)             src/Molecule.cpp:274-301
+ *   orc_crankshaft       crankshaft(angle, flip, ri)  :303-329
+ *                        (rcos, rsin = cos, sin of angle/2)
+ *   orc_flex_move        flex(axis, angle, ri, nb) on the given branch
+ *                        (ascending residue indices)  :331-395
+ * Pinned by the explicit coordinates of tests/testMolecule.cpp:146-283. */
+void orc_local_translate(float *pos, float *rel, float c[3], int n, int ri, const float v[3], float box);
+void orc_crankshaft(float *pos, float *rel, float c[3], int n, int ri, double rcos, double rsin, int flip,
+                    float box);
+void orc_flex_move(float *pos, float *rel, float c[3], int n, int ri, const double axis[3], double rcos,
+                   double rsin, int n_branch, const int *branch, float box);
+
+/* ---------------------------------------------------------------------
  * Rigid-body MC (device algorithm restated on the CPU).
  *
  * The device keeps replicas resident and proposes moves from counter-based
@@ -102,6 +168,7 @@ typedef struct {
     int *rep_at_rank;         /* R   replica holding temperature rank k */
     int64_t *exchange_round;  /* 1 */
     int64_t *exchange_stats;  /* 2: tests, accepted swaps */
+    const orc_flex *flex;     /* NULL: rigid molecules only */
 } orc_remc;
 
 /* Initialise positions (pos = rel + center), the molecule-pair matrix and
@@ -113,6 +180,15 @@ void orc_remc_init(orc_remc *s);
  * vector (float) or unit axis (double), and the Metropolis uniform. */
 void orc_mc_proposal(uint64_t seed, int r, int64_t step, int n_mol,
                      int *mol, int *is_rotate, double axis[3], double *u_accept);
+
+/* The same draw with flexible molecules: kind 0 translate, 1 rotate,
+ * 2 flex, 3 local (src/Molecule.cpp:451-466: equal weights over the four,
+ * or over rotate/flex/local when the flexible molecule is the only one);
+ * rigid molecules keep the Bernoulli(0.5) of orc_mc_proposal.  extra[4]
+ * are the words that pick the residue, the neighbour, crankshaft vs
+ * translate and the crankshaft direction. */
+void orc_mc_proposal_flex(uint64_t seed, int r, int64_t step, int n_mol, const int *mol_flexible, int *mol,
+                          int *kind, double axis[3], double *u_accept, uint32_t extra[4]);
 
 /* Applies one rigid move to n residues (translation when is_rotate == 0):
  * the move code of orc_remc_mc_steps, exposed for the move tests.  rcos,

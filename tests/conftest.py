@@ -63,6 +63,30 @@ def golden_system(golden, name):
 
 GOLDEN_SYSTEMS = ["cfg0", "mixed8", "mixed8wrap", "tight4"]
 
+FLEX_CASES = ["ala8", "angiotensin", "ubq_seg", "conf1_flex", "diubq", "linked", "linked_mixed"]
+FLEX_ENERGY_CASES = ["ala8", "angiotensin", "ubq_seg", "conf1_flex", "linked", "linked_mixed"]
+
+
+@pytest.fixture(scope="session")
+def flex_golden():
+    with np.load(GOLDEN / "flex_vectors.npz", allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def flex_case(fg, name):
+    """A flexible-linker golden case as a system dict (inputs only)."""
+    g = lambda k: fg[f"{name}_{k}"]  # noqa: E731
+    mol_off = g("mol_off")
+    M = len(mol_off) - 1
+    segs = [[] for _ in range(M)]
+    so, sr = g("seg_off"), g("seg_res")
+    for k, m in enumerate(g("seg_mol")):
+        segs[int(m)].append([int(v) for v in sr[so[k]:so[k + 1]]])
+    return dict(name=name, mol_off=mol_off, types=g("types"), pos=g("pos"), box=float(g("box")[0]),
+                chain_code=g("chain_code"), chain_index=g("chain_index"), res_seq=g("res_seq"),
+                all_flexible=[int(v) for v in g("all_flexible")], segments=segs, eps=g("eps"), sigma=g("sigma"),
+                charge=g("charge"), torsion_v=g("torsion_v"), torsion_delta=g("torsion_delta"))
+
 
 @pytest.fixture(scope="session")
 def gpu_engine_factory():
