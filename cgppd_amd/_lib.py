@@ -29,6 +29,10 @@ SIGNATURES = {
     "cgppd_set_force_field": (C.c_int, [_ctxp, _f32p, _f32p, _f32p]),
     "cgppd_set_box": (C.c_int, [_ctxp, C.c_float]),
     "cgppd_set_system": (C.c_int, [_ctxp, C.c_int, _i32p, _i32p, C.c_int]),
+    "cgppd_set_torsion_table": (C.c_int, [_ctxp, _f64p, _f64p]),
+    "cgppd_set_topology": (C.c_int, [_ctxp, C.c_int, _i32p, _i32p, C.c_int, C.c_int, _i32p, _i32p]),
+    "cgppd_energy_internal": (C.c_int, [_ctxp, C.c_int, _f64p]),
+    "cgppd_topology_list": (C.c_int, [C.c_int, _i32p, _i32p, C.c_int, C.c_int, _i32p, _i32p, C.c_int, _i32p, _i32p]),
     "cgppd_set_replicas": (C.c_int, [_ctxp, C.c_int, _f32p, _f32p, _f64p, _f32p, _f32p, _i32p, C.c_uint64]),
     "cgppd_upload_positions": (C.c_int, [_ctxp, C.c_int, _f32p]),
     "cgppd_update_molecule": (C.c_int, [_ctxp, C.c_int, C.c_int, _f32p]),
@@ -92,3 +96,39 @@ def f64(a):
 
 def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+TOPOLOGY_KINDS = {"bonds": (0, 2), "angles": (1, 3), "torsions": (2, 4), "local": (3, 1), "crankshaft": (4, 1),
+                  "flex": (5, 1), "segment_bonds": (6, 2), "indirect": (7, 2), "domain_of": (8, 1),
+                  "flex_nb": (9, 1), "flex_nb_off": (10, 1), "branch": (11, 1), "branch_off": (12, 1),
+                  "segbond_of": (13, 1), "counts": (14, 1)}
+
+
+def _segments_csr(segments):
+    off = np.zeros(len(segments) + 1, dtype=np.int32)
+    flat = []
+    for k, sg in enumerate(segments):
+        flat.extend(int(v) for v in sg)
+        off[k + 1] = len(flat)
+    return off, np.asarray(flat if flat else [0], dtype=np.int32)
+
+
+def topology_list(chain_code, chain_index, all_flexible, segments, kind):
+    """Host-only query of the library's topology builder
+    (cgppd_topology_list): one list of one molecule, shaped (-1, width)."""
+    lib = load()
+    code, idx = i32(chain_code), i32(chain_index)
+    off, res = _segments_csr(segments)
+    k, width = TOPOLOGY_KINDS[kind]
+    n = C.c_int()
+    args = (len(code), ptr(code, C.c_int), ptr(idx, C.c_int), int(bool(all_flexible)), len(segments),
+            ptr(off, C.c_int), ptr(res, C.c_int), k)
+    rc = lib.cgppd_topology_list(*args, None, C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"cgppd_topology_list failed ({rc})")
+    out = np.zeros(max(n.value, 1), dtype=np.int32)
+    rc = lib.cgppd_topology_list(*args, ptr(out, C.c_int), C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"cgppd_topology_list failed ({rc})")
+    out = out[:n.value]
+    return out.reshape(-1, width) if width > 1 else out

@@ -56,7 +56,7 @@ def hipcc():
 
 def build_hip(verbose=False):
     tmp = HIP_LIB.with_suffix(".so.tmp")
-    _run([hipcc(), *HIP_FLAGS, "-o", tmp, CSRC / "cgppd_hip.hip"], verbose)
+    _run([hipcc(), *HIP_FLAGS, "-o", tmp, CSRC / "cgppd_hip.hip", CSRC / "topology.cpp"], verbose)
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 
@@ -83,7 +83,7 @@ def build_oracle(verbose=False):
     present, oracle/_ref."""
     _run(["make", "-s", "-C", ROOT / "oracle", "oracle"], verbose)
     if Path("/root/reference/src/Potential.cpp").exists():
-        _run(["make", "-s", "-C", ROOT / "oracle", "ref"], verbose)
+        _run(["make", "-s", "-C", ROOT / "oracle", "ref", "ref_flex"], verbose)
 
 
 def build_all(verbose=False):

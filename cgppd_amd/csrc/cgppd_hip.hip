@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "cgppd_hip.h"
+#include "topology.h"
 
 namespace {
 
@@ -103,6 +104,7 @@ __device__ __forceinline__ double unit53(uint32_t hi, uint32_t lo)
 struct Proposal {
     int mol;
     int rotate;
+    uint32_t kind_word;  // picks translate/rotate/flex/local for flexible molecules
     double axis[3];  // raw (u-0.5, 0.5-u, u-0.5)
     double u;        // Metropolis uniform
 };
@@ -119,6 +121,7 @@ __device__ __forceinline__ Proposal draw_proposal(uint64_t seed, int r, int64_t 
     Proposal p;
     p.mol = (int)(((uint64_t)w[0] * (uint64_t)n_mol) >> 32);
     p.rotate = w[1] < 0x80000000u ? 1 : 0;
+    p.kind_word = w[1];
     p.axis[0] = unit32(v[0]) - 0.5;
     p.axis[1] = 0.5 - unit32(v[1]);
     p.axis[2] = unit32(v[2]) - 0.5;
@@ -248,6 +251,29 @@ struct PairParams {
     float4 *prop_center;  // R
     int *prop_mol;        // R
     int *prop_nib;        // R: i-blocks of the moved molecule
+    // flexible molecules (null / 0 when the system has none).  Residue
+    // indices are device indices (global); lists are CSR per molecule.
+    int n_flex;                 // number of flexible molecules
+    const int *flex_mols;       // n_flex: the flexible molecules
+    const int *flex_item_off;   // n_flex + 1: full-energy items per replica
+    const int *mol_flex;        // M: 1 = flexible
+    const int4 *rtag;           // N: chain key, rigid-domain id, segment-bond id, caller index in the molecule
+    const int *bond_off;        // M + 1
+    const int2 *bonds;
+    const int *angle_off;
+    const int4 *angles;         // (i, j, k, -)
+    const int *tors_off;
+    const int4 *tors;           // (i, j, k, l)
+    const int *ind_off;
+    const int2 *ind;            // indirect-neighbour pairs
+    const int *local_off, *local;
+    const int *crank_off;
+    const int4 *crank;          // (residue before, residue, residue after, -) in the caller's order
+    const int *flexr_off, *flexr;  // flex residues
+    const int *nb_off;          // per flex residue entry: first neighbour entry
+    const int *br_off, *br;     // per neighbour entry: residues the flex move rotates (caller order)
+    const double *tor_v, *tor_d;   // 20 x 20 x 4 torsion table (kcal/mol, radians)
+    double *out_int;            // R * 5 raw internal terms (full energy)
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -417,6 +443,153 @@ __device__ __forceinline__ void segment(const float4 *__restrict__ jrec, int js,
         pair_segment<SEPARATE, true>(jrec, js, je, jbase, p, pi.x, pi.y, pi.z, lane_off, tab, s0, c0, s1, c1);
 }
 
+// ---------------------------------------------------------------------------
+// Flexible molecules: intra-molecular pairs and bonded terms
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int mulhi_n(uint32_t w, int n) { return (int)(((uint64_t)w * (uint64_t)n) >> 32); }
+
+// The GPU kernel's exclusion of an intra-molecular pair
+// (src/CudaFunctions.cu:447-476): same rigid domain, same segment bond, or
+// the same chain with caller indices less than 4 apart.
+__device__ __forceinline__ bool intra_excluded(int4 a, int4 b)
+{
+    return (a.y != 0 && a.y == b.y) || (a.z != 0 && a.z == b.z) || (a.x == b.x && abs(a.w - b.w) < 4);
+}
+
+__device__ __forceinline__ uint32_t lane_col(float4 pi) { return (__float_as_uint(pi.w) >> kRowShift & 0x1Fu) << kColShift; }
+
+// One FP32 pair term between residues pi and pj (per-pair minimum image).
+__device__ __forceinline__ void pair_any(float4 pi, float4 pj, const char *__restrict__ tab, const PairParams &p,
+                                         float &lj, float &dh)
+{
+    const Coef cf = *reinterpret_cast<const Coef *>(tab + ((__float_as_uint(pj.w) & kRowMask) | lane_col(pi)));
+    pair_term<true, true>(pj, pi.x, pi.y, pi.z, cf, p.box, p.inv_box, lj, dh);
+}
+
+// Intra-molecular item: this lane's residue a = ib*64 + lane of a molecule
+// of n residues (molecule-local indices, coordinates src[], tags tg[])
+// against the residues b > a of chunk jc (256), wave w taking 64 of them.
+// Kahan-compensated FP32 sums of LJ and DH.
+__device__ __forceinline__ void intra_item(const float4 *__restrict__ src, const int4 *__restrict__ tg, int n, int ib,
+                                           int jc, const PairParams &p, const char *__restrict__ tab, float4 *jrec,
+                                           int4 *jtag, float &s0, float &c0, float &s1, float &c1, bool &valid)
+{
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    __syncthreads();  // previous item finished reading the staging arrays
+    {
+        const int j = jc * kJChunk + threadIdx.x;
+        if (j < n) {
+            jrec[threadIdx.x] = src[j];
+            jtag[threadIdx.x] = tg[j];
+        }
+    }
+    const int a = ib * kIBlock + lane;
+    valid = a < n;
+    const float4 pa = src[valid ? a : 0];
+    const int4 ta = tg[valid ? a : 0];
+    __syncthreads();
+    const int j0 = jc * kJChunk + wave * kWave;
+    const int j1 = min(n, j0 + kWave);
+    for (int b = j0; b < j1; ++b) {
+        const float4 pb = jrec[b - jc * kJChunk];
+        const int4 tb = jtag[b - jc * kJChunk];
+        float lj, dh;
+        pair_any(pa, pb, tab, p, lj, dh);
+        if (valid && b > a && !intra_excluded(ta, tb)) {
+            kahan_add(s0, c0, lj);
+            kahan_add(s1, c1, dh);
+        }
+    }
+}
+
+// src/Quaternion.cpp:95-112, double arithmetic on a float vector.
+__device__ __forceinline__ float3 qrot(const double q[4], float vx_, float vy_, float vz_)
+{
+#pragma clang fp contract(off)
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    const double w2 = w * w, x2 = x * x, y2 = y * y, z2 = z * z;
+    const double wx = w * x, wy = w * y, wz = w * z, xy = x * y, xz = x * z, yz = y * z;
+    const double vx = vx_, vy = vy_, vz = vz_;
+    return make_float3((float)(vx * (w2 + x2 - y2 - z2) + (vy * (xy - wz) + vz * (wy + xz)) * 2.0),
+                       (float)((vx * (wz + xy) + vz * (yz - wx)) * 2.0 + vy * (w2 - x2 + y2 - z2)),
+                       (float)((vx * (xz - wy) + vy * (yz + wx)) * 2.0 + vz * (w2 - x2 - y2 + z2)));
+}
+
+__device__ __forceinline__ float fmag(float x, float y, float z)
+{
+#pragma clang fp contract(off)
+    return sqrtf(x * x + y * y + z * z);
+}
+
+// Bonded terms of molecule m (whose residues start at device index i0;
+// coordinates src[] molecule-local) with the reference's float vectors:
+// bond src/Potential.cpp:45-55, angle :57-71 (src/vector3f.cpp:137-143),
+// torsion :73-98, and the indirect-neighbour pair terms that
+// Molecule::E(false) subtracts (src/Molecule.cpp:669-683).  Every thread
+// takes a strided share; t[5] = indirect LJ and DH (negated), bond sum,
+// sum of log(angle term), torsion sum.
+__device__ void bonded_terms(const float4 *__restrict__ src, int i0, int m, const PairParams &p,
+                             const char *__restrict__ tab, double t[5])
+{
+#pragma clang fp contract(off)
+    for (int k = 0; k < 5; ++k) t[k] = 0.0;
+    for (int k = p.ind_off[m] + threadIdx.x; k < p.ind_off[m + 1]; k += kWG) {
+        const int2 e = p.ind[k];
+        float lj, dh;
+        pair_any(src[e.x - i0], src[e.y - i0], tab, p, lj, dh);
+        t[0] -= (double)lj;
+        t[1] -= (double)dh;
+    }
+    for (int k = p.bond_off[m] + threadIdx.x; k < p.bond_off[m + 1]; k += kWG) {
+        const int2 e = p.bonds[k];
+        const float4 a = src[e.x - i0], b = src[e.y - i0];
+        float X = b.x - a.x, Y = b.y - a.y, Z = b.z - a.z;
+        X = X - p.box * roundf(X / p.box);
+        Y = Y - p.box * roundf(Y / p.box);
+        Z = Z - p.box * roundf(Z / p.box);
+        const double r = (double)sqrtf(X * X + Y * Y + Z * Z);
+        t[2] += (r - (double)3.81f) * (r - (double)3.81f);
+    }
+    for (int k = p.angle_off[m] + threadIdx.x; k < p.angle_off[m + 1]; k += kWG) {
+        const int4 e = p.angles[k];
+        const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0];
+        const float abx = a.x - b.x, aby = a.y - b.y, abz = a.z - b.z;
+        const float cbx = c.x - b.x, cby = c.y - b.y, cbz = c.z - b.z;
+        const double cmag = (double)(fmag(cbx, cby, cbz) * fmag(abx, aby, abz));
+        float th = 0.0f;
+        if (!(cmag < 1e-26)) th = (float)acos((double)(abx * cbx + aby * cby + abz * cbz) / cmag);
+        const double th_d = (double)th;
+        const double da = th_d - (double)1.6f, db = th_d - (double)2.27f;
+        t[3] += log(exp(-(double)0.1f * ((double)106.4f * da * da + (double)4.3f)) +
+                    exp(-(double)0.1f * ((double)26.3f * db * db)));
+    }
+    for (int k = p.tors_off[m] + threadIdx.x; k < p.tors_off[m + 1]; k += kWG) {
+        const int4 e = p.tors[k];
+        const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0], d = src[e.w - i0];
+        const float b1x = b.x - a.x, b1y = b.y - a.y, b1z = b.z - a.z;
+        const float b2x = c.x - b.x, b2y = c.y - b.y, b2z = c.z - b.z;
+        const float b3x = d.x - c.x, b3y = d.y - c.y, b3z = d.z - c.z;
+        const float x23 = b2y * b3z - b2z * b3y, y23 = b2z * b3x - b2x * b3z, z23 = b2x * b3y - b2y * b3x;
+        const float x12 = b1y * b2z - b1z * b2y, y12 = b1z * b2x - b1x * b2z, z12 = b1x * b2y - b1y * b2x;
+        const float yy = fmag(b2x, b2y, b2z) * (b1x * x23 + b1y * y23 + b1z * z23);
+        const float xx = x12 * x23 + y12 * y23 + z12 * z23;
+        const double phi = (double)atan2f(yy, xx);
+        const int tj = (int)(__float_as_uint(b.w) >> kRowShift & 0x1Fu);
+        const int tk = (int)(__float_as_uint(c.w) >> kRowShift & 0x1Fu);
+        const double *V = p.tor_v + (tj * 20 + tk) * 4;
+        const double *D = p.tor_d + (tj * 20 + tk) * 4;
+        t[4] += (1 + cos(phi - D[0])) * V[0] + (1 + cos(2 * phi - D[1])) * V[1] + (1 + cos(3 * phi - D[2])) * V[2] +
+                (1 + cos(4 * phi - D[3])) * V[3];
+    }
+}
+
+// kcal/mol of bonded_terms' vector when the pair table is scaled (the
+// indirect-neighbour terms are then already kcal/mol): src/Potential.cpp:203-210.
+__device__ __forceinline__ double bonded_kcal(const double t[5])
+{
+    return t[0] + t[1] + t[2] * (0.5 * 378.0) + t[3] * -(double)10.0f + t[4];
+}
+
 // Maps a persistent workgroup's k-th item to (replica, item-in-replica).
 // With xcd_map, workgroup b runs on XCD b % 8 and only takes items of
 // replicas r with r % 8 == b % 8, so a replica's coordinates live in one L2.
@@ -516,6 +689,70 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
 }
 
 
+// Full energy of flexible molecules: the intra-molecular pairs the GPU
+// kernel of the reference counts (src/CudaFunctions.cu:447-476) and the
+// bonded terms (src/Molecule.cpp:636-699) of every flexible molecule of the
+// replicas [rep_begin, rep_begin + rep_count).  Items per replica: for each
+// flexible molecule, its (i-block, j-chunk) pairs then one bonded item
+// (flex_item_off).  SCALED: kcal/mol into the MC matrix diagonal, with the
+// indirect neighbours subtracted and the bonded terms added; otherwise raw
+// LJ/DH into out_lj/out_dh (EonDevice), the raw internal terms into out_int
+// and the intra-molecular total into out_mat's diagonal.
+template <bool SCALED>
+__global__ __launch_bounds__(kWG) void intra_energy_kernel(PairParams p)
+{
+    __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
+    __shared__ float4 jrec[kJChunk];
+    __shared__ int4 jtag[kJChunk];
+    load_table(p.table, tab);
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int per_rep = p.flex_item_off[p.n_flex];
+    const size_t mm = (size_t)p.n_mol * p.n_mol;
+    for (ItemIter it(p, per_rep); it.valid(); it.next()) {
+        const int r = it.replica();
+        const int q = it.item();
+        int f = 0;
+        while (q >= p.flex_item_off[f + 1]) ++f;
+        const int m = p.flex_mols[f];
+        const int i0 = p.mol_off[m], n = p.mol_off[m + 1] - i0;
+        const int nib = (n + kIBlock - 1) / kIBlock, njc = (n + kJChunk - 1) / kJChunk;
+        const float4 *src = p.pos + (size_t)r * p.stride + i0;
+        const int local = q - p.flex_item_off[f];
+        double *diag = p.emat + r * mm + (size_t)m * p.n_mol + m;
+        if (local < nib * njc) {
+            float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+            bool valid;
+            intra_item(src, p.rtag + i0, n, local / njc, local % njc, p, tab, jrec, jtag, s0, c0, s1, c1, valid);
+            double v0 = valid ? (double)s0 - (double)c0 : 0.0, v1 = valid ? (double)s1 - (double)c1 : 0.0;
+            v0 = wave_sum(v0);
+            v1 = wave_sum(v1);
+            if (lane == 0) {
+                if (SCALED) {
+                    atomicAdd(diag, v0 + v1);
+                } else {
+                    atomicAdd(p.out_lj + r, v0);
+                    atomicAdd(p.out_dh + r, v1);
+                    if (p.out_mat)
+                        atomicAdd(p.out_mat + r * mm + (size_t)m * p.n_mol + m,
+                                  v0 * (double)p.rt_to_kcal + v1 * (double)p.dh_factor);
+                }
+            }
+        } else {
+            double t[5];
+            bonded_terms(src, i0, m, p, tab, t);
+            if (SCALED) {
+                const double v = wave_sum(bonded_kcal(t));
+                if (lane == 0) atomicAdd(diag, v);
+            } else {
+                for (int k = 0; k < 5; ++k) t[k] = wave_sum(t[k]);
+                if (lane == 0)
+                    for (int k = 0; k < 5; ++k) atomicAdd(p.out_int + (size_t)r * 5 + k, t[k]);
+            }
+        }
+    }
+}
+
 // MC step, phase 1 (pair kernel): the delta-E row of every replica's moved
 // molecule against all other molecules, FP32 pair terms only.  The proposed
 // residue positions of the moved molecule were laid out by the previous
@@ -524,14 +761,28 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
 // i-block, 256-residue j-chunk) triples, listed by prefix sum over the
 // replicas' i-block counts and split into equal contiguous ranges -- one per
 // workgroup, workgroups ordered by XCD so a replica's items share an L2.
+// FLEX: a flexible moved molecule adds its intra-molecular (i-block,
+// j-chunk) items and one bonded item, summed into the row's diagonal entry.
+template <bool FLEX>
 __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ float4 jrec[kJChunk];
+    __shared__ int4 jtag[FLEX ? kJChunk : 1];
     __shared__ int item_off[CGPPD_MAX_REPLICAS + 1];
     load_table(p.table, tab);
     const int R = p.rep_count, JC = p.jc_count;
-    for (int r = threadIdx.x; r < R; r += kWG) item_off[r + 1] = p.prop_nib[r] * JC;
+    for (int r = threadIdx.x; r < R; r += kWG) {
+        int cnt = p.prop_nib[r] * JC;
+        if (FLEX) {
+            const int m = p.prop_mol[r];
+            if (p.mol_flex[m]) {
+                const int n = p.mol_off[m + 1] - p.mol_off[m];
+                cnt += p.prop_nib[r] * ((n + kJChunk - 1) / kJChunk) + 1;
+            }
+        }
+        item_off[r + 1] = cnt;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         item_off[0] = 0;
@@ -549,6 +800,27 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     for (int k = k0; k < k1; ++k) {
         while (k >= item_off[r + 1]) ++r;
         const int local = k - item_off[r];
+        if (FLEX && local >= p.prop_nib[r] * JC) {
+            const int m = p.prop_mol[r];
+            const int i0 = p.mol_off[m], n = p.mol_off[m + 1] - i0;
+            const int nib = p.prop_nib[r], njc = (n + kJChunk - 1) / kJChunk;
+            const int l2 = local - nib * JC;
+            const float4 *src = p.prop_pos + (size_t)r * p.max_n;
+            double v;
+            if (l2 < nib * njc) {
+                float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+                bool valid;
+                intra_item(src, p.rtag + i0, n, l2 / njc, l2 % njc, p, tab, jrec, jtag, s0, c0, s1, c1, valid);
+                v = valid ? ((double)s0 - (double)c0) + ((double)s1 - (double)c1) : 0.0;
+            } else {
+                double t[5];
+                bonded_terms(src, i0, m, p, tab, t);
+                v = bonded_kcal(t);
+            }
+            v = wave_sum(v);
+            if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + m, v);
+            continue;
+        }
         const int ib = local / JC, jc = local % JC;
         const int m = p.prop_mol[r];
         const int n_m = p.mol_off[m + 1] - p.mol_off[m];
@@ -583,12 +855,187 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     }
 }
 
+// A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
+// out as the proposal (whole workgroup; src/Molecule.cpp:274-448).  The
+// current coordinates are copied, thread 0 moves the residues that change
+// and sums their displacement in the caller's order, then every residue's
+// relative position takes the centre shift and the molecule is wrapped
+// (src/Molecule.cpp:242-252).  Bit-identical to the CPU restatement: no
+// FMA contraction, correctly rounded division and square root.
+__device__ void propose_flexible(const PairParams &p, int r, int64_t step, const Proposal &pr, int kind)
+{
+#pragma clang fp contract(off)
+    __shared__ float sh_vf[3], sh_nc[3];
+    __shared__ int sh_moved;
+    const int m = pr.mol;
+    const int i0 = p.mol_off[m], n = p.mol_off[m + 1] - i0;
+    const float4 *pos = p.pos + (size_t)r * p.stride + i0;
+    const float4 *rel = p.rel + (size_t)r * p.stride + i0;
+    float4 *pp = p.prop_pos + (size_t)r * p.max_n;
+    float4 *pq = p.prop_rel + (size_t)r * p.max_n;
+    for (int k = threadIdx.x; k < n; k += kWG) {
+        pp[k] = pos[k];
+        pq[k] = rel[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t x[4];
+        philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, kTagMC | 2u, (uint32_t)p.seed,
+               (uint32_t)(p.seed >> 32), x);
+        const double rcos = p.rcos[r], rsin = p.rsin[r];
+        float acc[3] = {0.f, 0.f, 0.f};
+        int moved = 0;
+        if (kind == 2) {
+            // flex(axis, rotate step, ri, neighbour): src/Molecule.cpp:331-413
+            const int nf = p.flexr_off[m + 1] - p.flexr_off[m];
+            if (nf > 0) {
+                const int e = p.flexr_off[m] + mulhi_n(x[0], nf);
+                const int deg = p.nb_off[e + 1] - p.nb_off[e];
+                const int k = p.nb_off[e] + mulhi_n(x[1], deg);
+                const double ax = pr.axis[0], ay = pr.axis[1], az = pr.axis[2];
+                const double mag = sqrt(ax * ax + ay * ay + az * az);
+                const double q[4] = {rcos, rsin * (ax / mag), rsin * (ay / mag), rsin * (az / mag)};
+                const float4 c0 = pos[p.flexr[e] - i0];
+                for (int b = p.br_off[k]; b < p.br_off[k + 1]; ++b) {
+                    const int li = p.br[b] - i0;
+                    const float4 pi = pos[li];
+                    const float3 rr = qrot(q, pi.x - c0.x, pi.y - c0.y, pi.z - c0.z);
+                    const float dx = rr.x + c0.x - pi.x, dy = rr.y + c0.y - pi.y, dz = rr.z + c0.z - pi.z;
+                    acc[0] = acc[0] + dx;
+                    acc[1] = acc[1] + dy;
+                    acc[2] = acc[2] + dz;
+                    const float4 qi = rel[li];
+                    pp[li] = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, pi.w);
+                    pq[li] = make_float4(qi.x + dx, qi.y + dy, qi.z + dz, qi.w);
+                }
+                moved = 1;
+            }
+        } else {
+            // one local move, src/Molecule.cpp:415-448
+            int li = -1;
+            float v[3];
+            if (x[2] < 0x80000000u) {
+                // crankshaft(rotate step, flip, ri): src/Molecule.cpp:303-329
+                const int nc = p.crank_off[m + 1] - p.crank_off[m];
+                if (nc > 0) {
+                    const int4 cr = p.crank[p.crank_off[m] + mulhi_n(x[0], nc)];
+                    const float4 a = pos[cr.x - i0], b = pos[cr.y - i0], c = pos[cr.z - i0];
+                    double ax = (double)(c.x - a.x), ay = (double)(c.y - a.y), az = (double)(c.z - a.z);
+                    const double mag = sqrt(ax * ax + ay * ay + az * az);
+                    ax /= mag;
+                    ay /= mag;
+                    az /= mag;
+                    const double sn = x[3] < 0x80000000u ? -rsin : rsin;
+                    double q[4] = {rcos, sn * ax, sn * ay, sn * az};
+                    const double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+                    q[0] /= qm;
+                    q[1] /= qm;
+                    q[2] /= qm;
+                    q[3] /= qm;
+                    const float3 rr = qrot(q, b.x - a.x, b.y - a.y, b.z - a.z);
+                    v[0] = rr.x + a.x - b.x;
+                    v[1] = rr.y + a.y - b.y;
+                    v[2] = rr.z + a.z - b.z;
+                    li = cr.y - i0;
+                }
+            } else {
+                // local translate by 0.1 x the translation step: src/Molecule.cpp:422-430
+                const int nl = p.local_off[m + 1] - p.local_off[m];
+                if (nl > 0) {
+                    li = p.local[p.local_off[m] + mulhi_n(x[0], nl)] - i0;
+                    float n0 = (float)pr.axis[0], n1 = (float)pr.axis[1], n2 = (float)pr.axis[2];
+                    const float mag = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+                    if (mag != 0.0f) {
+                        n0 = n0 / mag;
+                        n1 = n1 / mag;
+                        n2 = n2 / mag;
+                    }
+                    const double sc = (double)0.1f * (double)p.tstep[r];
+                    v[0] = (float)((double)n0 * sc);
+                    v[1] = (float)((double)n1 * sc);
+                    v[2] = (float)((double)n2 * sc);
+                }
+            }
+            if (li >= 0) {
+                // translate(v, ri): src/Molecule.cpp:274-301
+                const float4 pi = pos[li], qi = rel[li];
+                pp[li] = make_float4(pi.x + v[0], pi.y + v[1], pi.z + v[2], pi.w);
+                pq[li] = make_float4(qi.x + v[0], qi.y + v[1], qi.z + v[2], qi.w);
+                acc[0] = v[0];
+                acc[1] = v[1];
+                acc[2] = v[2];
+                moved = 1;
+            }
+        }
+        const float4 c = p.center[(size_t)r * p.n_mol + m];
+        const float fn = (float)n;
+        for (int k = 0; k < 3; ++k) sh_vf[k] = moved ? acc[k] / fn : 0.0f;
+        sh_nc[0] = c.x + sh_vf[0];
+        sh_nc[1] = c.y + sh_vf[1];
+        sh_nc[2] = c.z + sh_vf[2];
+        sh_moved = moved;
+    }
+    __syncthreads();
+    const float vf0 = sh_vf[0], vf1 = sh_vf[1], vf2 = sh_vf[2];
+    float c0 = sh_nc[0], c1 = sh_nc[1], c2 = sh_nc[2];
+    if (sh_moved) {
+        const float w0 = fmodf(c0, p.box), w1 = fmodf(c1, p.box), w2 = fmodf(c2, p.box);
+        const bool wrap = w0 != c0 || w1 != c1 || w2 != c2;
+        for (int k = threadIdx.x; k < n; k += kWG) {
+            float4 q = pq[k];
+            q.x = q.x - vf0;
+            q.y = q.y - vf1;
+            q.z = q.z - vf2;
+            pq[k] = q;
+            if (wrap) pp[k] = make_float4(q.x + w0, q.y + w1, q.z + w2, pp[k].w);
+        }
+        if (wrap) {
+            c0 = w0;
+            c1 = w1;
+            c2 = w2;
+        }
+    }
+    if (threadIdx.x == 0) {
+        p.prop_mol[r] = m;
+        p.prop_nib[r] = (n + kIBlock - 1) / kIBlock;
+        p.prop_center[r] = make_float4(c0, c1, c2, 0.0f);
+    }
+}
+
 // Lays out the proposal of MC step `step` for replica r (whole workgroup):
 // molecule, new centre, and the proposed positions of its residues in
 // device order (src/Molecule.cpp:146-266, src/Quaternion.cpp:12-112).
+// Flexible molecules draw translate / rotate / flex / local with equal
+// weights (rotate / flex / local when the molecule is alone,
+// src/Molecule.cpp:451-466, src/Replica.cpp:309-334).
 __device__ void propose(const PairParams &p, int r, int64_t step)
 {
     const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
+    if (p.n_flex && p.mol_flex[pr.mol]) {
+        const int kind = p.n_mol > 1 ? mulhi_n(pr.kind_word, 4) : 1 + mulhi_n(pr.kind_word, 3);
+        if (kind >= 2) {
+            propose_flexible(p, r, step, pr, kind);
+            return;
+        }
+        Proposal rigid = pr;
+        rigid.rotate = kind;
+        const Move mv = make_move(rigid, p.center[(size_t)r * p.n_mol + pr.mol], p.tstep[r], p.rcos[r], p.rsin[r],
+                                  p.box);
+        const int i_beg = p.mol_off[pr.mol], i_end = p.mol_off[pr.mol + 1];
+        const float4 *rel = p.rel + (size_t)r * p.stride;
+        for (int i = i_beg + threadIdx.x; i < i_end; i += kWG) {
+            float4 rn, pn;
+            move_residue(mv, rel[i], rn, pn);
+            p.prop_pos[(size_t)r * p.max_n + (i - i_beg)] = pn;
+            p.prop_rel[(size_t)r * p.max_n + (i - i_beg)] = rn;
+        }
+        if (threadIdx.x == 0) {
+            p.prop_mol[r] = pr.mol;
+            p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
+            p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+        }
+        return;
+    }
     const Move mv = make_move(pr, p.center[(size_t)r * p.n_mol + pr.mol], p.tstep[r], p.rcos[r], p.rsin[r], p.box);
     const int i_beg = p.mol_off[pr.mol], i_end = p.mol_off[pr.mol + 1];
     const float4 *rel = p.rel + (size_t)r * p.stride;
@@ -622,9 +1069,10 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
         const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
         double *row = p.rowbuf + (size_t)r * M;
         double *mat = p.emat + (size_t)r * M * M;
+        // the diagonal entry is the moved molecule's internal energy when it
+        // is flexible (0 in both the row and the matrix otherwise)
         double d = 0.0;
-        for (int J = threadIdx.x; J < M; J += kWG)
-            if (J != m) d += row[J] - mat[(size_t)m * M + J];
+        for (int J = threadIdx.x; J < M; J += kWG) d += row[J] - mat[(size_t)m * M + J];
         d = wave_sum(d);
         if (lane == 0) red[wave] = d;
         __syncthreads();
@@ -646,12 +1094,13 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
             if (acc) p.energy[r] += delta;
             accept_flag = acc;
             const unsigned long long n_m = (unsigned long long)(p.mol_off[m + 1] - p.mol_off[m]);
-            atomicAdd(p.pair_evals, n_m * ((unsigned long long)p.n_res - n_m));
+            unsigned long long pairs = n_m * ((unsigned long long)p.n_res - n_m);
+            if (p.n_flex && p.mol_flex[m]) pairs += n_m * (n_m - 1) / 2;
+            atomicAdd(p.pair_evals, pairs);
         }
         __syncthreads();
         if (accept_flag) {
             for (int J = threadIdx.x; J < M; J += kWG) {
-                if (J == m) continue;
                 const double v = row[J];
                 mat[(size_t)m * M + J] = v;
                 mat[(size_t)J * M + m] = v;
@@ -674,8 +1123,9 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
     propose(p, r, next);
 }
 
-// After the scaled full-energy pass: mirror the upper triangle of each
-// replica's matrix and set the running total to the sum over I < J.
+// After the scaled full-energy passes: mirror the upper triangle of each
+// replica's matrix and set the running total to the sum over I <= J (the
+// diagonal holds flexible molecules' internal energies).
 __global__ __launch_bounds__(kWG) void prepare_finish_kernel(PairParams p)
 {
     __shared__ double red[kWavesPerWG];
@@ -690,7 +1140,7 @@ __global__ __launch_bounds__(kWG) void prepare_finish_kernel(PairParams p)
             mat[(size_t)J * M + I] = v;
             e += v;
         } else if (I == J) {
-            mat[k] = 0.0;
+            e += mat[k];  // internal energy of a flexible molecule, else 0
         }
     }
     e = wave_sum(e);
@@ -818,6 +1268,25 @@ struct cgppd_ctx {
     double *d_out_mat = nullptr;  // R*M*M
     double *h_out = nullptr;      // pinned 2R
 
+    // flexible molecules: per-molecule topology (n = 0: rigid), the chain
+    // index of every residue (caller order), the torsion table, and the
+    // device lists derived from them (rebuilt when topo_dirty)
+    std::vector<cgppd_topo::Molecule> topo;
+    std::vector<int> chain_index;
+    double tor_v[1600] = {0.0}, tor_d[1600] = {0.0};
+    bool topo_dirty = true;
+    int n_flex = 0;
+    int *d_flex_mols = nullptr, *d_flex_item_off = nullptr, *d_mol_flex = nullptr;
+    int4 *d_rtag = nullptr;
+    int *d_bond_off = nullptr, *d_angle_off = nullptr, *d_tors_off = nullptr, *d_ind_off = nullptr;
+    int2 *d_bonds = nullptr, *d_ind = nullptr;
+    int4 *d_angles = nullptr, *d_tors = nullptr, *d_crank = nullptr;
+    int *d_local_off = nullptr, *d_local = nullptr, *d_crank_off = nullptr;
+    int *d_flexr_off = nullptr, *d_flexr = nullptr, *d_nb_off = nullptr, *d_br_off = nullptr, *d_br = nullptr;
+    double *d_tor = nullptr;      // 2 x 1600: V then delta
+    double *d_out_int = nullptr;  // R * 5
+    double *d_out_scratch = nullptr;  // 2R: LJ/DH of internal-only queries
+
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
@@ -920,6 +1389,8 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_rep_at_rank);
     dev_free(ctx->d_out);
     dev_free(ctx->d_out_mat);
+    dev_free(ctx->d_out_int);
+    dev_free(ctx->d_out_scratch);
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
     ctx->h_out = nullptr;
     ctx->have_rep = false;
@@ -997,6 +1468,7 @@ int build_order(cgppd_ctx *ctx)
         }
     }
     run_start.push_back(N);
+    ctx->topo_dirty = true;
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_run_of, N))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_run_start, run_start.size()))) return rc;
@@ -1042,6 +1514,100 @@ void split_molecule(const cgppd_ctx *ctx, int i0, int n, const float *xyz, float
     }
     centre = make_float4(cx, cy, cz, 0.0f);
     radius = molecule_radius(rel, n);
+}
+
+template <typename T>
+int upload_vec(cgppd_ctx *ctx, T **d, const std::vector<T> &h)
+{
+    int rc = dev_alloc(ctx, d, h.size());
+    if (rc) return rc;
+    if (!h.empty()) CG_HIP(ctx, hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return CGPPD_OK;
+}
+
+// Device lists of the flexible molecules in device residue indices: the
+// exclusion tags (the reference packs them into pos.w / meta.w,
+// src/Residue.cpp:41-59), bonded lists and move sets.
+int sync_topology(cgppd_ctx *ctx)
+{
+    if (!ctx->topo_dirty) return CGPPD_OK;
+    const int M = ctx->M, N = ctx->N;
+    if ((int)ctx->topo.size() != M) ctx->topo.assign(M, cgppd_topo::Molecule());
+    if ((int)ctx->chain_index.size() != N) ctx->chain_index.assign(N, 0);
+    std::vector<int4> rtag(N);
+    std::vector<int> mol_flex(M, 0), flex_mols, flex_item_off{0};
+    std::vector<int> bond_off{0}, angle_off{0}, tors_off{0}, ind_off{0}, local_off{0}, crank_off{0}, flexr_off{0};
+    std::vector<int> nb_off{0}, br_off{0}, local, flexr, br;
+    std::vector<int2> bonds, ind;
+    std::vector<int4> angles, tors, crank;
+    const auto &inv = ctx->inv;
+    for (int m = 0; m < M; ++m) {
+        const int i0 = ctx->mol_off[m], n = ctx->mol_off[m + 1] - i0;
+        const cgppd_topo::Molecule &t = ctx->topo[m];
+        const bool flex = t.flexible && t.n == n;
+        for (int k = 0; k < n; ++k) {
+            int4 g = make_int4(0, 0, 0, k);
+            if (flex) {
+                g.x = k < t.vertex_count ? ctx->chain_index[i0 + k] + 1 : 0;
+                g.y = t.domain_of[k] + 1;
+                g.z = t.segbond_of[k] + 1;
+            }
+            rtag[inv[i0 + k]] = g;
+        }
+        if (flex) {
+            mol_flex[m] = 1;
+            flex_mols.push_back(m);
+            const int nib = (n + kIBlock - 1) / kIBlock, njc = (n + kJChunk - 1) / kJChunk;
+            flex_item_off.push_back(flex_item_off.back() + nib * njc + 1);
+            auto d = [&](int v) { return inv[i0 + v]; };
+            for (const auto &b : t.bonds) bonds.push_back(make_int2(d(b[0]), d(b[1])));
+            for (const auto &a : t.angles) angles.push_back(make_int4(d(a[0]), d(a[1]), d(a[2]), 0));
+            for (const auto &q : t.torsions) tors.push_back(make_int4(d(q[0]), d(q[1]), d(q[2]), d(q[3])));
+            for (const auto &q : t.indirect) ind.push_back(make_int2(d(q[0]), d(q[1])));
+            for (int v : t.local) local.push_back(d(v));
+            // crankshaft uses the residues before and after in the caller's
+            // order (src/Molecule.cpp:306,318); residues at either end of the
+            // molecule have no such pair and are left out
+            for (int v : t.crankshaft)
+                if (v > 0 && v < n - 1) crank.push_back(make_int4(d(v - 1), d(v), d(v + 1), 0));
+            for (size_t e = 0; e < t.flex.size(); ++e) {
+                flexr.push_back(d(t.flex[e]));
+                for (int k = t.flex_nb_off[e]; k < t.flex_nb_off[e + 1]; ++k) {
+                    for (int b = t.branch_off[k]; b < t.branch_off[k + 1]; ++b) br.push_back(d(t.branch[b]));
+                    br_off.push_back((int)br.size());
+                }
+                nb_off.push_back(nb_off.back() + (t.flex_nb_off[e + 1] - t.flex_nb_off[e]));
+            }
+        }
+        bond_off.push_back((int)bonds.size());
+        angle_off.push_back((int)angles.size());
+        tors_off.push_back((int)tors.size());
+        ind_off.push_back((int)ind.size());
+        local_off.push_back((int)local.size());
+        crank_off.push_back((int)crank.size());
+        flexr_off.push_back((int)flexr.size());
+    }
+    std::vector<double> tor(3200);
+    std::copy(ctx->tor_v, ctx->tor_v + 1600, tor.begin());
+    std::copy(ctx->tor_d, ctx->tor_d + 1600, tor.begin() + 1600);
+    int rc;
+    if ((rc = upload_vec(ctx, &ctx->d_rtag, rtag)) || (rc = upload_vec(ctx, &ctx->d_mol_flex, mol_flex)) ||
+        (rc = upload_vec(ctx, &ctx->d_flex_mols, flex_mols)) ||
+        (rc = upload_vec(ctx, &ctx->d_flex_item_off, flex_item_off)) ||
+        (rc = upload_vec(ctx, &ctx->d_bond_off, bond_off)) || (rc = upload_vec(ctx, &ctx->d_bonds, bonds)) ||
+        (rc = upload_vec(ctx, &ctx->d_angle_off, angle_off)) || (rc = upload_vec(ctx, &ctx->d_angles, angles)) ||
+        (rc = upload_vec(ctx, &ctx->d_tors_off, tors_off)) || (rc = upload_vec(ctx, &ctx->d_tors, tors)) ||
+        (rc = upload_vec(ctx, &ctx->d_ind_off, ind_off)) || (rc = upload_vec(ctx, &ctx->d_ind, ind)) ||
+        (rc = upload_vec(ctx, &ctx->d_local_off, local_off)) || (rc = upload_vec(ctx, &ctx->d_local, local)) ||
+        (rc = upload_vec(ctx, &ctx->d_crank_off, crank_off)) || (rc = upload_vec(ctx, &ctx->d_crank, crank)) ||
+        (rc = upload_vec(ctx, &ctx->d_flexr_off, flexr_off)) || (rc = upload_vec(ctx, &ctx->d_flexr, flexr)) ||
+        (rc = upload_vec(ctx, &ctx->d_nb_off, nb_off)) || (rc = upload_vec(ctx, &ctx->d_br_off, br_off)) ||
+        (rc = upload_vec(ctx, &ctx->d_br, br)) || (rc = upload_vec(ctx, &ctx->d_tor, tor)))
+        return rc;
+    ctx->n_flex = (int)flex_mols.size();
+    ctx->topo_dirty = false;
+    destroy_graph(ctx);
+    return CGPPD_OK;
 }
 
 PairParams base_params(cgppd_ctx *ctx)
@@ -1093,6 +1659,31 @@ PairParams base_params(cgppd_ctx *ctx)
     p.counters = ctx->d_counters;
     p.steps = ctx->d_steps;
     p.pair_evals = ctx->d_pair_evals;
+    p.n_flex = ctx->n_flex;
+    p.flex_mols = ctx->d_flex_mols;
+    p.flex_item_off = ctx->d_flex_item_off;
+    p.mol_flex = ctx->d_mol_flex;
+    p.rtag = ctx->d_rtag;
+    p.bond_off = ctx->d_bond_off;
+    p.bonds = ctx->d_bonds;
+    p.angle_off = ctx->d_angle_off;
+    p.angles = ctx->d_angles;
+    p.tors_off = ctx->d_tors_off;
+    p.tors = ctx->d_tors;
+    p.ind_off = ctx->d_ind_off;
+    p.ind = ctx->d_ind;
+    p.local_off = ctx->d_local_off;
+    p.local = ctx->d_local;
+    p.crank_off = ctx->d_crank_off;
+    p.crank = ctx->d_crank;
+    p.flexr_off = ctx->d_flexr_off;
+    p.flexr = ctx->d_flexr;
+    p.nb_off = ctx->d_nb_off;
+    p.br_off = ctx->d_br_off;
+    p.br = ctx->d_br;
+    p.tor_v = ctx->d_tor;
+    p.tor_d = ctx->d_tor ? ctx->d_tor + 1600 : nullptr;
+    p.out_int = ctx->d_out_int;
     return p;
 }
 
@@ -1119,6 +1710,10 @@ int check_ready(cgppd_ctx *ctx, bool need_rep)
     if (!ctx->have_box) return fail(ctx, CGPPD_E_STATE, "box not set");
     if (!ctx->have_sys) return fail(ctx, CGPPD_E_STATE, "system not set");
     if (need_rep && !ctx->have_rep) return fail(ctx, CGPPD_E_STATE, "replicas not set");
+    if (ctx->topo_dirty) {
+        CG_HIP(ctx, hipSetDevice(ctx->device));
+        return sync_topology(ctx);
+    }
     return CGPPD_OK;
 }
 
@@ -1142,6 +1737,36 @@ int launch_full(cgppd_ctx *ctx, int r_begin, int r_count, int n_active, bool sca
     return CGPPD_OK;
 }
 
+// Intra-molecular pairs and bonded terms of the flexible molecules of
+// replicas [r_begin, r_begin + r_count); out_lj/out_dh receive the raw
+// intra-molecular LJ/DH (unscaled launches).
+int launch_intra(cgppd_ctx *ctx, int r_begin, int r_count, bool scaled, double *mat, double *out_lj,
+                 double *out_dh)
+{
+    if (ctx->n_flex == 0) return CGPPD_OK;
+    PairParams p = base_params(ctx);
+    p.rep_begin = r_begin;
+    p.rep_count = r_count;
+    p.out_mat = mat;
+    p.out_lj = out_lj;
+    p.out_dh = out_dh;
+    p.table = scaled ? ctx->d_table_scaled : ctx->d_table_raw;
+    p.xcd_map = 0;
+    int per_rep = 0;
+    for (int m = 0; m < ctx->M; ++m)
+        if (ctx->topo[m].flexible) {
+            const int n = ctx->mol_off[m + 1] - ctx->mol_off[m];
+            per_rep += ((n + kIBlock - 1) / kIBlock) * ((n + kJChunk - 1) / kJChunk) + 1;
+        }
+    const int g = grid_for(ctx, (long long)per_rep * r_count, false);
+    if (scaled)
+        hipLaunchKernelGGL(intra_energy_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    else
+        hipLaunchKernelGGL(intra_energy_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
 int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
 {
     if (n == 0) return CGPPD_OK;
@@ -1161,10 +1786,16 @@ int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_sto
     // 6 workgroups per CU measured best for the pair kernel (4: -9 %, 8: -6 %)
     const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 6;
     const int g = std::max(8, ctx->n_cu * per_cu / 8 * 8);
-    if (ev_start)
-        hipExtLaunchKernelGGL(mc_pair_kernel, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
-    else
-        hipLaunchKernelGGL(mc_pair_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    if (ctx->n_flex) {
+        if (ev_start)
+            hipExtLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+        else
+            hipLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    } else if (ev_start) {
+        hipExtLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+    } else {
+        hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    }
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
@@ -1347,8 +1978,113 @@ int cgppd_set_system(cgppd_ctx *ctx, int n_mol, const int *mol_off, const int *t
     if ((rc = dev_alloc(ctx, &ctx->d_mol_of, N))) return rc;
     CG_HIP(ctx, hipMemcpy(ctx->d_mol_off, mol_off, sizeof(int) * (n_mol + 1), hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_mol_of, mol_of.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    ctx->topo.assign(n_mol, cgppd_topo::Molecule());
+    ctx->chain_index.assign(N, 0);
+    ctx->topo_dirty = true;
+    ctx->n_flex = 0;
     ctx->have_sys = true;
     if (ctx->have_ff && (rc = build_order(ctx))) return rc;
+    return CGPPD_OK;
+}
+
+int cgppd_set_torsion_table(cgppd_ctx *ctx, const double *v, const double *delta)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (!v || !delta) return fail(ctx, CGPPD_E_ARG, "null torsion table");
+    std::memcpy(ctx->tor_v, v, sizeof(ctx->tor_v));
+    std::memcpy(ctx->tor_d, delta, sizeof(ctx->tor_d));
+    ctx->topo_dirty = true;
+    ctx->mc_ready = false;
+    return CGPPD_OK;
+}
+
+static int read_segments(int n, int n_segments, const int *seg_off, const int *seg_res,
+                         std::vector<std::vector<int>> &segs, std::string &err)
+{
+    if (n_segments < 0) {
+        err = "negative segment count";
+        return CGPPD_E_ARG;
+    }
+    if (n_segments > 0 && (!seg_off || !seg_res)) {
+        err = "null segment arrays";
+        return CGPPD_E_ARG;
+    }
+    for (int k = 0; k < n_segments; ++k) {
+        if (seg_off[k + 1] < seg_off[k] || (k == 0 && seg_off[0] != 0)) {
+            err = "segment offsets must start at 0 and not decrease";
+            return CGPPD_E_ARG;
+        }
+        segs.emplace_back(seg_res + seg_off[k], seg_res + seg_off[k + 1]);
+        for (int v : segs.back())
+            if (v < 0 || v >= n) {
+                err = "segment residue " + std::to_string(v) + " outside the molecule";
+                return CGPPD_E_ARG;
+            }
+    }
+    return CGPPD_OK;
+}
+
+int cgppd_set_topology(cgppd_ctx *ctx, int molecule, const int *chain_code, const int *chain_index, int all_flexible,
+                       int n_segments, const int *seg_off, const int *seg_res)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (!ctx->have_sys) return fail(ctx, CGPPD_E_STATE, "system not set");
+    if (molecule < 0 || molecule >= ctx->M) return fail(ctx, CGPPD_E_ARG, "molecule %d out of range", molecule);
+    if (!chain_code || !chain_index) return fail(ctx, CGPPD_E_ARG, "null chain arrays");
+    const int i0 = ctx->mol_off[molecule], n = ctx->mol_off[molecule + 1] - i0;
+    std::vector<std::vector<int>> segs;
+    std::string err;
+    int rc = read_segments(n, n_segments, seg_off, seg_res, segs, err);
+    if (rc) return fail(ctx, rc, "%s", err.c_str());
+    int nchains = 0;
+    for (int k = 0; k < n; ++k) {
+        if (chain_index[k] < 0) return fail(ctx, CGPPD_E_ARG, "negative chain index");
+        nchains = std::max(nchains, chain_index[k] + 1);
+    }
+    cgppd_topo::Molecule t;
+    if (!cgppd_topo::build(n, chain_code, all_flexible != 0, segs, nchains, t, err))
+        return fail(ctx, CGPPD_E_ARG, "%s", err.c_str());
+    if ((int)ctx->topo.size() != ctx->M) ctx->topo.assign(ctx->M, cgppd_topo::Molecule());
+    ctx->topo[molecule] = std::move(t);
+    std::copy(chain_index, chain_index + n, ctx->chain_index.begin() + i0);
+    ctx->topo_dirty = true;
+    ctx->mc_ready = false;
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_topology_list(int n_res, const int *chain_code, const int *chain_index, int all_flexible, int n_segments,
+                        const int *seg_off, const int *seg_res, int kind, int *out, int *count)
+{
+    if (n_res < 1 || !chain_code || !chain_index || !count) return CGPPD_E_ARG;
+    std::vector<std::vector<int>> segs;
+    std::string err;
+    if (read_segments(n_res, n_segments, seg_off, seg_res, segs, err)) return CGPPD_E_ARG;
+    int nchains = 0;
+    for (int k = 0; k < n_res; ++k) nchains = std::max(nchains, chain_index[k] + 1);
+    cgppd_topo::Molecule t;
+    if (!cgppd_topo::build(n_res, chain_code, all_flexible != 0, segs, nchains, t, err)) return CGPPD_E_ARG;
+    std::vector<int> v;
+    switch (kind) {
+    case 0: for (const auto &a : t.bonds) v.insert(v.end(), a.begin(), a.end()); break;
+    case 1: for (const auto &a : t.angles) v.insert(v.end(), a.begin(), a.end()); break;
+    case 2: for (const auto &a : t.torsions) v.insert(v.end(), a.begin(), a.end()); break;
+    case 3: v = t.local; break;
+    case 4: v = t.crankshaft; break;
+    case 5: v = t.flex; break;
+    case 6: for (const auto &a : t.segment_bonds) v.insert(v.end(), a.begin(), a.end()); break;
+    case 7: for (const auto &a : t.indirect) v.insert(v.end(), a.begin(), a.end()); break;
+    case 8: v = t.domain_of; break;
+    case 9: v = t.flex_nb; break;
+    case 10: v = t.flex_nb_off; break;
+    case 11: v = t.branch; break;
+    case 12: v = t.branch_off; break;
+    case 13: v = t.segbond_of; break;
+    case 14: v = {t.vertex_count, t.num_chains}; break;
+    default: return CGPPD_E_ARG;
+    }
+    *count = (int)v.size();
+    if (out) std::copy(v.begin(), v.end(), out);
     return CGPPD_OK;
 }
 
@@ -1430,6 +2166,8 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_rank_of, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_rep_at_rank, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_out, (size_t)2 * R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_out_int, (size_t)5 * R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_out_scratch, (size_t)2 * R))) return rc;
     CG_HIP(ctx, hipHostMalloc((void **)&ctx->h_out, sizeof(double) * 2 * R, hipHostMallocDefault));
 
     CG_HIP(ctx, hipMemcpy(ctx->d_pos, hpos.data(), sizeof(float4) * hpos.size(), hipMemcpyHostToDevice));
@@ -1505,7 +2243,7 @@ int cgppd_set_image_shift(cgppd_ctx *ctx, int enable)
     return CGPPD_OK;
 }
 
-static int energy_common(cgppd_ctx *ctx, int replica, int n_active)
+static int energy_common(cgppd_ctx *ctx, int replica, int n_active, bool intra)
 {
     int rc = check_ready(ctx, true);
     if (rc) return rc;
@@ -1513,24 +2251,53 @@ static int energy_common(cgppd_ctx *ctx, int replica, int n_active)
     CG_HIP(ctx, hipSetDevice(ctx->device));
     CG_HIP(ctx, hipMemsetAsync(ctx->d_out + replica, 0, sizeof(double), ctx->stream));
     CG_HIP(ctx, hipMemsetAsync(ctx->d_out + ctx->R + replica, 0, sizeof(double), ctx->stream));
-    return launch_full(ctx, replica, 1, n_active, false, nullptr);
+    if ((rc = launch_full(ctx, replica, 1, n_active, false, nullptr))) return rc;
+    // flexible molecules' intra-molecular pairs, as EonDevice counts them
+    // (the non-crowder variant counts inter-molecular pairs only,
+    // src/CudaFunctions.cu:757-961)
+    if (intra && (rc = launch_intra(ctx, replica, 1, false, nullptr, ctx->d_out, ctx->d_out + ctx->R))) return rc;
+    return CGPPD_OK;
+}
+
+int cgppd_energy_internal(cgppd_ctx *ctx, int replica, double *out)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R || !out) return fail(ctx, CGPPD_E_ARG, "bad replica or null output");
+    for (int k = 0; k < 5; ++k) out[k] = 0.0;
+    if (ctx->n_flex == 0) return CGPPD_OK;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = zero_dev(ctx, ctx->d_out_int, (size_t)5 * ctx->R))) return rc;
+    if ((rc = launch_intra(ctx, replica, 1, false, nullptr, ctx->d_out_scratch, ctx->d_out_scratch + ctx->R)))
+        return rc;
+    double t[5];
+    CG_HIP(ctx, hipMemcpyAsync(t, ctx->d_out_int + (size_t)5 * replica, sizeof(t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // src/Potential.cpp:203-210
+    out[0] = t[0] * (double)ctx->rt_to_kcal;
+    out[1] = t[1] * (double)ctx->dh_factor;
+    out[2] = t[2] * 0.5 * 378.0;
+    out[3] = t[3] * -10.0;
+    out[4] = t[4];
+    return CGPPD_OK;
 }
 
 int cgppd_energy(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total)
 {
-    int rc = energy_common(ctx, replica, ctx ? ctx->M : 0);
+    int rc = energy_common(ctx, replica, ctx ? ctx->M : 0, true);
     if (rc) return rc;
     return energy_outputs(ctx, replica, lj, dh, total);
 }
 
 int cgppd_energy_nc(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total)
 {
-    int rc = energy_common(ctx, replica, ctx ? ctx->n_nc : 0);
+    int rc = energy_common(ctx, replica, ctx ? ctx->n_nc : 0, false);
     if (rc) return rc;
     return energy_outputs(ctx, replica, lj, dh, total);
 }
 
-int cgppd_energy_async(cgppd_ctx *ctx, int replica) { return energy_common(ctx, replica, ctx ? ctx->M : 0); }
+int cgppd_energy_async(cgppd_ctx *ctx, int replica) { return energy_common(ctx, replica, ctx ? ctx->M : 0, true); }
 
 int cgppd_energy_wait(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total)
 {
@@ -1551,6 +2318,7 @@ int cgppd_energy_all(cgppd_ctx *ctx, double *lj, double *dh, double *total, doub
     if ((rc = zero_dev(ctx, ctx->d_out, (size_t)2 * R))) return rc;
     if (mat && (rc = zero_dev(ctx, ctx->d_out_mat, (size_t)R * M * M))) return rc;
     if ((rc = launch_full(ctx, 0, R, M, false, mat ? ctx->d_out_mat : nullptr))) return rc;
+    if ((rc = launch_intra(ctx, 0, R, false, mat ? ctx->d_out_mat : nullptr, ctx->d_out, ctx->d_out + R))) return rc;
     CG_HIP(ctx, hipMemcpyAsync(ctx->h_out, ctx->d_out, sizeof(double) * 2 * R, hipMemcpyDeviceToHost, ctx->stream));
     if (mat)
         CG_HIP(ctx, hipMemcpyAsync(mat, ctx->d_out_mat, sizeof(double) * R * M * M, hipMemcpyDeviceToHost, ctx->stream));
@@ -1564,10 +2332,8 @@ int cgppd_energy_all(cgppd_ctx *ctx, double *lj, double *dh, double *total, doub
     if (mat)
         for (int r = 0; r < R; ++r) {
             double *m = mat + (size_t)r * M * M;
-            for (int I = 0; I < M; ++I) {
-                m[(size_t)I * M + I] = 0.0;
+            for (int I = 0; I < M; ++I)
                 for (int J = I + 1; J < M; ++J) m[(size_t)J * M + I] = m[(size_t)I * M + J];
-            }
         }
     return CGPPD_OK;
 }
@@ -1579,6 +2345,7 @@ int cgppd_mc_prepare(cgppd_ctx *ctx)
     CG_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = zero_dev(ctx, ctx->d_emat, (size_t)ctx->R * ctx->M * ctx->M))) return rc;
     if ((rc = launch_full(ctx, 0, ctx->R, ctx->M, true, nullptr))) return rc;
+    if ((rc = launch_intra(ctx, 0, ctx->R, true, nullptr, nullptr, nullptr))) return rc;
     PairParams p = base_params(ctx);
     hipLaunchKernelGGL(prepare_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p);
     CG_HIP(ctx, hipGetLastError());

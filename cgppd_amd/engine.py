@@ -64,6 +64,38 @@ class Engine:
         self._check(self.lib.cgppd_set_system(self.h, n_mol, ptr(mo, C.c_int), ptr(ty, C.c_int), nc), "set_system")
         self.n_mol, self.n_res = n_mol, int(mo[-1])
 
+    def set_torsion_table(self, v, delta):
+        vv, dd = f64(v).reshape(-1), f64(delta).reshape(-1)
+        self._check(self.lib.cgppd_set_torsion_table(self.h, ptr(vv, C.c_double), ptr(dd, C.c_double)),
+                    "set_torsion_table")
+
+    def set_topology(self, molecule, chain_code, chain_index, all_flexible=False, segments=()):
+        code, idx = i32(chain_code), i32(chain_index)
+        off, res = _lib._segments_csr(list(segments))
+        self._check(self.lib.cgppd_set_topology(self.h, int(molecule), ptr(code, C.c_int), ptr(idx, C.c_int),
+                                                int(bool(all_flexible)), len(segments), ptr(off, C.c_int),
+                                                ptr(res, C.c_int)), "set_topology")
+
+    def set_flexible_spec(self, spec):
+        """Torsion table and the flexible molecules' topology of a spec
+        (SystemSpec or dict with mol_off, chain_code, chain_index,
+        all_flexible, segments, torsion_v, torsion_delta)."""
+        g = spec if isinstance(spec, dict) else spec.__dict__
+        self.set_torsion_table(g["torsion_v"], g["torsion_delta"])
+        mol_off = np.asarray(g["mol_off"])
+        for m in range(len(mol_off) - 1):
+            if g["all_flexible"][m] or g["segments"][m]:
+                a, b = int(mol_off[m]), int(mol_off[m + 1])
+                self.set_topology(m, g["chain_code"][a:b], g["chain_index"][a:b], g["all_flexible"][m],
+                                  g["segments"][m])
+
+    def energy_internal(self, r=0):
+        """internal_molecule_E(false) of replica r: kcal/mol LJ, DH (indirect
+        neighbours, negated), bond, angle, torsion."""
+        out = np.zeros(5)
+        self._check(self.lib.cgppd_energy_internal(self.h, int(r), ptr(out, C.c_double)), "energy_internal")
+        return out
+
     def set_replicas(self, centers, rel, temperature=None, translate_step=None, rotate_step=None, label=None,
                      seed=0):
         c, r = f32(centers), f32(rel)
@@ -84,6 +116,8 @@ class Engine:
         self.set_force_field(spec.ff.eps, spec.ff.sigma, spec.ff.charge)
         self.set_box(spec.box)
         self.set_system(spec.mol_off, spec.types)
+        if getattr(spec, "flexible", False):
+            self.set_flexible_spec(spec)
         temps = spec.temperatures() if R == spec.n_rep else np.linspace(spec.t_min, spec.t_max, R)
         ts, rs = spec.step_sizes()
         if R != spec.n_rep:

@@ -63,6 +63,56 @@ int cgppd_set_box(cgppd_ctx *ctx, float box);
  * Resets all replica state. */
 int cgppd_set_system(cgppd_ctx *ctx, int n_mol, const int *mol_off, const int *type, int n_noncrowder);
 
+/* Flexible linkers (the reference's FLEXIBLE_LINKS build) ----------------- */
+
+/* Torsion table: v[20*20*4] in kcal/mol and delta[20*20*4] in radians,
+ * indexed [type_j][type_k][n-1] for the middle residues j, k of a torsion.
+ * Replaces TorsionalLookupMatrix::loadData (src/TorsionalLookupMatrix.cpp:27-52)
+ * and its per-molecule copies (src/Molecule.cpp:30). */
+int cgppd_set_torsion_table(cgppd_ctx *ctx, const double *v, const double *delta);
+
+/* Flexible topology of one molecule; call after cgppd_set_system (which
+ * makes every molecule rigid) and before cgppd_mc_prepare.
+ *   chain_code[n_m]   PDB chain identifier of each residue: consecutive
+ *                     residues with equal codes are bonded (src/Geometry.cpp:40-46)
+ *   chain_index[n_m]  chain number of each residue (TER records before it,
+ *                     src/Molecule.cpp:557)
+ *   all_flexible      moldata::all_flexible (src/structures.h:67)
+ *   n_segments, seg_off[n_segments+1], seg_res[seg_off[n_segments]]
+ *                     moldata::segments, residue indices 0-based within the
+ *                     molecule (src/Simulation.cpp:350-374 subtracts 1)
+ * The library builds the bonded graph, move sets and exclusion UIDs: replaces
+ * Graph::init and Graph::assign_uids (src/Geometry.cpp:34-302, called from
+ * src/Molecule.cpp:57 and src/Replica.cpp:30-38) and the pos.w / meta.w
+ * packing of the exclusion tags (src/Residue.cpp:41-59, src/Replica.cpp:753-775).
+ * With all_flexible = 0 and no segments the molecule is rigid.  Energies
+ * then include the molecule's intra-molecular pairs that the GPU kernel of
+ * the reference counts (src/CudaFunctions.cu:447-476), and MC steps use the
+ * flex and local moves (src/Molecule.cpp:274-466). */
+int cgppd_set_topology(cgppd_ctx *ctx, int molecule, const int *chain_code, const int *chain_index,
+                       int all_flexible, int n_segments, const int *seg_off, const int *seg_res);
+
+/* internal_molecule_E(false) of one replica (src/Replica.cpp:532-560 over
+ * src/Molecule.cpp:636-699), kcal/mol: out[5] = LJ and DH of the indirect
+ * neighbours (negated), bond, angle, torsion.  cgppd_energy's total plus the
+ * sum of out[] is the replica's total energy, as
+ * Replica::EonDevice() + internal_molecule_E(false).total() (src/Replica.cpp:362-371). */
+int cgppd_energy_internal(cgppd_ctx *ctx, int replica, double *out);
+
+/* Host-only topology query (no GPU, no context): the lists cgppd_set_topology
+ * derives for one molecule of n_res residues.
+ *   kind 0 bonds (2 ints each), 1 angles (3), 2 torsions (4), 3 local,
+ *   4 crankshaft, 5 flex residues, 6 segment bonds (2), 7 indirect
+ *   neighbours (2), 8 rigid-domain index of each residue (-1: none),
+ *   9 neighbours of each flex residue, 10 their offsets (n_flex + 1),
+ *   11 residues each flex move rotates, 12 their offsets (one per
+ *   neighbour entry, + 1), 13 segment-bond index of each residue (-1: none),
+ *   14 {vertex count, number of chains}.
+ * *count receives the number of ints; out may be NULL to size the buffer.
+ * New: the reference's Graph members (src/Geometry.h:137-176). */
+int cgppd_topology_list(int n_res, const int *chain_code, const int *chain_index, int all_flexible,
+                        int n_segments, const int *seg_off, const int *seg_res, int kind, int *out, int *count);
+
 /* Replica state.  Replaces ReplicaDataToDevice (src/Replica.cpp:705-830) for
  * R replicas at once plus Replica::init_child_replica's ladders
  * (src/Replica.cpp:79-98).  Arrays are row-major by replica:
@@ -124,9 +174,11 @@ int cgppd_energy_all(cgppd_ctx *ctx, double *lj, double *dh, double *total, doub
 int cgppd_mc_prepare(cgppd_ctx *ctx);
 
 /* n_steps MC steps on every replica, all on the device: molecule pick,
- * rigid translate/rotate (src/Molecule.cpp:146-266,468-504), delta-E of the
- * moved molecule against all others, Metropolis (src/Replica.cpp:385-410)
- * and state update.  Replaces Replica::MCSearch (src/Replica.cpp:350-414)
+ * rigid translate/rotate (src/Molecule.cpp:146-266,468-504), for flexible
+ * molecules also flex / local translate / crankshaft (src/Molecule.cpp:274-466),
+ * delta-E of the moved molecule against all others (plus its internal
+ * energy when flexible), Metropolis (src/Replica.cpp:385-410) and state
+ * update.  Replaces Replica::MCSearch (src/Replica.cpp:350-414)
  * and MCSearchMutate/Evaluate/AcceptReject (src/Replica.cpp:643-701). */
 int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps);
 

@@ -232,8 +232,13 @@ class Flex:
         for m, g in enumerate(self.graphs):
             a = int(mol_off[m])
             flexible = bool(self.mol_flexible[m])
+            n_m = int(mol_off[m + 1]) - a
+            # crankshaft needs the residues before and after in the caller's
+            # order (src/Molecule.cpp:306,318): residues at the molecule's ends
+            # are left out (the reference would read outside the molecule)
+            crank = [v for v in g.crankshaft if 0 < v < n_m - 1]
             src = {"bond": g.bonds, "angle": g.angles, "tors": g.torsions, "ind": g.indirect, "local": g.local,
-                   "crank": g.crankshaft, "flexr": g.flex}
+                   "crank": crank, "flexr": g.flex}
             for k, items in src.items():
                 off, flat = lists[k]
                 if flexible:
