@@ -45,7 +45,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50000, help="timed MC steps (all replicas each)")
     ap.add_argument("--warmup", type=int, default=2000)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 4])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 4],
+                    help="0: two chains, 1: 40-chain crowded box (default), 2: linked domains (flexible), "
+                         "4: 256-chain crowded box")
     ap.add_argument("--exchange-every", type=int, default=1000)
     ap.add_argument("--replicas", type=int, default=0, help="override the configuration's replica count")
     ap.add_argument("--wg-per-cu", type=int, default=0)
@@ -196,8 +198,11 @@ def main():
         "config": {
             "workload": spec.name,
             "n_mol": spec.n_mol, "n_res": spec.n_res, "replicas": R, "box_A": round(spec.box, 2),
-            "exchange_every": per, "moves": "rigid translate 2 A / rotate 0.2 rad",
+            "exchange_every": per,
+            "moves": ("translate 2 A / rotate 0.2 rad / flex / local translate + crankshaft"
+                      if getattr(spec, "flexible", False) else "rigid translate 2 A / rotate 0.2 rad"),
             "pair_terms": "fp32, Kahan over 4-term groups per lane, fp64 row reduction",
+            "flexible": bool(getattr(spec, "flexible", False)),
             "image_shift": not args.no_image_shift,
         },
         "pairs_per_s": pairs / wall,

@@ -152,12 +152,83 @@ Potential::Potential(double LJ_kcal, double DH_kcal)
     LJ = LJ_kcal / rt_to_kcalmol();
     DH = DH_kcal / dh_conversion_factor();
 }
+Potential::Potential(double LJ_kcal, double DH_kcal, double bond_kcal, double angle_kcal, double torsion_kcal)
+    : Potential(LJ_kcal, DH_kcal)
+{
+    bond = bond_kcal / (0.5 * 378);
+    angle = std::exp(angle_kcal / -10.0);
+    torsion = torsion_kcal;
+}
 double Potential::total_LJ() const { return LJ * rt_to_kcalmol(); }
 double Potential::total_DH() const { return DH * dh_conversion_factor(); }
-double Potential::total() const { return DH * dh_conversion_factor() + LJ * rt_to_kcalmol(); }
+double Potential::total_bond() const { return bond * 0.5 * 378; }
+double Potential::total_angle() const { return std::log(angle) * -10.0; }
+double Potential::total_torsion() const { return torsion; }
+double Potential::total() const
+{
+    // src/Potential.cpp:203-210
+    return DH * dh_conversion_factor() + LJ * rt_to_kcalmol() + bond * 0.5 * 378 + std::log(angle) * -10.0 + torsion;
+}
 bool Potential::almost_equal(const Potential &p, float eps) const
 {
-    return std::fabs(LJ - p.LJ) < eps && std::fabs(DH - p.DH) < eps;
+    return std::fabs(LJ - p.LJ) < eps && std::fabs(DH - p.DH) < eps && std::fabs(bond - p.bond) < eps &&
+           std::fabs(angle - p.angle) < eps && std::fabs(torsion - p.torsion) < eps;
+}
+
+bool TorsionalLookupMatrix::loadData(const char *filename, const AminoAcids &aminoAcids)
+{
+    std::ifstream in(filename);
+    if (!in.good()) throw error(std::string("Failed to open torsional data file: ") + filename);
+    std::string line;
+    while (getline_c(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        char a0[8] = {0}, a1[8] = {0};
+        double v = 0.0, sigma = 0.0;
+        int n = 0;
+        if (std::sscanf(line.c_str(), "%3s %3s %lf %d %lf", a0, a1, &v, &n, &sigma) != 5) continue;
+        const int i = aminoAcids.getAminoAcidIndex(a0), j = aminoAcids.getAminoAcidIndex(a1);
+        if (i < 0 || j < 0 || n < 1 || n > 4) throw error("bad torsion table line: " + line);
+        V[i][j][n - 1] = v;
+        Sigma[i][j][n - 1] = sigma / 57.29577951308232;  // degrees to radians
+    }
+    return true;
+}
+
+// Bonded terms with float vectors and double potentials (src/Potential.cpp:45-98).
+static double bond_term(const Residue &a, const Residue &b, float L)
+{
+    const double r = b.distance(a, L);
+    return (r - (double)3.81f) * (r - (double)3.81f);
+}
+
+static float fmagnitude(float x, float y, float z) { return std::sqrt(x * x + y * y + z * z); }
+
+static double angle_term(const Residue &a, const Residue &b, const Residue &c)
+{
+    const float abx = a.position.x - b.position.x, aby = a.position.y - b.position.y, abz = a.position.z - b.position.z;
+    const float cbx = c.position.x - b.position.x, cby = c.position.y - b.position.y, cbz = c.position.z - b.position.z;
+    const double cmag = (double)(fmagnitude(cbx, cby, cbz) * fmagnitude(abx, aby, abz));
+    float th = 0.0f;
+    if (!(cmag < 1e-26)) th = (float)std::acos((double)(abx * cbx + aby * cby + abz * cbz) / cmag);
+    const double t = th, da = t - (double)1.6f, db = t - (double)2.27f;
+    return std::exp(-(double)0.1f * ((double)106.4f * da * da + (double)4.3f)) +
+           std::exp(-(double)0.1f * ((double)26.3f * db * db));
+}
+
+static double torsion_term(const Residue &a, const Residue &b, const Residue &c, const Residue &d,
+                           const TorsionalLookupMatrix &tor)
+{
+    const float b1x = b.position.x - a.position.x, b1y = b.position.y - a.position.y, b1z = b.position.z - a.position.z;
+    const float b2x = c.position.x - b.position.x, b2y = c.position.y - b.position.y, b2z = c.position.z - b.position.z;
+    const float b3x = d.position.x - c.position.x, b3y = d.position.y - c.position.y, b3z = d.position.z - c.position.z;
+    const float x23 = b2y * b3z - b2z * b3y, y23 = b2z * b3x - b2x * b3z, z23 = b2x * b3y - b2y * b3x;
+    const float x12 = b1y * b2z - b1z * b2y, y12 = b1z * b2x - b1x * b2z, z12 = b1x * b2y - b1y * b2x;
+    const double phi = (double)std::atan2(fmagnitude(b2x, b2y, b2z) * (b1x * x23 + b1y * y23 + b1z * z23),
+                                          x12 * x23 + y12 * y23 + z12 * z23);
+    const int j = b.aminoAcidIndex, k = c.aminoAcidIndex;
+    double e = 0.0;
+    for (int n = 1; n <= 4; ++n) e += (1 + std::cos(n * phi - tor.getSigma(j, k, n))) * tor.getV(j, k, n);
+    return e;
 }
 
 double calculate_LJ(int ti, int tj, double r, const AminoAcids &aa)
@@ -295,11 +366,99 @@ void Molecule::rotate(const double axis[3], double angle)
     }
 }
 
-void Molecule::init(const moldata &mol, const AminoAcids &aa, int idx, float bv)
+static std::vector<int> topology_list(const std::vector<int> &code, const std::vector<int> &chain, bool all_flexible,
+                                      const std::vector<std::vector<int>> &segments, int kind)
+{
+    std::vector<int> off{0}, res;
+    for (const auto &sg : segments) {
+        res.insert(res.end(), sg.begin(), sg.end());
+        off.push_back((int)res.size());
+    }
+    if (res.empty()) res.push_back(0);
+    int n = 0;
+    int rc = cgppd_topology_list((int)code.size(), code.data(), chain.data(), all_flexible ? 1 : 0,
+                                 (int)segments.size(), off.data(), res.data(), kind, nullptr, &n);
+    if (rc) throw error("cgppd_topology_list failed (bad segment?)");
+    std::vector<int> out(std::max(n, 1));
+    cgppd_topology_list((int)code.size(), code.data(), chain.data(), all_flexible ? 1 : 0, (int)segments.size(),
+                        off.data(), res.data(), kind, out.data(), &n);
+    out.resize(n);
+    return out;
+}
+
+void Molecule::init_graph()
+{
+    // src/Molecule.cpp:57-61 (Graph::init); the library's builder restates
+    // src/Geometry.cpp:34-192
+    is_flexible = all_flexible || !segments.empty();
+    bonds.clear();
+    angles.clear();
+    torsions.clear();
+    indirect.clear();
+    domain_of.assign(residueCount, -1);
+    segbond_of.assign(residueCount, -1);
+    std::vector<int> code, chain;
+    for (const Residue &r : Residues) {
+        code.push_back((unsigned char)r.chainId);
+        chain.push_back(r.chain_int_id);
+    }
+    bonds = topology_list(code, chain, all_flexible, segments, 0);
+    angles = topology_list(code, chain, all_flexible, segments, 1);
+    torsions = topology_list(code, chain, all_flexible, segments, 2);
+    indirect = topology_list(code, chain, all_flexible, segments, 7);
+    domain_of = topology_list(code, chain, all_flexible, segments, 8);
+    segbond_of = topology_list(code, chain, all_flexible, segments, 13);
+    vertex_count = topology_list(code, chain, all_flexible, segments, 14)[0];
+}
+
+Potential Molecule::E(bool include_LJ_and_DH, const AminoAcids &aa, const TorsionalLookupMatrix *tor) const
+{
+    if (!tor) tor = torsion_data;
+    Potential p;
+    if (!is_flexible) return p;
+    const int n = residueCount;
+    auto indirect_pair = [&](int i, int j) {
+        for (size_t k = 0; k + 1 < indirect.size(); k += 2)
+            if (indirect[k] == i && indirect[k + 1] == j) return true;
+        return false;
+    };
+    if (include_LJ_and_DH) {
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                const Residue &a = Residues[i], &b = Residues[j];
+                if (a.same_rigid_domain_as(b) || a.chain_neighbour_of(b) || a.bonded_to(b) || indirect_pair(i, j))
+                    continue;
+                const double r = a.distance(b, bounding_value) + (double)EPS;
+                p.increment_LJ(calculate_LJ(a.aminoAcidIndex, b.aminoAcidIndex, r, aa));
+                p.increment_DH(calculate_DH(a.electrostaticCharge, b.electrostaticCharge, r));
+            }
+    } else {
+        for (size_t k = 0; k + 1 < indirect.size(); k += 2) {
+            const Residue &a = Residues[indirect[k]], &b = Residues[indirect[k + 1]];
+            const double r = a.distance(b, bounding_value) + (double)EPS;
+            p.increment_LJ(-calculate_LJ(a.aminoAcidIndex, b.aminoAcidIndex, r, aa));
+            p.increment_DH(-calculate_DH(a.electrostaticCharge, b.electrostaticCharge, r));
+        }
+    }
+    for (size_t k = 0; k + 1 < bonds.size(); k += 2)
+        p.increment_bond(bond_term(Residues[bonds[k]], Residues[bonds[k + 1]], bounding_value));
+    for (size_t k = 0; k + 2 < angles.size(); k += 3)
+        p.increment_angle(angle_term(Residues[angles[k]], Residues[angles[k + 1]], Residues[angles[k + 2]]));
+    if (!tor && !torsions.empty()) throw error("flexible molecule without a torsion table");
+    for (size_t k = 0; k + 3 < torsions.size(); k += 4)
+        p.increment_torsion(torsion_term(Residues[torsions[k]], Residues[torsions[k + 1]], Residues[torsions[k + 2]],
+                                         Residues[torsions[k + 3]], *tor));
+    return p;
+}
+
+void Molecule::init(const moldata &mol, const AminoAcids &aa, int idx, float bv, const TorsionalLookupMatrix *tor)
 {
     index = idx;
     bounding_value = bv;
     name = mol.name;
+    torsion_data = tor;
+    all_flexible = mol.all_flexible;
+    segments = mol.segments;
     initFromPDB(mol.pdbfilename.c_str(), aa);
     if (mol.translate) {
         if (mol.px || mol.py || mol.pz) translate(Vector3f{mol.px, mol.py, mol.pz});
@@ -317,6 +476,7 @@ void Molecule::init(const moldata &mol, const AminoAcids &aa, int idx, float bv)
         }
     }
     crowder = mol.crowder;
+    init_graph();
 }
 
 // ---------------------------------------------------------------------------
@@ -352,7 +512,7 @@ void DeviceContext::check(int rc, const char *what) const
 void Replica::loadMolecule(const moldata &mol)
 {
     Molecule m;
-    m.init(mol, aminoAcids, moleculeCount, boundingValue);
+    m.init(mol, aminoAcids, moleculeCount, boundingValue, &torsions);
     residueCount += m.residueCount;
     max_residue_count = std::max(max_residue_count, m.residueCount);
     if (!mol.crowder) {
@@ -369,7 +529,26 @@ void Replica::init_first_replica(const argdata &parameters, const AminoAcids &am
     boundingValue = parameters.bound;
     molecules.clear();
     moleculeCount = residueCount = nonCrowderCount = nonCrowderResidues = max_residue_count = 0;
+    bool flexible = false;
+    for (const moldata &m : parameters.mdata) flexible = flexible || m.all_flexible || !m.segments.empty();
+    if (flexible) torsions.loadData(parameters.torsionFile.c_str(), amino_acid_data);
     for (const moldata &m : parameters.mdata) loadMolecule(m);
+    // replica-wide UIDs from 1 (src/Replica.cpp:30-38, src/Geometry.cpp:256-302)
+    int chain_offset = 1, domain_offset = 1, bond_offset = 1;
+    for (Molecule &mol : molecules) {
+        int n_domains = 0, n_bonds = 0;
+        for (int d : mol.domain_of) n_domains = std::max(n_domains, d + 1);
+        for (int b : mol.segbond_of) n_bonds = std::max(n_bonds, b + 1);
+        for (int i = 0; i < mol.residueCount; ++i) {
+            Residue &r = mol.Residues[i];
+            r.rigid_domain_UID = mol.domain_of[i] >= 0 ? domain_offset + mol.domain_of[i] : 0;
+            r.segment_bond_UID = mol.segbond_of[i] >= 0 ? bond_offset + mol.segbond_of[i] : 0;
+            r.chain_UID = i < mol.vertex_count ? chain_offset + r.chain_int_id : 0;
+        }
+        chain_offset += mol.chainCount;
+        domain_offset += n_domains;
+        bond_offset += n_bonds;
+    }
     // the device requires non-crowders first (src/Replica.cpp:924)
     bool seen_crowder = false;
     for (const Molecule &m : molecules) {
@@ -378,10 +557,28 @@ void Replica::init_first_replica(const argdata &parameters, const AminoAcids &am
     }
 }
 
+Potential Replica::internal_molecule_E(bool include_LJ_and_DH) const
+{
+    // src/Replica.cpp:532-560
+    Potential p;
+    for (const Molecule &m : molecules)
+        if (!m.crowder) p += m.E(include_LJ_and_DH, aminoAcids, &torsions);
+    return p;
+}
+
+bool Replica::has_flexible() const
+{
+    for (const Molecule &m : molecules)
+        if (m.is_flexible) return true;
+    return false;
+}
+
 void Replica::copy(const Replica &r)
 {
     aminoAcids = r.aminoAcids;
+    torsions = r.torsions;
     molecules = r.molecules;
+    for (Molecule &m : molecules) m.torsion_data = &torsions;
     moleculeCount = r.moleculeCount;
     residueCount = r.residueCount;
     nonCrowderCount = r.nonCrowderCount;
@@ -742,6 +939,7 @@ void Simulation::loadArgsFromFile()
             else if (key == "seed") parameters.seed = std::strtoull(value.c_str(), nullptr, 10);
             else if (key == "aminoacids") parameters.aminoAcidFile = value;
             else if (key == "pairpotentials") parameters.pairPotentialFile = value;
+            else if (key == "torsions") parameters.torsionFile = value;
             else std::fprintf(stderr, "warning: unknown parameter: %s\n", key.c_str());
         }
     }
@@ -800,6 +998,35 @@ void Simulation::setup_device(int device)
     dev_->check(cgppd_set_system(dev_->ctx(), initialReplica.moleculeCount, off.data(), types.data(),
                                  initialReplica.nonCrowderCount),
                 "cgppd_set_system");
+    if (initialReplica.has_flexible()) {
+        // TorsionalLookupMatrix + the molecules' graphs (src/Molecule.cpp:30,57)
+        std::vector<double> v(1600), d(1600);
+        const TorsionalLookupMatrix &t = initialReplica.torsions;
+        for (int a = 0; a < AA_COUNT; ++a)
+            for (int b = 0; b < AA_COUNT; ++b)
+                for (int n = 0; n < 4; ++n) {
+                    v[(a * AA_COUNT + b) * 4 + n] = t.V[a][b][n];
+                    d[(a * AA_COUNT + b) * 4 + n] = t.Sigma[a][b][n];
+                }
+        dev_->check(cgppd_set_torsion_table(dev_->ctx(), v.data(), d.data()), "cgppd_set_torsion_table");
+        for (int m = 0; m < initialReplica.moleculeCount; ++m) {
+            const Molecule &mol = initialReplica.molecules[m];
+            if (!mol.is_flexible) continue;
+            std::vector<int> code, chain, seg_off{0}, seg_res;
+            for (const Residue &r : mol.Residues) {
+                code.push_back((unsigned char)r.chainId);
+                chain.push_back(r.chain_int_id);
+            }
+            for (const auto &sg : mol.segments) {
+                seg_res.insert(seg_res.end(), sg.begin(), sg.end());
+                seg_off.push_back((int)seg_res.size());
+            }
+            if (seg_res.empty()) seg_res.push_back(0);
+            dev_->check(cgppd_set_topology(dev_->ctx(), m, code.data(), chain.data(), mol.all_flexible ? 1 : 0,
+                                           (int)mol.segments.size(), seg_off.data(), seg_res.data()),
+                        "cgppd_set_topology");
+        }
+    }
     const int R = (int)replica.size(), M = initialReplica.moleculeCount, N = initialReplica.residueCount;
     std::vector<float> centres, rel;
     std::vector<double> temps(R);

@@ -16,6 +16,8 @@
 #define CGPPD_HOST_H
 
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -69,25 +71,48 @@ public:
 };
 
 // ---------------------------------------------------------------------------
-// Potential accumulator (src/Potential.h:19-92): raw LJ [RT] and raw DH.
+// Potential accumulator (src/Potential.h:19-92) of the flexible-linker build:
+// raw LJ [RT], raw DH, raw bond sum sum (r - R0)^2, the product of the angle
+// terms and the torsion sum; rigid systems leave the last three at 0, 1, 0.
 // ---------------------------------------------------------------------------
 class Potential {
 public:
-    double LJ = 0.0, DH = 0.0;
+    double LJ = 0.0, DH = 0.0, bond = 0.0, angle = 1.0, torsion = 0.0;
     Potential() = default;
     Potential(double LJ_kcal, double DH_kcal);  // stores raw, as src/Potential.cpp:116-126
+    Potential(double LJ_kcal, double DH_kcal, double bond_kcal, double angle_kcal, double torsion_kcal);
     void increment_LJ(double v) { LJ += v; }
     void increment_DH(double v) { DH += v; }
+    void increment_bond(double v) { bond += v; }
+    void increment_angle(double v) { angle *= v; }
+    void increment_torsion(double v) { torsion += v; }
     Potential &operator+=(const Potential &o)
     {
         LJ += o.LJ;
         DH += o.DH;
+        bond += o.bond;
+        angle *= o.angle;
+        torsion += o.torsion;
         return *this;
     }
     double total_LJ() const;
     double total_DH() const;
+    double total_bond() const;
+    double total_angle() const;
+    double total_torsion() const;
     double total() const;
     bool almost_equal(const Potential &p, float eps = 0.00001f) const;
+};
+
+// Torsion table (src/TorsionalLookupMatrix.cpp:21-52): V [kcal/mol] and the
+// phase [rad] for the middle residue types of a torsion, n = 1..4.
+class TorsionalLookupMatrix {
+public:
+    double V[AA_COUNT][AA_COUNT][4] = {};
+    double Sigma[AA_COUNT][AA_COUNT][4] = {};
+    bool loadData(const char *filename, const AminoAcids &aminoAcids);  // throws on a missing file
+    double getV(int ai, int aj, int n) const { return V[ai][aj][n - 1]; }
+    double getSigma(int ai, int aj, int n) const { return Sigma[ai][aj][n - 1]; }
 };
 
 // CPU pair terms with the reference's rounding (src/Potential.cpp:5-42).
@@ -107,8 +132,14 @@ struct Residue {
     int chain_int_id = 0;
     int resSeq = -1;
     int moleculeId = -1;
+    // replica-wide UIDs (src/Geometry.cpp:256-302), 0 = none
+    int chain_UID = 0, rigid_domain_UID = 0, segment_bond_UID = 0;
     // src/Residue.cpp:62-82: FP32 minimum image, sqrtf
     double distance(const Residue &r, float bounding_value = 0.0f) const;
+    // src/Residue.cpp:116-129
+    bool chain_neighbour_of(const Residue &o) const { return chain_UID == o.chain_UID && std::abs(resSeq - o.resSeq) < 4; }
+    bool same_rigid_domain_as(const Residue &o) const { return rigid_domain_UID && rigid_domain_UID == o.rigid_domain_UID; }
+    bool bonded_to(const Residue &o) const { return segment_bond_UID && segment_bond_UID == o.segment_bond_UID; }
 };
 
 struct moldata {
@@ -131,8 +162,22 @@ public:
     float bounding_value = 0.f;
     bool crowder = false;
     std::string filename, name;
+    // flexible linkers: the bonded graph from the library's topology builder
+    // (cgppd_topology_list, src/Geometry.cpp:34-192)
+    bool is_flexible = false, all_flexible = false;
+    std::vector<std::vector<int>> segments;
+    std::vector<int> bonds, angles, torsions, indirect;  // 2, 3, 4, 2 residue indices each
+    std::vector<int> domain_of, segbond_of;              // per residue, -1 = none
+    int vertex_count = 0;
+    const TorsionalLookupMatrix *torsion_data = nullptr;
 
-    void init(const moldata &mol, const AminoAcids &aa, int index, float bounding_value);
+    void init(const moldata &mol, const AminoAcids &aa, int index, float bounding_value,
+              const TorsionalLookupMatrix *torsions = nullptr);
+    void init_graph();
+    // src/Molecule.cpp:636-699: intra-molecular pairs (include_LJ_and_DH) or
+    // minus the indirect neighbours, plus the bonded terms
+    // (tor: the torsion table; the molecule's own torsion_data when null)
+    Potential E(bool include_LJ_and_DH, const AminoAcids &aa, const TorsionalLookupMatrix *tor = nullptr) const;
     void initFromPDB(const char *pdbfilename, const AminoAcids &aa);
     void setPosition(Vector3f c);                     // src/Molecule.cpp:183-191
     void translate(Vector3f v);                       // src/Molecule.cpp:146-159
@@ -171,6 +216,7 @@ struct argdata {
     bool use_graph = true;     // new: replay exchange intervals from a hipGraph
     std::string aminoAcidFile = "data/AminoAcids";
     std::string pairPotentialFile = "data/pair_potentials_MJ1996";
+    std::string torsionFile = "data/torsional_pair_potentials";  // src/definitions.h:7
     std::vector<moldata> mdata;
     std::map<std::string, int> mdata_map;
 };
@@ -223,9 +269,12 @@ public:
     void copy(const Replica &r);
     void exchangeReplicas(Replica &r);  // src/Replica.cpp:188-210
 
-    // CPU energies (src/Replica.cpp:421-528)
+    // CPU energies (src/Replica.cpp:421-560)
     Potential E() const;
     Potential E(const Molecule *a, const Molecule *b) const;
+    Potential internal_molecule_E(bool include_LJ_and_DH) const;
+    bool has_flexible() const;
+    TorsionalLookupMatrix torsions;
 
     // device entry points (src/Replica.cpp:705-935); no CPU fallback
     void setup_device(DeviceContext *dev, int slot);

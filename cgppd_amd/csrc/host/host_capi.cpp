@@ -104,6 +104,22 @@ int cgppd_host_replica_E(void *h, int r, double *lj, double *dh, double *total)
     });
 }
 
+// Replica::internal_molecule_E(include) (CPU): raw LJ, DH, bond, angle
+// product, torsion and the total
+int cgppd_host_replica_internal(void *h, int r, int include, double *out)
+{
+    return guarded([&] {
+        auto *s = static_cast<cgppd::Simulation *>(h);
+        const cgppd::Potential p = s->replica.at(r).internal_molecule_E(include != 0);
+        out[0] = p.LJ;
+        out[1] = p.DH;
+        out[2] = p.bond;
+        out[3] = p.angle;
+        out[4] = p.torsion;
+        out[5] = p.total();
+    });
+}
+
 // device set-up, then Replica::EonDevice / EonDeviceNC / async
 int cgppd_host_setup_device(void *h, int device)
 {

@@ -154,8 +154,14 @@ class SystemSpec:
         return pos
 
     def pairs_full(self) -> int:
+        """Residue pairs of one full evaluation: every inter-molecular pair
+        plus every intra-molecular pair of flexible molecules."""
         n = self.lengths.astype(np.int64)
-        return int((n.sum() ** 2 - (n * n).sum()) // 2)
+        pairs = int((n.sum() ** 2 - (n * n).sum()) // 2)
+        for m in range(self.n_mol):
+            if self.molecule_is_flexible(m):
+                pairs += int(n[m] * (n[m] - 1) // 2)
+        return pairs
 
     def temperatures(self) -> np.ndarray:
         """src/Simulation.cpp:590 + src/Replica.cpp:86: float ratio, double

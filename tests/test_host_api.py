@@ -23,7 +23,7 @@ def host():
     lib.cgppd_host_simulation_create.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_void_p)]
     for n in ("cgppd_host_simulation_destroy", "cgppd_host_simulation_sizes", "cgppd_host_replica_state",
               "cgppd_host_replica_E", "cgppd_host_setup_device", "cgppd_host_replica_EonDevice",
-              "cgppd_host_simulation_run"):
+              "cgppd_host_simulation_run", "cgppd_host_replica_internal"):
         getattr(lib, n).argtypes = None
     return lib
 
@@ -118,6 +118,61 @@ def test_replica_E_matches_oracle_on_reference_conformation(host, tmp_path, mj_f
     rt, dhc, _ = oracle.constants()
     assert abs(lj.value - elj / np.float32(rt)) < 1e-5
     assert abs(dh.value - edh / np.float32(dhc)) < 1e-5
+    host.cgppd_host_simulation_destroy(h)
+
+
+def flexible_conf1_config(tmp_path):
+    """tests/testReplica.cpp:30-58: data/conf1 with 1b's residues 73-76
+    (1-based in the argfile) a flexible segment, box 118.4."""
+    cfg = tmp_path / "config"
+    cfg.write_text("replicas 1\nboundary 118.4\n"
+                   f"aminoacids {DATA / 'AminoAcids'}\npairpotentials {DATA / 'pair_potentials_MJ1996'}\n"
+                   f"torsions {DATA / 'torsional_pair_potentials'}\n\nfiles\n"
+                   f"t(0,0,0) r(0,0,0,0) {DATA / 'conf1' / '1a.pdb'} a\n"
+                   f"t(0,0,0) r(0,0,0,0) {DATA / 'conf1' / '1b.pdb'} b\n\nsegments\nb 73 74 75 76\n")
+    return cfg
+
+
+def raw_potential(lj, dh, bond, angle, torsion):
+    """Potential(LJ, DH, bond, angle, torsion) in kcal/mol -> raw (src/Potential.cpp:116-126)."""
+    rt, dhc, _ = oracle.constants()
+    return np.array([lj / np.float32(rt), dh / np.float32(dhc), bond / (0.5 * 378), np.exp(angle / -10.0), torsion])
+
+
+def test_flexible_replica_E_matches_reference_test(host, tmp_path):
+    """(replica.E() + replica.internal_molecule_E(true)).almost_equal(expected)
+    (tests/testReplica.cpp:81-82) on the C++ host API's CPU path."""
+    h = create(host, ["cgppd", "-f", str(flexible_conf1_config(tmp_path))])
+    lj, dh, tot = C.c_double(), C.c_double(), C.c_double()
+    host.cgppd_host_replica_E(h, 0, C.byref(lj), C.byref(dh), C.byref(tot))
+    out = np.zeros(6)
+    assert host.cgppd_host_replica_internal(h, 0, 1, out.ctypes.data_as(C.c_void_p)) == 0, \
+        host.cgppd_host_last_error()
+    got = np.array([lj.value + out[0], dh.value + out[1], out[2], out[3], out[4]])
+    want = raw_potential(-0.068205, -0.240383, 0.180426, 1.555213, 5.420859)
+    assert (np.abs(got - want) < 1e-5).all(), (got, want)
+    rigid = raw_potential(-0.079673, -0.212202, 0.0, 0.0, 0.0)
+    assert abs(lj.value - rigid[0]) < 1e-5 and abs(dh.value - rigid[1]) < 1e-5
+    host.cgppd_host_simulation_destroy(h)
+
+
+@pytest.mark.gpu
+def test_flexible_device_entry_points(host, tmp_path, gpu_engine_factory):
+    """tests/testReplica.cpp:84-91: EonDevice() + internal_molecule_E(false).total()
+    == Approx(flexible total), EonDeviceNC() == Approx(rigid total), and the
+    asynchronous path."""
+    h = create(host, ["cgppd", "-f", str(flexible_conf1_config(tmp_path))])
+    assert host.cgppd_host_setup_device(h, 0) == 0, host.cgppd_host_last_error()
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    assert host.cgppd_host_replica_EonDevice(h, 0, C.byref(a), C.byref(b), C.byref(c)) == 0, \
+        host.cgppd_host_last_error()
+    out = np.zeros(6)
+    assert host.cgppd_host_replica_internal(h, 0, 0, out.ctypes.data_as(C.c_void_p)) == 0
+    flexible = -0.068205 - 0.240383 + 0.180426 + 1.555213 + 5.420859
+    approx = lambda x, y: abs(x - y) < 100 * np.finfo(np.float32).eps * (1 + max(abs(x), abs(y))) + 5e-6  # noqa: E731
+    assert approx(a.value + out[5], flexible), (a.value + out[5], flexible)
+    assert approx(c.value + out[5], flexible)
+    assert approx(b.value, -0.079673 - 0.212202)
     host.cgppd_host_simulation_destroy(h)
 
 
