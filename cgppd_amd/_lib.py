@@ -56,6 +56,20 @@ SIGNATURES = {
     "cgppd_last_kernel_ms": (C.c_int, [_ctxp, _f64p, _i64p]),
     "cgppd_model_constants": (C.c_int, [_f32p, _f32p, _f32p]),
     "cgppd_get_sizes": (C.c_int, [_ctxp, _i32p, _i32p, _i32p]),
+    "cgppd_surrogate_param_count": (C.c_int, [_i32p]),
+    "cgppd_surrogate_init": (C.c_int, [_ctxp, C.c_uint64]),
+    "cgppd_surrogate_set_params": (C.c_int, [_ctxp, _f32p]),
+    "cgppd_surrogate_get_params": (C.c_int, [_ctxp, _f32p]),
+    "cgppd_surrogate_predict": (C.c_int, [_ctxp, _f32p, C.c_int, _f32p]),
+    "cgppd_surrogate_gradient": (C.c_int, [_ctxp, _f32p, _f32p, C.c_int, _f32p, _f64p]),
+    "cgppd_surrogate_add_samples": (C.c_int, [_ctxp, _f32p, _f32p, C.c_int]),
+    "cgppd_surrogate_collect": (C.c_int, [_ctxp]),
+    "cgppd_surrogate_sample_count": (C.c_int, [_ctxp, _i64p]),
+    "cgppd_surrogate_clear_samples": (C.c_int, [_ctxp]),
+    "cgppd_surrogate_train": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_float, C.c_uint64, _f64p]),
+    "cgppd_surrogate_features": (C.c_int, [_ctxp, C.c_int, _f32p]),
+    "cgppd_set_delayed_acceptance": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_get_stage1_rejections": (C.c_int, [_ctxp, _i64p]),
 }
 
 _lib = None

@@ -34,6 +34,7 @@
 
 #include "cgppd_hip.h"
 #include "topology.h"
+#include "mlp.h"
 
 namespace {
 
@@ -49,6 +50,8 @@ constexpr uint32_t kRowMask = 0x1Fu << kRowShift;
 constexpr uint32_t kChargedBit = 0x80000000u;
 constexpr uint32_t kTagMC = 0x4D430000u;
 constexpr uint32_t kTagEX = 0x45580000u;
+constexpr uint32_t kTagSG = 0x53470000u;  // surrogate minibatch draws
+constexpr uint32_t kTagDA = 0x44410000u;  // delayed-acceptance stage-one uniforms
 #define CGPPD_MAX_MOLECULES 1024  // molecules per system
 #define CGPPD_MAX_REPLICAS 1024   // replicas per context (LDS item offsets of the pair kernel)
 
@@ -274,6 +277,19 @@ struct PairParams {
     const int *br_off, *br;     // per neighbour entry: residues the flex move rotates (caller order)
     const double *tor_v, *tor_d;   // 20 x 20 x 4 torsion table (kcal/mol, radians)
     double *out_int;            // R * 5 raw internal terms (full energy)
+    // delayed acceptance with the surrogate (da = 0: plain Metropolis)
+    int da;
+    int *s1;                    // R: stage one accepted the pending proposal
+    double *s_dS;               // R: surrogate change of the pending proposal
+    float *smat;                // R * M * M surrogate pair energies of the current state
+    float *srow;                // R * M: surrogate row of the pending proposal
+    float *frames;              // R * M * 9 body frames of the current state
+    float *prop_frame;          // R * 9 body frame of the moved molecule
+    const int2 *anchor;         // M: anchor residues (device indices)
+    const float *comp;          // M * 20 residue-type fractions
+    const float *sp;            // surrogate FP32 masters
+    const unsigned short *swb;  // surrogate BF16 weights
+    long long *s1_rej;          // R: stage-one rejections
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -781,6 +797,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
                 cnt += p.prop_nib[r] * ((n + kJChunk - 1) / kJChunk) + 1;
             }
         }
+        if (p.da && !p.s1[r]) cnt = 0;  // rejected by the surrogate's stage one
         item_off[r + 1] = cnt;
     }
     __syncthreads();
@@ -1082,21 +1099,29 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
             const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
             long long *cnt = p.counters + 3 * (size_t)r;
             int acc = 0;
-            if (delta < 0.0) {
-                acc = 1;
-                cnt[0] += 1;
-            } else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r]))) {
-                acc = 1;
-                cnt[1] += 1;
+            if (p.da && !p.s1[r]) {
+                // rejected at stage one: no exact row was evaluated
+                p.s1_rej[r] += 1;
             } else {
-                cnt[2] += 1;
+                // plain Metropolis, or delayed acceptance stage two with
+                // min(1, exp(-(dE - dS) / kT)) (Christen & Fox 2005)
+                const double d2 = p.da ? delta - p.s_dS[r] : delta;
+                if (d2 < 0.0) {
+                    acc = 1;
+                    cnt[0] += 1;
+                } else if (pr.u < exp(-d2 * 4184.0 / ((double)kRgas * p.temperature[r]))) {
+                    acc = 1;
+                    cnt[1] += 1;
+                } else {
+                    cnt[2] += 1;
+                }
+                const unsigned long long n_m = (unsigned long long)(p.mol_off[m + 1] - p.mol_off[m]);
+                unsigned long long pairs = n_m * ((unsigned long long)p.n_res - n_m);
+                if (p.n_flex && p.mol_flex[m]) pairs += n_m * (n_m - 1) / 2;
+                atomicAdd(p.pair_evals, pairs);
             }
             if (acc) p.energy[r] += delta;
             accept_flag = acc;
-            const unsigned long long n_m = (unsigned long long)(p.mol_off[m + 1] - p.mol_off[m]);
-            unsigned long long pairs = n_m * ((unsigned long long)p.n_res - n_m);
-            if (p.n_flex && p.mol_flex[m]) pairs += n_m * (n_m - 1) / 2;
-            atomicAdd(p.pair_evals, pairs);
         }
         __syncthreads();
         if (accept_flag) {
@@ -1113,6 +1138,16 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
                 rel[k] = p.prop_rel[(size_t)r * p.max_n + k];
             }
             if (threadIdx.x == 0) p.center[(size_t)r * M + m] = p.prop_center[r];
+            if (p.da) {
+                float *sm = p.smat + (size_t)r * M * M;
+                for (int J = threadIdx.x; J < M; J += kWG) {
+                    if (J == m) continue;
+                    const float v = p.srow[(size_t)r * M + J];
+                    sm[(size_t)m * M + J] = v;
+                    sm[(size_t)J * M + m] = v;
+                }
+                if (threadIdx.x < 9) p.frames[((size_t)r * M + m) * 9 + threadIdx.x] = p.prop_frame[(size_t)r * 9 + threadIdx.x];
+            }
         }
         __syncthreads();  // every thread read the row before it is cleared
         for (int J = threadIdx.x; J < M; J += kWG) row[J] = 0.0;
@@ -1223,6 +1258,315 @@ __global__ void fill_zero_kernel(double *p, size_t n)
         p[k] = 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Surrogate: inference, gradients and Adam (mlp.h holds the tile code)
+// ---------------------------------------------------------------------------
+
+// Predictions for rows [32 g, 32 g + 32) of x (BF16 bits, n x 80).
+__global__ __launch_bounds__(mlp::kThreads) void mlp_predict_kernel(const unsigned short *x, int n,
+                                                                    const float *master, const unsigned short *wb,
+                                                                    float *y)
+{
+    __shared__ mlp::Tile T;
+    const int base = blockIdx.x * mlp::kRows;
+    const int nv = min(mlp::kRows, n - base);
+    for (int e = threadIdx.x; e < mlp::kRows * mlp::kFeatPad; e += mlp::kThreads) {
+        const int k = e / mlp::kFeatPad, f = e % mlp::kFeatPad;
+        T.x[e] = (k < nv && f < mlp::kFeat) ? x[(size_t)(base + k) * mlp::kFeat + f] : (unsigned short)0;
+    }
+    __syncthreads();
+    mlp::forward(T, master, wb, nv);
+    if (threadIdx.x < nv) y[base + threadIdx.x] = T.y[threadIdx.x];
+}
+
+// Gradient of 0.5 mean((y - clip(t))^2) over one minibatch, added into grad;
+// loss (mean) added into loss[0].  Each workgroup takes 32 samples of the
+// buffer (sx: BF16 features, st: targets): rows [32 g, 32 g + 32) when
+// sequential, else drawn with replacement from Philox(step, sample, tag).
+__global__ __launch_bounds__(mlp::kThreads) void mlp_grad_kernel(const unsigned short *sx, const float *st,
+                                                                 int n_samples, int sequential, uint64_t seed,
+                                                                 uint64_t step, float batch, const float *master,
+                                                                 const unsigned short *wb, float *grad, double *loss)
+{
+    __shared__ mlp::Tile T;
+    __shared__ float dy[mlp::kRows];
+    __shared__ int idx[mlp::kRows];
+    const int g = blockIdx.x;
+    const int nv = sequential ? min(mlp::kRows, n_samples - g * mlp::kRows) : mlp::kRows;
+    if (threadIdx.x < mlp::kRows) {
+        int i = g * mlp::kRows + threadIdx.x;
+        if (!sequential) {
+            uint32_t w[4];
+            philox((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)i, kTagSG, (uint32_t)seed, (uint32_t)(seed >> 32),
+                   w);
+            i = mulhi_n(w[0], n_samples);
+        }
+        idx[threadIdx.x] = threadIdx.x < nv ? i : 0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < mlp::kRows * mlp::kFeatPad; e += mlp::kThreads) {
+        const int k = e / mlp::kFeatPad, f = e % mlp::kFeatPad;
+        T.x[e] = (k < nv && f < mlp::kFeat) ? sx[(size_t)idx[k] * mlp::kFeat + f] : (unsigned short)0;
+    }
+    __syncthreads();
+    mlp::forward(T, master, wb, nv);
+    if (threadIdx.x < 64) {
+        double l = 0.0;
+        if (threadIdx.x < nv) {
+            const float t = fminf(fmaxf(st[idx[threadIdx.x]], -mlp::kTargetClip), mlp::kTargetClip);
+            const float d = T.y[threadIdx.x] - t;
+            dy[threadIdx.x] = d / batch;
+            l = 0.5 * (double)d * (double)d / (double)batch;
+        } else if (threadIdx.x < mlp::kRows) {
+            dy[threadIdx.x] = 0.f;
+        }
+        l = wave_sum(l);
+        if (threadIdx.x == 0) atomicAdd(loss, l);
+    }
+    __syncthreads();
+    mlp::backward(T, wb, dy, grad);
+}
+
+// Adam on the FP32 masters (Kingma & Ba 2015, algorithm 1); bc1, bc2 are
+// 1 - beta^t.  The gradient is cleared.  No FMA contraction, so the update
+// matches an FP32 CPU restatement operation for operation.
+__global__ void mlp_adam_kernel(float *master, float *m, float *v, float *grad, float lr, float bc1, float bc2)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mlp::kParams) return;
+    const float g = grad[i];
+    const float mi = 0.9f * m[i] + 0.1f * g;
+    const float vi = 0.999f * v[i] + 0.001f * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / bc1, vh = vi / bc2;
+    master[i] = master[i] - lr * mh / (sqrtf(vh) + 1e-8f);
+    grad[i] = 0.f;
+}
+
+// Body frame (rows e1, e2, e3) of a molecule from the relative positions
+// of its two anchor residues; the identity when they coincide or degenerate.
+__device__ inline void make_frame(float4 a, float4 b, bool same, float f[9])
+{
+    for (int k = 0; k < 9; ++k) f[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+    if (same) return;
+    const float n1 = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    if (n1 == 0.0f) return;
+    const float e1x = a.x / n1, e1y = a.y / n1, e1z = a.z / n1;
+    const float d = b.x * e1x + b.y * e1y + b.z * e1z;
+    const float px = b.x - d * e1x, py = b.y - d * e1y, pz = b.z - d * e1z;
+    const float n2 = sqrtf(px * px + py * py + pz * pz);
+    if (n2 == 0.0f) return;
+    const float e2x = px / n2, e2y = py / n2, e2z = pz / n2;
+    f[0] = e1x;
+    f[1] = e1y;
+    f[2] = e1z;
+    f[3] = e2x;
+    f[4] = e2y;
+    f[5] = e2z;
+    f[6] = e1y * e2z - e1z * e2y;
+    f[7] = e1z * e2x - e1x * e2z;
+    f[8] = e1x * e2y - e1y * e2x;
+}
+
+// The 80 features of the pair (I, J), I < J, stored through put(f, value).
+template <typename Put>
+__device__ inline void pair_feature_row(float4 ci, float4 cj, const float *fi, const float *fj, const float *hi,
+                                        const float *hj, float box, float inv_box, Put put)
+{
+    float dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
+    dx -= box * rintf(dx * inv_box);
+    dy -= box * rintf(dy * inv_box);
+    dz -= box * rintf(dz * inv_box);
+    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float ux = dist > 0.f ? dx / dist : 0.f, uy = dist > 0.f ? dy / dist : 0.f,
+                uz = dist > 0.f ? dz / dist : 0.f;
+    for (int k = 0; k < 32; ++k) {
+        const float t = (dist - 3.0f * (float)k) * (1.0f / 3.0f);
+        put(k, expf(-0.5f * t * t));
+    }
+    for (int i = 0; i < 3; ++i) {
+        put(32 + i, fi[3 * i] * ux + fi[3 * i + 1] * uy + fi[3 * i + 2] * uz);
+        put(35 + i, -(fj[3 * i] * ux + fj[3 * i + 1] * uy + fj[3 * i + 2] * uz));
+    }
+    for (int t = 0; t < 20; ++t) {
+        put(38 + t, hi[t]);
+        put(58 + t, hj[t]);
+    }
+    put(78, 0.f);
+    put(79, 0.f);
+}
+
+// Pair q of a replica's M(M-1)/2 pairs in row-major (I, J) order.
+__device__ inline void pair_of(int q, int M, int &I, int &J)
+{
+    I = 0;
+    while (q >= M - 1 - I) {
+        q -= M - 1 - I;
+        ++I;
+    }
+    J = I + 1 + q;
+}
+
+// Body frames of every molecule of every replica (current coordinates).
+__global__ void frames_kernel(PairParams p)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.rep_count * p.n_mol) return;
+    const int r = k / p.n_mol, m = k % p.n_mol;
+    const int2 an = p.anchor[m];
+    const float4 *rel = p.rel + (size_t)r * p.stride;
+    float f[9];
+    make_frame(rel[an.x], rel[an.y], an.x == an.y, f);
+    for (int i = 0; i < 9; ++i) p.frames[(size_t)k * 9 + i] = f[i];
+}
+
+// Features of every pair of replicas [rep_begin, rep_begin + rep_count):
+// BF16 rows into xb (+ targets from the MC matrix into t) or FP32 rows into xf.
+__global__ void pair_features_kernel(PairParams p, unsigned short *xb, float *t, float *xf)
+{
+    const int P = p.n_mol * (p.n_mol - 1) / 2;
+    const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= (long long)p.rep_count * P) return;
+    const int rr = (int)(k / P), q = (int)(k % P), r = p.rep_begin + rr;
+    int I, J;
+    pair_of(q, p.n_mol, I, J);
+    const size_t rm = (size_t)r * p.n_mol;
+    const float *fI = p.frames + (rm + I) * 9, *fJ = p.frames + (rm + J) * 9;
+    if (xb) {
+        unsigned short *row = xb + (size_t)k * mlp::kFeat;
+        pair_feature_row(p.center[rm + I], p.center[rm + J], fI, fJ, p.comp + I * 20, p.comp + J * 20, p.box,
+                         p.inv_box, [&](int f, float v) { row[f] = mlp::f2bf(v); });
+        t[k] = (float)p.emat[(rm + I) * p.n_mol + J];
+    } else {
+        float *row = xf + (size_t)k * mlp::kFeat;
+        pair_feature_row(p.center[rm + I], p.center[rm + J], fI, fJ, p.comp + I * 20, p.comp + J * 20, p.box,
+                         p.inv_box, [&](int f, float v) { row[f] = v; });
+    }
+}
+
+// Surrogate pair matrix of the current state: 32 pairs per workgroup over
+// all replicas' pairs; smat[r][I][J] = smat[r][J][I] = S(features(I, J)).
+__global__ __launch_bounds__(mlp::kThreads) void surrogate_matrix_kernel(PairParams p)
+{
+    __shared__ mlp::Tile T;
+    __shared__ int pr[mlp::kRows], pi[mlp::kRows], pj[mlp::kRows];
+    const int M = p.n_mol, P = M * (M - 1) / 2;
+    const long long base = (long long)blockIdx.x * mlp::kRows, total = (long long)p.rep_count * P;
+    const int nv = (int)min((long long)mlp::kRows, total - base);
+    if (threadIdx.x < mlp::kRows) {
+        const int k = threadIdx.x;
+        unsigned short *row = T.x + k * mlp::kFeatPad;
+        for (int f = mlp::kFeat; f < mlp::kFeatPad; ++f) row[f] = 0;
+        if (k < nv) {
+            const long long g = base + k;
+            const int r = (int)(g / P);
+            int I, J;
+            pair_of((int)(g % P), M, I, J);
+            pr[k] = r;
+            pi[k] = I;
+            pj[k] = J;
+            const size_t rm = (size_t)r * M;
+            pair_feature_row(p.center[rm + I], p.center[rm + J], p.frames + (rm + I) * 9, p.frames + (rm + J) * 9,
+                             p.comp + I * 20, p.comp + J * 20, p.box, p.inv_box,
+                             [&](int f, float v) { row[f] = mlp::f2bf(v); });
+        } else {
+            for (int f = 0; f < mlp::kFeat; ++f) row[f] = 0;
+        }
+    }
+    __syncthreads();
+    mlp::forward(T, p.sp, p.swb, nv);
+    if (threadIdx.x < nv) {
+        const int k = threadIdx.x;
+        const size_t rmm = (size_t)pr[k] * M * M;
+        p.smat[rmm + (size_t)pi[k] * M + pj[k]] = T.y[k];
+        p.smat[rmm + (size_t)pj[k] * M + pi[k]] = T.y[k];
+    }
+}
+
+// Delayed acceptance, stage one (one workgroup per replica): the surrogate
+// row of the pending proposal against every other molecule, its change dS
+// against the current surrogate matrix, and the stage-one test
+// min(1, exp(-dS / kT)) with its own Philox uniform.
+__global__ __launch_bounds__(mlp::kThreads) void stage1_kernel(PairParams p)
+{
+    __shared__ mlp::Tile T;
+    __shared__ float fnew[9];
+    __shared__ double red[mlp::kThreads / 64];
+    const int r = blockIdx.x, M = p.n_mol, m = p.prop_mol[r];
+    if (threadIdx.x == 0) {
+        const int2 an = p.anchor[m];
+        const int i0 = p.mol_off[m];
+        const float4 *pq = p.prop_rel + (size_t)r * p.max_n;
+        make_frame(pq[an.x - i0], pq[an.y - i0], an.x == an.y, fnew);
+        for (int i = 0; i < 9; ++i) p.prop_frame[(size_t)r * 9 + i] = fnew[i];
+    }
+    __syncthreads();
+    const float4 cm = p.prop_center[r];
+    const size_t rm = (size_t)r * M;
+    double acc = 0.0;
+    for (int base = 0; base < M - 1; base += mlp::kRows) {
+        const int nv = min(mlp::kRows, M - 1 - base);
+        if (threadIdx.x < mlp::kRows) {
+            const int k = threadIdx.x;
+            unsigned short *row = T.x + k * mlp::kFeatPad;
+            for (int f = mlp::kFeat; f < mlp::kFeatPad; ++f) row[f] = 0;
+            if (k < nv) {
+                const int q = base + k, J = q < m ? q : q + 1;
+                auto put = [&](int f, float v) { row[f] = mlp::f2bf(v); };
+                if (m < J)
+                    pair_feature_row(cm, p.center[rm + J], fnew, p.frames + (rm + J) * 9, p.comp + m * 20,
+                                     p.comp + J * 20, p.box, p.inv_box, put);
+                else
+                    pair_feature_row(p.center[rm + J], cm, p.frames + (rm + J) * 9, fnew, p.comp + J * 20,
+                                     p.comp + m * 20, p.box, p.inv_box, put);
+            } else {
+                for (int f = 0; f < mlp::kFeat; ++f) row[f] = 0;
+            }
+        }
+        __syncthreads();
+        mlp::forward(T, p.sp, p.swb, nv);
+        if (threadIdx.x < nv) {
+            const int q = base + threadIdx.x, J = q < m ? q : q + 1;
+            const float y = T.y[threadIdx.x];
+            p.srow[rm + J] = y;
+            acc += (double)y - (double)p.smat[(rm + m) * M + J];
+        }
+        __syncthreads();  // the next tile reuses T
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double dS = 0.0;
+        for (int w = 0; w < mlp::kThreads / 64; ++w) dS += red[w];
+        const uint64_t step = (uint64_t)p.steps[r];
+        uint32_t w4[4];
+        philox((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)r, kTagDA, (uint32_t)p.seed, (uint32_t)(p.seed >> 32),
+               w4);
+        const double u1 = unit53(w4[0], w4[1]);
+        const double beta = 4184.0 / ((double)kRgas * p.temperature[r]);
+        p.s1[r] = (dS < 0.0 || u1 < exp(-dS * beta)) ? 1 : 0;
+        p.s_dS[r] = dS;
+    }
+}
+
+// BF16 copies of the weights for the matrix products (transposed, padded).
+__global__ void mlp_pack_kernel(const float *master, unsigned short *wb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < mlp::kHidden * mlp::kFeatPad) {
+        const int n = i / mlp::kFeatPad, k = i % mlp::kFeatPad;
+        wb[mlp::kBW1 + i] = k < mlp::kFeat ? mlp::f2bf(master[mlp::kOW1 + k * mlp::kHidden + n]) : (unsigned short)0;
+    } else if (i < mlp::kBw3) {
+        const int j = i - mlp::kBW2, n = j / mlp::kHidden, k = j % mlp::kHidden;
+        wb[i] = mlp::f2bf(master[mlp::kOW2 + k * mlp::kHidden + n]);
+    } else if (i < mlp::kBf16Params) {
+        wb[i] = mlp::f2bf(master[mlp::kOw3 + (i - mlp::kBw3)]);
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1286,6 +1630,29 @@ struct cgppd_ctx {
     double *d_tor = nullptr;      // 2 x 1600: V then delta
     double *d_out_int = nullptr;  // R * 5
     double *d_out_scratch = nullptr;  // 2R: LJ/DH of internal-only queries
+
+    // surrogate: FP32 masters, Adam moments, gradient, packed BF16 weights,
+    // the sample buffer (BF16 features, FP32 targets) and a scratch batch
+    bool have_sur = false;
+    long long adam_t = 0;
+    float *d_sp = nullptr, *d_sm = nullptr, *d_sv = nullptr, *d_sg = nullptr;
+    unsigned short *d_swb = nullptr;
+    double *d_sloss = nullptr;
+    unsigned short *d_sx = nullptr;
+    float *d_st = nullptr;
+    size_t s_cap = 0, s_n = 0;
+    unsigned short *d_tx = nullptr;
+    float *d_tt = nullptr, *d_ty = nullptr;
+    size_t t_cap = 0;
+    // delayed acceptance: anchors and composition of every molecule (set with
+    // the replicas), the surrogate pair matrix, frames and stage-one state
+    int2 *d_anchor = nullptr;
+    float *d_comp = nullptr;
+    bool da = false, smat_dirty = true;
+    float *d_smat = nullptr, *d_srow = nullptr, *d_frames = nullptr, *d_prop_frame = nullptr;
+    int *d_s1 = nullptr;
+    double *d_s_dS = nullptr;
+    long long *d_s1_rej = nullptr;
 
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -1391,6 +1758,17 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_out_mat);
     dev_free(ctx->d_out_int);
     dev_free(ctx->d_out_scratch);
+    dev_free(ctx->d_anchor);
+    dev_free(ctx->d_comp);
+    dev_free(ctx->d_smat);
+    dev_free(ctx->d_srow);
+    dev_free(ctx->d_frames);
+    dev_free(ctx->d_prop_frame);
+    dev_free(ctx->d_s1);
+    dev_free(ctx->d_s_dS);
+    dev_free(ctx->d_s1_rej);
+    ctx->da = false;
+    ctx->smat_dirty = true;
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
     ctx->h_out = nullptr;
     ctx->have_rep = false;
@@ -1684,7 +2062,94 @@ PairParams base_params(cgppd_ctx *ctx)
     p.tor_v = ctx->d_tor;
     p.tor_d = ctx->d_tor ? ctx->d_tor + 1600 : nullptr;
     p.out_int = ctx->d_out_int;
+    p.da = ctx->da ? 1 : 0;
+    p.s1 = ctx->d_s1;
+    p.s_dS = ctx->d_s_dS;
+    p.smat = ctx->d_smat;
+    p.srow = ctx->d_srow;
+    p.frames = ctx->d_frames;
+    p.prop_frame = ctx->d_prop_frame;
+    p.anchor = ctx->d_anchor;
+    p.comp = ctx->d_comp;
+    p.sp = ctx->d_sp;
+    p.swb = ctx->d_swb;
+    p.s1_rej = ctx->d_s1_rej;
     return p;
+}
+
+// Body frames of every molecule of every replica into d_frames.
+int launch_frames(cgppd_ctx *ctx)
+{
+    PairParams p = base_params(ctx);
+    const int n = ctx->R * ctx->M;
+    hipLaunchKernelGGL(frames_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, p);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+// Frames and the surrogate pair matrix of the current state.
+int refresh_surrogate_state(cgppd_ctx *ctx)
+{
+    int rc = launch_frames(ctx);
+    if (rc) return rc;
+    const long long pairs = (long long)ctx->R * ctx->M * (ctx->M - 1) / 2;
+    CG_HIP(ctx, hipMemsetAsync(ctx->d_smat, 0, sizeof(float) * (size_t)ctx->R * ctx->M * ctx->M, ctx->stream));
+    if (pairs > 0) {
+        PairParams p = base_params(ctx);
+        hipLaunchKernelGGL(surrogate_matrix_kernel, dim3((unsigned)((pairs + mlp::kRows - 1) / mlp::kRows)),
+                           dim3(mlp::kThreads), 0, ctx->stream, p);
+        CG_HIP(ctx, hipGetLastError());
+    }
+    ctx->smat_dirty = false;
+    return CGPPD_OK;
+}
+
+// Anchor residues (device indices) and residue-type fractions of every
+// molecule, from the relative positions rel0 (caller order, one replica):
+// anchor a is the residue farthest from the centre (first of equals), b the
+// one farthest from the line through a; molecules of fewer than 3 residues
+// use their first residue twice.
+int set_anchors(cgppd_ctx *ctx, const float *rel0)
+{
+    const int M = ctx->M;
+    std::vector<int2> an(M);
+    std::vector<float> comp((size_t)M * 20, 0.0f);
+    for (int m = 0; m < M; ++m) {
+        const int s0 = ctx->mol_off[m], n = ctx->mol_off[m + 1] - s0;
+        const float *x = rel0 + (size_t)s0 * 3;
+        int a = 0, b = 0;
+        if (n >= 3) {
+            double best = -1.0;
+            for (int k = 0; k < n; ++k) {
+                const double v[3] = {x[3 * k], x[3 * k + 1], x[3 * k + 2]};
+                const double d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (d > best) {
+                    best = d;
+                    a = k;
+                }
+            }
+            const double ax = x[3 * a], ay = x[3 * a + 1], az = x[3 * a + 2];
+            const double na = std::max(std::sqrt(ax * ax + ay * ay + az * az), 1e-30);
+            const double e[3] = {ax / na, ay / na, az / na};
+            best = -1.0;
+            for (int k = 0; k < n; ++k) {
+                const double v[3] = {x[3 * k], x[3 * k + 1], x[3 * k + 2]};
+                const double d = v[0] * e[0] + v[1] * e[1] + v[2] * e[2];
+                const double px = v[0] - d * e[0], py = v[1] - d * e[1], pz = v[2] - d * e[2];
+                const double q = px * px + py * py + pz * pz;
+                if (q > best) {
+                    best = q;
+                    b = k;
+                }
+            }
+        }
+        an[m] = make_int2(ctx->inv[s0 + a], ctx->inv[s0 + b]);
+        for (int k = 0; k < n; ++k) comp[(size_t)m * 20 + ctx->type[s0 + k]] += 1.0f;
+        for (int t = 0; t < 20; ++t) comp[(size_t)m * 20 + t] = (float)((double)comp[(size_t)m * 20 + t] / (double)n);
+    }
+    int rc;
+    if ((rc = upload_vec(ctx, &ctx->d_anchor, an))) return rc;
+    return upload_vec(ctx, &ctx->d_comp, comp);
 }
 
 int grid_for(const cgppd_ctx *ctx, long long items, bool xcd)
@@ -1808,9 +2273,20 @@ int launch_finish(cgppd_ctx *ctx, int propose_only)
     return CGPPD_OK;
 }
 
+int launch_stage1(cgppd_ctx *ctx)
+{
+    if (!ctx->da) return CGPPD_OK;
+    PairParams p = base_params(ctx);
+    hipLaunchKernelGGL(stage1_kernel, dim3(ctx->R), dim3(mlp::kThreads), 0, ctx->stream, p);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
 int launch_mc_step(cgppd_ctx *ctx)
 {
-    int rc = launch_pair(ctx);
+    int rc = launch_stage1(ctx);
+    if (rc) return rc;
+    rc = launch_pair(ctx);
     if (rc) return rc;
     return launch_finish(ctx, 0);
 }
@@ -1883,6 +2359,23 @@ int cgppd_create(cgppd_ctx **out, int device)
     return CGPPD_OK;
 }
 
+static void free_surrogate(cgppd_ctx *ctx)
+{
+    dev_free(ctx->d_sp);
+    dev_free(ctx->d_sm);
+    dev_free(ctx->d_sv);
+    dev_free(ctx->d_sg);
+    dev_free(ctx->d_swb);
+    dev_free(ctx->d_sloss);
+    dev_free(ctx->d_sx);
+    dev_free(ctx->d_st);
+    dev_free(ctx->d_tx);
+    dev_free(ctx->d_tt);
+    dev_free(ctx->d_ty);
+    ctx->s_cap = ctx->s_n = ctx->t_cap = 0;
+    ctx->have_sur = false;
+}
+
 int cgppd_destroy(cgppd_ctx *ctx)
 {
     if (!ctx) return CGPPD_E_ARG;
@@ -1896,6 +2389,18 @@ int cgppd_destroy(cgppd_ctx *ctx)
     dev_free(ctx->d_run_of);
     dev_free(ctx->d_run_start);
     dev_free(ctx->d_run_code);
+    for (int **q : {&ctx->d_flex_mols, &ctx->d_flex_item_off, &ctx->d_mol_flex, &ctx->d_bond_off, &ctx->d_angle_off,
+                    &ctx->d_tors_off, &ctx->d_ind_off, &ctx->d_local_off, &ctx->d_local, &ctx->d_crank_off,
+                    &ctx->d_flexr_off, &ctx->d_flexr, &ctx->d_nb_off, &ctx->d_br_off, &ctx->d_br})
+        dev_free(*q);
+    dev_free(ctx->d_rtag);
+    dev_free(ctx->d_bonds);
+    dev_free(ctx->d_ind);
+    dev_free(ctx->d_angles);
+    dev_free(ctx->d_tors);
+    dev_free(ctx->d_crank);
+    dev_free(ctx->d_tor);
+    free_surrogate(ctx);
     for (hipEvent_t ev : ctx->events) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2191,6 +2696,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     CG_HIP(ctx, hipMemset(ctx->d_ex_stats, 0, sizeof(unsigned long long) * 2));
     CG_HIP(ctx, hipMemset(ctx->d_pair_evals, 0, sizeof(unsigned long long)));
     CG_HIP(ctx, hipMemset(ctx->d_out, 0, sizeof(double) * 2 * R));
+    if ((rc = set_anchors(ctx, rel))) return rc;
     ctx->have_rep = true;
     ctx->mc_ready = false;
     return CGPPD_OK;
@@ -2215,6 +2721,7 @@ static int upload_molecules(cgppd_ctx *ctx, int replica, int m0, int m1, const f
                                ctx->stream));
     CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->mc_ready = false;
+    ctx->smat_dirty = true;
     return CGPPD_OK;
 }
 
@@ -2352,6 +2859,7 @@ int cgppd_mc_prepare(cgppd_ctx *ctx)
     if ((rc = launch_finish(ctx, 1))) return rc;
     CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->mc_ready = true;
+    ctx->smat_dirty = true;
     return CGPPD_OK;
 }
 
@@ -2372,9 +2880,11 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
     if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before MC steps");
     if (n_steps < 0) return fail(ctx, CGPPD_E_ARG, "negative step count");
     CG_HIP(ctx, hipSetDevice(ctx->device));
+    if (ctx->da && ctx->smat_dirty && (rc = refresh_surrogate_state(ctx))) return rc;
     if (ctx->timing && n_steps > 0) {
         if ((rc = ensure_events(ctx, (size_t)2 * n_steps))) return rc;
         for (int k = 0; k < n_steps; ++k) {
+            if ((rc = launch_stage1(ctx))) return rc;
             if ((rc = launch_pair(ctx, ctx->events[2 * k], ctx->events[2 * k + 1]))) return rc;
             if ((rc = launch_finish(ctx, 0))) return rc;
         }
@@ -2410,6 +2920,7 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
     if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before running");
     if (n_intervals < 0 || steps_per_interval < 0) return fail(ctx, CGPPD_E_ARG, "negative counts");
     CG_HIP(ctx, hipSetDevice(ctx->device));
+    if (ctx->da && ctx->smat_dirty && (rc = refresh_surrogate_state(ctx))) return rc;
     if (!use_graph) {
         for (int k = 0; k < n_intervals; ++k) {
             if ((rc = cgppd_mc_steps(ctx, steps_per_interval))) return rc;
@@ -2559,4 +3070,330 @@ int cgppd_get_sizes(const cgppd_ctx *ctx, int *n_mol, int *n_res, int *n_rep)
     return CGPPD_OK;
 }
 
-}  // extern "C"
+// ---------------------------------------------------------------------------
+// Surrogate
+// ---------------------------------------------------------------------------
+static int ensure_frames(cgppd_ctx *ctx)
+{
+    if (ctx->d_frames) return CGPPD_OK;
+    return dev_alloc(ctx, &ctx->d_frames, (size_t)ctx->R * ctx->M * 9);
+}
+
+static int ensure_surrogate(cgppd_ctx *ctx)
+{
+    if (ctx->have_sur) return CGPPD_OK;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_sp, mlp::kParams)) || (rc = dev_alloc(ctx, &ctx->d_sm, mlp::kParams)) ||
+        (rc = dev_alloc(ctx, &ctx->d_sv, mlp::kParams)) || (rc = dev_alloc(ctx, &ctx->d_sg, mlp::kParams)) ||
+        (rc = dev_alloc(ctx, &ctx->d_swb, mlp::kBf16Params)) || (rc = dev_alloc(ctx, &ctx->d_sloss, 1)))
+        return rc;
+    CG_HIP(ctx, hipMemset(ctx->d_sp, 0, sizeof(float) * mlp::kParams));
+    CG_HIP(ctx, hipMemset(ctx->d_sm, 0, sizeof(float) * mlp::kParams));
+    CG_HIP(ctx, hipMemset(ctx->d_sv, 0, sizeof(float) * mlp::kParams));
+    CG_HIP(ctx, hipMemset(ctx->d_sg, 0, sizeof(float) * mlp::kParams));
+    CG_HIP(ctx, hipMemset(ctx->d_swb, 0, sizeof(unsigned short) * mlp::kBf16Params));
+    ctx->have_sur = true;
+    ctx->adam_t = 0;
+    return CGPPD_OK;
+}
+
+static int pack_weights(cgppd_ctx *ctx)
+{
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((mlp::kBf16Params + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sp,
+                       ctx->d_swb);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+// scratch rows for host-provided batches (features as BF16 bits)
+static int stage_rows(cgppd_ctx *ctx, const float *x, const float *t, int n)
+{
+    if ((size_t)n > ctx->t_cap) {
+        int rc;
+        const size_t cap = std::max<size_t>((size_t)n, 4096);
+        if ((rc = dev_alloc(ctx, &ctx->d_tx, cap * mlp::kFeat)) || (rc = dev_alloc(ctx, &ctx->d_tt, cap)) ||
+            (rc = dev_alloc(ctx, &ctx->d_ty, cap)))
+            return rc;
+        ctx->t_cap = cap;
+    }
+    std::vector<unsigned short> hx((size_t)n * mlp::kFeat);
+    for (size_t i = 0; i < hx.size(); ++i) hx[i] = mlp::f2bf(x[i]);
+    CG_HIP(ctx, hipMemcpy(ctx->d_tx, hx.data(), sizeof(unsigned short) * hx.size(), hipMemcpyHostToDevice));
+    if (t) CG_HIP(ctx, hipMemcpy(ctx->d_tt, t, sizeof(float) * n, hipMemcpyHostToDevice));
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_set_params(cgppd_ctx *ctx, const float *params)
+{
+    if (!ctx || !params) return CGPPD_E_ARG;
+    int rc = ensure_surrogate(ctx);
+    if (rc) return rc;
+    CG_HIP(ctx, hipMemcpy(ctx->d_sp, params, sizeof(float) * mlp::kParams, hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemset(ctx->d_sm, 0, sizeof(float) * mlp::kParams));
+    CG_HIP(ctx, hipMemset(ctx->d_sv, 0, sizeof(float) * mlp::kParams));
+    CG_HIP(ctx, hipMemset(ctx->d_sg, 0, sizeof(float) * mlp::kParams));
+    ctx->adam_t = 0;
+    if ((rc = pack_weights(ctx))) return rc;
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->smat_dirty = true;
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_init(cgppd_ctx *ctx, uint64_t seed)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    // He-normal weights (Box-Muller over Philox uniforms), zero biases
+    std::vector<float> p(mlp::kParams, 0.0f);
+    auto normal = [&](uint32_t i) {
+        uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)}, w[4];
+        uint32_t c0 = i, c1 = 0, c2 = 0, c3 = kTagSG | 0xFFFFu;
+        for (int round = 0; round < 10; ++round) {
+            const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+            const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ key[0], n2 = (uint32_t)(p0 >> 32) ^ c3 ^ key[1];
+            c0 = n0;
+            c1 = (uint32_t)p1;
+            c2 = n2;
+            c3 = (uint32_t)p0;
+            key[0] += 0x9E3779B9u;
+            key[1] += 0xBB67AE85u;
+        }
+        w[0] = c0;
+        w[1] = c1;
+        const double u1 = ((double)w[0] + 1.0) / 4294967297.0, u2 = (double)w[1] / 4294967296.0;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int i = 0; i < mlp::kFeat * mlp::kHidden; ++i)
+        p[mlp::kOW1 + i] = (float)(normal(i) * std::sqrt(2.0 / mlp::kFeat));
+    for (int i = 0; i < mlp::kHidden * mlp::kHidden; ++i)
+        p[mlp::kOW2 + i] = (float)(normal(mlp::kOW2 + i) * std::sqrt(2.0 / mlp::kHidden));
+    for (int i = 0; i < mlp::kHidden; ++i) p[mlp::kOw3 + i] = (float)(normal(mlp::kOw3 + i) * std::sqrt(1.0 / mlp::kHidden));
+    return cgppd_surrogate_set_params(ctx, p.data());
+}
+
+int cgppd_surrogate_get_params(cgppd_ctx *ctx, float *params)
+{
+    if (!ctx || !params) return CGPPD_E_ARG;
+    if (!ctx->have_sur) return fail(ctx, CGPPD_E_STATE, "surrogate not initialised");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CG_HIP(ctx, hipMemcpy(params, ctx->d_sp, sizeof(float) * mlp::kParams, hipMemcpyDeviceToHost));
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_param_count(int *count)
+{
+    if (!count) return CGPPD_E_ARG;
+    *count = mlp::kParams;
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_predict(cgppd_ctx *ctx, const float *x, int n, float *y)
+{
+    if (!ctx || !x || !y || n < 0) return CGPPD_E_ARG;
+    if (!ctx->have_sur) return fail(ctx, CGPPD_E_STATE, "surrogate not initialised");
+    if (n == 0) return CGPPD_OK;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = stage_rows(ctx, x, nullptr, n);
+    if (rc) return rc;
+    const int g = (n + mlp::kRows - 1) / mlp::kRows;
+    hipLaunchKernelGGL(mlp_predict_kernel, dim3(g), dim3(mlp::kThreads), 0, ctx->stream, ctx->d_tx, n, ctx->d_sp,
+                       ctx->d_swb, ctx->d_ty);
+    CG_HIP(ctx, hipGetLastError());
+    CG_HIP(ctx, hipMemcpyAsync(y, ctx->d_ty, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_gradient(cgppd_ctx *ctx, const float *x, const float *t, int n, float *grad, double *loss)
+{
+    if (!ctx || !x || !t || !grad || n < 1) return CGPPD_E_ARG;
+    if (!ctx->have_sur) return fail(ctx, CGPPD_E_STATE, "surrogate not initialised");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = stage_rows(ctx, x, t, n);
+    if (rc) return rc;
+    CG_HIP(ctx, hipMemsetAsync(ctx->d_sg, 0, sizeof(float) * mlp::kParams, ctx->stream));
+    CG_HIP(ctx, hipMemsetAsync(ctx->d_sloss, 0, sizeof(double), ctx->stream));
+    const int g = (n + mlp::kRows - 1) / mlp::kRows;
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(g), dim3(mlp::kThreads), 0, ctx->stream, ctx->d_tx, ctx->d_tt, n, 1,
+                       (uint64_t)0, (uint64_t)0, (float)n, ctx->d_sp, ctx->d_swb, ctx->d_sg, ctx->d_sloss);
+    CG_HIP(ctx, hipGetLastError());
+    CG_HIP(ctx, hipMemcpyAsync(grad, ctx->d_sg, sizeof(float) * mlp::kParams, hipMemcpyDeviceToHost, ctx->stream));
+    double l = 0.0;
+    CG_HIP(ctx, hipMemcpyAsync(&l, ctx->d_sloss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CG_HIP(ctx, hipMemsetAsync(ctx->d_sg, 0, sizeof(float) * mlp::kParams, ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (loss) *loss = l;
+    return CGPPD_OK;
+}
+
+static int reserve_samples(cgppd_ctx *ctx, size_t need)
+{
+    if (need <= ctx->s_cap) return CGPPD_OK;
+    const size_t cap = std::max(need, std::max<size_t>(ctx->s_cap * 2, (size_t)1 << 16));
+    unsigned short *nx = nullptr;
+    float *nt = nullptr;
+    CG_HIP(ctx, hipMalloc((void **)&nx, sizeof(unsigned short) * cap * mlp::kFeat));
+    CG_HIP(ctx, hipMalloc((void **)&nt, sizeof(float) * cap));
+    if (ctx->s_n) {
+        CG_HIP(ctx, hipMemcpyAsync(nx, ctx->d_sx, sizeof(unsigned short) * ctx->s_n * mlp::kFeat,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        CG_HIP(ctx, hipMemcpyAsync(nt, ctx->d_st, sizeof(float) * ctx->s_n, hipMemcpyDeviceToDevice, ctx->stream));
+        CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    dev_free(ctx->d_sx);
+    dev_free(ctx->d_st);
+    ctx->d_sx = nx;
+    ctx->d_st = nt;
+    ctx->s_cap = cap;
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_add_samples(cgppd_ctx *ctx, const float *x, const float *t, int n)
+{
+    if (!ctx || !x || !t || n < 0) return CGPPD_E_ARG;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = reserve_samples(ctx, ctx->s_n + (size_t)n);
+    if (rc) return rc;
+    std::vector<unsigned short> hx((size_t)n * mlp::kFeat);
+    for (size_t i = 0; i < hx.size(); ++i) hx[i] = mlp::f2bf(x[i]);
+    CG_HIP(ctx, hipMemcpy(ctx->d_sx + ctx->s_n * mlp::kFeat, hx.data(), sizeof(unsigned short) * hx.size(),
+                          hipMemcpyHostToDevice));
+    CG_HIP(ctx, hipMemcpy(ctx->d_st + ctx->s_n, t, sizeof(float) * n, hipMemcpyHostToDevice));
+    ctx->s_n += n;
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_sample_count(cgppd_ctx *ctx, int64_t *count)
+{
+    if (!ctx || !count) return CGPPD_E_ARG;
+    *count = (int64_t)ctx->s_n;
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_clear_samples(cgppd_ctx *ctx)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    ctx->s_n = 0;
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_train(cgppd_ctx *ctx, int steps, int batch, float lr, uint64_t seed, double *mean_loss)
+{
+    if (!ctx || steps < 0 || batch < mlp::kRows || batch % mlp::kRows) return fail(ctx, CGPPD_E_ARG, "batch must be a positive multiple of 32");
+    if (!ctx->have_sur) return fail(ctx, CGPPD_E_STATE, "surrogate not initialised");
+    if (ctx->s_n == 0) return fail(ctx, CGPPD_E_STATE, "no training samples");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipMemsetAsync(ctx->d_sloss, 0, sizeof(double), ctx->stream));
+    const int g = batch / mlp::kRows;
+    for (int k = 0; k < steps; ++k) {
+        const uint64_t t = (uint64_t)++ctx->adam_t;
+        hipLaunchKernelGGL(mlp_grad_kernel, dim3(g), dim3(mlp::kThreads), 0, ctx->stream, ctx->d_sx, ctx->d_st,
+                           (int)ctx->s_n, 0, seed, t, (float)batch, ctx->d_sp, ctx->d_swb, ctx->d_sg, ctx->d_sloss);
+        CG_HIP(ctx, hipGetLastError());
+        const float bc1 = (float)(1.0 - std::pow(0.9, (double)t)), bc2 = (float)(1.0 - std::pow(0.999, (double)t));
+        hipLaunchKernelGGL(mlp_adam_kernel, dim3((mlp::kParams + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sp,
+                           ctx->d_sm, ctx->d_sv, ctx->d_sg, lr, bc1, bc2);
+        CG_HIP(ctx, hipGetLastError());
+        int rc = pack_weights(ctx);
+        if (rc) return rc;
+    }
+    double l = 0.0;
+    CG_HIP(ctx, hipMemcpyAsync(&l, ctx->d_sloss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (mean_loss) *mean_loss = steps ? l / steps : 0.0;
+    ctx->smat_dirty = true;
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_collect(cgppd_ctx *ctx)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before collecting samples");
+    const int M = ctx->M, R = ctx->R;
+    const size_t P = (size_t)M * (M - 1) / 2, n = P * R;
+    if (n == 0) return CGPPD_OK;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_frames(ctx))) return rc;
+    if ((rc = reserve_samples(ctx, ctx->s_n + n))) return rc;
+    if ((rc = launch_frames(ctx))) return rc;
+    PairParams p = base_params(ctx);
+    hipLaunchKernelGGL(pair_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p,
+                       ctx->d_sx + ctx->s_n * mlp::kFeat, ctx->d_st + ctx->s_n, (float *)nullptr);
+    CG_HIP(ctx, hipGetLastError());
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->s_n += n;
+    return CGPPD_OK;
+}
+
+int cgppd_surrogate_features(cgppd_ctx *ctx, int replica, float *x)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (replica < 0 || replica >= ctx->R || !x) return fail(ctx, CGPPD_E_ARG, "bad replica or null output");
+    const int M = ctx->M;
+    const size_t P = (size_t)M * (M - 1) / 2;
+    if (P == 0) return CGPPD_OK;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_frames(ctx))) return rc;
+    if ((rc = launch_frames(ctx))) return rc;
+    float *d = nullptr;
+    CG_HIP(ctx, hipMalloc((void **)&d, sizeof(float) * P * mlp::kFeat));
+    PairParams p = base_params(ctx);
+    p.rep_begin = replica;
+    p.rep_count = 1;
+    hipLaunchKernelGGL(pair_features_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, ctx->stream, p,
+                       (unsigned short *)nullptr, (float *)nullptr, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(x, d, sizeof(float) * P * mlp::kFeat, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, CGPPD_E_HIP, "surrogate features: %s", hipGetErrorString(e));
+    return CGPPD_OK;
+}
+
+int cgppd_set_delayed_acceptance(cgppd_ctx *ctx, int enable)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    destroy_graph(ctx);
+    if (!enable) {
+        ctx->da = false;
+        return CGPPD_OK;
+    }
+    if (!ctx->have_sur) return fail(ctx, CGPPD_E_STATE, "initialise or train the surrogate first");
+    const size_t R = ctx->R, M = ctx->M;
+    if (!ctx->d_smat) {
+        if ((rc = dev_alloc(ctx, &ctx->d_smat, R * M * M)) || (rc = dev_alloc(ctx, &ctx->d_srow, R * M)) ||
+            (rc = dev_alloc(ctx, &ctx->d_prop_frame, R * 9)) || (rc = dev_alloc(ctx, &ctx->d_s1, R)) ||
+            (rc = dev_alloc(ctx, &ctx->d_s_dS, R)) || (rc = dev_alloc(ctx, &ctx->d_s1_rej, R)))
+            return rc;
+        CG_HIP(ctx, hipMemset(ctx->d_s1_rej, 0, sizeof(long long) * R));
+        CG_HIP(ctx, hipMemset(ctx->d_s1, 0, sizeof(int) * R));
+    }
+    if ((rc = ensure_frames(ctx))) return rc;
+    ctx->da = true;
+    ctx->smat_dirty = true;
+    return CGPPD_OK;
+}
+
+int cgppd_get_stage1_rejections(cgppd_ctx *ctx, int64_t *rejections)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!rejections) return fail(ctx, CGPPD_E_ARG, "null output");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx->d_s1_rej) {
+        for (int r = 0; r < ctx->R; ++r) rejections[r] = 0;
+        return CGPPD_OK;
+    }
+    CG_HIP(ctx, hipMemcpy(rejections, ctx->d_s1_rej, sizeof(long long) * ctx->R, hipMemcpyDeviceToHost));
+    return CGPPD_OK;
+}
+
+}  // extern "C""

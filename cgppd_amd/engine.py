@@ -205,6 +205,90 @@ class Engine:
         self._check(self.lib.cgppd_get_energy_matrix(self.h, int(r), ptr(mat, C.c_double)), "get_energy_matrix")
         return mat
 
+    # surrogate ------------------------------------------------------------
+    PARAM_SHAPES = (("W1", (80, 128)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)), ("w3", (128,)),
+                    ("b3", (1,)))
+
+    @classmethod
+    def flatten_params(cls, p):
+        return np.concatenate([np.asarray(p[k], dtype=np.float32).reshape(-1) for k, _ in cls.PARAM_SHAPES])
+
+    @classmethod
+    def unflatten_params(cls, flat):
+        out, o = {}, 0
+        for k, shape in cls.PARAM_SHAPES:
+            n = int(np.prod(shape))
+            out[k] = np.asarray(flat[o:o + n], dtype=np.float32).reshape(shape)
+            o += n
+        return out
+
+    def surrogate_init(self, seed=0):
+        self._check(self.lib.cgppd_surrogate_init(self.h, C.c_uint64(int(seed))), "surrogate_init")
+
+    def surrogate_set_params(self, params):
+        flat = f32(self.flatten_params(params) if isinstance(params, dict) else params)
+        self._check(self.lib.cgppd_surrogate_set_params(self.h, ptr(flat, C.c_float)), "surrogate_set_params")
+
+    def surrogate_params(self):
+        n = C.c_int()
+        self.lib.cgppd_surrogate_param_count(C.byref(n))
+        flat = np.zeros(n.value, dtype=np.float32)
+        self._check(self.lib.cgppd_surrogate_get_params(self.h, ptr(flat, C.c_float)), "surrogate_get_params")
+        return self.unflatten_params(flat)
+
+    def surrogate_predict(self, x):
+        xx = f32(x).reshape(-1, 80)
+        y = np.zeros(len(xx), dtype=np.float32)
+        self._check(self.lib.cgppd_surrogate_predict(self.h, ptr(xx, C.c_float), len(xx), ptr(y, C.c_float)),
+                    "surrogate_predict")
+        return y
+
+    def surrogate_gradient(self, x, t):
+        xx, tt = f32(x).reshape(-1, 80), f32(t)
+        n = C.c_int()
+        self.lib.cgppd_surrogate_param_count(C.byref(n))
+        g = np.zeros(n.value, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.cgppd_surrogate_gradient(self.h, ptr(xx, C.c_float), ptr(tt, C.c_float), len(xx),
+                                                      ptr(g, C.c_float), C.byref(loss)), "surrogate_gradient")
+        return loss.value, self.unflatten_params(g)
+
+    def surrogate_add_samples(self, x, t):
+        xx, tt = f32(x).reshape(-1, 80), f32(t)
+        self._check(self.lib.cgppd_surrogate_add_samples(self.h, ptr(xx, C.c_float), ptr(tt, C.c_float), len(xx)),
+                    "surrogate_add_samples")
+
+    def surrogate_collect(self):
+        self._check(self.lib.cgppd_surrogate_collect(self.h), "surrogate_collect")
+
+    def surrogate_sample_count(self):
+        n = C.c_int64()
+        self._check(self.lib.cgppd_surrogate_sample_count(self.h, C.byref(n)), "surrogate_sample_count")
+        return n.value
+
+    def surrogate_clear_samples(self):
+        self._check(self.lib.cgppd_surrogate_clear_samples(self.h), "surrogate_clear_samples")
+
+    def surrogate_train(self, steps, batch=1024, lr=1e-3, seed=0):
+        loss = C.c_double()
+        self._check(self.lib.cgppd_surrogate_train(self.h, int(steps), int(batch), float(lr), C.c_uint64(int(seed)),
+                                                   C.byref(loss)), "surrogate_train")
+        return loss.value
+
+    def surrogate_features(self, r):
+        M = self.n_mol
+        x = np.zeros((M * (M - 1) // 2, 80), dtype=np.float32)
+        self._check(self.lib.cgppd_surrogate_features(self.h, int(r), ptr(x, C.c_float)), "surrogate_features")
+        return x
+
+    def set_delayed_acceptance(self, on):
+        self._check(self.lib.cgppd_set_delayed_acceptance(self.h, int(bool(on))), "set_delayed_acceptance")
+
+    def stage1_rejections(self):
+        out = np.zeros(self.n_rep, dtype=np.int64)
+        self._check(self.lib.cgppd_get_stage1_rejections(self.h, ptr(out, C.c_int64)), "get_stage1_rejections")
+        return out
+
     def set_launch(self, wg_per_cu):
         self._check(self.lib.cgppd_set_launch(self.h, int(wg_per_cu)), "set_launch")
 

@@ -233,6 +233,50 @@ int cgppd_model_constants(float *rt_to_kcal, float *dh_factor, float *r0_factor)
 /* Sizes of the current system: n_mol, n_res, n_rep (any NULL). */
 int cgppd_get_sizes(const cgppd_ctx *ctx, int *n_mol, int *n_res, int *n_rep);
 
+/* Learned surrogate of the pair energy (new: no reference counterpart) ---- *
+ * A molecule-pair MLP (80 features -> 128 -> 128 -> 1, SiLU) trained and
+ * evaluated by the library's BF16 MFMA kernels with FP32 accumulation; it
+ * drives stage one of a delayed-acceptance Metropolis test whose stage two
+ * uses the exact pair kernel.  Features of a pair I < J: 32 Gaussian RBFs
+ * of the minimum-image centre distance (centres 0, 3, ..., 93 A, width 3 A),
+ * the unit centre-to-centre vector in each molecule's body frame (2 x 3),
+ * each molecule's residue-type fractions (2 x 20), two zeros.
+ * Parameters (FP32, cgppd_surrogate_param_count of them): W1[80][128], b1[128],
+ * W2[128][128], b2[128], w3[128], b3. */
+int cgppd_surrogate_param_count(int *count);
+int cgppd_surrogate_init(cgppd_ctx *ctx, uint64_t seed);  /* He-normal weights, zero biases */
+int cgppd_surrogate_set_params(cgppd_ctx *ctx, const float *params);  /* resets Adam */
+int cgppd_surrogate_get_params(cgppd_ctx *ctx, float *params);
+/* Predictions y[n] for feature rows x[n*80] (rounded to BF16 on entry). */
+int cgppd_surrogate_predict(cgppd_ctx *ctx, const float *x, int n, float *y);
+/* Gradient of 0.5 mean((y - clip(t, -15, 15))^2) over the rows x[n*80],
+ * t[n] (parameter layout as above) and the loss; parameters unchanged. */
+int cgppd_surrogate_gradient(cgppd_ctx *ctx, const float *x, const float *t, int n, float *grad, double *loss);
+/* Training samples held on the device: append host rows, append every
+ * molecule pair of every replica from the current coordinates with its
+ * exact energy (kcal/mol, from the molecule-pair matrix of cgppd_mc_prepare
+ * and the MC steps), count, clear. */
+int cgppd_surrogate_add_samples(cgppd_ctx *ctx, const float *x, const float *t, int n);
+int cgppd_surrogate_collect(cgppd_ctx *ctx);
+int cgppd_surrogate_sample_count(cgppd_ctx *ctx, int64_t *count);
+int cgppd_surrogate_clear_samples(cgppd_ctx *ctx);
+/* steps Adam updates (beta 0.9 / 0.999, eps 1e-8) on minibatches of batch
+ * samples (a multiple of 32) drawn with replacement from the sample buffer;
+ * mean_loss = mean of the minibatch losses. */
+int cgppd_surrogate_train(cgppd_ctx *ctx, int steps, int batch, float lr, uint64_t seed, double *mean_loss);
+/* Features of every pair I < J of one replica, x[M(M-1)/2 * 80], pairs in
+ * row-major (I, J) order, as the device computes them. */
+int cgppd_surrogate_features(cgppd_ctx *ctx, int replica, float *x);
+/* Delayed acceptance (Christen & Fox 2005) for cgppd_mc_steps / cgppd_run:
+ * stage one accepts a proposal with probability min(1, exp(-dS / kT)) from
+ * the surrogate's change dS of the moved molecule's pair energies; only
+ * stage-one survivors get the exact delta-E row, and stage two accepts with
+ * min(1, exp(-(dE - dS) / kT)).  The sampled distribution is the exact one.
+ * Needs an initialised surrogate; enabling recomputes its pair matrix. */
+int cgppd_set_delayed_acceptance(cgppd_ctx *ctx, int enable);
+/* Stage-one rejections of each replica (R) since cgppd_set_replicas. */
+int cgppd_get_stage1_rejections(cgppd_ctx *ctx, int64_t *rejections);
+
 #ifdef __cplusplus
 }
 #endif
