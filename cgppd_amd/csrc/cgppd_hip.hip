@@ -1069,6 +1069,8 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
     }
 }
 
+__device__ void stage_one(const PairParams &p, int r);
+
 // MC step, phase 2 (one workgroup per replica): Metropolis on the finished
 // delta-E row (src/Replica.cpp:385-410, delta and exponent in FP64), apply
 // the proposal (src/Molecule.cpp:468-504), update the matrix row and column
@@ -1156,6 +1158,10 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
     const int64_t next = propose_only ? step : step + 1;
     if (!propose_only && threadIdx.x == 0) p.steps[r] = next;
     propose(p, r, next);
+    if (p.da && !propose_only) {
+        __syncthreads();  // the proposal is laid out
+        stage_one(p, r);
+    }
 }
 
 // After the scaled full-energy passes: mirror the upper triangle of each
@@ -1409,6 +1415,21 @@ __device__ inline void pair_of(int q, int M, int &I, int &J)
     J = I + 1 + q;
 }
 
+// The surrogate of a pair is 0 beyond centre distance r_I + r_J + 15 A
+// (r: bounding radius): such pairs are left out of training, of the
+// surrogate matrix and of stage one.
+constexpr float kSurrogatePad = 15.0f;
+
+__device__ inline bool surrogate_near(float4 ci, float4 cj, float ri, float rj, float box, float inv_box)
+{
+    float dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
+    dx -= box * rintf(dx * inv_box);
+    dy -= box * rintf(dy * inv_box);
+    dz -= box * rintf(dz * inv_box);
+    const float lim = ri + rj + kSurrogatePad;
+    return dx * dx + dy * dy + dz * dz < lim * lim;
+}
+
 // Body frames of every molecule of every replica (current coordinates).
 __global__ void frames_kernel(PairParams p)
 {
@@ -1424,26 +1445,45 @@ __global__ void frames_kernel(PairParams p)
 
 // Features of every pair of replicas [rep_begin, rep_begin + rep_count):
 // BF16 rows into xb (+ targets from the MC matrix into t) or FP32 rows into xf.
-__global__ void pair_features_kernel(PairParams p, unsigned short *xb, float *t, float *xf)
+// With slot != null only pairs with slot[k] >= 0 are written, to row slot[k].
+__global__ void pair_features_kernel(PairParams p, unsigned short *xb, float *t, float *xf, const int *slot)
 {
     const int P = p.n_mol * (p.n_mol - 1) / 2;
     const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (k >= (long long)p.rep_count * P) return;
+    const long long out = slot ? (long long)slot[k] : k;
+    if (out < 0) return;
     const int rr = (int)(k / P), q = (int)(k % P), r = p.rep_begin + rr;
     int I, J;
     pair_of(q, p.n_mol, I, J);
     const size_t rm = (size_t)r * p.n_mol;
     const float *fI = p.frames + (rm + I) * 9, *fJ = p.frames + (rm + J) * 9;
     if (xb) {
-        unsigned short *row = xb + (size_t)k * mlp::kFeat;
+        unsigned short *row = xb + (size_t)out * mlp::kFeat;
         pair_feature_row(p.center[rm + I], p.center[rm + J], fI, fJ, p.comp + I * 20, p.comp + J * 20, p.box,
                          p.inv_box, [&](int f, float v) { row[f] = mlp::f2bf(v); });
-        t[k] = (float)p.emat[(rm + I) * p.n_mol + J];
+        t[out] = (float)p.emat[(rm + I) * p.n_mol + J];
     } else {
-        float *row = xf + (size_t)k * mlp::kFeat;
+        float *row = xf + (size_t)out * mlp::kFeat;
         pair_feature_row(p.center[rm + I], p.center[rm + J], fI, fJ, p.comp + I * 20, p.comp + J * 20, p.box,
                          p.inv_box, [&](int f, float v) { row[f] = v; });
     }
+}
+
+// 1 for the pairs of replicas [rep_begin, rep_begin + rep_count) within the
+// surrogate's range, else 0.
+__global__ void pair_near_kernel(PairParams p, int *flag)
+{
+    const int P = p.n_mol * (p.n_mol - 1) / 2;
+    const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= (long long)p.rep_count * P) return;
+    const int r = p.rep_begin + (int)(k / P);
+    int I, J;
+    pair_of((int)(k % P), p.n_mol, I, J);
+    const size_t rm = (size_t)r * p.n_mol;
+    flag[k] = surrogate_near(p.center[rm + I], p.center[rm + J], p.radius[rm + I], p.radius[rm + J], p.box, p.inv_box)
+                  ? 1
+                  : 0;
 }
 
 // Surrogate pair matrix of the current state: 32 pairs per workgroup over
@@ -1451,7 +1491,7 @@ __global__ void pair_features_kernel(PairParams p, unsigned short *xb, float *t,
 __global__ __launch_bounds__(mlp::kThreads) void surrogate_matrix_kernel(PairParams p)
 {
     __shared__ mlp::Tile T;
-    __shared__ int pr[mlp::kRows], pi[mlp::kRows], pj[mlp::kRows];
+    __shared__ int pr[mlp::kRows], pi[mlp::kRows], pj[mlp::kRows], pn[mlp::kRows];
     const int M = p.n_mol, P = M * (M - 1) / 2;
     const long long base = (long long)blockIdx.x * mlp::kRows, total = (long long)p.rep_count * P;
     const int nv = (int)min((long long)mlp::kRows, total - base);
@@ -1468,6 +1508,8 @@ __global__ __launch_bounds__(mlp::kThreads) void surrogate_matrix_kernel(PairPar
             pi[k] = I;
             pj[k] = J;
             const size_t rm = (size_t)r * M;
+            pn[k] = surrogate_near(p.center[rm + I], p.center[rm + J], p.radius[rm + I], p.radius[rm + J], p.box,
+                                   p.inv_box);
             pair_feature_row(p.center[rm + I], p.center[rm + J], p.frames + (rm + I) * 9, p.frames + (rm + J) * 9,
                              p.comp + I * 20, p.comp + J * 20, p.box, p.inv_box,
                              [&](int f, float v) { row[f] = mlp::f2bf(v); });
@@ -1480,40 +1522,56 @@ __global__ __launch_bounds__(mlp::kThreads) void surrogate_matrix_kernel(PairPar
     if (threadIdx.x < nv) {
         const int k = threadIdx.x;
         const size_t rmm = (size_t)pr[k] * M * M;
-        p.smat[rmm + (size_t)pi[k] * M + pj[k]] = T.y[k];
-        p.smat[rmm + (size_t)pj[k] * M + pi[k]] = T.y[k];
+        const float y = pn[k] ? T.y[k] : 0.0f;
+        p.smat[rmm + (size_t)pi[k] * M + pj[k]] = y;
+        p.smat[rmm + (size_t)pj[k] * M + pi[k]] = y;
     }
 }
 
-// Delayed acceptance, stage one (one workgroup per replica): the surrogate
-// row of the pending proposal against every other molecule, its change dS
-// against the current surrogate matrix, and the stage-one test
+// Delayed acceptance, stage one for replica r (whole workgroup): the
+// surrogate row of the pending proposal against the molecules within range,
+// its change dS against the current surrogate matrix, and the stage-one test
 // min(1, exp(-dS / kT)) with its own Philox uniform.
-__global__ __launch_bounds__(mlp::kThreads) void stage1_kernel(PairParams p)
+__device__ void stage_one(const PairParams &p, int r)
 {
     __shared__ mlp::Tile T;
     __shared__ float fnew[9];
     __shared__ double red[mlp::kThreads / 64];
-    const int r = blockIdx.x, M = p.n_mol, m = p.prop_mol[r];
+    __shared__ int nearJ[CGPPD_MAX_MOLECULES];
+    __shared__ int n_near;
+    const int M = p.n_mol, m = p.prop_mol[r];
+    const float4 cm = p.prop_center[r];
+    const size_t rm = (size_t)r * M;
     if (threadIdx.x == 0) {
         const int2 an = p.anchor[m];
         const int i0 = p.mol_off[m];
         const float4 *pq = p.prop_rel + (size_t)r * p.max_n;
         make_frame(pq[an.x - i0], pq[an.y - i0], an.x == an.y, fnew);
         for (int i = 0; i < 9; ++i) p.prop_frame[(size_t)r * 9 + i] = fnew[i];
+        // molecules within the surrogate's range of the moved molecule
+        int c = 0;
+        for (int J = 0; J < M; ++J)
+            if (J != m && surrogate_near(cm, p.center[rm + J], p.radius[rm + m], p.radius[rm + J], p.box, p.inv_box))
+                nearJ[c++] = J;
+        n_near = c;
+    }
+    // the current row enters with a minus sign; the new row is 0 outside the range
+    double acc = 0.0;
+    for (int J = threadIdx.x; J < M; J += mlp::kThreads) {
+        if (J == m) continue;
+        acc -= (double)p.smat[(rm + m) * M + J];
+        p.srow[rm + J] = 0.0f;
     }
     __syncthreads();
-    const float4 cm = p.prop_center[r];
-    const size_t rm = (size_t)r * M;
-    double acc = 0.0;
-    for (int base = 0; base < M - 1; base += mlp::kRows) {
-        const int nv = min(mlp::kRows, M - 1 - base);
+    const int nn = n_near;
+    for (int base = 0; base < nn; base += mlp::kRows) {
+        const int nv = min(mlp::kRows, nn - base);
         if (threadIdx.x < mlp::kRows) {
             const int k = threadIdx.x;
             unsigned short *row = T.x + k * mlp::kFeatPad;
             for (int f = mlp::kFeat; f < mlp::kFeatPad; ++f) row[f] = 0;
             if (k < nv) {
-                const int q = base + k, J = q < m ? q : q + 1;
+                const int J = nearJ[base + k];
                 auto put = [&](int f, float v) { row[f] = mlp::f2bf(v); };
                 if (m < J)
                     pair_feature_row(cm, p.center[rm + J], fnew, p.frames + (rm + J) * 9, p.comp + m * 20,
@@ -1528,10 +1586,10 @@ __global__ __launch_bounds__(mlp::kThreads) void stage1_kernel(PairParams p)
         __syncthreads();
         mlp::forward(T, p.sp, p.swb, nv);
         if (threadIdx.x < nv) {
-            const int q = base + threadIdx.x, J = q < m ? q : q + 1;
+            const int J = nearJ[base + threadIdx.x];
             const float y = T.y[threadIdx.x];
             p.srow[rm + J] = y;
-            acc += (double)y - (double)p.smat[(rm + m) * M + J];
+            acc += (double)y;
         }
         __syncthreads();  // the next tile reuses T
     }
@@ -1551,6 +1609,8 @@ __global__ __launch_bounds__(mlp::kThreads) void stage1_kernel(PairParams p)
         p.s_dS[r] = dS;
     }
 }
+
+__global__ __launch_bounds__(mlp::kThreads) void stage1_kernel(PairParams p) { stage_one(p, blockIdx.x); }
 
 // BF16 copies of the weights for the matrix products (transposed, padded).
 __global__ void mlp_pack_kernel(const float *master, unsigned short *wb)
@@ -1649,6 +1709,7 @@ struct cgppd_ctx {
     int2 *d_anchor = nullptr;
     float *d_comp = nullptr;
     bool da = false, smat_dirty = true;
+    bool s1_valid = false;  // the pending proposals' stage-one results are current
     float *d_smat = nullptr, *d_srow = nullptr, *d_frames = nullptr, *d_prop_frame = nullptr;
     int *d_s1 = nullptr;
     double *d_s_dS = nullptr;
@@ -2101,6 +2162,7 @@ int refresh_surrogate_state(cgppd_ctx *ctx)
         CG_HIP(ctx, hipGetLastError());
     }
     ctx->smat_dirty = false;
+    ctx->s1_valid = false;
     return CGPPD_OK;
 }
 
@@ -2282,11 +2344,24 @@ int launch_stage1(cgppd_ctx *ctx)
     return CGPPD_OK;
 }
 
+// Before MC steps with delayed acceptance: the surrogate matrix and the
+// stage-one results of the pending proposals (the finish kernel computes
+// them for every later proposal).
+int ensure_stage1(cgppd_ctx *ctx)
+{
+    if (!ctx->da) return CGPPD_OK;
+    int rc;
+    if (ctx->smat_dirty && (rc = refresh_surrogate_state(ctx))) return rc;
+    if (!ctx->s1_valid) {
+        if ((rc = launch_stage1(ctx))) return rc;
+        ctx->s1_valid = true;
+    }
+    return CGPPD_OK;
+}
+
 int launch_mc_step(cgppd_ctx *ctx)
 {
-    int rc = launch_stage1(ctx);
-    if (rc) return rc;
-    rc = launch_pair(ctx);
+    int rc = launch_pair(ctx);
     if (rc) return rc;
     return launch_finish(ctx, 0);
 }
@@ -2860,6 +2935,7 @@ int cgppd_mc_prepare(cgppd_ctx *ctx)
     CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->mc_ready = true;
     ctx->smat_dirty = true;
+    ctx->s1_valid = false;
     return CGPPD_OK;
 }
 
@@ -2880,11 +2956,10 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
     if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before MC steps");
     if (n_steps < 0) return fail(ctx, CGPPD_E_ARG, "negative step count");
     CG_HIP(ctx, hipSetDevice(ctx->device));
-    if (ctx->da && ctx->smat_dirty && (rc = refresh_surrogate_state(ctx))) return rc;
+    if ((rc = ensure_stage1(ctx))) return rc;
     if (ctx->timing && n_steps > 0) {
         if ((rc = ensure_events(ctx, (size_t)2 * n_steps))) return rc;
         for (int k = 0; k < n_steps; ++k) {
-            if ((rc = launch_stage1(ctx))) return rc;
             if ((rc = launch_pair(ctx, ctx->events[2 * k], ctx->events[2 * k + 1]))) return rc;
             if ((rc = launch_finish(ctx, 0))) return rc;
         }
@@ -2908,6 +2983,7 @@ int cgppd_exchange(cgppd_ctx *ctx)
 {
     int rc = check_ready(ctx, true);
     if (rc) return rc;
+    ctx->s1_valid = false;
     if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before exchanges");
     CG_HIP(ctx, hipSetDevice(ctx->device));
     return launch_exchange(ctx);
@@ -2920,11 +2996,13 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
     if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before running");
     if (n_intervals < 0 || steps_per_interval < 0) return fail(ctx, CGPPD_E_ARG, "negative counts");
     CG_HIP(ctx, hipSetDevice(ctx->device));
-    if (ctx->da && ctx->smat_dirty && (rc = refresh_surrogate_state(ctx))) return rc;
+    if ((rc = ensure_stage1(ctx))) return rc;
     if (!use_graph) {
         for (int k = 0; k < n_intervals; ++k) {
             if ((rc = cgppd_mc_steps(ctx, steps_per_interval))) return rc;
             if ((rc = launch_exchange(ctx))) return rc;
+            // exchanged temperatures change the stage-one test
+            if ((rc = launch_stage1(ctx))) return rc;
         }
         return CGPPD_OK;
     }
@@ -2934,6 +3012,7 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
         int lrc = CGPPD_OK;
         for (int k = 0; k < steps_per_interval && !lrc; ++k) lrc = launch_mc_step(ctx);
         if (!lrc) lrc = launch_exchange(ctx);
+        if (!lrc) lrc = launch_stage1(ctx);  // exchanged temperatures change the stage-one test
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
         if (lrc) {
@@ -3318,14 +3397,36 @@ int cgppd_surrogate_collect(cgppd_ctx *ctx)
     if (n == 0) return CGPPD_OK;
     CG_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_frames(ctx))) return rc;
-    if ((rc = reserve_samples(ctx, ctx->s_n + n))) return rc;
     if ((rc = launch_frames(ctx))) return rc;
+    // pairs within the surrogate's range, in (replica, I, J) order
+    int *d_flag = nullptr;
+    CG_HIP(ctx, hipMalloc((void **)&d_flag, sizeof(int) * n));
     PairParams p = base_params(ctx);
-    hipLaunchKernelGGL(pair_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p,
-                       ctx->d_sx + ctx->s_n * mlp::kFeat, ctx->d_st + ctx->s_n, (float *)nullptr);
-    CG_HIP(ctx, hipGetLastError());
-    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->s_n += n;
+    hipLaunchKernelGGL(pair_near_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, d_flag);
+    std::vector<int> slot(n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(slot.data(), d_flag, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(d_flag);
+        return fail(ctx, CGPPD_E_HIP, "surrogate collect: %s", hipGetErrorString(e));
+    }
+    size_t kept = 0;
+    for (size_t k = 0; k < n; ++k) slot[k] = slot[k] ? (int)(ctx->s_n + kept++) : -1;
+    if ((rc = reserve_samples(ctx, ctx->s_n + kept))) {
+        (void)hipFree(d_flag);
+        return rc;
+    }
+    e = hipMemcpy(d_flag, slot.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess && kept > 0) {
+        hipLaunchKernelGGL(pair_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p,
+                           ctx->d_sx, ctx->d_st, (float *)nullptr, (const int *)d_flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_flag);
+    if (e != hipSuccess) return fail(ctx, CGPPD_E_HIP, "surrogate collect: %s", hipGetErrorString(e));
+    ctx->s_n += kept;
     return CGPPD_OK;
 }
 
@@ -3346,7 +3447,7 @@ int cgppd_surrogate_features(cgppd_ctx *ctx, int replica, float *x)
     p.rep_begin = replica;
     p.rep_count = 1;
     hipLaunchKernelGGL(pair_features_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, ctx->stream, p,
-                       (unsigned short *)nullptr, (float *)nullptr, d);
+                       (unsigned short *)nullptr, (float *)nullptr, d, (const int *)nullptr);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(x, d, sizeof(float) * P * mlp::kFeat, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -3378,6 +3479,7 @@ int cgppd_set_delayed_acceptance(cgppd_ctx *ctx, int enable)
     if ((rc = ensure_frames(ctx))) return rc;
     ctx->da = true;
     ctx->smat_dirty = true;
+    ctx->s1_valid = false;
     return CGPPD_OK;
 }
 

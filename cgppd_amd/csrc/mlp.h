@@ -73,6 +73,17 @@ __device__ inline f32x4 mma_tile(const unsigned short *A, int am, int ak, const 
 {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, kq = (lane >> 4) * 8;
+    if (ak == 1 && bk == 1) {
+        // K contiguous in both operands (row strides multiples of 8 elements,
+        // 16-byte aligned bases): one 16-byte load per operand
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const u16x8 a = *reinterpret_cast<const u16x8 *>(A + (m0 + r) * am + k0 + kq);
+            const u16x8 b = *reinterpret_cast<const u16x8 *>(B + (n0 + r) * bn + k0 + kq);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                          __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+        }
+        return acc;
+    }
     for (int k0 = 0; k0 < K; k0 += 32) {
         u16x8 a, b;
 #pragma unroll
@@ -87,11 +98,11 @@ __device__ inline f32x4 mma_tile(const unsigned short *A, int am, int ak, const 
 }
 
 struct Tile {
-    unsigned short x[kRows * kFeatPad];   // BF16 inputs
-    float z1[kRows * kHidden];            // pre-activations (then dZ1)
-    unsigned short a1[kRows * kHidden];   // BF16 activations (then BF16 dZ1)
-    float z2[kRows * kHidden];            // (then dZ2)
-    unsigned short a2[kRows * kHidden];   // (then BF16 dZ2)
+    alignas(16) unsigned short x[kRows * kFeatPad];   // BF16 inputs
+    alignas(16) float z1[kRows * kHidden];            // pre-activations (then dZ1)
+    alignas(16) unsigned short a1[kRows * kHidden];   // BF16 activations (then BF16 dZ1)
+    alignas(16) float z2[kRows * kHidden];            // (then dZ2)
+    alignas(16) unsigned short a2[kRows * kHidden];   // (then BF16 dZ2)
     float y[kRows];
 };
 
@@ -124,10 +135,15 @@ __device__ inline void forward(Tile &T, const float *master, const unsigned shor
     __syncthreads();
     dense_silu(T.a1, kHidden, wb + kBW2, master + kOb2, T.z2, T.a2);
     __syncthreads();
-    if (threadIdx.x < kRows) {
+    {
+        // output layer: 8 lanes per row, 16 products each, then a butterfly
+        const int row = threadIdx.x >> 3, part = threadIdx.x & 7;
         float s = 0.f;
-        for (int k = 0; k < kHidden; ++k) s += bf2f(T.a2[threadIdx.x * kHidden + k]) * bf2f(wb[kBw3 + k]);
-        T.y[threadIdx.x] = threadIdx.x < n ? s + master[kOb3] : 0.f;
+        for (int k = part * 16; k < part * 16 + 16; ++k) s += bf2f(T.a2[row * kHidden + k]) * bf2f(wb[kBw3 + k]);
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        s += __shfl_xor(s, 4, 64);
+        if (part == 0) T.y[row] = row < n ? s + master[kOb3] : 0.f;
     }
     __syncthreads();
 }

@@ -240,7 +240,10 @@ int cgppd_get_sizes(const cgppd_ctx *ctx, int *n_mol, int *n_res, int *n_rep);
  * uses the exact pair kernel.  Features of a pair I < J: 32 Gaussian RBFs
  * of the minimum-image centre distance (centres 0, 3, ..., 93 A, width 3 A),
  * the unit centre-to-centre vector in each molecule's body frame (2 x 3),
- * each molecule's residue-type fractions (2 x 20), two zeros.
+ * each molecule's residue-type fractions (2 x 20), two zeros.  The
+ * surrogate of a pair is 0 beyond centre distance r_I + r_J + 15 A (r: the
+ * molecules' bounding radii), so only nearby pairs are trained on and
+ * evaluated.
  * Parameters (FP32, cgppd_surrogate_param_count of them): W1[80][128], b1[128],
  * W2[128][128], b2[128], w3[128], b3. */
 int cgppd_surrogate_param_count(int *count);
@@ -253,9 +256,9 @@ int cgppd_surrogate_predict(cgppd_ctx *ctx, const float *x, int n, float *y);
  * t[n] (parameter layout as above) and the loss; parameters unchanged. */
 int cgppd_surrogate_gradient(cgppd_ctx *ctx, const float *x, const float *t, int n, float *grad, double *loss);
 /* Training samples held on the device: append host rows, append every
- * molecule pair of every replica from the current coordinates with its
- * exact energy (kcal/mol, from the molecule-pair matrix of cgppd_mc_prepare
- * and the MC steps), count, clear. */
+ * molecule pair within the surrogate's range of every replica from the
+ * current coordinates with its exact energy (kcal/mol, from the
+ * molecule-pair matrix of cgppd_mc_prepare and the MC steps), count, clear. */
 int cgppd_surrogate_add_samples(cgppd_ctx *ctx, const float *x, const float *t, int n);
 int cgppd_surrogate_collect(cgppd_ctx *ctx);
 int cgppd_surrogate_sample_count(cgppd_ctx *ctx, int64_t *count);

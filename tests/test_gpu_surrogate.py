@@ -125,7 +125,8 @@ def test_collect_and_train_reduce_loss(gpu_engine_factory):
         e.mc_steps(40)
         e.surrogate_collect()
     P = spec.n_mol * (spec.n_mol - 1) // 2
-    assert e.surrogate_sample_count() == 10 * spec.n_rep * P
+    # only pairs within the surrogate's range are kept
+    assert 0 < e.surrogate_sample_count() <= 10 * spec.n_rep * P
     first = e.surrogate_train(20, batch=256, lr=2e-3, seed=1)
     later = e.surrogate_train(400, batch=256, lr=2e-3, seed=2)
     last = e.surrogate_train(20, batch=256, lr=2e-3, seed=3)
@@ -134,13 +135,13 @@ def test_collect_and_train_reduce_loss(gpu_engine_factory):
     e.close()
 
 
-def test_delayed_acceptance_with_constant_surrogate_is_plain_metropolis(gpu_engine_factory):
-    """A constant surrogate has dS = 0: stage one always passes and stage two
-    is the plain Metropolis test, so the trajectory is the oracle's."""
+def test_delayed_acceptance_with_zero_surrogate_is_plain_metropolis(gpu_engine_factory):
+    """A surrogate that is 0 everywhere has dS = 0: stage one always passes
+    and stage two is the plain Metropolis test, so the trajectory is the
+    oracle's."""
     spec = small_spec()
     e = engine_with_system(gpu_engine_factory, spec, seed=21)
     zero = {k: np.zeros_like(v) for k, v in S.MLP(seed=0).p.items()}
-    zero["b3"][:] = 1.25
     e.surrogate_set_params(zero)
     e.set_delayed_acceptance(True)
     st = oracle.Remc(spec, seed=21)
