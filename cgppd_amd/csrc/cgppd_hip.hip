@@ -251,6 +251,7 @@ struct PairParams {
     int max_n;            // residues of the largest molecule (prop_* row stride)
     float4 *prop_pos;     // R * max_n: proposed positions, device order, w = code
     float4 *prop_rel;     // R * max_n: proposed relative positions
+    float4 *prop_delta;   // R * max_n: per-residue displacements of a flex move
     float4 *prop_center;  // R
     int *prop_mol;        // R
     int *prop_nib;        // R: i-blocks of the moved molecule
@@ -874,11 +875,13 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
 // out as the proposal (whole workgroup; src/Molecule.cpp:274-448).  The
-// current coordinates are copied, thread 0 moves the residues that change
-// and sums their displacement in the caller's order, then every residue's
-// relative position takes the centre shift and the molecule is wrapped
-// (src/Molecule.cpp:242-252).  Bit-identical to the CPU restatement: no
-// FMA contraction, correctly rounded division and square root.
+// current coordinates are copied, the residues that change are moved (a
+// flex branch in parallel, one residue per thread, each displacement kept
+// in prop_delta), thread 0 sums the displacements in the caller's order,
+// then every residue's relative position takes the centre shift and the
+// molecule is wrapped (src/Molecule.cpp:242-252).  Bit-identical to the CPU
+// restatement: no FMA contraction, correctly rounded division and square
+// root, the same summation order.
 __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const Proposal &pr, int kind)
 {
 #pragma clang fp contract(off)
@@ -895,95 +898,121 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
         pq[k] = rel[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t x[4];
-        philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, kTagMC | 2u, (uint32_t)p.seed,
-               (uint32_t)(p.seed >> 32), x);
-        const double rcos = p.rcos[r], rsin = p.rsin[r];
-        float acc[3] = {0.f, 0.f, 0.f};
-        int moved = 0;
-        if (kind == 2) {
-            // flex(axis, rotate step, ri, neighbour): src/Molecule.cpp:331-413
-            const int nf = p.flexr_off[m + 1] - p.flexr_off[m];
-            if (nf > 0) {
-                const int e = p.flexr_off[m] + mulhi_n(x[0], nf);
-                const int deg = p.nb_off[e + 1] - p.nb_off[e];
-                const int k = p.nb_off[e] + mulhi_n(x[1], deg);
-                const double ax = pr.axis[0], ay = pr.axis[1], az = pr.axis[2];
-                const double mag = sqrt(ax * ax + ay * ay + az * az);
-                const double q[4] = {rcos, rsin * (ax / mag), rsin * (ay / mag), rsin * (az / mag)};
-                const float4 c0 = pos[p.flexr[e] - i0];
-                for (int b = p.br_off[k]; b < p.br_off[k + 1]; ++b) {
-                    const int li = p.br[b] - i0;
-                    const float4 pi = pos[li];
-                    const float3 rr = qrot(q, pi.x - c0.x, pi.y - c0.y, pi.z - c0.z);
-                    const float dx = rr.x + c0.x - pi.x, dy = rr.y + c0.y - pi.y, dz = rr.z + c0.z - pi.z;
-                    acc[0] = acc[0] + dx;
-                    acc[1] = acc[1] + dy;
-                    acc[2] = acc[2] + dz;
-                    const float4 qi = rel[li];
-                    pp[li] = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, pi.w);
-                    pq[li] = make_float4(qi.x + dx, qi.y + dy, qi.z + dz, qi.w);
+    // every thread draws the same words (the choices below are uniform)
+    uint32_t x[4];
+    philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, kTagMC | 2u, (uint32_t)p.seed,
+           (uint32_t)(p.seed >> 32), x);
+    const double rcos = p.rcos[r], rsin = p.rsin[r];
+    float acc[3] = {0.f, 0.f, 0.f};
+    int moved = 0;
+    if (kind == 2) {
+        // flex(axis, rotate step, ri, neighbour): src/Molecule.cpp:331-413
+        const int nf = p.flexr_off[m + 1] - p.flexr_off[m];
+        if (nf > 0) {
+            const int e = p.flexr_off[m] + mulhi_n(x[0], nf);
+            const int deg = p.nb_off[e + 1] - p.nb_off[e];
+            const int k = p.nb_off[e] + mulhi_n(x[1], deg);
+            const double ax = pr.axis[0], ay = pr.axis[1], az = pr.axis[2];
+            const double mag = sqrt(ax * ax + ay * ay + az * az);
+            const double q[4] = {rcos, rsin * (ax / mag), rsin * (ay / mag), rsin * (az / mag)};
+            const float4 c0 = pos[p.flexr[e] - i0];
+            const int b0 = p.br_off[k], nb = p.br_off[k + 1] - b0;
+            float4 *dl = p.prop_delta + (size_t)r * p.max_n;
+            for (int t = threadIdx.x; t < nb; t += kWG) {
+                const int li = p.br[b0 + t] - i0;
+                const float4 pi = pos[li];
+                const float3 rr = qrot(q, pi.x - c0.x, pi.y - c0.y, pi.z - c0.z);
+                const float dx = rr.x + c0.x - pi.x, dy = rr.y + c0.y - pi.y, dz = rr.z + c0.z - pi.z;
+                const float4 qi = rel[li];
+                pp[li] = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, pi.w);
+                pq[li] = make_float4(qi.x + dx, qi.y + dy, qi.z + dz, qi.w);
+                dl[t] = make_float4(dx, dy, dz, 0.0f);
+            }
+            __syncthreads();  // the displacements are in place
+            if (threadIdx.x == 0) {
+                int t = 0;
+                for (; t + 4 <= nb; t += 4) {
+                    const float4 d0 = dl[t], d1 = dl[t + 1], d2 = dl[t + 2], d3 = dl[t + 3];
+                    acc[0] = acc[0] + d0.x;
+                    acc[1] = acc[1] + d0.y;
+                    acc[2] = acc[2] + d0.z;
+                    acc[0] = acc[0] + d1.x;
+                    acc[1] = acc[1] + d1.y;
+                    acc[2] = acc[2] + d1.z;
+                    acc[0] = acc[0] + d2.x;
+                    acc[1] = acc[1] + d2.y;
+                    acc[2] = acc[2] + d2.z;
+                    acc[0] = acc[0] + d3.x;
+                    acc[1] = acc[1] + d3.y;
+                    acc[2] = acc[2] + d3.z;
                 }
-                moved = 1;
+                for (; t < nb; ++t) {
+                    const float4 d = dl[t];
+                    acc[0] = acc[0] + d.x;
+                    acc[1] = acc[1] + d.y;
+                    acc[2] = acc[2] + d.z;
+                }
+            }
+            moved = 1;
+        }
+    } else if (threadIdx.x == 0) {
+        // one local move, src/Molecule.cpp:415-448
+        int li = -1;
+        float v[3];
+        if (x[2] < 0x80000000u) {
+            // crankshaft(rotate step, flip, ri): src/Molecule.cpp:303-329
+            const int nc = p.crank_off[m + 1] - p.crank_off[m];
+            if (nc > 0) {
+                const int4 cr = p.crank[p.crank_off[m] + mulhi_n(x[0], nc)];
+                const float4 a = pos[cr.x - i0], b = pos[cr.y - i0], c = pos[cr.z - i0];
+                double ax = (double)(c.x - a.x), ay = (double)(c.y - a.y), az = (double)(c.z - a.z);
+                const double mag = sqrt(ax * ax + ay * ay + az * az);
+                ax /= mag;
+                ay /= mag;
+                az /= mag;
+                const double sn = x[3] < 0x80000000u ? -rsin : rsin;
+                double q[4] = {rcos, sn * ax, sn * ay, sn * az};
+                const double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+                q[0] /= qm;
+                q[1] /= qm;
+                q[2] /= qm;
+                q[3] /= qm;
+                const float3 rr = qrot(q, b.x - a.x, b.y - a.y, b.z - a.z);
+                v[0] = rr.x + a.x - b.x;
+                v[1] = rr.y + a.y - b.y;
+                v[2] = rr.z + a.z - b.z;
+                li = cr.y - i0;
             }
         } else {
-            // one local move, src/Molecule.cpp:415-448
-            int li = -1;
-            float v[3];
-            if (x[2] < 0x80000000u) {
-                // crankshaft(rotate step, flip, ri): src/Molecule.cpp:303-329
-                const int nc = p.crank_off[m + 1] - p.crank_off[m];
-                if (nc > 0) {
-                    const int4 cr = p.crank[p.crank_off[m] + mulhi_n(x[0], nc)];
-                    const float4 a = pos[cr.x - i0], b = pos[cr.y - i0], c = pos[cr.z - i0];
-                    double ax = (double)(c.x - a.x), ay = (double)(c.y - a.y), az = (double)(c.z - a.z);
-                    const double mag = sqrt(ax * ax + ay * ay + az * az);
-                    ax /= mag;
-                    ay /= mag;
-                    az /= mag;
-                    const double sn = x[3] < 0x80000000u ? -rsin : rsin;
-                    double q[4] = {rcos, sn * ax, sn * ay, sn * az};
-                    const double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-                    q[0] /= qm;
-                    q[1] /= qm;
-                    q[2] /= qm;
-                    q[3] /= qm;
-                    const float3 rr = qrot(q, b.x - a.x, b.y - a.y, b.z - a.z);
-                    v[0] = rr.x + a.x - b.x;
-                    v[1] = rr.y + a.y - b.y;
-                    v[2] = rr.z + a.z - b.z;
-                    li = cr.y - i0;
+            // local translate by 0.1 x the translation step: src/Molecule.cpp:422-430
+            const int nl = p.local_off[m + 1] - p.local_off[m];
+            if (nl > 0) {
+                li = p.local[p.local_off[m] + mulhi_n(x[0], nl)] - i0;
+                float n0 = (float)pr.axis[0], n1 = (float)pr.axis[1], n2 = (float)pr.axis[2];
+                const float mag = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+                if (mag != 0.0f) {
+                    n0 = n0 / mag;
+                    n1 = n1 / mag;
+                    n2 = n2 / mag;
                 }
-            } else {
-                // local translate by 0.1 x the translation step: src/Molecule.cpp:422-430
-                const int nl = p.local_off[m + 1] - p.local_off[m];
-                if (nl > 0) {
-                    li = p.local[p.local_off[m] + mulhi_n(x[0], nl)] - i0;
-                    float n0 = (float)pr.axis[0], n1 = (float)pr.axis[1], n2 = (float)pr.axis[2];
-                    const float mag = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
-                    if (mag != 0.0f) {
-                        n0 = n0 / mag;
-                        n1 = n1 / mag;
-                        n2 = n2 / mag;
-                    }
-                    const double sc = (double)0.1f * (double)p.tstep[r];
-                    v[0] = (float)((double)n0 * sc);
-                    v[1] = (float)((double)n1 * sc);
-                    v[2] = (float)((double)n2 * sc);
-                }
-            }
-            if (li >= 0) {
-                // translate(v, ri): src/Molecule.cpp:274-301
-                const float4 pi = pos[li], qi = rel[li];
-                pp[li] = make_float4(pi.x + v[0], pi.y + v[1], pi.z + v[2], pi.w);
-                pq[li] = make_float4(qi.x + v[0], qi.y + v[1], qi.z + v[2], qi.w);
-                acc[0] = v[0];
-                acc[1] = v[1];
-                acc[2] = v[2];
-                moved = 1;
+                const double sc = (double)0.1f * (double)p.tstep[r];
+                v[0] = (float)((double)n0 * sc);
+                v[1] = (float)((double)n1 * sc);
+                v[2] = (float)((double)n2 * sc);
             }
         }
+        if (li >= 0) {
+            // translate(v, ri): src/Molecule.cpp:274-301
+            const float4 pi = pos[li], qi = rel[li];
+            pp[li] = make_float4(pi.x + v[0], pi.y + v[1], pi.z + v[2], pi.w);
+            pq[li] = make_float4(qi.x + v[0], qi.y + v[1], qi.z + v[2], qi.w);
+            acc[0] = v[0];
+            acc[1] = v[1];
+            acc[2] = v[2];
+            moved = 1;
+        }
+    }
+    if (threadIdx.x == 0) {
         const float4 c = p.center[(size_t)r * p.n_mol + m];
         const float fn = (float)n;
         for (int k = 0; k < 3; ++k) sh_vf[k] = moved ? acc[k] / fn : 0.0f;
@@ -1662,7 +1691,7 @@ struct cgppd_ctx {
     double *d_rcos = nullptr, *d_rsin = nullptr;
     float *d_tstep = nullptr;
     unsigned *d_done = nullptr;
-    float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr;
+    float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr, *d_prop_delta = nullptr;
     int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
     unsigned long long *d_ex_stats = nullptr;
@@ -1804,6 +1833,7 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_done);
     dev_free(ctx->d_prop_pos);
     dev_free(ctx->d_prop_rel);
+    dev_free(ctx->d_prop_delta);
     dev_free(ctx->d_prop_center);
     dev_free(ctx->d_prop_mol);
     dev_free(ctx->d_prop_nib);
@@ -2086,6 +2116,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.max_n = ctx->max_n;
     p.prop_pos = ctx->d_prop_pos;
     p.prop_rel = ctx->d_prop_rel;
+    p.prop_delta = ctx->d_prop_delta;
     p.prop_center = ctx->d_prop_center;
     p.prop_mol = ctx->d_prop_mol;
     p.prop_nib = ctx->d_prop_nib;
@@ -2734,6 +2765,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_done, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_pos, (size_t)R * ctx->max_n))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_rel, (size_t)R * ctx->max_n))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_delta, (size_t)R * ctx->max_n))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_center, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_mol, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
