@@ -10,10 +10,13 @@ all others (FP32 pair terms, Kahan, FP64 reduction) and applies Metropolis.
 Exchange intervals are replayed from a captured hipGraph.
 
 value = MC steps/s summed over all replicas (replicas x steps / wall), with
-the replicas already resident in HBM.  The roofline object refers to the
-dominant kernel (mc_pair_kernel): algorithmic FP32 flops = 35 per inter-
-molecular residue pair, pairs counted on the device, divided by the kernel's
-mean duration measured with HIP events on the context stream.  Its HBM
+the replicas already resident in HBM.  Before the warm-up the autotuner
+picks the pair kernel's layout and launch shape on the pending proposals.
+The roofline object refers to the dominant kernel (mc_row_kernel in the
+j-lane layout, mc_pair_kernel in the i-lane layout): algorithmic FP32 flops
+= 35 per inter-molecular residue pair, pairs counted on the device, divided
+by the kernel's mean duration measured with HIP events recorded by the
+dispatches themselves.  Its HBM
 traffic per launch comes from two rocprofv3 --pmc passes (FETCH_SIZE,
 WRITE_SIZE), scaled by what the same counters read for a plain device copy
 of known size (--copy-probe-mib), and is read from the committed summary in
@@ -50,7 +53,10 @@ def parse():
                          "4: 256-chain crowded box")
     ap.add_argument("--exchange-every", type=int, default=1000)
     ap.add_argument("--replicas", type=int, default=0, help="override the configuration's replica count")
-    ap.add_argument("--wg-per-cu", type=int, default=0)
+    ap.add_argument("--wg-per-cu", type=int, default=0, help="workgroups per CU of the pair kernel (0: tuned)")
+    ap.add_argument("--layout", default="auto", choices=["auto", "ilanes", "jlanes"],
+                    help="pair-kernel layout (auto: tuned)")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the pair-kernel autotuner")
     ap.add_argument("--no-image-shift", action="store_true", help="per-pair minimum image everywhere")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -62,14 +68,16 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_SUMMARY = ROOT / "profiles" / "mc_pair_kernel_hbm_traffic_crowded40.txt"
+KERNEL_NAMES = {"ilanes": "mc_pair_kernel", "jlanes": "mc_row_kernel"}
 
 
-def committed_traffic(workload):
+def committed_traffic(kernel, workload):
     """HBM bytes per pair-kernel launch from the committed PMC summary
-    (line 'bytes_per_launch <workload> <value>'), or None."""
+    profiles/<kernel>_hbm_traffic_<workload>.txt (line
+    'bytes_per_launch <workload> <value>'), or None."""
+    path = ROOT / "profiles" / f"{kernel}_hbm_traffic_{workload}.txt"
     try:
-        for line in TRAFFIC_SUMMARY.read_text().splitlines():
+        for line in path.read_text().splitlines():
             f = line.split()
             if len(f) == 3 and f[0] == "bytes_per_launch" and f[1] == workload:
                 return float(f[2])
@@ -143,10 +151,19 @@ def main():
     e.load_spec(spec, seed=args.seed)
     if args.wg_per_cu:
         e.set_launch(args.wg_per_cu)
+    if args.layout != "auto":
+        e.set_pair_layout(args.layout)
     if args.no_image_shift:
         e.set_image_shift(False)
     e.mc_prepare()
     R = spec.n_rep
+    tuned = None
+    if not args.no_autotune and not (args.wg_per_cu and args.layout != "auto"):
+        # the autotuner times the candidates on the pending proposals; the
+        # caller's explicit choices stay in force
+        tuned = e.autotune(launches=20)
+        print(f"autotune: {tuned}", file=sys.stderr, flush=True)
+    layout = e.pair_layout()
 
     if args.profile_steps:
         e.mc_steps(args.profile_steps)
@@ -204,18 +221,21 @@ def main():
             "pair_terms": "fp32, Kahan over 4-term groups per lane, fp64 row reduction",
             "flexible": bool(getattr(spec, "flexible", False)),
             "image_shift": not args.no_image_shift,
+            "pair_layout": layout,
+            "wg_per_cu": e.pair_wg_per_cu(),
+            "autotuned": tuned is not None,
         },
         "pairs_per_s": pairs / wall,
         "acceptance": float((acc[0] + acc[1]) / max(1, acc.sum())),
         "exchange_acceptance": float(s["exchange"][1] / max(1, s["exchange"][0])),
         "roofline": {
             "bound": "valu",
-            "kernel": "mc_pair_kernel",
+            "kernel": KERNEL_NAMES[layout],
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": committed_traffic(spec.name),
+            "traffic": committed_traffic(KERNEL_NAMES[layout], spec.name),
             "kernel_ms": kern_ms,
             "pairs_per_launch": pairs_timed / n_timed,
             "flops_per_pair": FLOPS_PER_PAIR,

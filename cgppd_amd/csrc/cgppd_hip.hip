@@ -48,6 +48,12 @@ constexpr int kColShift = 4;              // 16-byte entries
 constexpr int kTableBytes = CGPPD_AA_COUNT << kRowShift;
 constexpr uint32_t kRowMask = 0x1Fu << kRowShift;
 constexpr uint32_t kChargedBit = 0x80000000u;
+// Fixed-point row partials: 2^-32 kcal/mol units in 64-bit integers, so
+// that their atomic sum does not depend on the order of the additions.
+// Partials of magnitude 2^17 kcal/mol or more (clashes) take the FP64 path.
+constexpr double kFixScale = 4294967296.0;
+constexpr double kFixInv = 1.0 / 4294967296.0;
+constexpr double kFixMax = 131072.0;
 constexpr uint32_t kTagMC = 0x4D430000u;
 constexpr uint32_t kTagEX = 0x45580000u;
 constexpr uint32_t kTagSG = 0x53470000u;  // surrogate minibatch draws
@@ -69,6 +75,18 @@ struct Coef {
     float A;     // -4 E_ij (raw RT) or -4 E_ij * RT_to_kcal (scaled)
     float rc2;   // (sigma_ij * 2^(1/6))^2 when E_ij > 0, else -1
     float qq;    // q_i q_j (raw) or q_i q_j * DH factor (scaled)
+};
+
+// Pair coefficients of the j-lane row kernel (kcal/mol), in the form that
+// needs the fewest operations per pair: with u = 1/r^6,
+//   attractive branch  A (s^12 - s^6) = u (B u - C),  B = A sigma^12, C = A sigma^6
+//   repulsive branch   -A/2 - A (s^12 - s^6) = K - u (B u - C),  K = -A/2
+// (A = -4 E_ij RT_to_kcal).  The Debye-Hueckel coefficient q_i q_j DH is
+// formed from per-residue charges instead (one product per run and lane).
+struct RowCoef {
+    float B, C;
+    float rc2;   // as in Coef
+    float K;
 };
 
 // ---------------------------------------------------------------------------
@@ -235,9 +253,12 @@ struct PairParams {
     double *out_lj, *out_dh;  // R
     double *out_mat;          // R * M * M (may be null)
     float rt_to_kcal, dh_factor;
+    const RowCoef *row_table;  // 20 x 20, entry tj * 20 + ti
+    float charge[CGPPD_AA_COUNT];
     // MC state
     uint64_t seed;
     double *rowbuf;       // R * M
+    unsigned long long *rowfix;  // R * M: row partials in fixed point (kFixScale units, two's complement)
     unsigned *done;       // R
     double *emat;         // R * M * M
     double *energy;       // R
@@ -873,6 +894,240 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Delta-E row, j-lane layout (systems without flexible molecules).  For
+// replica r with molecule m (n_m residues) moved, the j axis is every
+// residue outside m in device order (n_j = n_res - n_m), cut into blocks
+// of 64 j residues, one per lane.  One unit of work is an (i, j-block)
+// step: all 64 lanes evaluate their j residue against residue i of m's
+// proposal, read as a wave-uniform record through the scalar cache.  The
+// units of all replicas are laid end to end (replica, j-block, i) and every
+// wave takes an equal contiguous share, so the grid is balanced to within
+// one unit and waves never wait for one another.  Residue i's type is
+// uniform along m's constant-type runs, so each lane loads its table entry
+// once per run; uncharged runs skip the Debye-Hueckel term.  Per lane the
+// terms are summed in groups of four into a Kahan-compensated FP32
+// accumulator; at the end of a j-block visit the lanes of each molecule J
+// are reduced in FP64 and added atomically into the row.
+// ---------------------------------------------------------------------------
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) f4_t const_f4;
+
+// i record of the proposal through the scalar cache (the proposal is not
+// written while the row kernel runs)
+__device__ __forceinline__ f4_t scalar_rec(const float4 *base, int i)
+{
+    return ((const_f4 *)base)[i];
+}
+
+// One pair term of the row kernel: wave-uniform i record (xi, yi, zi),
+// this lane's j (xj, yj, zj); qq = q_i q_j DH for charged i runs.
+template <bool MINIMG, bool CHARGED>
+__device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj, float yj, float zj,
+                                          const RowCoef &cf, float qq, float box, float inv_box)
+{
+    float dx = xi - xj;
+    float dy = yi - yj;
+    float dz = zi - zj;
+    if (MINIMG) {
+        dx = fmaf(-box, rintf(dx * inv_box), dx);
+        dy = fmaf(-box, rintf(dy * inv_box), dy);
+        dz = fmaf(-box, rintf(dz * inv_box), dz);
+    }
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    float rinv = 0.0f, rinv2;
+    if (CHARGED) {
+        rinv = __builtin_amdgcn_rsqf(r2);
+        rinv2 = rinv * rinv;
+    } else {
+        rinv2 = __builtin_amdgcn_rcpf(r2);
+    }
+    const float u = rinv2 * rinv2 * rinv2;
+    const float att = fmaf(cf.B, u, -cf.C) * u;
+    const float rep = cf.K - att;
+    float e = r2 < cf.rc2 ? rep : att;
+    if (CHARGED) e = fmaf(qq * __builtin_amdgcn_exp2f(r2 * rinv * kDHExp2), rinv, e);
+    return e;
+}
+
+// A run [a, b) of i residues of one type (wave-uniform) against this lane's
+// j residue: terms summed plainly in groups of eight, group sums added into
+// a Kahan-compensated accumulator.
+template <bool MINIMG, bool CHARGED>
+__device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, float xj, float yj, float zj,
+                                        const RowCoef &cf, float qq, float box, float inv_box, float &s0, float &c0)
+{
+    int i = a;
+    for (; i + 8 <= b; i += 8) {
+        f4_t r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) r[u] = scalar_rec(irec, i + u);
+        float g = row_term<MINIMG, CHARGED>(r[0].x, r[0].y, r[0].z, xj, yj, zj, cf, qq, box, inv_box);
+#pragma unroll
+        for (int u = 1; u < 8; ++u) g += row_term<MINIMG, CHARGED>(r[u].x, r[u].y, r[u].z, xj, yj, zj, cf, qq, box, inv_box);
+        kahan_add(s0, c0, g);
+    }
+    if (i < b) {
+        f4_t r0 = scalar_rec(irec, i);
+        float g = row_term<MINIMG, CHARGED>(r0.x, r0.y, r0.z, xj, yj, zj, cf, qq, box, inv_box);
+        for (++i; i < b; ++i) {
+            const f4_t ri = scalar_rec(irec, i);
+            g += row_term<MINIMG, CHARGED>(ri.x, ri.y, ri.z, xj, yj, zj, cf, qq, box, inv_box);
+        }
+        kahan_add(s0, c0, g);
+    }
+}
+
+template <bool MINIMG>
+__device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec, int i0, int ia, int ib,
+                                         float xj, float yj, float zj, uint32_t col, float qj,
+                                         const char *__restrict__ tab, float &s0, float &c0)
+{
+    int run = __builtin_amdgcn_readfirstlane(p.run_of[i0 + ia]);
+    int a = ia;
+    while (a < ib) {
+        const int b = __builtin_amdgcn_readfirstlane(min(ib, p.run_start[run + 1] - i0));
+        const uint32_t code = __builtin_amdgcn_readfirstlane(p.run_code[run]);
+        const RowCoef cf = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | col));
+        if (code & kChargedBit) {
+            const float qq = p.charge[(code & kRowMask) >> kRowShift] * p.dh_factor * qj;
+            row_run<MINIMG, true>(irec, a, b, xj, yj, zj, cf, qq, p.box, p.inv_box, s0, c0);
+        } else {
+            row_run<MINIMG, false>(irec, a, b, xj, yj, zj, cf, 0.0f, p.box, p.inv_box, s0, c0);
+        }
+        a = b;
+        ++run;
+    }
+}
+
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) void mc_row_kernel(PairParams p)
+{
+    __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
+    __shared__ long long unit_off[CGPPD_MAX_REPLICAS + 1];
+    long long *part = unit_off;  // scan partials, consumed before unit_off is written
+    // entry e = tj * 20 + ti (i = the moved residue) goes to row ti, column tj
+    for (int e = threadIdx.x; e < CGPPD_AA_COUNT * CGPPD_AA_COUNT; e += kWG) {
+        const int tj = e / CGPPD_AA_COUNT, ti = e % CGPPD_AA_COUNT;
+        *reinterpret_cast<RowCoef *>(tab + ((ti << kRowShift) | (tj << kColShift))) = p.row_table[e];
+    }
+    const int R = p.rep_count;
+    // units per replica, then an exclusive scan (each thread owns up to
+    // four consecutive replicas)
+    constexpr int kPer = (CGPPD_MAX_REPLICAS + kWG - 1) / kWG;
+    long long own[kPer];
+    long long sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int r = threadIdx.x * kPer + q;
+        long long c = 0;
+        if (r < R && !(p.da && !p.s1[r])) {
+            const int m = p.prop_mol[r];
+            const int n_m = p.mol_off[m + 1] - p.mol_off[m];
+            const int nj = p.n_res - n_m;
+            c = (long long)((nj + kWave - 1) / kWave) * n_m;
+        }
+        own[q] = c;
+        sum += c;
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < kWG; o <<= 1) {
+        const long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    {
+        long long run = part[threadIdx.x] - sum;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int r = threadIdx.x * kPer + q;
+            if (r <= R) unit_off[r] = run;
+            run += own[q];
+        }
+        if (threadIdx.x == kWG - 1 && R == kPer * kWG) unit_off[R] = run;
+    }
+    __syncthreads();
+    const long long total = unit_off[R];
+    const int G = gridDim.x, b = blockIdx.x;
+    const int lb = (p.xcd_map && (G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long W = (long long)G * kWavesPerWG;
+    const long long wv = (long long)lb * kWavesPerWG + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    long long P = total * wv / W;
+    const long long P1 = total * (wv + 1) / W;
+    if (P >= P1) return;
+    // replica holding unit P (binary search; wave-uniform)
+    int r;
+    {
+        int lo = 0, hi = R;  // unit_off[lo] <= P < unit_off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (unit_off[mid] <= P) lo = mid; else hi = mid;
+        }
+        r = __builtin_amdgcn_readfirstlane(lo);
+    }
+    const int M = p.n_mol;
+    while (P < P1) {
+        while (P >= unit_off[r + 1]) ++r;
+        const int m = __builtin_amdgcn_readfirstlane(p.prop_mol[r]);
+        const int i0 = __builtin_amdgcn_readfirstlane(p.mol_off[m]);
+        const int n_m = __builtin_amdgcn_readfirstlane(p.mol_off[m + 1]) - i0;
+        const long long local = P - unit_off[r];
+        const int jb = __builtin_amdgcn_readfirstlane((int)(local / n_m));
+        const int ia = __builtin_amdgcn_readfirstlane((int)(local % n_m));
+        const int ib = __builtin_amdgcn_readfirstlane((int)min((long long)n_m, (long long)ia + (P1 - P)));
+        // this lane's j residue
+        const int nj = p.n_res - n_m;
+        const int jp = jb * kWave + lane;
+        const bool valid = jp < nj;
+        const int j = jp + (jp >= i0 ? n_m : 0);
+        float4 rj = make_float4(0.f, 0.f, 0.f, 0.f);
+        int J = 0x7fffffff;
+        if (valid) {
+            rj = p.pos[(size_t)r * p.stride + j];
+            J = p.mol_of[j];
+        }
+        const uint32_t col = lane_col(rj);
+        const float qj = p.charge[col >> kColShift];
+        // one image shift for the whole visit when every lane's molecule
+        // allows it (the shift moves j next to m's proposed centre)
+        bool fast = false;
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        if (p.fast_image && valid) {
+            const float4 cm = p.prop_center[r];
+            fast = segment_shift(cm, p.center[(size_t)r * M + J],
+                                 p.radius[(size_t)r * M + m] + p.radius[(size_t)r * M + J], p.box, p.inv_box, sx, sy,
+                                 sz);
+        }
+        const bool all_fast = __all(fast || !valid);
+        const float4 *irec = p.prop_pos + (size_t)r * p.max_n;
+        float s0 = 0.f, c0 = 0.f;
+        if (all_fast)
+            row_runs<false>(p, irec, i0, ia, ib, rj.x + sx, rj.y + sy, rj.z + sz, col, qj, tab, s0, c0);
+        else
+            row_runs<true>(p, irec, i0, ia, ib, rj.x, rj.y, rj.z, col, qj, tab, s0, c0);
+        // FP64 reduction per molecule J (J is non-decreasing across the lanes)
+        const double v = valid ? (double)s0 - (double)c0 : 0.0;
+        int Jc = __builtin_amdgcn_readfirstlane(J);
+        for (;;) {
+            const double t = wave_sum(J == Jc ? v : 0.0);
+            if (lane == 0) {
+                const size_t at = (size_t)r * M + Jc;
+                if (fabs(t) < kFixMax)
+                    atomicAdd(p.rowfix + at, (unsigned long long)__double2ll_rn(t * kFixScale));
+                else
+                    unsafeAtomicAdd(p.rowbuf + at, t);
+            }
+            const unsigned long long rest = __ballot(valid && J > Jc);
+            if (rest == 0) break;
+            Jc = __builtin_amdgcn_readlane(J, __builtin_ctzll(rest));
+        }
+        P += ib - ia;
+    }
+}
+
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
 // out as the proposal (whole workgroup; src/Molecule.cpp:274-448).  The
 // current coordinates are copied, the residues that change are moved (a
@@ -1116,11 +1371,23 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
         const int m = p.prop_mol[r];
         const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
         double *row = p.rowbuf + (size_t)r * M;
+        unsigned long long *fix = p.rowfix + (size_t)r * M;
         double *mat = p.emat + (size_t)r * M * M;
         // the diagonal entry is the moved molecule's internal energy when it
         // is flexible (0 in both the row and the matrix otherwise)
+        // (row entries are completed with their fixed-point part here; the
+        // same thread reads entry J again below)
         double d = 0.0;
-        for (int J = threadIdx.x; J < M; J += kWG) d += row[J] - mat[(size_t)m * M + J];
+        for (int J = threadIdx.x; J < M; J += kWG) {
+            double v = row[J];
+            const unsigned long long f = fix[J];
+            if (f) {
+                v += (double)(long long)f * kFixInv;
+                row[J] = v;
+                fix[J] = 0ull;
+            }
+            d += v - mat[(size_t)m * M + J];
+        }
         d = wave_sum(d);
         if (lane == 0) red[wave] = d;
         __syncthreads();
@@ -1222,7 +1489,10 @@ __global__ __launch_bounds__(kWG) void prepare_finish_kernel(PairParams p)
         p.energy[r] = t;
         p.done[r] = 0u;
     }
-    for (int J = threadIdx.x; J < M; J += kWG) p.rowbuf[(size_t)r * M + J] = 0.0;
+    for (int J = threadIdx.x; J < M; J += kWG) {
+        p.rowbuf[(size_t)r * M + J] = 0.0;
+        p.rowfix[(size_t)r * M + J] = 0ull;
+    }
 }
 
 struct ExchangeParams {
@@ -1683,6 +1953,7 @@ struct cgppd_ctx {
     uint64_t seed = 0;
 
     Coef *d_table_raw = nullptr, *d_table_scaled = nullptr;
+    RowCoef *d_row_table = nullptr;
     int *d_mol_off = nullptr, *d_mol_of = nullptr;
     float4 *d_pos = nullptr, *d_rel = nullptr, *d_center = nullptr;
     float *d_radius = nullptr;
@@ -1749,6 +2020,8 @@ struct cgppd_ctx {
     int graph_steps = 0;
 
     int wg_per_cu = 0;
+    int pair_layout = CGPPD_LAYOUT_AUTO;  // caller's choice
+    int tuned_layout = 0, tuned_wg = 0;    // set by cgppd_autotune
     bool timing = false;
     std::vector<hipEvent_t> events;
     double t_mean_ms = 0.0;
@@ -1870,6 +2143,7 @@ void free_replicas(cgppd_ctx *ctx)
 int build_tables(cgppd_ctx *ctx)
 {
     std::vector<Coef> raw(400), scaled(400);
+    std::vector<RowCoef> rowt(400);
     for (int tj = 0; tj < 20; ++tj)
         for (int ti = 0; ti < 20; ++ti) {
             // src/Potential.cpp:14-17, float arithmetic as in the reference
@@ -1886,12 +2160,21 @@ int build_tables(cgppd_ctx *ctx)
             s.A = c.A * ctx->rt_to_kcal;
             s.qq = c.qq * ctx->dh_factor;
             scaled[tj * 20 + ti] = s;
+            RowCoef w;
+            const float s6 = c.sig2 * c.sig2 * c.sig2;
+            w.B = s.A * (s6 * s6);
+            w.C = s.A * s6;
+            w.rc2 = c.rc2;
+            w.K = -0.5f * s.A;
+            rowt[tj * 20 + ti] = w;
         }
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_table_raw, 400))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_table_scaled, 400))) return rc;
     CG_HIP(ctx, hipMemcpy(ctx->d_table_raw, raw.data(), sizeof(Coef) * 400, hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_table_scaled, scaled.data(), sizeof(Coef) * 400, hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(ctx, &ctx->d_row_table, 400))) return rc;
+    CG_HIP(ctx, hipMemcpy(ctx->d_row_table, rowt.data(), sizeof(RowCoef) * 400, hipMemcpyHostToDevice));
     return CGPPD_OK;
 }
 
@@ -2110,8 +2393,11 @@ PairParams base_params(cgppd_ctx *ctx)
     p.out_mat = nullptr;
     p.rt_to_kcal = ctx->rt_to_kcal;
     p.dh_factor = ctx->dh_factor;
+    p.row_table = ctx->d_row_table;
+    for (int t = 0; t < CGPPD_AA_COUNT; ++t) p.charge[t] = ctx->charge[t];
     p.seed = ctx->seed;
     p.rowbuf = ctx->d_rowbuf;
+    p.rowfix = reinterpret_cast<unsigned long long *>(ctx->d_rowbuf + (size_t)ctx->R * ctx->M);
     p.done = ctx->d_done;
     p.max_n = ctx->max_n;
     p.prop_pos = ctx->d_prop_pos;
@@ -2334,6 +2620,25 @@ int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
     return CGPPD_OK;
 }
 
+// Layout of the pair kernel: the caller's choice, else the autotuned one,
+// else j-lanes (flexible systems always run the i-lane kernel).
+int pair_layout_of(const cgppd_ctx *ctx)
+{
+    if (ctx->n_flex) return CGPPD_LAYOUT_ILANES;
+    if (ctx->pair_layout != CGPPD_LAYOUT_AUTO) return ctx->pair_layout;
+    return ctx->tuned_layout ? ctx->tuned_layout : CGPPD_LAYOUT_JLANES;
+}
+
+// Workgroups per CU: the caller's choice, else the autotuned one (when it
+// was tuned for this layout), else 6 for i-lanes (measured best of 4..8 on
+// a 40-chain system) and 8 for j-lanes (the occupancy limit).
+int pair_wg_per_cu(const cgppd_ctx *ctx, int layout)
+{
+    if (ctx->wg_per_cu > 0) return ctx->wg_per_cu;
+    if (ctx->tuned_wg > 0 && ctx->tuned_layout == layout) return ctx->tuned_wg;
+    return layout == CGPPD_LAYOUT_ILANES ? 6 : 8;
+}
+
 // ev_start/ev_stop (optional) are recorded by the dispatch itself, so their
 // elapsed time is the kernel's execution time without separate event packets.
 int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr)
@@ -2341,18 +2646,22 @@ int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_sto
     PairParams p = base_params(ctx);
     p.table = ctx->d_table_scaled;
     p.xcd_map = 1;
-    // 6 workgroups per CU measured best for the pair kernel (4: -9 %, 8: -6 %)
-    const int per_cu = ctx->wg_per_cu > 0 ? ctx->wg_per_cu : 6;
-    const int g = std::max(8, ctx->n_cu * per_cu / 8 * 8);
+    const int layout = pair_layout_of(ctx);
+    const int g = std::max(8, ctx->n_cu * pair_wg_per_cu(ctx, layout) / 8 * 8);
     if (ctx->n_flex) {
         if (ev_start)
             hipExtLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
         else
             hipLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+    } else if (layout == CGPPD_LAYOUT_ILANES) {
+        if (ev_start)
+            hipExtLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+        else
+            hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
     } else if (ev_start) {
-        hipExtLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+        hipExtLaunchKernelGGL(mc_row_kernel, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
     } else {
-        hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+        hipLaunchKernelGGL(mc_row_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
     }
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
@@ -2489,6 +2798,7 @@ int cgppd_destroy(cgppd_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_replicas(ctx);
     dev_free(ctx->d_table_raw);
+    dev_free(ctx->d_row_table);
     dev_free(ctx->d_table_scaled);
     dev_free(ctx->d_mol_off);
     dev_free(ctx->d_mol_of);
@@ -2755,7 +3065,8 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_rel, (size_t)R * S))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_center, (size_t)R * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_radius, (size_t)R * M))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_rowbuf, (size_t)R * M))) return rc;
+    // FP64 rows, then the fixed-point rows (same size)
+    if ((rc = dev_alloc(ctx, &ctx->d_rowbuf, (size_t)2 * R * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_emat, (size_t)R * M * M))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_energy, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_temp, R))) return rc;
@@ -2793,7 +3104,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     CG_HIP(ctx, hipMemcpy(ctx->d_label, lab.data(), sizeof(int) * R, hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_rank_of, rank_of.data(), sizeof(int) * R, hipMemcpyHostToDevice));
     CG_HIP(ctx, hipMemcpy(ctx->d_rep_at_rank, at_rank.data(), sizeof(int) * R, hipMemcpyHostToDevice));
-    CG_HIP(ctx, hipMemset(ctx->d_rowbuf, 0, sizeof(double) * R * M));
+    CG_HIP(ctx, hipMemset(ctx->d_rowbuf, 0, sizeof(double) * 2 * R * M));
     CG_HIP(ctx, hipMemset(ctx->d_emat, 0, sizeof(double) * R * M * M));
     CG_HIP(ctx, hipMemset(ctx->d_energy, 0, sizeof(double) * R));
     CG_HIP(ctx, hipMemset(ctx->d_done, 0, sizeof(unsigned) * R));
@@ -3154,6 +3465,83 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu)
     if (!ctx || wg_per_cu < 0 || wg_per_cu > 64) return CGPPD_E_ARG;
     ctx->wg_per_cu = wg_per_cu;
     destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout)
+{
+    if (!ctx || layout < CGPPD_LAYOUT_AUTO || layout > CGPPD_LAYOUT_JLANES) return CGPPD_E_ARG;
+    ctx->pair_layout = layout;
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    const int l = pair_layout_of(ctx);
+    if (layout) *layout = l;
+    if (wg_per_cu) *wg_per_cu = pair_wg_per_cu(ctx, l);
+    return CGPPD_OK;
+}
+
+int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, double *best_ms)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!ctx->mc_ready) return fail(ctx, CGPPD_E_STATE, "call cgppd_mc_prepare before tuning");
+    if (launches < 1 || launches > 100000) return fail(ctx, CGPPD_E_ARG, "launches must be in [1, 100000]");
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure_stage1(ctx))) return rc;
+    if ((rc = ensure_events(ctx, (size_t)2 * launches))) return rc;
+    // the caller's settings are lifted while candidates are timed
+    const int keep_layout = ctx->pair_layout, keep_wg = ctx->wg_per_cu;
+    const int layouts[2] = {CGPPD_LAYOUT_JLANES, CGPPD_LAYOUT_ILANES};
+    const int n_layouts = ctx->n_flex ? 1 : 2;
+    int best_l = 0, best_w = 0;
+    double best = 1e300;
+    for (int li = 0; li < n_layouts && rc == CGPPD_OK; ++li) {
+        for (int w = 2; w <= 8 && rc == CGPPD_OK; ++w) {
+            ctx->pair_layout = ctx->n_flex ? CGPPD_LAYOUT_AUTO : layouts[li];
+            ctx->wg_per_cu = w;
+            // one untimed launch, then the timed ones
+            if ((rc = launch_pair(ctx))) break;
+            for (int k = 0; k < launches; ++k)
+                if ((rc = launch_pair(ctx, ctx->events[2 * k], ctx->events[2 * k + 1]))) break;
+            if (rc) break;
+            if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+                rc = fail(ctx, CGPPD_E_HIP, "autotune: stream synchronisation failed");
+                break;
+            }
+            double sum = 0.0;
+            for (int k = 0; k < launches; ++k) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]) != hipSuccess) {
+                    rc = fail(ctx, CGPPD_E_HIP, "autotune: event timing failed");
+                    break;
+                }
+                sum += ms;
+            }
+            const double mean = sum / launches;
+            if (rc == CGPPD_OK && mean < best) {
+                best = mean;
+                best_l = ctx->n_flex ? CGPPD_LAYOUT_ILANES : layouts[li];
+                best_w = w;
+            }
+        }
+    }
+    ctx->pair_layout = keep_layout;
+    ctx->wg_per_cu = keep_wg;
+    // the timed launches added into the pending rows: clear them
+    const int zrc = zero_dev(ctx, ctx->d_rowbuf, (size_t)2 * ctx->R * ctx->M);
+    if (rc) return rc;
+    if (zrc) return zrc;
+    ctx->tuned_layout = best_l;
+    ctx->tuned_wg = best_w;
+    destroy_graph(ctx);
+    if (layout) *layout = best_l;
+    if (wg_per_cu) *wg_per_cu = best_w;
+    if (best_ms) *best_ms = best;
     return CGPPD_OK;
 }
 

@@ -292,6 +292,35 @@ class Engine:
     def set_launch(self, wg_per_cu):
         self._check(self.lib.cgppd_set_launch(self.h, int(wg_per_cu)), "set_launch")
 
+    LAYOUTS = {"auto": 0, "ilanes": 1, "jlanes": 2}
+
+    def set_pair_layout(self, layout):
+        """Pair-kernel layout: "auto", "ilanes" or "jlanes" (or 0 / 1 / 2)."""
+        code = self.LAYOUTS[layout] if isinstance(layout, str) else int(layout)
+        self._check(self.lib.cgppd_set_pair_layout(self.h, code), "set_pair_layout")
+
+    def _pair_launch(self):
+        lay, wg = C.c_int(), C.c_int()
+        self._check(self.lib.cgppd_get_pair_launch(self.h, C.byref(lay), C.byref(wg)), "get_pair_launch")
+        return lay.value, wg.value
+
+    def pair_layout(self):
+        """The layout the next pair-kernel launch uses ("ilanes" or "jlanes")."""
+        names = {v: k for k, v in self.LAYOUTS.items()}
+        return names[self._pair_launch()[0]]
+
+    def pair_wg_per_cu(self):
+        return self._pair_launch()[1]
+
+    def autotune(self, launches=20):
+        """Times the pair-kernel candidates on the pending proposals and keeps
+        the fastest; returns {"layout", "wg_per_cu", "ms"}."""
+        lay, wg, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.cgppd_autotune(self.h, int(launches), C.byref(lay), C.byref(wg), C.byref(ms)),
+                    "autotune")
+        names = {v: k for k, v in self.LAYOUTS.items()}
+        return {"layout": names[lay.value], "wg_per_cu": wg.value, "ms": ms.value}
+
     def set_image_shift(self, on):
         self._check(self.lib.cgppd_set_image_shift(self.h, int(bool(on))), "set_image_shift")
 

@@ -222,6 +222,30 @@ int cgppd_get_energy_matrix(cgppd_ctx *ctx, int replica, double *mat);
  * HIP events on the context stream (0 when not measured);
  * cgppd_set_timing(ctx, 1) enables the measurement. */
 int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
+
+/* Layout of the pair kernel for systems without flexible molecules.
+ * Replaces the reference's block-size choice (Replica::init_first_replica,
+ * src/Replica.cpp:59-68: 32 threads below 1024 residues, else 64, or -d).
+ *   CGPPD_LAYOUT_AUTO   the autotuned choice, else j-lanes
+ *   CGPPD_LAYOUT_ILANES lanes hold the moved molecule's residues, the other
+ *                       residues are staged through LDS in 256-residue chunks
+ *   CGPPD_LAYOUT_JLANES lanes hold the other residues, the moved molecule's
+ *                       residues are read as wave-uniform records; every
+ *                       wave takes an equal share of the work
+ * Systems with flexible molecules always use the i-lane kernel. */
+#define CGPPD_LAYOUT_AUTO 0
+#define CGPPD_LAYOUT_ILANES 1
+#define CGPPD_LAYOUT_JLANES 2
+int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout);
+/* The layout and workgroups per CU the next pair-kernel launch uses. */
+int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu);
+
+/* Autotuner (replaces the reference's fixed block size, same lines as
+ * above): times `launches` pair-kernel launches on the pending proposals
+ * (after cgppd_mc_prepare) for every layout and workgroups-per-CU
+ * candidate, keeps the fastest for this context and reports it (any
+ * pointer may be NULL).  The MC state is unchanged. */
+int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, double *best_ms);
 int cgppd_set_timing(cgppd_ctx *ctx, int enable);
 int cgppd_last_kernel_ms(cgppd_ctx *ctx, double *mean_ms, int64_t *launches);
 

@@ -126,3 +126,97 @@ def test_config1_mc_steps_keep_totals_consistent(gpu_engine_factory):
     assert (s["steps"] == 200).all()
     assert s["counters"].sum() == 200 * spec.n_rep
     e.close()
+
+
+@pytest.mark.parametrize("layout", ["ilanes", "jlanes"])
+def test_pair_layouts_match_oracle(gpu_engine_factory, layout):
+    """Both pair-kernel layouts give the oracle's trajectory; j-lanes also
+    on a system whose molecules are shorter than a wave (many molecules per
+    j-block) and with the per-pair minimum image forced."""
+    from cgppd_amd import synth
+    for spec in (small_spec(), synth.make_system("tiny", [3, 1, 9, 2, 70, 5, 1, 12], 60.0, 6, 288.0, 450.0, 7, 2.0,
+                                                 0.2)):
+        for shift in (True, False):
+            print(f"layout {layout} spec {spec.name} image_shift {shift}", flush=True)
+            e, st = gpu_and_oracle(gpu_engine_factory, spec, 41)
+            e.set_pair_layout(layout)
+            e.set_image_shift(shift)
+            e.mc_steps(120)
+            st.mc_steps(120)
+            s = e.stats()
+            # rotations of the one-residue molecules leave the energy
+            # unchanged: the oracle's delta is exactly 0 there and the FP32
+            # row's is a rounding residue of either sign, so only the split
+            # of accepted moves between "dE < 0" and "Metropolis" may differ
+            c, o = s["counters"], st.counters
+            np.testing.assert_array_equal(c[:, 0] + c[:, 1], o[:, 0] + o[:, 1])
+            np.testing.assert_array_equal(c[:, 2], o[:, 2])
+            if spec.name != "tiny":
+                np.testing.assert_array_equal(c, o)
+            np.testing.assert_allclose(s["energy"], st.energy, rtol=1e-5, atol=1e-8)
+            for r in range(spec.n_rep):
+                np.testing.assert_array_equal(e.coordinates(r)[0], st.pos[r])
+            e.close()
+
+
+def test_pair_layouts_agree_on_config1(gpu_engine_factory):
+    """40-chain system: the two layouts make the same decisions and their
+    running totals agree far below the parity bar."""
+    from cgppd_amd import synth
+    spec = synth.config(1)
+    out = {}
+    for layout in ("ilanes", "jlanes"):
+        e = gpu_engine_factory()
+        e.load_spec(spec, seed=12)
+        e.mc_prepare()
+        e.set_pair_layout(layout)
+        e.mc_steps(300)
+        s = e.stats()
+        out[layout] = (s, e.coordinates(5)[0])
+        e.close()
+    (sa, pa), (sb, pb) = out["ilanes"], out["jlanes"]
+    # molecules far from all others have |dE| near the FP32 rounding level:
+    # the decision agrees, its "dE < 0" / "Metropolis" label may not
+    ca, cb = sa["counters"], sb["counters"]
+    np.testing.assert_array_equal(ca[:, 0] + ca[:, 1], cb[:, 0] + cb[:, 1])
+    np.testing.assert_array_equal(ca[:, 2], cb[:, 2])
+    np.testing.assert_array_equal(pa, pb)
+    # the layouts use different FP32 forms of the pair term and different
+    # summation orders: totals of ~0.1 kcal/mol agree to a few 1e-8
+    np.testing.assert_allclose(sa["energy"], sb["energy"], rtol=2e-6, atol=1e-7)
+
+
+def test_jlanes_rows_are_deterministic(gpu_engine_factory):
+    """The j-lane kernel sums row partials in fixed point: two runs of the
+    same configuration agree bit for bit."""
+    from cgppd_amd import synth
+    spec = synth.config(1)
+    res = []
+    for _ in range(2):
+        e = gpu_engine_factory()
+        e.load_spec(spec, seed=4)
+        e.mc_prepare()
+        e.set_pair_layout("jlanes")
+        e.mc_steps(200)
+        res.append(e.stats()["energy"].copy())
+        e.close()
+    np.testing.assert_array_equal(res[0], res[1])
+
+
+def test_autotune_keeps_state(gpu_engine_factory):
+    """Tuning times the candidates on the pending proposals without changing
+    the MC state: the tuned engine follows the untuned engine's trajectory."""
+    spec = small_spec()
+    a, st = gpu_and_oracle(gpu_engine_factory, spec, 19)
+    best = a.autotune(launches=3)
+    print("autotune", best, flush=True)
+    assert best["layout"] in ("ilanes", "jlanes")
+    assert 2 <= best["wg_per_cu"] <= 8 and best["ms"] > 0
+    a.mc_steps(150)
+    st.mc_steps(150)
+    s = a.stats()
+    np.testing.assert_array_equal(s["counters"], st.counters)
+    np.testing.assert_allclose(s["energy"], st.energy, rtol=1e-5, atol=1e-8)
+    for r in range(spec.n_rep):
+        np.testing.assert_array_equal(a.coordinates(r)[0], st.pos[r])
+    a.close()
