@@ -276,6 +276,7 @@ struct PairParams {
     float4 *prop_center;  // R
     int *prop_mol;        // R
     int *prop_nib;        // R: i-blocks of the moved molecule
+    unsigned *prop_units; // R: (i, j-block) units of the row kernel for the pending proposal
     // flexible molecules (null / 0 when the system has none).  Residue
     // indices are device indices (global); lists are CSR per molecule.
     int n_flex;                 // number of flexible molecules
@@ -920,20 +921,34 @@ __device__ __forceinline__ f4_t scalar_rec(const float4 *base, int i)
     return ((const_f4 *)base)[i];
 }
 
+// Image handling of the row kernel, wave-uniform per j-block visit.  The
+// lanes' j coordinates are shifted by their molecule's centre image (the
+// multiple of L nearest to the moved molecule's centre), so that on every
+// axis |dx| < L.  On an axis where some lane's molecule pair is not
+// confined to one image (the bounding-sphere test of segment_shift), the
+// per-pair minimum image is min(|dx|, L - |dx|): the same value as
+// dx - L rint(dx / L) for |dx| < L, in two operations.  kImgFull: the
+// bounding spheres are too large for |dx| < L; unshifted coordinates and
+// the per-pair rint on every axis.
+constexpr int kImgFull = 8;
+
+template <int IMG>
+__device__ __forceinline__ float image_axis(float d, float box, float inv_box, int axis_bit)
+{
+    if (IMG == kImgFull) return fmaf(-box, rintf(d * inv_box), d);
+    if (IMG & axis_bit) return fminf(fabsf(d), box - fabsf(d));
+    return d;
+}
+
 // One pair term of the row kernel: wave-uniform i record (xi, yi, zi),
 // this lane's j (xj, yj, zj); qq = q_i q_j DH for charged i runs.
-template <bool MINIMG, bool CHARGED>
+template <int IMG, bool CHARGED>
 __device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj, float yj, float zj,
                                           const RowCoef &cf, float qq, float box, float inv_box)
 {
-    float dx = xi - xj;
-    float dy = yi - yj;
-    float dz = zi - zj;
-    if (MINIMG) {
-        dx = fmaf(-box, rintf(dx * inv_box), dx);
-        dy = fmaf(-box, rintf(dy * inv_box), dy);
-        dz = fmaf(-box, rintf(dz * inv_box), dz);
-    }
+    const float dx = image_axis<IMG>(xi - xj, box, inv_box, 1);
+    const float dy = image_axis<IMG>(yi - yj, box, inv_box, 2);
+    const float dz = image_axis<IMG>(zi - zj, box, inv_box, 4);
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     float rinv = 0.0f, rinv2;
     if (CHARGED) {
@@ -950,113 +965,109 @@ __device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj
     return e;
 }
 
-// A run [a, b) of i residues of one type (wave-uniform) against this lane's
-// j residue: terms summed plainly in groups of eight, group sums added into
-// a Kahan-compensated accumulator.
-template <bool MINIMG, bool CHARGED>
-__device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, float xj, float yj, float zj,
-                                        const RowCoef &cf, float qq, float box, float inv_box, float &s0, float &c0)
+// The two j residues a lane holds (64 apart) with their table entries and
+// Debye-Hueckel coefficients for the current run.
+struct LaneJ {
+    float x[2], y[2], z[2];
+    uint32_t col[2];
+    float q[2];
+};
+
+// A run [a, b) of i residues of one type (wave-uniform) against the lane's
+// two j residues: terms summed plainly in groups of up to eight per j,
+// group sums added into Kahan-compensated accumulators.  Records are read
+// eight at a time; the proposal buffer is padded so that a short group may
+// read past the molecule's last residue (those terms are skipped).
+template <int IMG, bool CHARGED>
+__device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, const LaneJ &lj,
+                                        const RowCoef &c0f, const RowCoef &c1f, float qq0, float qq1, float box,
+                                        float inv_box, float (&s)[2], float (&c)[2])
 {
-    int i = a;
-    for (; i + 8 <= b; i += 8) {
+    for (int i = a; i < b; i += 8) {
         f4_t r[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) r[u] = scalar_rec(irec, i + u);
-        float g = row_term<MINIMG, CHARGED>(r[0].x, r[0].y, r[0].z, xj, yj, zj, cf, qq, box, inv_box);
+        const int n = b - i;
+        float g0 = row_term<IMG, CHARGED>(r[0].x, r[0].y, r[0].z, lj.x[0], lj.y[0], lj.z[0], c0f, qq0, box, inv_box);
+        float g1 = row_term<IMG, CHARGED>(r[0].x, r[0].y, r[0].z, lj.x[1], lj.y[1], lj.z[1], c1f, qq1, box, inv_box);
 #pragma unroll
-        for (int u = 1; u < 8; ++u) g += row_term<MINIMG, CHARGED>(r[u].x, r[u].y, r[u].z, xj, yj, zj, cf, qq, box, inv_box);
-        kahan_add(s0, c0, g);
-    }
-    if (i < b) {
-        f4_t r0 = scalar_rec(irec, i);
-        float g = row_term<MINIMG, CHARGED>(r0.x, r0.y, r0.z, xj, yj, zj, cf, qq, box, inv_box);
-        for (++i; i < b; ++i) {
-            const f4_t ri = scalar_rec(irec, i);
-            g += row_term<MINIMG, CHARGED>(ri.x, ri.y, ri.z, xj, yj, zj, cf, qq, box, inv_box);
-        }
-        kahan_add(s0, c0, g);
+        for (int u = 1; u < 8; ++u)
+            if (u < n) {
+                g0 += row_term<IMG, CHARGED>(r[u].x, r[u].y, r[u].z, lj.x[0], lj.y[0], lj.z[0], c0f, qq0, box, inv_box);
+                g1 += row_term<IMG, CHARGED>(r[u].x, r[u].y, r[u].z, lj.x[1], lj.y[1], lj.z[1], c1f, qq1, box, inv_box);
+            }
+        kahan_add(s[0], c[0], g0);
+        kahan_add(s[1], c[1], g1);
     }
 }
 
-template <bool MINIMG>
+template <int IMG>
 __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec, int i0, int ia, int ib,
-                                         float xj, float yj, float zj, uint32_t col, float qj,
-                                         const char *__restrict__ tab, float &s0, float &c0)
+                                         const LaneJ &lj, const char *__restrict__ tab, float (&s)[2], float (&c)[2])
 {
     int run = __builtin_amdgcn_readfirstlane(p.run_of[i0 + ia]);
     int a = ia;
     while (a < ib) {
         const int b = __builtin_amdgcn_readfirstlane(min(ib, p.run_start[run + 1] - i0));
         const uint32_t code = __builtin_amdgcn_readfirstlane(p.run_code[run]);
-        const RowCoef cf = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | col));
+        const RowCoef c0f = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[0]));
+        const RowCoef c1f = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[1]));
         if (code & kChargedBit) {
-            const float qq = p.charge[(code & kRowMask) >> kRowShift] * p.dh_factor * qj;
-            row_run<MINIMG, true>(irec, a, b, xj, yj, zj, cf, qq, p.box, p.inv_box, s0, c0);
+            const float qi = p.charge[(code & kRowMask) >> kRowShift] * p.dh_factor;
+            row_run<IMG, true>(irec, a, b, lj, c0f, c1f, qi * lj.q[0], qi * lj.q[1], p.box, p.inv_box, s, c);
         } else {
-            row_run<MINIMG, false>(irec, a, b, xj, yj, zj, cf, 0.0f, p.box, p.inv_box, s0, c0);
+            row_run<IMG, false>(irec, a, b, lj, c0f, c1f, 0.0f, 0.0f, p.box, p.inv_box, s, c);
         }
         a = b;
         ++run;
     }
 }
 
+// Residues per j-block of the row kernel: two per lane.
+constexpr int kRowJ = 2 * kWave;
+
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) void mc_row_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ long long unit_off[CGPPD_MAX_REPLICAS + 1];
-    long long *part = unit_off;  // scan partials, consumed before unit_off is written
     // entry e = tj * 20 + ti (i = the moved residue) goes to row ti, column tj
     for (int e = threadIdx.x; e < CGPPD_AA_COUNT * CGPPD_AA_COUNT; e += kWG) {
         const int tj = e / CGPPD_AA_COUNT, ti = e % CGPPD_AA_COUNT;
         *reinterpret_cast<RowCoef *>(tab + ((ti << kRowShift) | (tj << kColShift))) = p.row_table[e];
     }
     const int R = p.rep_count;
-    // units per replica, then an exclusive scan (each thread owns up to
-    // four consecutive replicas)
-    constexpr int kPer = (CGPPD_MAX_REPLICAS + kWG - 1) / kWG;
-    long long own[kPer];
-    long long sum = 0;
+    // unit offsets of the replicas: an inclusive scan by wave 0, 64
+    // replicas at a time
+    if (threadIdx.x < kWave) {
+        const int lane0 = threadIdx.x;
+        long long carry = 0;
+        for (int base = 0; base < R; base += kWave) {
+            const int r = base + lane0;
+            long long c = 0;
+            if (r < R && !(p.da && !p.s1[r])) c = p.prop_units[r];
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        const int r = threadIdx.x * kPer + q;
-        long long c = 0;
-        if (r < R && !(p.da && !p.s1[r])) {
-            const int m = p.prop_mol[r];
-            const int n_m = p.mol_off[m + 1] - p.mol_off[m];
-            const int nj = p.n_res - n_m;
-            c = (long long)((nj + kWave - 1) / kWave) * n_m;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const long long t = __shfl_up(c, o, kWave);
+                if (lane0 >= o) c += t;
+            }
+            if (r < R) unit_off[r + 1] = carry + c;
+            carry += __shfl(c, kWave - 1, kWave);
         }
-        own[q] = c;
-        sum += c;
-    }
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (int o = 1; o < kWG; o <<= 1) {
-        const long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    {
-        long long run = part[threadIdx.x] - sum;
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int r = threadIdx.x * kPer + q;
-            if (r <= R) unit_off[r] = run;
-            run += own[q];
-        }
-        if (threadIdx.x == kWG - 1 && R == kPer * kWG) unit_off[R] = run;
+        if (lane0 == 0) unit_off[0] = 0;
     }
     __syncthreads();
     const long long total = unit_off[R];
     const int G = gridDim.x, b = blockIdx.x;
     const int lb = (p.xcd_map && (G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
     const int lane = threadIdx.x & (kWave - 1);
-    const long long W = (long long)G * kWavesPerWG;
-    const long long wv = (long long)lb * kWavesPerWG + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    long long P = total * wv / W;
-    const long long P1 = total * (wv + 1) / W;
+    const int W = G * kWavesPerWG;
+    const int wv = lb * kWavesPerWG + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    // share boundaries floor(total w / W) in FP64 (exact while total W
+    // < 2^53; every wave evaluates the same expression for a shared
+    // boundary, so the shares tile [0, total) exactly)
+    const double per = (double)total / (double)W;
+    long long P = wv == 0 ? 0 : (long long)floor((double)wv * per);
+    const long long P1 = wv + 1 == W ? total : (long long)floor((double)(wv + 1) * per);
     if (P >= P1) return;
     // replica holding unit P (binary search; wave-uniform)
     int r;
@@ -1069,50 +1080,101 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         r = __builtin_amdgcn_readfirstlane(lo);
     }
     const int M = p.n_mol;
+    // position of unit P inside its replica: j-block jb, residue ia of m
+    // (one division for the first visit, then the visits advance)
+    int jb = -1, ia = 0;
     while (P < P1) {
-        while (P >= unit_off[r + 1]) ++r;
+        if (jb < 0 || P >= unit_off[r + 1]) {
+            while (P >= unit_off[r + 1]) ++r;
+            jb = -1;
+        }
         const int m = __builtin_amdgcn_readfirstlane(p.prop_mol[r]);
         const int i0 = __builtin_amdgcn_readfirstlane(p.mol_off[m]);
         const int n_m = __builtin_amdgcn_readfirstlane(p.mol_off[m + 1]) - i0;
-        const long long local = P - unit_off[r];
-        const int jb = __builtin_amdgcn_readfirstlane((int)(local / n_m));
-        const int ia = __builtin_amdgcn_readfirstlane((int)(local % n_m));
+        if (jb < 0) {
+            // units per replica fit in 32 bits (set_replicas bounds them)
+            const unsigned local = (unsigned)(P - unit_off[r]);
+            jb = __builtin_amdgcn_readfirstlane((int)(local / (unsigned)n_m));
+            ia = __builtin_amdgcn_readfirstlane((int)(local % (unsigned)n_m));
+        }
         const int ib = __builtin_amdgcn_readfirstlane((int)min((long long)n_m, (long long)ia + (P1 - P)));
-        // this lane's j residue
+        // this lane's two j residues (jb * 128 + lane and + 64)
         const int nj = p.n_res - n_m;
-        const int jp = jb * kWave + lane;
-        const bool valid = jp < nj;
-        const int j = jp + (jp >= i0 ? n_m : 0);
-        float4 rj = make_float4(0.f, 0.f, 0.f, 0.f);
-        int J = 0x7fffffff;
-        if (valid) {
-            rj = p.pos[(size_t)r * p.stride + j];
-            J = p.mol_of[j];
+        LaneJ lj;
+        bool valid[2];
+        int J[2];
+        float4 rj[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int jp = jb * kRowJ + h * kWave + lane;
+            valid[h] = jp < nj;
+            const int j = jp + (jp >= i0 ? n_m : 0);
+            rj[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            J[h] = 0x7fffffff;
+            if (valid[h]) {
+                rj[h] = p.pos[(size_t)r * p.stride + j];
+                J[h] = p.mol_of[j];
+            }
+            lj.col[h] = lane_col(rj[h]);
+            lj.q[h] = p.charge[lj.col[h] >> kColShift];
         }
-        const uint32_t col = lane_col(rj);
-        const float qj = p.charge[col >> kColShift];
-        // one image shift for the whole visit when every lane's molecule
-        // allows it (the shift moves j next to m's proposed centre)
-        bool fast = false;
-        float sx = 0.f, sy = 0.f, sz = 0.f;
-        if (p.fast_image && valid) {
+        // image handling of this visit (see image_axis): the centre image
+        // of each j residue's molecule, and the axes on which some lane
+        // needs the per-pair minimum image
+        int img = kImgFull;
+        float sh[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        if (p.fast_image) {
+            int need = 0;
             const float4 cm = p.prop_center[r];
-            fast = segment_shift(cm, p.center[(size_t)r * M + J],
-                                 p.radius[(size_t)r * M + m] + p.radius[(size_t)r * M + J], p.box, p.inv_box, sx, sy,
-                                 sz);
+            const float rm = p.radius[(size_t)r * M + m];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (!valid[h]) continue;
+                const float4 cj = p.center[(size_t)r * M + J[h]];
+                const float rsum = rm + p.radius[(size_t)r * M + J[h]];
+                const float lim = 0.5f * p.box - (rsum * 1.0001f + 0.01f);
+                const float kx = rintf((cm.x - cj.x) * p.inv_box), ky = rintf((cm.y - cj.y) * p.inv_box),
+                            kz = rintf((cm.z - cj.z) * p.inv_box);
+                const float ex = fmaf(-p.box, kx, cm.x - cj.x), ey = fmaf(-p.box, ky, cm.y - cj.y),
+                            ez = fmaf(-p.box, kz, cm.z - cj.z);
+                sh[h][0] = p.box * kx;
+                sh[h][1] = p.box * ky;
+                sh[h][2] = p.box * kz;
+                need |= (fabsf(ex) < lim ? 0 : 1) | (fabsf(ey) < lim ? 0 : 2) | (fabsf(ez) < lim ? 0 : 4);
+                // |dx| < L after the shift needs rsum < L/2 (with margin)
+                if (!(rsum * 1.0001f + 0.01f < 0.45f * p.box)) need |= kImgFull;
+            }
+            const bool full = __any(need & kImgFull);
+            img = full ? kImgFull
+                       : (__any(need & 1) ? 1 : 0) | (__any(need & 2) ? 2 : 0) | (__any(need & 4) ? 4 : 0);
         }
-        const bool all_fast = __all(fast || !valid);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bool shift = img != kImgFull;
+            lj.x[h] = shift ? rj[h].x + sh[h][0] : rj[h].x;
+            lj.y[h] = shift ? rj[h].y + sh[h][1] : rj[h].y;
+            lj.z[h] = shift ? rj[h].z + sh[h][2] : rj[h].z;
+        }
         const float4 *irec = p.prop_pos + (size_t)r * p.max_n;
-        float s0 = 0.f, c0 = 0.f;
-        if (all_fast)
-            row_runs<false>(p, irec, i0, ia, ib, rj.x + sx, rj.y + sy, rj.z + sz, col, qj, tab, s0, c0);
-        else
-            row_runs<true>(p, irec, i0, ia, ib, rj.x, rj.y, rj.z, col, qj, tab, s0, c0);
-        // FP64 reduction per molecule J (J is non-decreasing across the lanes)
-        const double v = valid ? (double)s0 - (double)c0 : 0.0;
-        int Jc = __builtin_amdgcn_readfirstlane(J);
+        float sa[2] = {0.f, 0.f}, ca[2] = {0.f, 0.f};
+        switch (img) {
+        case 0: row_runs<0>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 1: row_runs<1>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 2: row_runs<2>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 3: row_runs<3>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 4: row_runs<4>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 5: row_runs<5>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 6: row_runs<6>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 7: row_runs<7>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        default: row_runs<kImgFull>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        }
+        // FP64 reduction per molecule J: J[0] and then J[1] are
+        // non-decreasing across the lanes, and J[0] <= J[1] lane by lane
+        const double v0 = valid[0] ? (double)sa[0] - (double)ca[0] : 0.0;
+        const double v1 = valid[1] ? (double)sa[1] - (double)ca[1] : 0.0;
+        int Jc = __builtin_amdgcn_readfirstlane(J[0]);
         for (;;) {
-            const double t = wave_sum(J == Jc ? v : 0.0);
+            const double t = wave_sum((J[0] == Jc ? v0 : 0.0) + (J[1] == Jc ? v1 : 0.0));
             if (lane == 0) {
                 const size_t at = (size_t)r * M + Jc;
                 if (fabs(t) < kFixMax)
@@ -1120,12 +1182,32 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 else
                     unsafeAtomicAdd(p.rowbuf + at, t);
             }
-            const unsigned long long rest = __ballot(valid && J > Jc);
-            if (rest == 0) break;
-            Jc = __builtin_amdgcn_readlane(J, __builtin_ctzll(rest));
+            const unsigned long long rest0 = __ballot(valid[0] && J[0] > Jc);
+            if (rest0) {
+                Jc = __builtin_amdgcn_readlane(J[0], __builtin_ctzll(rest0));
+                continue;
+            }
+            const unsigned long long rest1 = __ballot(valid[1] && J[1] > Jc);
+            if (rest1 == 0) break;
+            Jc = __builtin_amdgcn_readlane(J[1], __builtin_ctzll(rest1));
         }
         P += ib - ia;
+        // the next visit starts at the next j-block of this replica, or at
+        // the first unit of a later one
+        if (ib == n_m) {
+            ++jb;
+            ia = 0;
+        } else {
+            ia = ib;
+        }
     }
+}
+
+// Units of the row kernel for a moved molecule of n residues: j-blocks
+// (kRowJ residues) of the other residues times n (32 bits: the host keeps the product in range).
+__device__ __forceinline__ unsigned row_units(const PairParams &p, int n)
+{
+    return (unsigned)((p.n_res - n + kRowJ - 1) / kRowJ) * (unsigned)n;
 }
 
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
@@ -1299,6 +1381,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     if (threadIdx.x == 0) {
         p.prop_mol[r] = m;
         p.prop_nib[r] = (n + kIBlock - 1) / kIBlock;
+        p.prop_units[r] = row_units(p, n);
         p.prop_center[r] = make_float4(c0, c1, c2, 0.0f);
     }
 }
@@ -1333,6 +1416,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
         if (threadIdx.x == 0) {
             p.prop_mol[r] = pr.mol;
             p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
+            p.prop_units[r] = row_units(p, i_end - i_beg);
             p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
         }
         return;
@@ -1349,6 +1433,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
     if (threadIdx.x == 0) {
         p.prop_mol[r] = pr.mol;
         p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
+        p.prop_units[r] = row_units(p, i_end - i_beg);
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
 }
@@ -1457,6 +1542,146 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
     if (p.da && !propose_only) {
         __syncthreads();  // the proposal is laid out
         stage_one(p, r);
+    }
+}
+
+// MC step, phase 2 for systems without flexible molecules and without
+// delayed acceptance (one workgroup per replica): the same Metropolis,
+// apply and next proposal as mc_finish_kernel, with every global load the
+// step needs issued at the start.  The next step's molecule is known from
+// its Philox counter before the decision, so its relative positions and
+// centre are read together with the row, the matrix row and the pending
+// proposal; they are read again only when the next molecule is the one
+// just moved and the move was accepted.
+__global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[kWavesPerWG];
+    __shared__ int accept_flag;
+    const int r = blockIdx.x, t = threadIdx.x;
+    const int M = p.n_mol;
+    const int64_t step = p.steps[r];
+    const int m = p.prop_mol[r];
+    const int i_beg = p.mol_off[m], n_m = p.mol_off[m + 1] - i_beg;
+    const Proposal nx = draw_proposal(p.seed, r, step + 1, M);
+    const int m2 = nx.mol;
+    const int j_beg = p.mol_off[m2], n_2 = p.mol_off[m2 + 1] - j_beg;
+    double *row = p.rowbuf + (size_t)r * M;
+    unsigned long long *fix = p.rowfix + (size_t)r * M;
+    double *mat = p.emat + (size_t)r * M * M;
+    float4 *pos = p.pos + (size_t)r * p.stride;
+    float4 *rel = p.rel + (size_t)r * p.stride;
+    const float4 *pp = p.prop_pos + (size_t)r * p.max_n;
+    const float4 *pq = p.prop_rel + (size_t)r * p.max_n;
+    // loads of the whole step (the first element of each per-thread loop)
+    double rv = 0.0, mv_old = 0.0;
+    unsigned long long fv = 0ull;
+    if (t < M) {
+        rv = row[t];
+        fv = fix[t];
+        mv_old = mat[(size_t)m * M + t];
+    }
+    float4 a_pos = make_float4(0.f, 0.f, 0.f, 0.f), a_rel = a_pos, n_rel = a_pos;
+    if (t < n_m) {
+        a_pos = pp[t];
+        a_rel = pq[t];
+    }
+    if (t < n_2) n_rel = rel[j_beg + t];
+    float4 c2 = p.center[(size_t)r * M + m2];
+    const float tstep = p.tstep[r];
+    const double rcos = p.rcos[r], rsin = p.rsin[r];
+    // Metropolis (src/Replica.cpp:385-410, delta and exponent in FP64)
+    double d = 0.0;
+    if (t < M) {
+        if (fv) rv += (double)(long long)fv * kFixInv;
+        d = rv - mv_old;
+    }
+    for (int J = t + kWG; J < M; J += kWG) {
+        double v = row[J];
+        const unsigned long long f = fix[J];
+        if (f) v += (double)(long long)f * kFixInv;
+        d += v - mat[(size_t)m * M + J];
+    }
+    d = wave_sum(d);
+    if ((t & (kWave - 1)) == 0) red[t / kWave] = d;
+    __syncthreads();
+    if (t == 0) {
+        double delta = 0.0;
+        for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
+        const Proposal pr = draw_proposal(p.seed, r, step, M);
+        long long *cnt = p.counters + 3 * (size_t)r;
+        int acc = 0;
+        if (delta < 0.0) {
+            acc = 1;
+            cnt[0] += 1;
+        } else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r]))) {
+            acc = 1;
+            cnt[1] += 1;
+        } else {
+            cnt[2] += 1;
+        }
+        const unsigned long long nm = (unsigned long long)n_m;
+        atomicAdd(p.pair_evals, nm * ((unsigned long long)p.n_res - nm));
+        if (acc) p.energy[r] += delta;
+        accept_flag = acc;
+        p.steps[r] = step + 1;
+    }
+    __syncthreads();
+    const int acc = accept_flag;
+    if (acc) {
+        // apply (src/Molecule.cpp:468-504) and the matrix row and column
+        if (t < M) {
+            mat[(size_t)m * M + t] = rv;
+            mat[(size_t)t * M + m] = rv;
+        }
+        for (int J = t + kWG; J < M; J += kWG) {
+            double v = row[J];
+            const unsigned long long f = fix[J];
+            if (f) v += (double)(long long)f * kFixInv;
+            mat[(size_t)m * M + J] = v;
+            mat[(size_t)J * M + m] = v;
+        }
+        if (t < n_m) {
+            pos[i_beg + t] = a_pos;
+            rel[i_beg + t] = a_rel;
+        }
+        for (int k = t + kWG; k < n_m; k += kWG) {
+            pos[i_beg + k] = pp[k];
+            rel[i_beg + k] = pq[k];
+        }
+        if (t == 0) p.center[(size_t)r * M + m] = p.prop_center[r];
+    }
+    __syncthreads();  // every thread read the row (and the proposal) before they are replaced
+    for (int J = t; J < M; J += kWG) {
+        row[J] = 0.0;
+        fix[J] = 0ull;
+    }
+    // the next proposal (src/Molecule.cpp:146-266)
+    if (acc && m2 == m) {
+        __syncthreads();  // the applied move is in place
+        if (t < n_2) n_rel = rel[j_beg + t];
+        c2 = p.center[(size_t)r * M + m2];
+    }
+    const Move mv = make_move(nx, c2, tstep, rcos, rsin, p.box);
+    float4 *qp = p.prop_pos + (size_t)r * p.max_n;
+    float4 *qq = p.prop_rel + (size_t)r * p.max_n;
+    if (t < n_2) {
+        float4 rn, pn;
+        move_residue(mv, n_rel, rn, pn);
+        qp[t] = pn;
+        qq[t] = rn;
+    }
+    for (int k = t + kWG; k < n_2; k += kWG) {
+        float4 rn, pn;
+        move_residue(mv, rel[j_beg + k], rn, pn);
+        qp[k] = pn;
+        qq[k] = rn;
+    }
+    if (t == 0) {
+        p.prop_mol[r] = m2;
+        p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
+        p.prop_units[r] = row_units(p, n_2);
+        p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
 }
 
@@ -1964,6 +2189,7 @@ struct cgppd_ctx {
     unsigned *d_done = nullptr;
     float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr, *d_prop_delta = nullptr;
     int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
+    unsigned *d_prop_units = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
     unsigned long long *d_ex_stats = nullptr;
     unsigned long long *d_pair_evals = nullptr;
@@ -2110,6 +2336,7 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_prop_center);
     dev_free(ctx->d_prop_mol);
     dev_free(ctx->d_prop_nib);
+    dev_free(ctx->d_prop_units);
     dev_free(ctx->d_counters);
     dev_free(ctx->d_steps);
     dev_free(ctx->d_ex_round);
@@ -2406,6 +2633,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.prop_center = ctx->d_prop_center;
     p.prop_mol = ctx->d_prop_mol;
     p.prop_nib = ctx->d_prop_nib;
+    p.prop_units = ctx->d_prop_units;
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
     p.temperature = ctx->d_temp;
@@ -2625,6 +2853,8 @@ int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
 int pair_layout_of(const cgppd_ctx *ctx)
 {
     if (ctx->n_flex) return CGPPD_LAYOUT_ILANES;
+    // the row kernel counts one replica's units in 32 bits
+    if ((long long)((ctx->N + kWave - 1) / kWave) * ctx->max_n >= (1ll << 32)) return CGPPD_LAYOUT_ILANES;
     if (ctx->pair_layout != CGPPD_LAYOUT_AUTO) return ctx->pair_layout;
     return ctx->tuned_layout ? ctx->tuned_layout : CGPPD_LAYOUT_JLANES;
 }
@@ -2670,7 +2900,10 @@ int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_sto
 int launch_finish(cgppd_ctx *ctx, int propose_only)
 {
     PairParams p = base_params(ctx);
-    hipLaunchKernelGGL(mc_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, propose_only);
+    if (!propose_only && !ctx->n_flex && !ctx->da)
+        hipLaunchKernelGGL(mc_finish_rigid_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p);
+    else
+        hipLaunchKernelGGL(mc_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, propose_only);
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
@@ -3074,12 +3307,14 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_rsin, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_tstep, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_done, R))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_prop_pos, (size_t)R * ctx->max_n))) return rc;
+    // 8 records of padding: the row kernel reads i records in groups of 8
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_pos, (size_t)R * ctx->max_n + 8))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_rel, (size_t)R * ctx->max_n))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_delta, (size_t)R * ctx->max_n))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_center, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_mol, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_units, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_counters, (size_t)3 * R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_steps, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_ex_round, 1))) return rc;

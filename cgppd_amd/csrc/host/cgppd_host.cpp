@@ -1070,6 +1070,19 @@ void Simulation::initSamplingFiles()
     std::fprintf(exchangeFrequencyFile, "# step exchange_fraction accumulative_fraction\n");
 }
 
+// src/Simulation.cpp:1181-1194: the molecule files of the run, in order
+void Simulation::writeFileIndex()
+{
+    const std::string path = "output/" + parameters.prefix + "/fileindex";
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) throw error("cannot create " + path);
+    std::fprintf(f, "index molecule_file_path crowder(Y/N)\n");
+    for (size_t i = 0; i < parameters.mdata.size(); ++i)
+        std::fprintf(f, "%2d %s %s", (int)i, parameters.mdata[i].pdbfilename.c_str(),
+                     parameters.mdata[i].crowder ? "Y\n" : "N\n");
+    std::fclose(f);
+}
+
 void Simulation::closeSamplingFiles()
 {
     for (FILE **f : {&fractionBoundFile, &acceptanceRatioFile, &exchangeFrequencyFile, &boundConformationsFile}) {
@@ -1117,6 +1130,7 @@ void Simulation::run()
 {
     if (!dev_) setup_device(parameters.gpuoffset);
     initSamplingFiles();
+    writeFileIndex();
     dev_->check(cgppd_mc_prepare(dev_->ctx()), "cgppd_mc_prepare");
     const int per = std::max(1, parameters.MCsteps / parameters.REsteps);
     const int R = (int)replica.size();

@@ -337,6 +337,7 @@ private:
          *boundConformationsFile = nullptr;
     void initSamplingFiles();
     void closeSamplingFiles();
+    void writeFileIndex();
     void sync_from_device();
 };
 

@@ -222,3 +222,9 @@ def test_cli_runs(tmp_path, gpu_engine_factory):
     assert res.returncode == 0, res.stderr
     assert "2 replicas" in res.stdout
     assert (tmp_path / "output").exists()
+    # src/Simulation.cpp:1181-1194: one line per molecule file after the header
+    idx = list((tmp_path / "output").glob("*/fileindex"))
+    assert len(idx) == 1
+    lines = idx[0].read_text().splitlines()
+    assert lines[0] == "index molecule_file_path crowder(Y/N)"
+    assert len(lines) == 1 + spec.n_mol and all(l.split()[-1] == "N" for l in lines[1:])
