@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--exchange-every", type=int, default=1000)
     ap.add_argument("--replicas", type=int, default=0, help="override the configuration's replica count")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="workgroups per CU of the pair kernel (0: tuned)")
-    ap.add_argument("--layout", default="auto", choices=["auto", "ilanes", "jlanes"],
+    ap.add_argument("--layout", default="auto", choices=["auto", "ilanes", "jlanes", "jlanes4"],
                     help="pair-kernel layout (auto: tuned)")
     ap.add_argument("--no-autotune", action="store_true", help="skip the pair-kernel autotuner")
     ap.add_argument("--no-image-shift", action="store_true", help="per-pair minimum image everywhere")
@@ -68,7 +68,7 @@ def parse():
     return ap.parse_args()
 
 
-KERNEL_NAMES = {"ilanes": "mc_pair_kernel", "jlanes": "mc_row_kernel"}
+KERNEL_NAMES = {"ilanes": "mc_pair_kernel", "jlanes": "mc_row_kernel", "jlanes4": "mc_row_kernel"}
 
 
 def committed_traffic(kernel, workload):

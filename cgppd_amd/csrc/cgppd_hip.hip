@@ -276,7 +276,8 @@ struct PairParams {
     float4 *prop_center;  // R
     int *prop_mol;        // R
     int *prop_nib;        // R: i-blocks of the moved molecule
-    unsigned *prop_units; // R: (i, j-block) units of the row kernel for the pending proposal
+    int *prop_n;          // R: residues of the moved molecule
+    int row_j;            // j residues per j-block of the row kernel (64 x residues per lane)
     // flexible molecules (null / 0 when the system has none).  Residue
     // indices are device indices (global); lists are CSR per molecule.
     int n_flex;                 // number of flexible molecules
@@ -965,68 +966,78 @@ __device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj
     return e;
 }
 
-// The two j residues a lane holds (64 apart) with their table entries and
+// The JPL j residues a lane holds (64 apart) with their table entries and
 // Debye-Hueckel coefficients for the current run.
+template <int JPL>
 struct LaneJ {
-    float x[2], y[2], z[2];
-    uint32_t col[2];
-    float q[2];
+    float x[JPL], y[JPL], z[JPL];
+    uint32_t col[JPL];
+    float q[JPL];
 };
 
 // A run [a, b) of i residues of one type (wave-uniform) against the lane's
-// two j residues: terms summed plainly in groups of up to eight per j,
-// group sums added into Kahan-compensated accumulators.  Records are read
-// eight at a time; the proposal buffer is padded so that a short group may
-// read past the molecule's last residue (those terms are skipped).
-template <int IMG, bool CHARGED>
-__device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, const LaneJ &lj,
-                                        const RowCoef &c0f, const RowCoef &c1f, float qq0, float qq1, float box,
-                                        float inv_box, float (&s)[2], float (&c)[2])
+// j residues: terms summed plainly in groups of up to eight per j, group
+// sums added into Kahan-compensated accumulators.  Records are read eight
+// at a time; the proposal buffer is padded so that a short group may read
+// past the molecule's last residue (those terms are skipped).
+template <int IMG, bool CHARGED, int JPL>
+__device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, const LaneJ<JPL> &lj,
+                                        const RowCoef (&cf)[JPL], const float (&qq)[JPL], float box, float inv_box,
+                                        float (&s)[JPL], float (&c)[JPL])
 {
     for (int i = a; i < b; i += 8) {
         f4_t r[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) r[u] = scalar_rec(irec, i + u);
         const int n = b - i;
-        float g0 = row_term<IMG, CHARGED>(r[0].x, r[0].y, r[0].z, lj.x[0], lj.y[0], lj.z[0], c0f, qq0, box, inv_box);
-        float g1 = row_term<IMG, CHARGED>(r[0].x, r[0].y, r[0].z, lj.x[1], lj.y[1], lj.z[1], c1f, qq1, box, inv_box);
+        float g[JPL];
+#pragma unroll
+        for (int h = 0; h < JPL; ++h)
+            g[h] = row_term<IMG, CHARGED>(r[0].x, r[0].y, r[0].z, lj.x[h], lj.y[h], lj.z[h], cf[h], qq[h], box, inv_box);
 #pragma unroll
         for (int u = 1; u < 8; ++u)
             if (u < n) {
-                g0 += row_term<IMG, CHARGED>(r[u].x, r[u].y, r[u].z, lj.x[0], lj.y[0], lj.z[0], c0f, qq0, box, inv_box);
-                g1 += row_term<IMG, CHARGED>(r[u].x, r[u].y, r[u].z, lj.x[1], lj.y[1], lj.z[1], c1f, qq1, box, inv_box);
+#pragma unroll
+                for (int h = 0; h < JPL; ++h)
+                    g[h] += row_term<IMG, CHARGED>(r[u].x, r[u].y, r[u].z, lj.x[h], lj.y[h], lj.z[h], cf[h], qq[h],
+                                                   box, inv_box);
             }
-        kahan_add(s[0], c[0], g0);
-        kahan_add(s[1], c[1], g1);
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], g[h]);
     }
 }
 
-template <int IMG>
+template <int IMG, int JPL>
 __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec, int i0, int ia, int ib,
-                                         const LaneJ &lj, const char *__restrict__ tab, float (&s)[2], float (&c)[2])
+                                         const LaneJ<JPL> &lj, const char *__restrict__ tab, float (&s)[JPL],
+                                         float (&c)[JPL])
 {
     int run = __builtin_amdgcn_readfirstlane(p.run_of[i0 + ia]);
     int a = ia;
     while (a < ib) {
         const int b = __builtin_amdgcn_readfirstlane(min(ib, p.run_start[run + 1] - i0));
         const uint32_t code = __builtin_amdgcn_readfirstlane(p.run_code[run]);
-        const RowCoef c0f = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[0]));
-        const RowCoef c1f = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[1]));
+        RowCoef cf[JPL];
+        float qq[JPL];
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) cf[h] = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[h]));
         if (code & kChargedBit) {
             const float qi = p.charge[(code & kRowMask) >> kRowShift] * p.dh_factor;
-            row_run<IMG, true>(irec, a, b, lj, c0f, c1f, qi * lj.q[0], qi * lj.q[1], p.box, p.inv_box, s, c);
+#pragma unroll
+            for (int h = 0; h < JPL; ++h) qq[h] = qi * lj.q[h];
+            row_run<IMG, true, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
         } else {
-            row_run<IMG, false>(irec, a, b, lj, c0f, c1f, 0.0f, 0.0f, p.box, p.inv_box, s, c);
+#pragma unroll
+            for (int h = 0; h < JPL; ++h) qq[h] = 0.0f;
+            row_run<IMG, false, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
         }
         a = b;
         ++run;
     }
 }
 
-// Residues per j-block of the row kernel: two per lane.
-constexpr int kRowJ = 2 * kWave;
-
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) void mc_row_kernel(PairParams p)
+template <int JPL>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4 : 8, 8))) void mc_row_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ long long unit_off[CGPPD_MAX_REPLICAS + 1];
@@ -1044,7 +1055,10 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         for (int base = 0; base < R; base += kWave) {
             const int r = base + lane0;
             long long c = 0;
-            if (r < R && !(p.da && !p.s1[r])) c = p.prop_units[r];
+            if (r < R && !(p.da && !p.s1[r])) {
+                const int n = p.prop_n[r];
+                c = (long long)((p.n_res - n + JPL * kWave - 1) / (JPL * kWave)) * n;
+            }
 #pragma unroll
             for (int o = 1; o < kWave; o <<= 1) {
                 const long long t = __shfl_up(c, o, kWave);
@@ -1098,15 +1112,15 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             ia = __builtin_amdgcn_readfirstlane((int)(local % (unsigned)n_m));
         }
         const int ib = __builtin_amdgcn_readfirstlane((int)min((long long)n_m, (long long)ia + (P1 - P)));
-        // this lane's two j residues (jb * 128 + lane and + 64)
+        // this lane's j residues (jb * 64 JPL + 64 h + lane, h < JPL)
         const int nj = p.n_res - n_m;
-        LaneJ lj;
-        bool valid[2];
-        int J[2];
-        float4 rj[2];
+        LaneJ<JPL> lj;
+        bool valid[JPL];
+        int J[JPL];
+        float4 rj[JPL];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int jp = jb * kRowJ + h * kWave + lane;
+        for (int h = 0; h < JPL; ++h) {
+            const int jp = jb * (JPL * kWave) + h * kWave + lane;
             valid[h] = jp < nj;
             const int j = jp + (jp >= i0 ? n_m : 0);
             rj[h] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1122,13 +1136,14 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         // of each j residue's molecule, and the axes on which some lane
         // needs the per-pair minimum image
         int img = kImgFull;
-        float sh[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        float sh[JPL][3];
         if (p.fast_image) {
             int need = 0;
             const float4 cm = p.prop_center[r];
             const float rm = p.radius[(size_t)r * M + m];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < JPL; ++h) {
+                sh[h][0] = sh[h][1] = sh[h][2] = 0.0f;
                 if (!valid[h]) continue;
                 const float4 cj = p.center[(size_t)r * M + J[h]];
                 const float rsum = rm + p.radius[(size_t)r * M + J[h]];
@@ -1148,33 +1163,39 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             img = full ? kImgFull
                        : (__any(need & 1) ? 1 : 0) | (__any(need & 2) ? 2 : 0) | (__any(need & 4) ? 4 : 0);
         }
+        const bool shift = img != kImgFull;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const bool shift = img != kImgFull;
+        for (int h = 0; h < JPL; ++h) {
             lj.x[h] = shift ? rj[h].x + sh[h][0] : rj[h].x;
             lj.y[h] = shift ? rj[h].y + sh[h][1] : rj[h].y;
             lj.z[h] = shift ? rj[h].z + sh[h][2] : rj[h].z;
         }
         const float4 *irec = p.prop_pos + (size_t)r * p.max_n;
-        float sa[2] = {0.f, 0.f}, ca[2] = {0.f, 0.f};
+        float sa[JPL], ca[JPL];
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) sa[h] = ca[h] = 0.0f;
         switch (img) {
-        case 0: row_runs<0>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 1: row_runs<1>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 2: row_runs<2>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 3: row_runs<3>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 4: row_runs<4>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 5: row_runs<5>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 6: row_runs<6>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 7: row_runs<7>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        default: row_runs<kImgFull>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 0: row_runs<0, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 1: row_runs<1, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 2: row_runs<2, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 3: row_runs<3, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 4: row_runs<4, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 5: row_runs<5, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 6: row_runs<6, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 7: row_runs<7, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        default: row_runs<kImgFull, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
         }
-        // FP64 reduction per molecule J: J[0] and then J[1] are
-        // non-decreasing across the lanes, and J[0] <= J[1] lane by lane
-        const double v0 = valid[0] ? (double)sa[0] - (double)ca[0] : 0.0;
-        const double v1 = valid[1] ? (double)sa[1] - (double)ca[1] : 0.0;
+        // FP64 reduction per molecule J: across the lanes J[h] is
+        // non-decreasing for each h, and every J[h] <= every J[h + 1]
+        double v[JPL];
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) v[h] = valid[h] ? (double)sa[h] - (double)ca[h] : 0.0;
         int Jc = __builtin_amdgcn_readfirstlane(J[0]);
         for (;;) {
-            const double t = wave_sum((J[0] == Jc ? v0 : 0.0) + (J[1] == Jc ? v1 : 0.0));
+            double part = 0.0;
+#pragma unroll
+            for (int h = 0; h < JPL; ++h) part += J[h] == Jc ? v[h] : 0.0;
+            const double t = wave_sum(part);
             if (lane == 0) {
                 const size_t at = (size_t)r * M + Jc;
                 if (fabs(t) < kFixMax)
@@ -1182,14 +1203,15 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 else
                     unsafeAtomicAdd(p.rowbuf + at, t);
             }
-            const unsigned long long rest0 = __ballot(valid[0] && J[0] > Jc);
-            if (rest0) {
-                Jc = __builtin_amdgcn_readlane(J[0], __builtin_ctzll(rest0));
-                continue;
+            int next = -1;
+#pragma unroll
+            for (int h = 0; h < JPL; ++h) {
+                if (next >= 0) break;
+                const unsigned long long rest = __ballot(valid[h] && J[h] > Jc);
+                if (rest) next = __builtin_amdgcn_readlane(J[h], __builtin_ctzll(rest));
             }
-            const unsigned long long rest1 = __ballot(valid[1] && J[1] > Jc);
-            if (rest1 == 0) break;
-            Jc = __builtin_amdgcn_readlane(J[1], __builtin_ctzll(rest1));
+            if (next < 0) break;
+            Jc = next;
         }
         P += ib - ia;
         // the next visit starts at the next j-block of this replica, or at
@@ -1201,13 +1223,6 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             ia = ib;
         }
     }
-}
-
-// Units of the row kernel for a moved molecule of n residues: j-blocks
-// (kRowJ residues) of the other residues times n (32 bits: the host keeps the product in range).
-__device__ __forceinline__ unsigned row_units(const PairParams &p, int n)
-{
-    return (unsigned)((p.n_res - n + kRowJ - 1) / kRowJ) * (unsigned)n;
 }
 
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
@@ -1381,7 +1396,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     if (threadIdx.x == 0) {
         p.prop_mol[r] = m;
         p.prop_nib[r] = (n + kIBlock - 1) / kIBlock;
-        p.prop_units[r] = row_units(p, n);
+        p.prop_n[r] = n;
         p.prop_center[r] = make_float4(c0, c1, c2, 0.0f);
     }
 }
@@ -1416,7 +1431,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
         if (threadIdx.x == 0) {
             p.prop_mol[r] = pr.mol;
             p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
-            p.prop_units[r] = row_units(p, i_end - i_beg);
+            p.prop_n[r] = i_end - i_beg;
             p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
         }
         return;
@@ -1433,7 +1448,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
     if (threadIdx.x == 0) {
         p.prop_mol[r] = pr.mol;
         p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
-        p.prop_units[r] = row_units(p, i_end - i_beg);
+        p.prop_n[r] = i_end - i_beg;
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
 }
@@ -1680,7 +1695,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
     if (t == 0) {
         p.prop_mol[r] = m2;
         p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
-        p.prop_units[r] = row_units(p, n_2);
+        p.prop_n[r] = n_2;
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
 }
@@ -2189,7 +2204,7 @@ struct cgppd_ctx {
     unsigned *d_done = nullptr;
     float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr, *d_prop_delta = nullptr;
     int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
-    unsigned *d_prop_units = nullptr;
+    int *d_prop_n = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
     unsigned long long *d_ex_stats = nullptr;
     unsigned long long *d_pair_evals = nullptr;
@@ -2336,7 +2351,7 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_prop_center);
     dev_free(ctx->d_prop_mol);
     dev_free(ctx->d_prop_nib);
-    dev_free(ctx->d_prop_units);
+    dev_free(ctx->d_prop_n);
     dev_free(ctx->d_counters);
     dev_free(ctx->d_steps);
     dev_free(ctx->d_ex_round);
@@ -2589,6 +2604,9 @@ int sync_topology(cgppd_ctx *ctx)
     return CGPPD_OK;
 }
 
+int pair_layout_of(const cgppd_ctx *ctx);
+int row_lanes_of(int layout);
+
 PairParams base_params(cgppd_ctx *ctx)
 {
     PairParams p;
@@ -2633,7 +2651,8 @@ PairParams base_params(cgppd_ctx *ctx)
     p.prop_center = ctx->d_prop_center;
     p.prop_mol = ctx->d_prop_mol;
     p.prop_nib = ctx->d_prop_nib;
-    p.prop_units = ctx->d_prop_units;
+    p.prop_n = ctx->d_prop_n;
+    p.row_j = kWave * row_lanes_of(pair_layout_of(ctx));
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
     p.temperature = ctx->d_temp;
@@ -2848,6 +2867,9 @@ int zero_dev(cgppd_ctx *ctx, double *p, size_t n)
     return CGPPD_OK;
 }
 
+// Residues per lane of the row kernel in a layout (2 outside the j-lane layouts).
+int row_lanes_of(int layout) { return layout == CGPPD_LAYOUT_JLANES4 ? 4 : 2; }
+
 // Layout of the pair kernel: the caller's choice, else the autotuned one,
 // else j-lanes (flexible systems always run the i-lane kernel).
 int pair_layout_of(const cgppd_ctx *ctx)
@@ -2888,10 +2910,12 @@ int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_sto
             hipExtLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
         else
             hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
-    } else if (ev_start) {
-        hipExtLaunchKernelGGL(mc_row_kernel, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
     } else {
-        hipLaunchKernelGGL(mc_row_kernel, dim3(g), dim3(kWG), 0, ctx->stream, p);
+        auto kern = layout == CGPPD_LAYOUT_JLANES4 ? mc_row_kernel<4> : mc_row_kernel<2>;
+        if (ev_start)
+            hipExtLaunchKernelGGL(kern, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+        else
+            hipLaunchKernelGGL(kern, dim3(g), dim3(kWG), 0, ctx->stream, p);
     }
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
@@ -3314,7 +3338,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_prop_center, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_mol, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_prop_units, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_n, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_counters, (size_t)3 * R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_steps, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_ex_round, 1))) return rc;
@@ -3705,7 +3729,7 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu)
 
 int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout)
 {
-    if (!ctx || layout < CGPPD_LAYOUT_AUTO || layout > CGPPD_LAYOUT_JLANES) return CGPPD_E_ARG;
+    if (!ctx || layout < CGPPD_LAYOUT_AUTO || layout > CGPPD_LAYOUT_JLANES4) return CGPPD_E_ARG;
     ctx->pair_layout = layout;
     destroy_graph(ctx);
     return CGPPD_OK;
@@ -3731,8 +3755,8 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     if ((rc = ensure_events(ctx, (size_t)2 * launches))) return rc;
     // the caller's settings are lifted while candidates are timed
     const int keep_layout = ctx->pair_layout, keep_wg = ctx->wg_per_cu;
-    const int layouts[2] = {CGPPD_LAYOUT_JLANES, CGPPD_LAYOUT_ILANES};
-    const int n_layouts = ctx->n_flex ? 1 : 2;
+    const int layouts[3] = {CGPPD_LAYOUT_JLANES, CGPPD_LAYOUT_JLANES4, CGPPD_LAYOUT_ILANES};
+    const int n_layouts = ctx->n_flex ? 1 : 3;
     int best_l = 0, best_w = 0;
     double best = 1e300;
     for (int li = 0; li < n_layouts && rc == CGPPD_OK; ++li) {

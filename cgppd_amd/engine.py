@@ -292,7 +292,7 @@ class Engine:
     def set_launch(self, wg_per_cu):
         self._check(self.lib.cgppd_set_launch(self.h, int(wg_per_cu)), "set_launch")
 
-    LAYOUTS = {"auto": 0, "ilanes": 1, "jlanes": 2}
+    LAYOUTS = {"auto": 0, "ilanes": 1, "jlanes": 2, "jlanes4": 3}
 
     def set_pair_layout(self, layout):
         """Pair-kernel layout: "auto", "ilanes" or "jlanes" (or 0 / 1 / 2)."""

@@ -229,13 +229,15 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
  *   CGPPD_LAYOUT_AUTO   the autotuned choice, else j-lanes
  *   CGPPD_LAYOUT_ILANES lanes hold the moved molecule's residues, the other
  *                       residues are staged through LDS in 256-residue chunks
- *   CGPPD_LAYOUT_JLANES lanes hold the other residues, the moved molecule's
- *                       residues are read as wave-uniform records; every
- *                       wave takes an equal share of the work
+ *   CGPPD_LAYOUT_JLANES lanes hold two of the other residues each, the moved
+ *                       molecule's residues are read as wave-uniform
+ *                       records; every wave takes an equal share of the work
+ *   CGPPD_LAYOUT_JLANES4 the same with four residues per lane
  * Systems with flexible molecules always use the i-lane kernel. */
 #define CGPPD_LAYOUT_AUTO 0
 #define CGPPD_LAYOUT_ILANES 1
 #define CGPPD_LAYOUT_JLANES 2
+#define CGPPD_LAYOUT_JLANES4 3
 int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout);
 /* The layout and workgroups per CU the next pair-kernel launch uses. */
 int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu);

@@ -128,7 +128,7 @@ def test_config1_mc_steps_keep_totals_consistent(gpu_engine_factory):
     e.close()
 
 
-@pytest.mark.parametrize("layout", ["ilanes", "jlanes"])
+@pytest.mark.parametrize("layout", ["ilanes", "jlanes", "jlanes4"])
 def test_pair_layouts_match_oracle(gpu_engine_factory, layout):
     """Both pair-kernel layouts give the oracle's trajectory; j-lanes also
     on a system whose molecules are shorter than a wave (many molecules per
@@ -165,7 +165,7 @@ def test_pair_layouts_agree_on_config1(gpu_engine_factory):
     from cgppd_amd import synth
     spec = synth.config(1)
     out = {}
-    for layout in ("ilanes", "jlanes"):
+    for layout in ("ilanes", "jlanes", "jlanes4"):
         e = gpu_engine_factory()
         e.load_spec(spec, seed=12)
         e.mc_prepare()
@@ -174,16 +174,18 @@ def test_pair_layouts_agree_on_config1(gpu_engine_factory):
         s = e.stats()
         out[layout] = (s, e.coordinates(5)[0])
         e.close()
-    (sa, pa), (sb, pb) = out["ilanes"], out["jlanes"]
-    # molecules far from all others have |dE| near the FP32 rounding level:
-    # the decision agrees, its "dE < 0" / "Metropolis" label may not
-    ca, cb = sa["counters"], sb["counters"]
-    np.testing.assert_array_equal(ca[:, 0] + ca[:, 1], cb[:, 0] + cb[:, 1])
-    np.testing.assert_array_equal(ca[:, 2], cb[:, 2])
-    np.testing.assert_array_equal(pa, pb)
-    # the layouts use different FP32 forms of the pair term and different
-    # summation orders: totals of ~0.1 kcal/mol agree to a few 1e-8
-    np.testing.assert_allclose(sa["energy"], sb["energy"], rtol=2e-6, atol=1e-7)
+    sa, pa = out["ilanes"]
+    for other in ("jlanes", "jlanes4"):
+        sb, pb = out[other]
+        # molecules far from all others have |dE| near the FP32 rounding
+        # level: the decision agrees, its "dE < 0" / "Metropolis" label may not
+        ca, cb = sa["counters"], sb["counters"]
+        np.testing.assert_array_equal(ca[:, 0] + ca[:, 1], cb[:, 0] + cb[:, 1])
+        np.testing.assert_array_equal(ca[:, 2], cb[:, 2])
+        np.testing.assert_array_equal(pa, pb)
+        # the layouts use different FP32 forms of the pair term and different
+        # summation orders: totals of ~0.1 kcal/mol agree to a few 1e-8
+        np.testing.assert_allclose(sa["energy"], sb["energy"], rtol=2e-6, atol=1e-7)
 
 
 def test_jlanes_rows_are_deterministic(gpu_engine_factory):
@@ -210,7 +212,7 @@ def test_autotune_keeps_state(gpu_engine_factory):
     a, st = gpu_and_oracle(gpu_engine_factory, spec, 19)
     best = a.autotune(launches=3)
     print("autotune", best, flush=True)
-    assert best["layout"] in ("ilanes", "jlanes")
+    assert best["layout"] in ("ilanes", "jlanes", "jlanes4")
     assert 2 <= best["wg_per_cu"] <= 8 and best["ms"] > 0
     a.mc_steps(150)
     st.mc_steps(150)
