@@ -162,7 +162,7 @@ def main():
         # the autotuner times the candidates on the pending proposals; the
         # caller's explicit choices stay in force
         tuned = e.autotune(launches=20)
-        print(f"autotune: {tuned}", file=sys.stderr, flush=True)
+        print(f"autotune: {json.dumps(tuned)}", file=sys.stderr, flush=True)
     layout = e.pair_layout()
 
     if args.profile_steps:

@@ -272,11 +272,11 @@ struct PairParams {
     int max_n;            // residues of the largest molecule (prop_* row stride)
     float4 *prop_pos;     // R * max_n: proposed positions, device order, w = code
     float4 *prop_rel;     // R * max_n: proposed relative positions
-    float4 *prop_delta;   // R * max_n: per-residue displacements of a flex move
     float4 *prop_center;  // R
     int *prop_mol;        // R
     int *prop_nib;        // R: i-blocks of the moved molecule
     int *prop_n;          // R: residues of the moved molecule
+    int *prop_items;      // R: items of the i-lane pair kernel for the pending proposal
     int row_j;            // j residues per j-block of the row kernel (64 x residues per lane)
     // flexible molecules (null / 0 when the system has none).  Residue
     // indices are device indices (global); lists are CSR per molecule.
@@ -812,22 +812,24 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     __shared__ int item_off[CGPPD_MAX_REPLICAS + 1];
     load_table(p.table, tab);
     const int R = p.rep_count, JC = p.jc_count;
-    for (int r = threadIdx.x; r < R; r += kWG) {
-        int cnt = p.prop_nib[r] * JC;
-        if (FLEX) {
-            const int m = p.prop_mol[r];
-            if (p.mol_flex[m]) {
-                const int n = p.mol_off[m + 1] - p.mol_off[m];
-                cnt += p.prop_nib[r] * ((n + kJChunk - 1) / kJChunk) + 1;
+    // item offsets of the replicas (counts recorded with each proposal;
+    // zero for a proposal rejected by the surrogate's stage one): an
+    // inclusive scan by wave 0, 64 replicas at a time
+    if (threadIdx.x < kWave) {
+        const int lane0 = threadIdx.x;
+        int carry = 0;
+        for (int base = 0; base < R; base += kWave) {
+            const int r = base + lane0;
+            int c = (r < R && !(p.da && !p.s1[r])) ? p.prop_items[r] : 0;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int t = __shfl_up(c, o, kWave);
+                if (lane0 >= o) c += t;
             }
+            if (r < R) item_off[r + 1] = carry + c;
+            carry += __shfl(c, kWave - 1, kWave);
         }
-        if (p.da && !p.s1[r]) cnt = 0;  // rejected by the surrogate's stage one
-        item_off[r + 1] = cnt;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        item_off[0] = 0;
-        for (int r = 0; r < R; ++r) item_off[r + 1] += item_off[r];
+        if (lane0 == 0) item_off[0] = 0;
     }
     __syncthreads();
     const int total = item_off[R];
@@ -837,7 +839,16 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
+    // replica of the first item (binary search), then advance
     int r = 0;
+    if (k0 < k1) {
+        int lo = 0, hi = R;  // item_off[lo] <= k0 < item_off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (item_off[mid] <= k0) lo = mid; else hi = mid;
+        }
+        r = lo;
+    }
     for (int k = k0; k < k1; ++k) {
         while (k >= item_off[r + 1]) ++r;
         const int local = k - item_off[r];
@@ -1225,11 +1236,22 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     }
 }
 
+// Items of the i-lane pair kernel for a moved molecule m of n residues:
+// (i-block, j-chunk) items, and for a flexible molecule its intra-molecular
+// items and one bonded item.
+__device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n)
+{
+    const int nib = (n + kIBlock - 1) / kIBlock;
+    int items = nib * p.jc_count;
+    if (p.n_flex && p.mol_flex[m]) items += nib * ((n + kJChunk - 1) / kJChunk) + 1;
+    return items;
+}
+
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
 // out as the proposal (whole workgroup; src/Molecule.cpp:274-448).  The
 // current coordinates are copied, the residues that change are moved (a
 // flex branch in parallel, one residue per thread, each displacement kept
-// in prop_delta), thread 0 sums the displacements in the caller's order,
+// in LDS), thread 0 sums the displacements in the caller's order,
 // then every residue's relative position takes the centre shift and the
 // molecule is wrapped (src/Molecule.cpp:242-252).  Bit-identical to the CPU
 // restatement: no FMA contraction, correctly rounded division and square
@@ -1269,41 +1291,45 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
             const double q[4] = {rcos, rsin * (ax / mag), rsin * (ay / mag), rsin * (az / mag)};
             const float4 c0 = pos[p.flexr[e] - i0];
             const int b0 = p.br_off[k], nb = p.br_off[k + 1] - b0;
-            float4 *dl = p.prop_delta + (size_t)r * p.max_n;
-            for (int t = threadIdx.x; t < nb; t += kWG) {
-                const int li = p.br[b0 + t] - i0;
-                const float4 pi = pos[li];
-                const float3 rr = qrot(q, pi.x - c0.x, pi.y - c0.y, pi.z - c0.z);
-                const float dx = rr.x + c0.x - pi.x, dy = rr.y + c0.y - pi.y, dz = rr.z + c0.z - pi.z;
-                const float4 qi = rel[li];
-                pp[li] = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, pi.w);
-                pq[li] = make_float4(qi.x + dx, qi.y + dy, qi.z + dz, qi.w);
-                dl[t] = make_float4(dx, dy, dz, 0.0f);
-            }
-            __syncthreads();  // the displacements are in place
-            if (threadIdx.x == 0) {
-                int t = 0;
-                for (; t + 4 <= nb; t += 4) {
-                    const float4 d0 = dl[t], d1 = dl[t + 1], d2 = dl[t + 2], d3 = dl[t + 3];
-                    acc[0] = acc[0] + d0.x;
-                    acc[1] = acc[1] + d0.y;
-                    acc[2] = acc[2] + d0.z;
-                    acc[0] = acc[0] + d1.x;
-                    acc[1] = acc[1] + d1.y;
-                    acc[2] = acc[2] + d1.z;
-                    acc[0] = acc[0] + d2.x;
-                    acc[1] = acc[1] + d2.y;
-                    acc[2] = acc[2] + d2.z;
-                    acc[0] = acc[0] + d3.x;
-                    acc[1] = acc[1] + d3.y;
-                    acc[2] = acc[2] + d3.z;
+            // chunks of kWG residues: each thread moves one and keeps its
+            // displacement in LDS, then thread 0 adds the chunk in order
+            __shared__ float3 sdl[kWG];
+            for (int base = 0; base < nb; base += kWG) {
+                const int t = base + threadIdx.x;
+                if (t < nb) {
+                    const int li = p.br[b0 + t] - i0;
+                    const float4 pi = pos[li];
+                    const float3 rr = qrot(q, pi.x - c0.x, pi.y - c0.y, pi.z - c0.z);
+                    const float dx = rr.x + c0.x - pi.x, dy = rr.y + c0.y - pi.y, dz = rr.z + c0.z - pi.z;
+                    const float4 qi = rel[li];
+                    pp[li] = make_float4(pi.x + dx, pi.y + dy, pi.z + dz, pi.w);
+                    pq[li] = make_float4(qi.x + dx, qi.y + dy, qi.z + dz, qi.w);
+                    sdl[threadIdx.x] = make_float3(dx, dy, dz);
                 }
-                for (; t < nb; ++t) {
-                    const float4 d = dl[t];
-                    acc[0] = acc[0] + d.x;
-                    acc[1] = acc[1] + d.y;
-                    acc[2] = acc[2] + d.z;
+                __syncthreads();  // the chunk's displacements are in place
+                if (threadIdx.x == 0) {
+                    // eight LDS reads in flight per step, then the ordered adds
+                    const int n = min(kWG, nb - base);
+                    int k = 0;
+                    for (; k + 8 <= n; k += 8) {
+                        float3 d[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) d[u] = sdl[k + u];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            acc[0] = acc[0] + d[u].x;
+                            acc[1] = acc[1] + d[u].y;
+                            acc[2] = acc[2] + d[u].z;
+                        }
+                    }
+                    for (; k < n; ++k) {
+                        const float3 d = sdl[k];
+                        acc[0] = acc[0] + d.x;
+                        acc[1] = acc[1] + d.y;
+                        acc[2] = acc[2] + d.z;
+                    }
                 }
+                __syncthreads();  // the chunk was read before the next one is written
             }
             moved = 1;
         }
@@ -1397,6 +1423,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
         p.prop_mol[r] = m;
         p.prop_nib[r] = (n + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n;
+        p.prop_items[r] = ilane_items(p, m, n);
         p.prop_center[r] = make_float4(c0, c1, c2, 0.0f);
     }
 }
@@ -1432,6 +1459,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
             p.prop_mol[r] = pr.mol;
             p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
             p.prop_n[r] = i_end - i_beg;
+            p.prop_items[r] = ilane_items(p, pr.mol, i_end - i_beg);
             p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
         }
         return;
@@ -1449,6 +1477,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
         p.prop_mol[r] = pr.mol;
         p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
         p.prop_n[r] = i_end - i_beg;
+        p.prop_items[r] = ilane_items(p, pr.mol, i_end - i_beg);
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
 }
@@ -1696,6 +1725,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         p.prop_mol[r] = m2;
         p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n_2;
+        p.prop_items[r] = ilane_items(p, m2, n_2);
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
     }
 }
@@ -2202,9 +2232,9 @@ struct cgppd_ctx {
     double *d_rcos = nullptr, *d_rsin = nullptr;
     float *d_tstep = nullptr;
     unsigned *d_done = nullptr;
-    float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr, *d_prop_delta = nullptr;
+    float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr;
     int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
-    int *d_prop_n = nullptr;
+    int *d_prop_n = nullptr, *d_prop_items = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
     unsigned long long *d_ex_stats = nullptr;
     unsigned long long *d_pair_evals = nullptr;
@@ -2263,6 +2293,7 @@ struct cgppd_ctx {
     int wg_per_cu = 0;
     int pair_layout = CGPPD_LAYOUT_AUTO;  // caller's choice
     int tuned_layout = 0, tuned_wg = 0;    // set by cgppd_autotune
+    double tune_ms[3][7] = {};             // its timings: j-lanes, j-lanes x4, i-lanes; 2..8 per CU (0: not run)
     bool timing = false;
     std::vector<hipEvent_t> events;
     double t_mean_ms = 0.0;
@@ -2347,11 +2378,11 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_done);
     dev_free(ctx->d_prop_pos);
     dev_free(ctx->d_prop_rel);
-    dev_free(ctx->d_prop_delta);
     dev_free(ctx->d_prop_center);
     dev_free(ctx->d_prop_mol);
     dev_free(ctx->d_prop_nib);
     dev_free(ctx->d_prop_n);
+    dev_free(ctx->d_prop_items);
     dev_free(ctx->d_counters);
     dev_free(ctx->d_steps);
     dev_free(ctx->d_ex_round);
@@ -2647,11 +2678,11 @@ PairParams base_params(cgppd_ctx *ctx)
     p.max_n = ctx->max_n;
     p.prop_pos = ctx->d_prop_pos;
     p.prop_rel = ctx->d_prop_rel;
-    p.prop_delta = ctx->d_prop_delta;
     p.prop_center = ctx->d_prop_center;
     p.prop_mol = ctx->d_prop_mol;
     p.prop_nib = ctx->d_prop_nib;
     p.prop_n = ctx->d_prop_n;
+    p.prop_items = ctx->d_prop_items;
     p.row_j = kWave * row_lanes_of(pair_layout_of(ctx));
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
@@ -3334,11 +3365,11 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     // 8 records of padding: the row kernel reads i records in groups of 8
     if ((rc = dev_alloc(ctx, &ctx->d_prop_pos, (size_t)R * ctx->max_n + 8))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_rel, (size_t)R * ctx->max_n))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_prop_delta, (size_t)R * ctx->max_n))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_center, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_mol, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_n, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_items, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_counters, (size_t)3 * R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_steps, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_ex_round, 1))) return rc;
@@ -3744,6 +3775,14 @@ int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu)
     return CGPPD_OK;
 }
 
+int cgppd_autotune_table(cgppd_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms) return CGPPD_E_ARG;
+    for (int l = 0; l < 3; ++l)
+        for (int w = 0; w < 7; ++w) ms[l * 7 + w] = ctx->tune_ms[l][w];
+    return CGPPD_OK;
+}
+
 int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, double *best_ms)
 {
     int rc = check_ready(ctx, true);
@@ -3759,6 +3798,8 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     const int n_layouts = ctx->n_flex ? 1 : 3;
     int best_l = 0, best_w = 0;
     double best = 1e300;
+    for (auto &row : ctx->tune_ms)
+        for (double &v : row) v = 0.0;
     for (int li = 0; li < n_layouts && rc == CGPPD_OK; ++li) {
         for (int w = 2; w <= 8 && rc == CGPPD_OK; ++w) {
             ctx->pair_layout = ctx->n_flex ? CGPPD_LAYOUT_AUTO : layouts[li];
@@ -3782,6 +3823,7 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
                 sum += ms;
             }
             const double mean = sum / launches;
+            ctx->tune_ms[ctx->n_flex ? 2 : li][w - 2] = mean;
             if (rc == CGPPD_OK && mean < best) {
                 best = mean;
                 best_l = ctx->n_flex ? CGPPD_LAYOUT_ILANES : layouts[li];

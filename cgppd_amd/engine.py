@@ -319,7 +319,12 @@ class Engine:
         self._check(self.lib.cgppd_autotune(self.h, int(launches), C.byref(lay), C.byref(wg), C.byref(ms)),
                     "autotune")
         names = {v: k for k, v in self.LAYOUTS.items()}
-        return {"layout": names[lay.value], "wg_per_cu": wg.value, "ms": ms.value}
+        table = np.zeros(21)
+        self._check(self.lib.cgppd_autotune_table(self.h, ptr(table, C.c_double)), "autotune_table")
+        timings = {f"{lay}/{w}": round(float(table[i * 7 + w - 2]) * 1e3, 2)
+                   for i, lay in enumerate(("jlanes", "jlanes4", "ilanes")) for w in range(2, 9)
+                   if table[i * 7 + w - 2] > 0}
+        return {"layout": names[lay.value], "wg_per_cu": wg.value, "ms": ms.value, "us_by_candidate": timings}
 
     def set_image_shift(self, on):
         self._check(self.lib.cgppd_set_image_shift(self.h, int(bool(on))), "set_image_shift")

@@ -248,6 +248,9 @@ int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu);
  * candidate, keeps the fastest for this context and reports it (any
  * pointer may be NULL).  The MC state is unchanged. */
 int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, double *best_ms);
+/* The last autotune's mean kernel times in ms, ms[3*7]: rows j-lanes,
+ * j-lanes x4, i-lanes; columns 2..8 workgroups per CU (0 = not timed). */
+int cgppd_autotune_table(cgppd_ctx *ctx, double *ms);
 int cgppd_set_timing(cgppd_ctx *ctx, int enable);
 int cgppd_last_kernel_ms(cgppd_ctx *ctx, double *mean_ms, int64_t *launches);
 

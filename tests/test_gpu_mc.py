@@ -222,3 +222,4 @@ def test_autotune_keeps_state(gpu_engine_factory):
     for r in range(spec.n_rep):
         np.testing.assert_array_equal(a.coordinates(r)[0], st.pos[r])
     a.close()
+
