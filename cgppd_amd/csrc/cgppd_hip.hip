@@ -55,6 +55,17 @@ constexpr double kFixScale = 4294967296.0;
 constexpr double kFixInv = 1.0 / 4294967296.0;
 constexpr double kFixMax = 131072.0;
 constexpr uint32_t kTagMC = 0x4D430000u;
+// How the pair kernel evaluates a flexible molecule's internal energy for a
+// pending proposal (PairParams::prop_moved.x).  kMovedOne: a local
+// translate or crankshaft whose centre shift did not wrap the molecule, so
+// every other residue keeps its stored coordinates bit for bit; only the
+// moved residue's pairs and the bonded terms are evaluated, new minus
+// current.  kMovedAll: every other proposal rewrites the coordinates of many
+// residues (rounded), and the whole internal energy is recomputed, as the
+// reference does.
+constexpr int kMovedNone = 0;  // a molecule without internal energy
+constexpr int kMovedOne = 2;   // molecule-local device index in .y
+constexpr int kMovedAll = 3;
 constexpr uint32_t kTagEX = 0x45580000u;
 constexpr uint32_t kTagSG = 0x53470000u;  // surrogate minibatch draws
 constexpr uint32_t kTagDA = 0x44410000u;  // delayed-acceptance stage-one uniforms
@@ -247,7 +258,7 @@ struct PairParams {
     float4 *pos;         // R * stride
     float4 *rel;         // R * stride
     float4 *center;      // R * n_mol
-    const float *radius; // R * n_mol, max |relative position| of each molecule
+    float *radius;       // R * n_mol, max |relative position| of each molecule
     int fast_image;      // use the per-segment image shift where it is exact
     // full-energy outputs (raw LJ/DH)
     double *out_lj, *out_dh;  // R
@@ -273,6 +284,8 @@ struct PairParams {
     float4 *prop_pos;     // R * max_n: proposed positions, device order, w = code
     float4 *prop_rel;     // R * max_n: proposed relative positions
     float4 *prop_center;  // R
+    float *prop_radius;   // R: bounding radius of the proposed molecule
+    int4 *prop_moved;     // R: {kind, moved residue, 0, 0} (flexible molecules)
     int *prop_mol;        // R
     int *prop_nib;        // R: i-blocks of the moved molecule
     int *prop_n;          // R: residues of the moved molecule
@@ -540,6 +553,26 @@ __device__ __forceinline__ void intra_item(const float4 *__restrict__ src, const
             kahan_add(s1, c1, dh);
         }
     }
+}
+
+// Internal-energy change item of a flexible molecule whose proposal moves
+// one residue a (molecule-local; proposed position src[a], current cur[a]):
+// thread t takes residue b = jc * 256 + t of the molecule, b != a, and
+// sums e(new_a, b) - e(cur_a, b) over the pairs the GPU kernel's exclusions
+// keep (Kahan-compensated FP32 per thread; the caller reduces the waves).
+__device__ __forceinline__ void moved_item(const float4 *__restrict__ src, const float4 *__restrict__ cur,
+                                           const int4 *__restrict__ tg, int n, int a, int jc, const PairParams &p,
+                                           const char *__restrict__ tab, float &s0, float &c0)
+{
+    const int b = jc * kJChunk + threadIdx.x;
+    if (b >= n || b == a) return;
+    const int4 ta = tg[a], tb = tg[b];
+    if (intra_excluded(ta, tb)) return;
+    const float4 pb = src[b];
+    float ljn, dhn, ljo, dho;
+    pair_any(src[a], pb, tab, p, ljn, dhn);
+    pair_any(cur[a], pb, tab, p, ljo, dho);
+    kahan_add(s0, c0, (ljn + dhn) - (ljo + dho));
 }
 
 // src/Quaternion.cpp:95-112, double arithmetic on a float vector.
@@ -858,16 +891,34 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
             const int nib = p.prop_nib[r], njc = (n + kJChunk - 1) / kJChunk;
             const int l2 = local - nib * JC;
             const float4 *src = p.prop_pos + (size_t)r * p.max_n;
+            const int4 mv = p.prop_moved[r];
             double v;
-            if (l2 < nib * njc) {
-                float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-                bool valid;
-                intra_item(src, p.rtag + i0, n, l2 / njc, l2 % njc, p, tab, jrec, jtag, s0, c0, s1, c1, valid);
-                v = valid ? ((double)s0 - (double)c0) + ((double)s1 - (double)c1) : 0.0;
+            if (mv.x == kMovedAll) {
+                // the whole internal energy of the proposal
+                if (l2 < nib * njc) {
+                    float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+                    bool valid;
+                    intra_item(src, p.rtag + i0, n, l2 / njc, l2 % njc, p, tab, jrec, jtag, s0, c0, s1, c1, valid);
+                    v = valid ? ((double)s0 - (double)c0) + ((double)s1 - (double)c1) : 0.0;
+                } else {
+                    double t[5];
+                    bonded_terms(src, i0, m, p, tab, t);
+                    v = bonded_kcal(t);
+                }
             } else {
-                double t[5];
-                bonded_terms(src, i0, m, p, tab, t);
-                v = bonded_kcal(t);
+                // its change: the moved residue's pairs and the bonded terms,
+                // new minus current
+                const float4 *cur = p.pos + (size_t)r * p.stride + i0;
+                if (l2 < njc) {
+                    float s0 = 0.f, c0 = 0.f;
+                    moved_item(src, cur, p.rtag + i0, n, mv.y, l2, p, tab, s0, c0);
+                    v = (double)s0 - (double)c0;
+                } else {
+                    double tn[5], to[5];
+                    bonded_terms(src, i0, m, p, tab, tn);
+                    bonded_terms(cur, i0, m, p, tab, to);
+                    v = bonded_kcal(tn) - bonded_kcal(to);
+                }
             }
             v = wave_sum(v);
             if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + m, v);
@@ -887,7 +938,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
         const float4 pn = p.prop_pos[(size_t)r * p.max_n + (valid ? il : 0)];
         const uint32_t lane_off = (__float_as_uint(pn.w) >> kRowShift & 0x1Fu) << kColShift;
         const float4 ci = p.prop_center[r];
-        const float ri = p.radius[(size_t)r * p.n_mol + m];
+        const float ri = p.prop_radius[r];
         __syncthreads();
         int js = jbase + wave * kWave;
         const int je = min(p.n_res, jbase + (wave + 1) * kWave);
@@ -1237,14 +1288,37 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
 }
 
 // Items of the i-lane pair kernel for a moved molecule m of n residues:
-// (i-block, j-chunk) items, and for a flexible molecule its intra-molecular
-// items and one bonded item.
-__device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n)
+// (i-block, j-chunk) items, and for a flexible molecule its internal-energy
+// items: for kMovedAll every intra-molecular (i-block, j-chunk) item and one
+// bonded item, for kMovedOne one item per j-chunk and one bonded-difference
+// item.
+__device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, int kind)
 {
     const int nib = (n + kIBlock - 1) / kIBlock;
     int items = nib * p.jc_count;
-    if (p.n_flex && p.mol_flex[m]) items += nib * ((n + kJChunk - 1) / kJChunk) + 1;
+    if (p.n_flex && p.mol_flex[m]) {
+        const int njc = (n + kJChunk - 1) / kJChunk;
+        items += (kind == kMovedOne ? njc : nib * njc) + 1;
+    }
     return items;
+}
+
+// Thread 0, after a proposal of molecule m (n residues) of replica r is
+// laid out: its i-lane items, the internal-energy record, the proposed
+// bounding radius, and the diagonal row entry the pair kernel adds the
+// internal energy to (the current one for kMovedOne, else 0).
+__device__ __forceinline__ void record_proposal(const PairParams &p, int r, int m, int n, int kind, int moved,
+                                                float radius_new)
+{
+    const bool flex = p.n_flex && p.mol_flex[m];
+    if (!flex) kind = kMovedNone;
+    p.prop_items[r] = ilane_items(p, m, n, kind);
+    p.prop_moved[r] = make_int4(kind, moved, 0, 0);
+    p.prop_radius[r] = radius_new;
+    if (flex) {
+        const size_t d = (size_t)r * p.n_mol + m;
+        p.rowbuf[d] = kind == kMovedOne ? p.emat[(size_t)r * p.n_mol * p.n_mol + (size_t)m * p.n_mol + m] : 0.0;
+    }
 }
 
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
@@ -1267,10 +1341,13 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     const float4 *rel = p.rel + (size_t)r * p.stride + i0;
     float4 *pp = p.prop_pos + (size_t)r * p.max_n;
     float4 *pq = p.prop_rel + (size_t)r * p.max_n;
+    __shared__ int sh_mone, sh_wrap;
+    __shared__ float sh_r2[kWavesPerWG];
     for (int k = threadIdx.x; k < n; k += kWG) {
         pp[k] = pos[k];
         pq[k] = rel[k];
     }
+    if (threadIdx.x == 0) sh_mone = -1;
     __syncthreads();
     // every thread draws the same words (the choices below are uniform)
     uint32_t x[4];
@@ -1388,6 +1465,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
             acc[1] = v[1];
             acc[2] = v[2];
             moved = 1;
+            sh_mone = li;
         }
     }
     if (threadIdx.x == 0) {
@@ -1402,6 +1480,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     __syncthreads();
     const float vf0 = sh_vf[0], vf1 = sh_vf[1], vf2 = sh_vf[2];
     float c0 = sh_nc[0], c1 = sh_nc[1], c2 = sh_nc[2];
+    float r2max = 0.0f;
     if (sh_moved) {
         const float w0 = fmodf(c0, p.box), w1 = fmodf(c1, p.box), w2 = fmodf(c2, p.box);
         const bool wrap = w0 != c0 || w1 != c1 || w2 != c2;
@@ -1412,19 +1491,33 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
             q.z = q.z - vf2;
             pq[k] = q;
             if (wrap) pp[k] = make_float4(q.x + w0, q.y + w1, q.z + w2, pp[k].w);
+            r2max = fmaxf(r2max, q.x * q.x + q.y * q.y + q.z * q.z);
         }
         if (wrap) {
             c0 = w0;
             c1 = w1;
             c2 = w2;
         }
+        if (threadIdx.x == 0) sh_wrap = wrap;
     }
+    // bounding radius of the proposed molecule (block maximum)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r2max = fmaxf(r2max, __shfl_xor(r2max, o, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) sh_r2[threadIdx.x / kWave] = r2max;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        float rad = p.radius[(size_t)r * p.n_mol + m];
+        if (sh_moved) {
+            float q2 = 0.0f;
+            for (int w = 0; w < kWavesPerWG; ++w) q2 = fmaxf(q2, sh_r2[w]);
+            rad = sqrtf(q2) * 1.0001f;
+        }
         p.prop_mol[r] = m;
         p.prop_nib[r] = (n + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n;
-        p.prop_items[r] = ilane_items(p, m, n);
         p.prop_center[r] = make_float4(c0, c1, c2, 0.0f);
+        const bool one = sh_moved && sh_mone >= 0 && !sh_wrap;
+        record_proposal(p, r, m, n, one ? kMovedOne : kMovedAll, one ? sh_mone : 0, rad);
     }
 }
 
@@ -1459,8 +1552,10 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
             p.prop_mol[r] = pr.mol;
             p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
             p.prop_n[r] = i_end - i_beg;
-            p.prop_items[r] = ilane_items(p, pr.mol, i_end - i_beg);
             p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+            // a rigid-body move keeps the radius; its rounded coordinates are
+            // evaluated in full
+            record_proposal(p, r, pr.mol, i_end - i_beg, kMovedAll, 0, p.radius[(size_t)r * p.n_mol + pr.mol]);
         }
         return;
     }
@@ -1477,8 +1572,8 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
         p.prop_mol[r] = pr.mol;
         p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
         p.prop_n[r] = i_end - i_beg;
-        p.prop_items[r] = ilane_items(p, pr.mol, i_end - i_beg);
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+        record_proposal(p, r, pr.mol, i_end - i_beg, kMovedNone, 0, p.radius[(size_t)r * p.n_mol + pr.mol]);
     }
 }
 
@@ -1564,7 +1659,10 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
                 pos[k] = p.prop_pos[(size_t)r * p.max_n + k];
                 rel[k] = p.prop_rel[(size_t)r * p.max_n + k];
             }
-            if (threadIdx.x == 0) p.center[(size_t)r * M + m] = p.prop_center[r];
+            if (threadIdx.x == 0) {
+                p.center[(size_t)r * M + m] = p.prop_center[r];
+                p.radius[(size_t)r * M + m] = p.prop_radius[r];
+            }
             if (p.da) {
                 float *sm = p.smat + (size_t)r * M * M;
                 for (int J = threadIdx.x; J < M; J += kWG) {
@@ -1725,8 +1823,8 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         p.prop_mol[r] = m2;
         p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n_2;
-        p.prop_items[r] = ilane_items(p, m2, n_2);
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+        record_proposal(p, r, m2, n_2, kMovedNone, 0, p.radius[(size_t)r * M + m2]);
     }
 }
 
@@ -2233,6 +2331,8 @@ struct cgppd_ctx {
     float *d_tstep = nullptr;
     unsigned *d_done = nullptr;
     float4 *d_prop_pos = nullptr, *d_prop_rel = nullptr, *d_prop_center = nullptr;
+    float *d_prop_radius = nullptr;
+    int4 *d_prop_moved = nullptr;
     int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
     int *d_prop_n = nullptr, *d_prop_items = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
@@ -2379,6 +2479,8 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_prop_pos);
     dev_free(ctx->d_prop_rel);
     dev_free(ctx->d_prop_center);
+    dev_free(ctx->d_prop_radius);
+    dev_free(ctx->d_prop_moved);
     dev_free(ctx->d_prop_mol);
     dev_free(ctx->d_prop_nib);
     dev_free(ctx->d_prop_n);
@@ -2679,6 +2781,8 @@ PairParams base_params(cgppd_ctx *ctx)
     p.prop_pos = ctx->d_prop_pos;
     p.prop_rel = ctx->d_prop_rel;
     p.prop_center = ctx->d_prop_center;
+    p.prop_radius = ctx->d_prop_radius;
+    p.prop_moved = ctx->d_prop_moved;
     p.prop_mol = ctx->d_prop_mol;
     p.prop_nib = ctx->d_prop_nib;
     p.prop_n = ctx->d_prop_n;
@@ -3366,6 +3470,8 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_prop_pos, (size_t)R * ctx->max_n + 8))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_rel, (size_t)R * ctx->max_n))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_center, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_radius, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_moved, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_mol, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_n, R))) return rc;
@@ -3792,6 +3898,17 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     CG_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_stage1(ctx))) return rc;
     if ((rc = ensure_events(ctx, (size_t)2 * launches))) return rc;
+    // the timed launches add into the pending rows, which already hold the
+    // proposals' starting values (a flexible molecule's internal energy):
+    // keep a copy to put back
+    const size_t row_n = (size_t)2 * ctx->R * ctx->M;
+    double *row_copy = nullptr;
+    CG_HIP(ctx, hipMalloc((void **)&row_copy, row_n * sizeof(double)));
+    if (hipMemcpyAsync(row_copy, ctx->d_rowbuf, row_n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream) !=
+        hipSuccess) {
+        (void)hipFree(row_copy);
+        return fail(ctx, CGPPD_E_HIP, "autotune: saving the rows failed");
+    }
     // the caller's settings are lifted while candidates are timed
     const int keep_layout = ctx->pair_layout, keep_wg = ctx->wg_per_cu;
     const int layouts[3] = {CGPPD_LAYOUT_JLANES, CGPPD_LAYOUT_JLANES4, CGPPD_LAYOUT_ILANES};
@@ -3833,10 +3950,12 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     }
     ctx->pair_layout = keep_layout;
     ctx->wg_per_cu = keep_wg;
-    // the timed launches added into the pending rows: clear them
-    const int zrc = zero_dev(ctx, ctx->d_rowbuf, (size_t)2 * ctx->R * ctx->M);
+    const hipError_t back =
+        hipMemcpyAsync(ctx->d_rowbuf, row_copy, row_n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    const hipError_t done = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(row_copy);
     if (rc) return rc;
-    if (zrc) return zrc;
+    if (back != hipSuccess || done != hipSuccess) return fail(ctx, CGPPD_E_HIP, "autotune: restoring the rows failed");
     ctx->tuned_layout = best_l;
     ctx->tuned_wg = best_w;
     destroy_graph(ctx);

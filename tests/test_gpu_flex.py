@@ -185,3 +185,20 @@ def test_flexible_running_total_and_graph(gpu_engine_factory):
         assert abs(sa["energy"][r] - tot) <= 1e-6 * max(1.0, abs(tot)), (r, sa["energy"][r], tot)
     a.close()
     b.close()
+
+
+def test_autotune_keeps_flexible_state(gpu_engine_factory):
+    """The autotuner's timed launches add into the pending rows, which hold
+    the starting internal energy of a proposal that moves one residue: the
+    tuned engine must still follow the oracle's trajectory (regression)."""
+    from cgppd_amd import synth
+    spec = synth.config(2)
+    spec.n_rep = 4
+    e, st = mc_pair(gpu_engine_factory, spec, 31)
+    best = e.autotune(launches=2)
+    print("autotune", best, flush=True)
+    for chunk in range(2):
+        e.mc_steps(60)
+        st.mc_steps(60)
+        compare_states(e, st, spec, split_counters=False)
+    e.close()
