@@ -254,6 +254,7 @@ struct PairParams {
     int rep_begin, rep_count;
     int xcd_map;  // items of replica r stay on XCD r % 8
     const Coef *table;   // 20 x 20 entries, row = j type
+    const Coef *table_scaled;  // the same in kcal/mol (finish kernels' bonded terms)
     float box, inv_box;
     float4 *pos;         // R * stride
     float4 *rel;         // R * stride
@@ -519,6 +520,15 @@ __device__ __forceinline__ void pair_any(float4 pi, float4 pj, const char *__res
     pair_term<true, true>(pj, pi.x, pi.y, pi.z, cf, p.box, p.inv_box, lj, dh);
 }
 
+// pair_any with the coefficients read from a global table (entry tj * 20 + ti).
+__device__ __forceinline__ void pair_global(float4 pi, float4 pj, const Coef *__restrict__ g, const PairParams &p,
+                                            float &lj, float &dh)
+{
+    const uint32_t ti = __float_as_uint(pi.w) >> kRowShift & 0x1Fu, tj = __float_as_uint(pj.w) >> kRowShift & 0x1Fu;
+    const Coef cf = g[tj * CGPPD_AA_COUNT + ti];
+    pair_term<true, true>(pj, pi.x, pi.y, pi.z, cf, p.box, p.inv_box, lj, dh);
+}
+
 // Intra-molecular item: this lane's residue a = ib*64 + lane of a molecule
 // of n residues (molecule-local indices, coordinates src[], tags tg[])
 // against the residues b > a of chunk jc (256), wave w taking 64 of them.
@@ -601,58 +611,106 @@ __device__ __forceinline__ float fmag(float x, float y, float z)
 // Molecule::E(false) subtracts (src/Molecule.cpp:669-683).  Every thread
 // takes a strided share; t[5] = indirect LJ and DH (negated), bond sum,
 // sum of log(angle term), torsion sum.
+// One bonded or indirect-neighbour term of each kind, from the molecule-
+// local coordinates src (the term's residues are device indices minus i0).
+__device__ __forceinline__ double bond_term(const float4 *__restrict__ src, int i0, int2 e, float box)
+{
+#pragma clang fp contract(off)
+    const float4 a = src[e.x - i0], b = src[e.y - i0];
+    float X = b.x - a.x, Y = b.y - a.y, Z = b.z - a.z;
+    X = X - box * roundf(X / box);
+    Y = Y - box * roundf(Y / box);
+    Z = Z - box * roundf(Z / box);
+    const double r = (double)sqrtf(X * X + Y * Y + Z * Z);
+    return (r - (double)3.81f) * (r - (double)3.81f);
+}
+
+__device__ __forceinline__ double angle_term(const float4 *__restrict__ src, int i0, int4 e)
+{
+#pragma clang fp contract(off)
+    const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0];
+    const float abx = a.x - b.x, aby = a.y - b.y, abz = a.z - b.z;
+    const float cbx = c.x - b.x, cby = c.y - b.y, cbz = c.z - b.z;
+    const double cmag = (double)(fmag(cbx, cby, cbz) * fmag(abx, aby, abz));
+    float th = 0.0f;
+    if (!(cmag < 1e-26)) th = (float)acos((double)(abx * cbx + aby * cby + abz * cbz) / cmag);
+    const double th_d = (double)th;
+    const double da = th_d - (double)1.6f, db = th_d - (double)2.27f;
+    return log(exp(-(double)0.1f * ((double)106.4f * da * da + (double)4.3f)) +
+               exp(-(double)0.1f * ((double)26.3f * db * db)));
+}
+
+__device__ __forceinline__ double torsion_term(const float4 *__restrict__ src, int i0, int4 e, const PairParams &p)
+{
+#pragma clang fp contract(off)
+    const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0], d = src[e.w - i0];
+    const float b1x = b.x - a.x, b1y = b.y - a.y, b1z = b.z - a.z;
+    const float b2x = c.x - b.x, b2y = c.y - b.y, b2z = c.z - b.z;
+    const float b3x = d.x - c.x, b3y = d.y - c.y, b3z = d.z - c.z;
+    const float x23 = b2y * b3z - b2z * b3y, y23 = b2z * b3x - b2x * b3z, z23 = b2x * b3y - b2y * b3x;
+    const float x12 = b1y * b2z - b1z * b2y, y12 = b1z * b2x - b1x * b2z, z12 = b1x * b2y - b1y * b2x;
+    const float yy = fmag(b2x, b2y, b2z) * (b1x * x23 + b1y * y23 + b1z * z23);
+    const float xx = x12 * x23 + y12 * y23 + z12 * z23;
+    const double phi = (double)atan2f(yy, xx);
+    const int tj = (int)(__float_as_uint(b.w) >> kRowShift & 0x1Fu);
+    const int tk = (int)(__float_as_uint(c.w) >> kRowShift & 0x1Fu);
+    const double *V = p.tor_v + (tj * 20 + tk) * 4;
+    const double *D = p.tor_d + (tj * 20 + tk) * 4;
+    return (1 + cos(phi - D[0])) * V[0] + (1 + cos(2 * phi - D[1])) * V[1] + (1 + cos(3 * phi - D[2])) * V[2] +
+           (1 + cos(4 * phi - D[3])) * V[3];
+}
+
+// Bonded terms of flexible molecule m and the indirect-neighbour pairs that
+// Molecule::E(false) subtracts (src/Molecule.cpp:669-683), from the
+// molecule-local coordinates src (whole workgroup, the terms spread over
+// its threads).
+// t[5] = indirect LJ and DH (negated), bond sum, sum of log(angle term),
+// torsion sum, this thread's share.  With old non-null, every term is
+// evaluated on both coordinate sets and t holds new minus old.
+// GLOBAL_TABLE: the indirect-neighbour pairs read the scaled table from
+// global memory (kernels without the LDS table), else the LDS table tab.
+template <bool GLOBAL_TABLE>
 __device__ void bonded_terms(const float4 *__restrict__ src, int i0, int m, const PairParams &p,
-                             const char *__restrict__ tab, double t[5])
+                             const char *__restrict__ tab, double t[5], const float4 *__restrict__ old = nullptr)
 {
 #pragma clang fp contract(off)
     for (int k = 0; k < 5; ++k) t[k] = 0.0;
-    for (int k = p.ind_off[m] + threadIdx.x; k < p.ind_off[m + 1]; k += kWG) {
-        const int2 e = p.ind[k];
-        float lj, dh;
-        pair_any(src[e.x - i0], src[e.y - i0], tab, p, lj, dh);
-        t[0] -= (double)lj;
-        t[1] -= (double)dh;
-    }
-    for (int k = p.bond_off[m] + threadIdx.x; k < p.bond_off[m + 1]; k += kWG) {
-        const int2 e = p.bonds[k];
-        const float4 a = src[e.x - i0], b = src[e.y - i0];
-        float X = b.x - a.x, Y = b.y - a.y, Z = b.z - a.z;
-        X = X - p.box * roundf(X / p.box);
-        Y = Y - p.box * roundf(Y / p.box);
-        Z = Z - p.box * roundf(Z / p.box);
-        const double r = (double)sqrtf(X * X + Y * Y + Z * Z);
-        t[2] += (r - (double)3.81f) * (r - (double)3.81f);
-    }
-    for (int k = p.angle_off[m] + threadIdx.x; k < p.angle_off[m + 1]; k += kWG) {
-        const int4 e = p.angles[k];
-        const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0];
-        const float abx = a.x - b.x, aby = a.y - b.y, abz = a.z - b.z;
-        const float cbx = c.x - b.x, cby = c.y - b.y, cbz = c.z - b.z;
-        const double cmag = (double)(fmag(cbx, cby, cbz) * fmag(abx, aby, abz));
-        float th = 0.0f;
-        if (!(cmag < 1e-26)) th = (float)acos((double)(abx * cbx + aby * cby + abz * cbz) / cmag);
-        const double th_d = (double)th;
-        const double da = th_d - (double)1.6f, db = th_d - (double)2.27f;
-        t[3] += log(exp(-(double)0.1f * ((double)106.4f * da * da + (double)4.3f)) +
-                    exp(-(double)0.1f * ((double)26.3f * db * db)));
-    }
-    for (int k = p.tors_off[m] + threadIdx.x; k < p.tors_off[m + 1]; k += kWG) {
-        const int4 e = p.tors[k];
-        const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0], d = src[e.w - i0];
-        const float b1x = b.x - a.x, b1y = b.y - a.y, b1z = b.z - a.z;
-        const float b2x = c.x - b.x, b2y = c.y - b.y, b2z = c.z - b.z;
-        const float b3x = d.x - c.x, b3y = d.y - c.y, b3z = d.z - c.z;
-        const float x23 = b2y * b3z - b2z * b3y, y23 = b2z * b3x - b2x * b3z, z23 = b2x * b3y - b2y * b3x;
-        const float x12 = b1y * b2z - b1z * b2y, y12 = b1z * b2x - b1x * b2z, z12 = b1x * b2y - b1y * b2x;
-        const float yy = fmag(b2x, b2y, b2z) * (b1x * x23 + b1y * y23 + b1z * z23);
-        const float xx = x12 * x23 + y12 * y23 + z12 * z23;
-        const double phi = (double)atan2f(yy, xx);
-        const int tj = (int)(__float_as_uint(b.w) >> kRowShift & 0x1Fu);
-        const int tk = (int)(__float_as_uint(c.w) >> kRowShift & 0x1Fu);
-        const double *V = p.tor_v + (tj * 20 + tk) * 4;
-        const double *D = p.tor_d + (tj * 20 + tk) * 4;
-        t[4] += (1 + cos(phi - D[0])) * V[0] + (1 + cos(2 * phi - D[1])) * V[1] + (1 + cos(3 * phi - D[2])) * V[2] +
-                (1 + cos(4 * phi - D[3])) * V[3];
+    // one index space over the terms, the costliest kinds first (torsions,
+    // angles, bonds, indirect pairs): thread t takes entries t, t + block, ...
+    const int nt = p.tors_off[m + 1] - p.tors_off[m], na = p.angle_off[m + 1] - p.angle_off[m];
+    const int nb = p.bond_off[m + 1] - p.bond_off[m], ni = p.ind_off[m + 1] - p.ind_off[m];
+    const int total = nt + na + nb + ni;
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+        if (q < nt) {
+            const int4 e = p.tors[p.tors_off[m] + q];
+            t[4] += torsion_term(src, i0, e, p);
+            if (old) t[4] -= torsion_term(old, i0, e, p);
+        } else if (q < nt + na) {
+            const int4 e = p.angles[p.angle_off[m] + q - nt];
+            t[3] += angle_term(src, i0, e);
+            if (old) t[3] -= angle_term(old, i0, e);
+        } else if (q < nt + na + nb) {
+            const int2 e = p.bonds[p.bond_off[m] + q - nt - na];
+            t[2] += bond_term(src, i0, e, p.box);
+            if (old) t[2] -= bond_term(old, i0, e, p.box);
+        } else {
+            const int2 e = p.ind[p.ind_off[m] + q - nt - na - nb];
+            float lj, dh;
+            if (GLOBAL_TABLE)
+                pair_global(src[e.x - i0], src[e.y - i0], p.table_scaled, p, lj, dh);
+            else
+                pair_any(src[e.x - i0], src[e.y - i0], tab, p, lj, dh);
+            t[0] -= (double)lj;
+            t[1] -= (double)dh;
+            if (old) {
+                if (GLOBAL_TABLE)
+                    pair_global(old[e.x - i0], old[e.y - i0], p.table_scaled, p, lj, dh);
+                else
+                    pair_any(old[e.x - i0], old[e.y - i0], tab, p, lj, dh);
+                t[0] += (double)lj;
+                t[1] += (double)dh;
+            }
+        }
     }
 }
 
@@ -813,7 +871,7 @@ __global__ __launch_bounds__(kWG) void intra_energy_kernel(PairParams p)
             }
         } else {
             double t[5];
-            bonded_terms(src, i0, m, p, tab, t);
+            bonded_terms<false>(src, i0, m, p, tab, t);
             if (SCALED) {
                 const double v = wave_sum(bonded_kcal(t));
                 if (lane == 0) atomicAdd(diag, v);
@@ -893,32 +951,19 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
             const float4 *src = p.prop_pos + (size_t)r * p.max_n;
             const int4 mv = p.prop_moved[r];
             double v;
+            // (the bonded terms were added to the diagonal when the proposal
+            // was laid out)
             if (mv.x == kMovedAll) {
-                // the whole internal energy of the proposal
-                if (l2 < nib * njc) {
-                    float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
-                    bool valid;
-                    intra_item(src, p.rtag + i0, n, l2 / njc, l2 % njc, p, tab, jrec, jtag, s0, c0, s1, c1, valid);
-                    v = valid ? ((double)s0 - (double)c0) + ((double)s1 - (double)c1) : 0.0;
-                } else {
-                    double t[5];
-                    bonded_terms(src, i0, m, p, tab, t);
-                    v = bonded_kcal(t);
-                }
+                // the intra-molecular pairs of the proposal
+                float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
+                bool valid;
+                intra_item(src, p.rtag + i0, n, l2 / njc, l2 % njc, p, tab, jrec, jtag, s0, c0, s1, c1, valid);
+                v = valid ? ((double)s0 - (double)c0) + ((double)s1 - (double)c1) : 0.0;
             } else {
-                // its change: the moved residue's pairs and the bonded terms,
-                // new minus current
-                const float4 *cur = p.pos + (size_t)r * p.stride + i0;
-                if (l2 < njc) {
-                    float s0 = 0.f, c0 = 0.f;
-                    moved_item(src, cur, p.rtag + i0, n, mv.y, l2, p, tab, s0, c0);
-                    v = (double)s0 - (double)c0;
-                } else {
-                    double tn[5], to[5];
-                    bonded_terms(src, i0, m, p, tab, tn);
-                    bonded_terms(cur, i0, m, p, tab, to);
-                    v = bonded_kcal(tn) - bonded_kcal(to);
-                }
+                // the moved residue's pairs, new minus current
+                float s0 = 0.f, c0 = 0.f;
+                moved_item(src, p.pos + (size_t)r * p.stride + i0, p.rtag + i0, n, mv.y, l2, p, tab, s0, c0);
+                v = (double)s0 - (double)c0;
             }
             v = wave_sum(v);
             if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + m, v);
@@ -1288,17 +1333,16 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
 }
 
 // Items of the i-lane pair kernel for a moved molecule m of n residues:
-// (i-block, j-chunk) items, and for a flexible molecule its internal-energy
-// items: for kMovedAll every intra-molecular (i-block, j-chunk) item and one
-// bonded item, for kMovedOne one item per j-chunk and one bonded-difference
-// item.
+// (i-block, j-chunk) items, and for a flexible molecule its intra-molecular
+// items: for kMovedAll every (i-block, j-chunk) item, for kMovedOne one item
+// per j-chunk.
 __device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, int kind)
 {
     const int nib = (n + kIBlock - 1) / kIBlock;
     int items = nib * p.jc_count;
     if (p.n_flex && p.mol_flex[m]) {
         const int njc = (n + kJChunk - 1) / kJChunk;
-        items += (kind == kMovedOne ? njc : nib * njc) + 1;
+        items += kind == kMovedOne ? njc : nib * njc;
     }
     return items;
 }
@@ -1306,9 +1350,11 @@ __device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, in
 // Thread 0, after a proposal of molecule m (n residues) of replica r is
 // laid out: its i-lane items, the internal-energy record, the proposed
 // bounding radius, and the diagonal row entry the pair kernel adds the
-// internal energy to (the current one for kMovedOne, else 0).
+// intra-molecular pairs to: the bonded energy of the proposal (kMovedAll),
+// or the current internal energy plus the bonded change (kMovedOne); bonded
+// is that value in kcal/mol.
 __device__ __forceinline__ void record_proposal(const PairParams &p, int r, int m, int n, int kind, int moved,
-                                                float radius_new)
+                                                float radius_new, double bonded)
 {
     const bool flex = p.n_flex && p.mol_flex[m];
     if (!flex) kind = kMovedNone;
@@ -1317,8 +1363,28 @@ __device__ __forceinline__ void record_proposal(const PairParams &p, int r, int 
     p.prop_radius[r] = radius_new;
     if (flex) {
         const size_t d = (size_t)r * p.n_mol + m;
-        p.rowbuf[d] = kind == kMovedOne ? p.emat[(size_t)r * p.n_mol * p.n_mol + (size_t)m * p.n_mol + m] : 0.0;
+        p.rowbuf[d] = (kind == kMovedOne ? p.emat[(size_t)r * p.n_mol * p.n_mol + (size_t)m * p.n_mol + m] : 0.0) + bonded;
     }
+}
+
+// Bonded energy (kcal/mol) of flexible molecule m's proposal of replica r,
+// minus that of its current coordinates when diff (whole workgroup; every
+// thread returns the sum).  The proposal is in place.
+__device__ double proposal_bonded(const PairParams &p, int r, int m, bool diff)
+{
+    __shared__ double sb[kWavesPerWG];
+    const int i0 = p.mol_off[m];
+    double tn[5];
+    bonded_terms<true>(p.prop_pos + (size_t)r * p.max_n, i0, m, p, nullptr, tn,
+                       diff ? p.pos + (size_t)r * p.stride + i0 : nullptr);
+    double v = bonded_kcal(tn);
+    v = wave_sum(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) sb[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < kWavesPerWG; ++w) t += sb[w];
+    __syncthreads();  // sb may be reused by the next call
+    return t;
 }
 
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
@@ -1505,6 +1571,8 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     for (int o = 32; o > 0; o >>= 1) r2max = fmaxf(r2max, __shfl_xor(r2max, o, kWave));
     if ((threadIdx.x & (kWave - 1)) == 0) sh_r2[threadIdx.x / kWave] = r2max;
     __syncthreads();
+    const bool one = sh_moved && sh_mone >= 0 && !sh_wrap;
+    const double bonded = proposal_bonded(p, r, m, one);
     if (threadIdx.x == 0) {
         float rad = p.radius[(size_t)r * p.n_mol + m];
         if (sh_moved) {
@@ -1516,8 +1584,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
         p.prop_nib[r] = (n + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n;
         p.prop_center[r] = make_float4(c0, c1, c2, 0.0f);
-        const bool one = sh_moved && sh_mone >= 0 && !sh_wrap;
-        record_proposal(p, r, m, n, one ? kMovedOne : kMovedAll, one ? sh_mone : 0, rad);
+        record_proposal(p, r, m, n, one ? kMovedOne : kMovedAll, one ? sh_mone : 0, rad, bonded);
     }
 }
 
@@ -1548,6 +1615,8 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
             p.prop_pos[(size_t)r * p.max_n + (i - i_beg)] = pn;
             p.prop_rel[(size_t)r * p.max_n + (i - i_beg)] = rn;
         }
+        __syncthreads();  // the proposal is in place
+        const double bonded = proposal_bonded(p, r, pr.mol, false);
         if (threadIdx.x == 0) {
             p.prop_mol[r] = pr.mol;
             p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
@@ -1555,7 +1624,8 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
             p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
             // a rigid-body move keeps the radius; its rounded coordinates are
             // evaluated in full
-            record_proposal(p, r, pr.mol, i_end - i_beg, kMovedAll, 0, p.radius[(size_t)r * p.n_mol + pr.mol]);
+            record_proposal(p, r, pr.mol, i_end - i_beg, kMovedAll, 0, p.radius[(size_t)r * p.n_mol + pr.mol],
+                            bonded);
         }
         return;
     }
@@ -1573,7 +1643,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
         p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
         p.prop_n[r] = i_end - i_beg;
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-        record_proposal(p, r, pr.mol, i_end - i_beg, kMovedNone, 0, p.radius[(size_t)r * p.n_mol + pr.mol]);
+        record_proposal(p, r, pr.mol, i_end - i_beg, kMovedNone, 0, p.radius[(size_t)r * p.n_mol + pr.mol], 0.0);
     }
 }
 
@@ -1824,7 +1894,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n_2;
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-        record_proposal(p, r, m2, n_2, kMovedNone, 0, p.radius[(size_t)r * M + m2]);
+        record_proposal(p, r, m2, n_2, kMovedNone, 0, p.radius[(size_t)r * M + m2], 0.0);
     }
 }
 
@@ -2759,6 +2829,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.rep_count = ctx->R;
     p.xcd_map = 0;
     p.table = ctx->d_table_raw;
+    p.table_scaled = ctx->d_table_scaled;
     p.box = ctx->box;
     p.inv_box = 1.0f / ctx->box;
     p.pos = ctx->d_pos;
