@@ -48,6 +48,11 @@ constexpr int kColShift = 4;              // 16-byte entries
 constexpr int kTableBytes = CGPPD_AA_COUNT << kRowShift;
 constexpr uint32_t kRowMask = 0x1Fu << kRowShift;
 constexpr uint32_t kChargedBit = 0x80000000u;
+// Residue word (the w lane of positions): type << kRowShift | charged bit |
+// (residues left in the residue's constant-type run, itself included,
+// capped at kRunMax) << kRunShift.
+constexpr int kRunShift = 14;
+constexpr uint32_t kRunMax = 0x1FFFFu;
 // Fixed-point row partials: 2^-32 kcal/mol units in 64-bit integers, so
 // that their atomic sum does not depend on the order of the additions.
 // Partials of magnitude 2^17 kcal/mol or more (clashes) take the FP64 path.
@@ -291,6 +296,7 @@ struct PairParams {
     int *prop_nib;        // R: i-blocks of the moved molecule
     int *prop_n;          // R: residues of the moved molecule
     int *prop_items;      // R: items of the i-lane pair kernel for the pending proposal
+    int4 *prop_hdr;       // R: {molecule, first residue, residues, bounding radius bits} of the proposal
     int row_j;            // j residues per j-block of the row kernel (64 x residues per lane)
     // flexible molecules (null / 0 when the system has none).  Residue
     // indices are device indices (global); lists are CSR per molecule.
@@ -314,6 +320,7 @@ struct PairParams {
     const int *nb_off;          // per flex residue entry: first neighbour entry
     const int *br_off, *br;     // per neighbour entry: residues the flex move rotates (caller order)
     const double *tor_v, *tor_d;   // 20 x 20 x 4 torsion table (kcal/mol, radians)
+    const double *tor_cd, *tor_sd;  // cos and sin of tor_d
     double *out_int;            // R * 5 raw internal terms (full energy)
     // delayed acceptance with the surrogate (da = 0: plain Metropolis)
     int da;
@@ -335,6 +342,58 @@ __device__ __forceinline__ double wave_sum(double v)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
+}
+
+// 64-bit DPP move: lane l receives the value of the lane the GFX9 control
+// CTRL names (row_shr:n = 0x110 + n, row_bcast:15 = 0x142, row_bcast:31 =
+// 0x143), or 0 where that lane is out of its row or ROW_MASK disables l's row.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ long long dpp_b64(long long v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROW_MASK, 0xf, false);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ long long readlane_b64(long long v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane((int)v, l), hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Inclusive prefix sums over the 64 lanes (every lane active): shifts by
+// 1, 2, 4, 8 within rows of 16, then row 0's total into row 1 and row 2's
+// into row 3 (row_bcast:15), then rows 0-1's total into rows 2-3
+// (row_bcast:31).  Six DPP steps instead of twelve LDS-crossbar shuffles.
+__device__ __forceinline__ long long wave_scan_i64(long long v)
+{
+    v += dpp_b64<0x111, 0xf>(v);
+    v += dpp_b64<0x112, 0xf>(v);
+    v += dpp_b64<0x114, 0xf>(v);
+    v += dpp_b64<0x118, 0xf>(v);
+    v += dpp_b64<0x142, 0xa>(v);
+    v += dpp_b64<0x143, 0xc>(v);
+    return v;
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    return __longlong_as_double(dpp_b64<CTRL, ROW_MASK>(__double_as_longlong(v)));
+}
+
+// Sum over the 64 lanes (every lane active), returned in every lane: the
+// DPP scan's last lane, read back.  A fixed summation order, so the result
+// is reproducible.
+__device__ __forceinline__ double wave_total(double v)
+{
+    v += dpp_f64<0x111, 0xf>(v);
+    v += dpp_f64<0x112, 0xf>(v);
+    v += dpp_f64<0x114, 0xf>(v);
+    v += dpp_f64<0x118, 0xf>(v);
+    v += dpp_f64<0x142, 0xa>(v);
+    v += dpp_f64<0x143, 0xc>(v);
+    return __longlong_as_double(readlane_b64(__double_as_longlong(v), kWave - 1));
 }
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
@@ -611,12 +670,11 @@ __device__ __forceinline__ float fmag(float x, float y, float z)
 // Molecule::E(false) subtracts (src/Molecule.cpp:669-683).  Every thread
 // takes a strided share; t[5] = indirect LJ and DH (negated), bond sum,
 // sum of log(angle term), torsion sum.
-// One bonded or indirect-neighbour term of each kind, from the molecule-
-// local coordinates src (the term's residues are device indices minus i0).
-__device__ __forceinline__ double bond_term(const float4 *__restrict__ src, int i0, int2 e, float box)
+// One bonded or indirect-neighbour term of each kind, from its residues'
+// coordinates.
+__device__ __forceinline__ double bond_term(float4 a, float4 b, float box)
 {
 #pragma clang fp contract(off)
-    const float4 a = src[e.x - i0], b = src[e.y - i0];
     float X = b.x - a.x, Y = b.y - a.y, Z = b.z - a.z;
     X = X - box * roundf(X / box);
     Y = Y - box * roundf(Y / box);
@@ -625,10 +683,9 @@ __device__ __forceinline__ double bond_term(const float4 *__restrict__ src, int 
     return (r - (double)3.81f) * (r - (double)3.81f);
 }
 
-__device__ __forceinline__ double angle_term(const float4 *__restrict__ src, int i0, int4 e)
+__device__ __forceinline__ double angle_term(float4 a, float4 b, float4 c)
 {
 #pragma clang fp contract(off)
-    const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0];
     const float abx = a.x - b.x, aby = a.y - b.y, abz = a.z - b.z;
     const float cbx = c.x - b.x, cby = c.y - b.y, cbz = c.z - b.z;
     const double cmag = (double)(fmag(cbx, cby, cbz) * fmag(abx, aby, abz));
@@ -636,14 +693,16 @@ __device__ __forceinline__ double angle_term(const float4 *__restrict__ src, int
     if (!(cmag < 1e-26)) th = (float)acos((double)(abx * cbx + aby * cby + abz * cbz) / cmag);
     const double th_d = (double)th;
     const double da = th_d - (double)1.6f, db = th_d - (double)2.27f;
-    return log(exp(-(double)0.1f * ((double)106.4f * da * da + (double)4.3f)) +
-               exp(-(double)0.1f * ((double)26.3f * db * db)));
+    // log(e^u + e^v) = max(u, v) + log1p(e^-|u - v|): one exp fewer, the
+    // same FP64 value to rounding
+    const double u = -(double)0.1f * ((double)106.4f * da * da + (double)4.3f);
+    const double v = -(double)0.1f * ((double)26.3f * db * db);
+    return fmax(u, v) + log1p(exp(-fabs(u - v)));
 }
 
-__device__ __forceinline__ double torsion_term(const float4 *__restrict__ src, int i0, int4 e, const PairParams &p)
+__device__ __forceinline__ double torsion_term(float4 a, float4 b, float4 c, float4 d, const PairParams &p)
 {
 #pragma clang fp contract(off)
-    const float4 a = src[e.x - i0], b = src[e.y - i0], c = src[e.z - i0], d = src[e.w - i0];
     const float b1x = b.x - a.x, b1y = b.y - a.y, b1z = b.z - a.z;
     const float b2x = c.x - b.x, b2y = c.y - b.y, b2z = c.z - b.z;
     const float b3x = d.x - c.x, b3y = d.y - c.y, b3z = d.z - c.z;
@@ -654,10 +713,18 @@ __device__ __forceinline__ double torsion_term(const float4 *__restrict__ src, i
     const double phi = (double)atan2f(yy, xx);
     const int tj = (int)(__float_as_uint(b.w) >> kRowShift & 0x1Fu);
     const int tk = (int)(__float_as_uint(c.w) >> kRowShift & 0x1Fu);
-    const double *V = p.tor_v + (tj * 20 + tk) * 4;
-    const double *D = p.tor_d + (tj * 20 + tk) * 4;
-    return (1 + cos(phi - D[0])) * V[0] + (1 + cos(2 * phi - D[1])) * V[1] + (1 + cos(3 * phi - D[2])) * V[2] +
-           (1 + cos(4 * phi - D[3])) * V[3];
+    const int e = (tj * 20 + tk) * 4;
+    const double *V = p.tor_v + e, *CD = p.tor_cd + e, *SD = p.tor_sd + e;
+    // cos(n phi - d_n) = cos(n phi) cos d_n + sin(n phi) sin d_n, with
+    // cos(n phi) and sin(n phi) from one sincos by the angle-addition
+    // formulas (the same FP64 value as four cos calls, to rounding)
+    double s1, c1;
+    sincos(phi, &s1, &c1);
+    const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * (s1 * c1);
+    const double c3 = c2 * c1 - s2 * s1, s3 = s2 * c1 + c2 * s1;
+    const double c4 = c2 * c2 - s2 * s2, s4 = 2.0 * (s2 * c2);
+    return (1 + (c1 * CD[0] + s1 * SD[0])) * V[0] + (1 + (c2 * CD[1] + s2 * SD[1])) * V[1] +
+           (1 + (c3 * CD[2] + s3 * SD[2])) * V[2] + (1 + (c4 * CD[3] + s4 * SD[3])) * V[3];
 }
 
 // Bonded terms of flexible molecule m and the indirect-neighbour pairs that
@@ -666,49 +733,104 @@ __device__ __forceinline__ double torsion_term(const float4 *__restrict__ src, i
 // its threads).
 // t[5] = indirect LJ and DH (negated), bond sum, sum of log(angle term),
 // torsion sum, this thread's share.  With old non-null, every term is
-// evaluated on both coordinate sets and t holds new minus old.
+// evaluated on both coordinate sets and t holds new minus old; with only
+// >= 0 as well (the molecule-local index of the one residue whose
+// coordinates differ between src and old), the terms without it are skipped.
 // GLOBAL_TABLE: the indirect-neighbour pairs read the scaled table from
 // global memory (kernels without the LDS table), else the LDS table tab.
 template <bool GLOBAL_TABLE>
 __device__ void bonded_terms(const float4 *__restrict__ src, int i0, int m, const PairParams &p,
-                             const char *__restrict__ tab, double t[5], const float4 *__restrict__ old = nullptr)
+                             const char *__restrict__ tab, double t[5], const float4 *__restrict__ old = nullptr,
+                             int only = -1)
 {
 #pragma clang fp contract(off)
     for (int k = 0; k < 5; ++k) t[k] = 0.0;
     // one index space over the terms, the costliest kinds first (torsions,
-    // angles, bonds, indirect pairs): thread t takes entries t, t + block, ...
-    const int nt = p.tors_off[m + 1] - p.tors_off[m], na = p.angle_off[m + 1] - p.angle_off[m];
-    const int nb = p.bond_off[m + 1] - p.bond_off[m], ni = p.ind_off[m + 1] - p.ind_off[m];
-    const int total = nt + na + nb + ni;
-    for (int q = threadIdx.x; q < total; q += blockDim.x) {
-        if (q < nt) {
-            const int4 e = p.tors[p.tors_off[m] + q];
-            t[4] += torsion_term(src, i0, e, p);
-            if (old) t[4] -= torsion_term(old, i0, e, p);
-        } else if (q < nt + na) {
-            const int4 e = p.angles[p.angle_off[m] + q - nt];
-            t[3] += angle_term(src, i0, e);
-            if (old) t[3] -= angle_term(old, i0, e);
-        } else if (q < nt + na + nb) {
-            const int2 e = p.bonds[p.bond_off[m] + q - nt - na];
-            t[2] += bond_term(src, i0, e, p.box);
-            if (old) t[2] -= bond_term(old, i0, e, p.box);
-        } else {
-            const int2 e = p.ind[p.ind_off[m] + q - nt - na - nb];
-            float lj, dh;
-            if (GLOBAL_TABLE)
-                pair_global(src[e.x - i0], src[e.y - i0], p.table_scaled, p, lj, dh);
-            else
-                pair_any(src[e.x - i0], src[e.y - i0], tab, p, lj, dh);
-            t[0] -= (double)lj;
-            t[1] -= (double)dh;
+    // angles, bonds, indirect pairs), each kind's range padded to whole
+    // waves so that a wave evaluates one kind at a time: thread t takes
+    // entries t, t + block, ..., kBatch at a time (their indices, then their
+    // coordinates, then the arithmetic)
+    const int to = p.tors_off[m], ao = p.angle_off[m], bo = p.bond_off[m], io = p.ind_off[m];
+    const int nt = p.tors_off[m + 1] - to, na = p.angle_off[m + 1] - ao;
+    const int nb = p.bond_off[m + 1] - bo, ni = p.ind_off[m + 1] - io;
+    const int qa = (nt + kWave - 1) & ~(kWave - 1), qb = qa + ((na + kWave - 1) & ~(kWave - 1));
+    const int qi = qb + ((nb + kWave - 1) & ~(kWave - 1)), total = qi + ni;
+    const int g = i0 + only;  // device index of the only moved residue
+    constexpr int kBatch = 4;
+    for (int q0 = threadIdx.x; q0 < total; q0 += kBatch * blockDim.x) {
+        int kind[kBatch];
+        int4 e[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int q = q0 + u * blockDim.x;
+            kind[u] = q >= total ? -1
+                      : q < qa   ? (q < nt ? 0 : -1)
+                      : q < qb   ? (q - qa < na ? 1 : -1)
+                      : q < qi   ? (q - qb < nb ? 2 : -1)
+                                 : 3;
+            // unused residue slots name the molecule's first residue
+            e[u] = make_int4(i0, i0, i0, i0);
+            if (kind[u] == 0) {
+                e[u] = p.tors[to + q];
+            } else if (kind[u] == 1) {
+                const int4 a = p.angles[ao + q - qa];
+                e[u] = make_int4(a.x, a.y, a.z, i0);
+            } else if (kind[u] >= 2) {
+                const int2 b = kind[u] == 2 ? p.bonds[bo + q - qb] : p.ind[io + q - qi];
+                e[u] = make_int4(b.x, b.y, i0, i0);
+            }
+        }
+        if (only >= 0) {
+            // one residue moved and nothing else did: the other terms'
+            // differences are exactly zero
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const bool has = e[u].x == g || e[u].y == g || (kind[u] <= 1 && e[u].z == g) || (kind[u] == 0 && e[u].w == g);
+                if (!has) kind[u] = -1;
+            }
+        }
+        float4 A[kBatch][4], O[kBatch][4];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (kind[u] < 0) continue;
+            A[u][0] = src[e[u].x - i0];
+            A[u][1] = src[e[u].y - i0];
+            A[u][2] = src[e[u].z - i0];
+            A[u][3] = src[e[u].w - i0];
             if (old) {
+                O[u][0] = old[e[u].x - i0];
+                O[u][1] = old[e[u].y - i0];
+                O[u][2] = old[e[u].z - i0];
+                O[u][3] = old[e[u].w - i0];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (kind[u] == 0) {
+                t[4] += torsion_term(A[u][0], A[u][1], A[u][2], A[u][3], p);
+                if (old) t[4] -= torsion_term(O[u][0], O[u][1], O[u][2], O[u][3], p);
+            } else if (kind[u] == 1) {
+                t[3] += angle_term(A[u][0], A[u][1], A[u][2]);
+                if (old) t[3] -= angle_term(O[u][0], O[u][1], O[u][2]);
+            } else if (kind[u] == 2) {
+                t[2] += bond_term(A[u][0], A[u][1], p.box);
+                if (old) t[2] -= bond_term(O[u][0], O[u][1], p.box);
+            } else if (kind[u] == 3) {
+                float lj, dh;
                 if (GLOBAL_TABLE)
-                    pair_global(old[e.x - i0], old[e.y - i0], p.table_scaled, p, lj, dh);
+                    pair_global(A[u][0], A[u][1], p.table_scaled, p, lj, dh);
                 else
-                    pair_any(old[e.x - i0], old[e.y - i0], tab, p, lj, dh);
-                t[0] += (double)lj;
-                t[1] += (double)dh;
+                    pair_any(A[u][0], A[u][1], tab, p, lj, dh);
+                t[0] -= (double)lj;
+                t[1] -= (double)dh;
+                if (old) {
+                    if (GLOBAL_TABLE)
+                        pair_global(O[u][0], O[u][1], p.table_scaled, p, lj, dh);
+                    else
+                        pair_any(O[u][0], O[u][1], tab, p, lj, dh);
+                    t[0] += (double)lj;
+                    t[1] += (double)dh;
+                }
             }
         }
     }
@@ -965,7 +1087,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
                 moved_item(src, p.pos + (size_t)r * p.stride + i0, p.rtag + i0, n, mv.y, l2, p, tab, s0, c0);
                 v = (double)s0 - (double)c0;
             }
-            v = wave_sum(v);
+            v = wave_total(v);
             if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + m, v);
             continue;
         }
@@ -995,7 +1117,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
                 segment<false>(jrec, js - jbase, se - jbase, jbase, pn, lane_off, tab, p, ci, ri,
                                p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
                 double v = valid ? (double)s0 - (double)c0 : 0.0;
-                v = wave_sum(v);
+                v = wave_total(v);
                 if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
             }
             js = se;
@@ -1115,21 +1237,23 @@ __device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, 
 }
 
 template <int IMG, int JPL>
-__device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec, int i0, int ia, int ib,
+__device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec,
+                                         const float *__restrict__ qtab, int ia, int ib,
                                          const LaneJ<JPL> &lj, const char *__restrict__ tab, float (&s)[JPL],
                                          float (&c)[JPL])
 {
-    int run = __builtin_amdgcn_readfirstlane(p.run_of[i0 + ia]);
     int a = ia;
     while (a < ib) {
-        const int b = __builtin_amdgcn_readfirstlane(min(ib, p.run_start[run + 1] - i0));
-        const uint32_t code = __builtin_amdgcn_readfirstlane(p.run_code[run]);
+        // the run [a, b) of one type: the first record's word gives the
+        // type, the charge flag and the rest of the run
+        const uint32_t code = __float_as_uint(scalar_rec(irec, a).w);
+        const int b = min(ib, a + max(1, (int)(code >> kRunShift & kRunMax)));
         RowCoef cf[JPL];
         float qq[JPL];
 #pragma unroll
         for (int h = 0; h < JPL; ++h) cf[h] = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[h]));
         if (code & kChargedBit) {
-            const float qi = p.charge[(code & kRowMask) >> kRowShift] * p.dh_factor;
+            const float qi = qtab[(code & kRowMask) >> kRowShift] * p.dh_factor;
 #pragma unroll
             for (int h = 0; h < JPL; ++h) qq[h] = qi * lj.q[h];
             row_run<IMG, true, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
@@ -1139,7 +1263,6 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
             row_run<IMG, false, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
         }
         a = b;
-        ++run;
     }
 }
 
@@ -1147,32 +1270,40 @@ template <int JPL>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4 : 8, 8))) void mc_row_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
-    __shared__ long long unit_off[CGPPD_MAX_REPLICAS + 1];
+    __shared__ float qtab[CGPPD_AA_COUNT];
+    // dynamic LDS (row_lds_bytes): the replicas' unit offsets, then for at
+    // most 64 replicas their proposal headers and centres
+    extern __shared__ __attribute__((aligned(16))) long long row_lds[];
+    const int R = p.rep_count;
+    const bool hdr_lds = R <= kWave;
+    long long *unit_off = row_lds;
+    int4 *s_hdr = reinterpret_cast<int4 *>(row_lds + ((R + 2) & ~1));
+    float4 *s_cen = reinterpret_cast<float4 *>(s_hdr + (hdr_lds ? R : 0));
     // entry e = tj * 20 + ti (i = the moved residue) goes to row ti, column tj
     for (int e = threadIdx.x; e < CGPPD_AA_COUNT * CGPPD_AA_COUNT; e += kWG) {
         const int tj = e / CGPPD_AA_COUNT, ti = e % CGPPD_AA_COUNT;
         *reinterpret_cast<RowCoef *>(tab + ((ti << kRowShift) | (tj << kColShift))) = p.row_table[e];
     }
-    const int R = p.rep_count;
+    if (threadIdx.x < CGPPD_AA_COUNT) qtab[threadIdx.x] = p.charge[threadIdx.x];
     // unit offsets of the replicas: an inclusive scan by wave 0, 64
-    // replicas at a time
+    // replicas at a time, from the proposal headers
     if (threadIdx.x < kWave) {
         const int lane0 = threadIdx.x;
         long long carry = 0;
         for (int base = 0; base < R; base += kWave) {
             const int r = base + lane0;
             long long c = 0;
-            if (r < R && !(p.da && !p.s1[r])) {
-                const int n = p.prop_n[r];
-                c = (long long)((p.n_res - n + JPL * kWave - 1) / (JPL * kWave)) * n;
+            if (r < R) {
+                const int4 hd = p.prop_hdr[r];
+                if (hdr_lds) {
+                    s_hdr[r] = hd;
+                    s_cen[r] = p.prop_center[r];
+                }
+                if (!(p.da && !p.s1[r])) c = (long long)((p.n_res - hd.z + JPL * kWave - 1) / (JPL * kWave)) * hd.z;
             }
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const long long t = __shfl_up(c, o, kWave);
-                if (lane0 >= o) c += t;
-            }
+            c = wave_scan_i64(c);
             if (r < R) unit_off[r + 1] = carry + c;
-            carry += __shfl(c, kWave - 1, kWave);
+            carry += readlane_b64(c, kWave - 1);
         }
         if (lane0 == 0) unit_off[0] = 0;
     }
@@ -1190,16 +1321,16 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     long long P = wv == 0 ? 0 : (long long)floor((double)wv * per);
     const long long P1 = wv + 1 == W ? total : (long long)floor((double)(wv + 1) * per);
     if (P >= P1) return;
-    // replica holding unit P (binary search; wave-uniform)
-    int r;
-    {
-        int lo = 0, hi = R;  // unit_off[lo] <= P < unit_off[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (unit_off[mid] <= P) lo = mid; else hi = mid;
-        }
-        r = __builtin_amdgcn_readfirstlane(lo);
+    // replica holding unit P: the number of replicas that end at or before
+    // P (unit_off is non-decreasing and P < unit_off[R]), 64 at a time
+    int r = 0;
+    for (int base = 0; base < R; base += kWave) {
+        const int k = base + lane;
+        const unsigned long long before = __ballot(k < R && unit_off[k + 1] <= P);
+        r = base + __popcll(before);
+        if (before != ~0ull) break;
     }
+    r = __builtin_amdgcn_readfirstlane(r);
     const int M = p.n_mol;
     // position of unit P inside its replica: j-block jb, residue ia of m
     // (one division for the first visit, then the visits advance)
@@ -1209,9 +1340,22 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
             while (P >= unit_off[r + 1]) ++r;
             jb = -1;
         }
-        const int m = __builtin_amdgcn_readfirstlane(p.prop_mol[r]);
-        const int i0 = __builtin_amdgcn_readfirstlane(p.mol_off[m]);
-        const int n_m = __builtin_amdgcn_readfirstlane(p.mol_off[m + 1]) - i0;
+        // the proposal: molecule, first residue, size, bounding radius, centre
+        int4 hd;
+        float4 cm;
+        if (hdr_lds) {
+            hd = s_hdr[r];
+            cm = s_cen[r];
+        } else {
+            hd = p.prop_hdr[r];
+            cm = p.prop_center[r];
+        }
+        const int i0 = __builtin_amdgcn_readfirstlane(hd.y);
+        const int n_m = __builtin_amdgcn_readfirstlane(hd.z);
+        const float rm = __int_as_float(__builtin_amdgcn_readfirstlane(hd.w));
+        cm.x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cm.x)));
+        cm.y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cm.y)));
+        cm.z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cm.z)));
         if (jb < 0) {
             // units per replica fit in 32 bits (set_replicas bounds them)
             const unsigned local = (unsigned)(P - unit_off[r]);
@@ -1237,38 +1381,33 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
                 J[h] = p.mol_of[j];
             }
             lj.col[h] = lane_col(rj[h]);
-            lj.q[h] = p.charge[lj.col[h] >> kColShift];
+            lj.q[h] = qtab[lj.col[h] >> kColShift];
         }
-        // image handling of this visit (see image_axis): the centre image
-        // of each j residue's molecule, and the axes on which some lane
-        // needs the per-pair minimum image
+        // image handling of this visit (see image_axis): each j residue is
+        // moved to its image nearest the proposal's centre cm; every moved
+        // residue i lies within rm of cm, so on an axis where
+        // |cm - x_j| < L/2 - rm (with margin) for every lane the plain
+        // difference is the minimum image, and elsewhere
+        // min(|dx|, L - |dx|) is (|dx| < L needs rm < L/2, with margin)
         int img = kImgFull;
         float sh[JPL][3];
-        if (p.fast_image) {
+        const float reach = rm * 1.0001f + 0.01f;
+        if (p.fast_image && reach < 0.45f * p.box) {
+            const float lim = 0.5f * p.box - reach;
             int need = 0;
-            const float4 cm = p.prop_center[r];
-            const float rm = p.radius[(size_t)r * M + m];
 #pragma unroll
             for (int h = 0; h < JPL; ++h) {
                 sh[h][0] = sh[h][1] = sh[h][2] = 0.0f;
                 if (!valid[h]) continue;
-                const float4 cj = p.center[(size_t)r * M + J[h]];
-                const float rsum = rm + p.radius[(size_t)r * M + J[h]];
-                const float lim = 0.5f * p.box - (rsum * 1.0001f + 0.01f);
-                const float kx = rintf((cm.x - cj.x) * p.inv_box), ky = rintf((cm.y - cj.y) * p.inv_box),
-                            kz = rintf((cm.z - cj.z) * p.inv_box);
-                const float ex = fmaf(-p.box, kx, cm.x - cj.x), ey = fmaf(-p.box, ky, cm.y - cj.y),
-                            ez = fmaf(-p.box, kz, cm.z - cj.z);
+                const float ox = cm.x - rj[h].x, oy = cm.y - rj[h].y, oz = cm.z - rj[h].z;
+                const float kx = rintf(ox * p.inv_box), ky = rintf(oy * p.inv_box), kz = rintf(oz * p.inv_box);
+                const float ex = fmaf(-p.box, kx, ox), ey = fmaf(-p.box, ky, oy), ez = fmaf(-p.box, kz, oz);
                 sh[h][0] = p.box * kx;
                 sh[h][1] = p.box * ky;
                 sh[h][2] = p.box * kz;
                 need |= (fabsf(ex) < lim ? 0 : 1) | (fabsf(ey) < lim ? 0 : 2) | (fabsf(ez) < lim ? 0 : 4);
-                // |dx| < L after the shift needs rsum < L/2 (with margin)
-                if (!(rsum * 1.0001f + 0.01f < 0.45f * p.box)) need |= kImgFull;
             }
-            const bool full = __any(need & kImgFull);
-            img = full ? kImgFull
-                       : (__any(need & 1) ? 1 : 0) | (__any(need & 2) ? 2 : 0) | (__any(need & 4) ? 4 : 0);
+            img = (__any(need & 1) ? 1 : 0) | (__any(need & 2) ? 2 : 0) | (__any(need & 4) ? 4 : 0);
         }
         const bool shift = img != kImgFull;
 #pragma unroll
@@ -1282,27 +1421,29 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
 #pragma unroll
         for (int h = 0; h < JPL; ++h) sa[h] = ca[h] = 0.0f;
         switch (img) {
-        case 0: row_runs<0, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 1: row_runs<1, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 2: row_runs<2, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 3: row_runs<3, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 4: row_runs<4, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 5: row_runs<5, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 6: row_runs<6, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        case 7: row_runs<7, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
-        default: row_runs<kImgFull, JPL>(p, irec, i0, ia, ib, lj, tab, sa, ca); break;
+        case 0: row_runs<0, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 1: row_runs<1, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 2: row_runs<2, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 3: row_runs<3, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 4: row_runs<4, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 5: row_runs<5, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 6: row_runs<6, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 7: row_runs<7, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        default: row_runs<kImgFull, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
         }
         // FP64 reduction per molecule J: across the lanes J[h] is
         // non-decreasing for each h, and every J[h] <= every J[h + 1]
         double v[JPL];
 #pragma unroll
         for (int h = 0; h < JPL; ++h) v[h] = valid[h] ? (double)sa[h] - (double)ca[h] : 0.0;
+        // (lane 0 holds the block's first j residue: when it has none, no
+        // lane has)
         int Jc = __builtin_amdgcn_readfirstlane(J[0]);
-        for (;;) {
+        while (Jc != 0x7fffffff) {
             double part = 0.0;
 #pragma unroll
             for (int h = 0; h < JPL; ++h) part += J[h] == Jc ? v[h] : 0.0;
-            const double t = wave_sum(part);
+            const double t = wave_total(part);
             if (lane == 0) {
                 const size_t at = (size_t)r * M + Jc;
                 if (fabs(t) < kFixMax)
@@ -1317,8 +1458,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
                 const unsigned long long rest = __ballot(valid[h] && J[h] > Jc);
                 if (rest) next = __builtin_amdgcn_readlane(J[h], __builtin_ctzll(rest));
             }
-            if (next < 0) break;
-            Jc = next;
+            Jc = next < 0 ? 0x7fffffff : next;
         }
         P += ib - ia;
         // the next visit starts at the next j-block of this replica, or at
@@ -1353,11 +1493,12 @@ __device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, in
 // intra-molecular pairs to: the bonded energy of the proposal (kMovedAll),
 // or the current internal energy plus the bonded change (kMovedOne); bonded
 // is that value in kcal/mol.
-__device__ __forceinline__ void record_proposal(const PairParams &p, int r, int m, int n, int kind, int moved,
-                                                float radius_new, double bonded)
+__device__ __forceinline__ void record_proposal(const PairParams &p, int r, int m, int i0, int n, int kind,
+                                                int moved, float radius_new, double bonded)
 {
     const bool flex = p.n_flex && p.mol_flex[m];
     if (!flex) kind = kMovedNone;
+    p.prop_hdr[r] = make_int4(m, i0, n, __float_as_int(radius_new));
     p.prop_items[r] = ilane_items(p, m, n, kind);
     p.prop_moved[r] = make_int4(kind, moved, 0, 0);
     p.prop_radius[r] = radius_new;
@@ -1368,23 +1509,68 @@ __device__ __forceinline__ void record_proposal(const PairParams &p, int r, int 
 }
 
 // Bonded energy (kcal/mol) of flexible molecule m's proposal of replica r,
-// minus that of its current coordinates when diff (whole workgroup; every
+// or, when only one residue moved (molecule-local index moved >= 0), the
+// proposal's minus that of the current coordinates (whole workgroup; every
 // thread returns the sum).  The proposal is in place.
-__device__ double proposal_bonded(const PairParams &p, int r, int m, bool diff)
+__device__ double proposal_bonded(const PairParams &p, int r, int m, int moved)
 {
     __shared__ double sb[kWavesPerWG];
     const int i0 = p.mol_off[m];
     double tn[5];
     bonded_terms<true>(p.prop_pos + (size_t)r * p.max_n, i0, m, p, nullptr, tn,
-                       diff ? p.pos + (size_t)r * p.stride + i0 : nullptr);
+                       moved >= 0 ? p.pos + (size_t)r * p.stride + i0 : nullptr, moved);
     double v = bonded_kcal(tn);
-    v = wave_sum(v);
+    v = wave_total(v);
     if ((threadIdx.x & (kWave - 1)) == 0) sb[threadIdx.x / kWave] = v;
     __syncthreads();
     double t = 0.0;
     for (int w = 0; w < kWavesPerWG; ++w) t += sb[w];
     __syncthreads();  // sb may be reused by the next call
     return t;
+}
+
+// The topology side of a flexible move (kind 2 flex, 3 local) of molecule
+// m, fixed by the molecule and the move's random words alone: the flex
+// move's pivot and branch, or the local move's crankshaft triple or
+// translated residue (device indices; -1 where the molecule has none).
+// It takes a chain of dependent index loads, which mc_finish_kernel
+// resolves for the next step while the current step is decided.
+struct FlexPlan {
+    uint32_t x[4];  // the move's Philox words
+    int pivot;      // flex: the flexible residue the branch turns about
+    int b0, nb;     // flex: the branch residues br[b0, b0 + nb)
+    int4 cr;        // crankshaft: residues i - 1, i, i + 1
+    int li;         // local translation: the residue
+};
+
+__device__ __forceinline__ void plan_flexible(const PairParams &p, int r, int64_t step, int m, int kind, FlexPlan &fp)
+{
+    philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, kTagMC | 2u, (uint32_t)p.seed,
+           (uint32_t)(p.seed >> 32), fp.x);
+    fp.pivot = -1;
+    fp.b0 = fp.nb = 0;
+    fp.cr = make_int4(-1, -1, -1, -1);
+    fp.li = -1;
+    if (kind == 2) {
+        // flex(axis, rotate step, ri, neighbour): src/Molecule.cpp:331-413
+        const int nf = p.flexr_off[m + 1] - p.flexr_off[m];
+        if (nf > 0) {
+            const int e = p.flexr_off[m] + mulhi_n(fp.x[0], nf);
+            const int deg = p.nb_off[e + 1] - p.nb_off[e];
+            const int k = p.nb_off[e] + mulhi_n(fp.x[1], deg);
+            fp.pivot = p.flexr[e];
+            fp.b0 = p.br_off[k];
+            fp.nb = p.br_off[k + 1] - fp.b0;
+        }
+    } else if (fp.x[2] < 0x80000000u) {
+        // crankshaft: src/Molecule.cpp:303-329, 415-448
+        const int nc = p.crank_off[m + 1] - p.crank_off[m];
+        if (nc > 0) fp.cr = p.crank[p.crank_off[m] + mulhi_n(fp.x[0], nc)];
+    } else {
+        // local translation: src/Molecule.cpp:422-430
+        const int nl = p.local_off[m + 1] - p.local_off[m];
+        if (nl > 0) fp.li = p.local[p.local_off[m] + mulhi_n(fp.x[0], nl)];
+    }
 }
 
 // A flexible move of molecule m (kind 2 flex, 3 local) of replica r, laid
@@ -1396,7 +1582,8 @@ __device__ double proposal_bonded(const PairParams &p, int r, int m, bool diff)
 // molecule is wrapped (src/Molecule.cpp:242-252).  Bit-identical to the CPU
 // restatement: no FMA contraction, correctly rounded division and square
 // root, the same summation order.
-__device__ void propose_flexible(const PairParams &p, int r, int64_t step, const Proposal &pr, int kind)
+__device__ void propose_flexible(const PairParams &p, int r, int64_t step, const Proposal &pr, int kind,
+                                 const FlexPlan *plan, const int *plan_br, bool copied)
 {
 #pragma clang fp contract(off)
     __shared__ float sh_vf[3], sh_nc[3];
@@ -1409,38 +1596,41 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     float4 *pq = p.prop_rel + (size_t)r * p.max_n;
     __shared__ int sh_mone, sh_wrap;
     __shared__ float sh_r2[kWavesPerWG];
-    for (int k = threadIdx.x; k < n; k += kWG) {
-        pp[k] = pos[k];
-        pq[k] = rel[k];
-    }
+    // the proposal starts as a copy of the current coordinates (already in
+    // place when copied)
+    if (!copied)
+        for (int k = threadIdx.x; k < n; k += kWG) {
+            pp[k] = pos[k];
+            pq[k] = rel[k];
+        }
     if (threadIdx.x == 0) sh_mone = -1;
+    // the move's topology (every thread resolves the same plan unless the
+    // caller did)
+    FlexPlan fp;
+    if (plan)
+        fp = *plan;
+    else
+        plan_flexible(p, r, step, m, kind, fp);
     __syncthreads();
-    // every thread draws the same words (the choices below are uniform)
-    uint32_t x[4];
-    philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, kTagMC | 2u, (uint32_t)p.seed,
-           (uint32_t)(p.seed >> 32), x);
+    const uint32_t x[4] = {fp.x[0], fp.x[1], fp.x[2], fp.x[3]};
     const double rcos = p.rcos[r], rsin = p.rsin[r];
     float acc[3] = {0.f, 0.f, 0.f};
     int moved = 0;
     if (kind == 2) {
         // flex(axis, rotate step, ri, neighbour): src/Molecule.cpp:331-413
-        const int nf = p.flexr_off[m + 1] - p.flexr_off[m];
-        if (nf > 0) {
-            const int e = p.flexr_off[m] + mulhi_n(x[0], nf);
-            const int deg = p.nb_off[e + 1] - p.nb_off[e];
-            const int k = p.nb_off[e] + mulhi_n(x[1], deg);
+        if (fp.pivot >= 0) {
             const double ax = pr.axis[0], ay = pr.axis[1], az = pr.axis[2];
             const double mag = sqrt(ax * ax + ay * ay + az * az);
             const double q[4] = {rcos, rsin * (ax / mag), rsin * (ay / mag), rsin * (az / mag)};
-            const float4 c0 = pos[p.flexr[e] - i0];
-            const int b0 = p.br_off[k], nb = p.br_off[k + 1] - b0;
+            const float4 c0 = pos[fp.pivot - i0];
+            const int b0 = fp.b0, nb = fp.nb;
             // chunks of kWG residues: each thread moves one and keeps its
             // displacement in LDS, then thread 0 adds the chunk in order
             __shared__ float3 sdl[kWG];
             for (int base = 0; base < nb; base += kWG) {
                 const int t = base + threadIdx.x;
                 if (t < nb) {
-                    const int li = p.br[b0 + t] - i0;
+                    const int li = (base == 0 && plan_br ? plan_br[threadIdx.x] : p.br[b0 + t]) - i0;
                     const float4 pi = pos[li];
                     const float3 rr = qrot(q, pi.x - c0.x, pi.y - c0.y, pi.z - c0.z);
                     const float dx = rr.x + c0.x - pi.x, dy = rr.y + c0.y - pi.y, dz = rr.z + c0.z - pi.z;
@@ -1482,9 +1672,8 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
         float v[3];
         if (x[2] < 0x80000000u) {
             // crankshaft(rotate step, flip, ri): src/Molecule.cpp:303-329
-            const int nc = p.crank_off[m + 1] - p.crank_off[m];
-            if (nc > 0) {
-                const int4 cr = p.crank[p.crank_off[m] + mulhi_n(x[0], nc)];
+            if (fp.cr.x >= 0) {
+                const int4 cr = fp.cr;
                 const float4 a = pos[cr.x - i0], b = pos[cr.y - i0], c = pos[cr.z - i0];
                 double ax = (double)(c.x - a.x), ay = (double)(c.y - a.y), az = (double)(c.z - a.z);
                 const double mag = sqrt(ax * ax + ay * ay + az * az);
@@ -1506,9 +1695,8 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
             }
         } else {
             // local translate by 0.1 x the translation step: src/Molecule.cpp:422-430
-            const int nl = p.local_off[m + 1] - p.local_off[m];
-            if (nl > 0) {
-                li = p.local[p.local_off[m] + mulhi_n(x[0], nl)] - i0;
+            if (fp.li >= 0) {
+                li = fp.li - i0;
                 float n0 = (float)pr.axis[0], n1 = (float)pr.axis[1], n2 = (float)pr.axis[2];
                 const float mag = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
                 if (mag != 0.0f) {
@@ -1572,7 +1760,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     if ((threadIdx.x & (kWave - 1)) == 0) sh_r2[threadIdx.x / kWave] = r2max;
     __syncthreads();
     const bool one = sh_moved && sh_mone >= 0 && !sh_wrap;
-    const double bonded = proposal_bonded(p, r, m, one);
+    const double bonded = proposal_bonded(p, r, m, one ? sh_mone : -1);
     if (threadIdx.x == 0) {
         float rad = p.radius[(size_t)r * p.n_mol + m];
         if (sh_moved) {
@@ -1584,7 +1772,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
         p.prop_nib[r] = (n + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n;
         p.prop_center[r] = make_float4(c0, c1, c2, 0.0f);
-        record_proposal(p, r, m, n, one ? kMovedOne : kMovedAll, one ? sh_mone : 0, rad, bonded);
+        record_proposal(p, r, m, i0, n, one ? kMovedOne : kMovedAll, one ? sh_mone : 0, rad, bonded);
     }
 }
 
@@ -1594,13 +1782,17 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
 // Flexible molecules draw translate / rotate / flex / local with equal
 // weights (rotate / flex / local when the molecule is alone,
 // src/Molecule.cpp:451-466, src/Replica.cpp:309-334).
-__device__ void propose(const PairParams &p, int r, int64_t step)
+// plan, plan_br and copied: the caller resolved the flexible move's
+// topology (plan_br: the first kWG branch residues of a flex move) and
+// already copied the current coordinates into the proposal buffers.
+__device__ void propose(const PairParams &p, int r, int64_t step, const FlexPlan *plan = nullptr,
+                        const int *plan_br = nullptr, bool copied = false)
 {
     const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
     if (p.n_flex && p.mol_flex[pr.mol]) {
         const int kind = p.n_mol > 1 ? mulhi_n(pr.kind_word, 4) : 1 + mulhi_n(pr.kind_word, 3);
         if (kind >= 2) {
-            propose_flexible(p, r, step, pr, kind);
+            propose_flexible(p, r, step, pr, kind, plan, plan_br, copied);
             return;
         }
         Proposal rigid = pr;
@@ -1616,7 +1808,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
             p.prop_rel[(size_t)r * p.max_n + (i - i_beg)] = rn;
         }
         __syncthreads();  // the proposal is in place
-        const double bonded = proposal_bonded(p, r, pr.mol, false);
+        const double bonded = proposal_bonded(p, r, pr.mol, -1);
         if (threadIdx.x == 0) {
             p.prop_mol[r] = pr.mol;
             p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
@@ -1624,7 +1816,7 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
             p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
             // a rigid-body move keeps the radius; its rounded coordinates are
             // evaluated in full
-            record_proposal(p, r, pr.mol, i_end - i_beg, kMovedAll, 0, p.radius[(size_t)r * p.n_mol + pr.mol],
+            record_proposal(p, r, pr.mol, i_beg, i_end - i_beg, kMovedAll, 0, p.radius[(size_t)r * p.n_mol + pr.mol],
                             bonded);
         }
         return;
@@ -1643,7 +1835,8 @@ __device__ void propose(const PairParams &p, int r, int64_t step)
         p.prop_nib[r] = (i_end - i_beg + kIBlock - 1) / kIBlock;
         p.prop_n[r] = i_end - i_beg;
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-        record_proposal(p, r, pr.mol, i_end - i_beg, kMovedNone, 0, p.radius[(size_t)r * p.n_mol + pr.mol], 0.0);
+        record_proposal(p, r, pr.mol, i_beg, i_end - i_beg, kMovedNone, 0, p.radius[(size_t)r * p.n_mol + pr.mol],
+                        0.0);
     }
 }
 
@@ -1658,9 +1851,18 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
 {
     __shared__ double red[kWavesPerWG];
     __shared__ int accept_flag;
+    __shared__ FlexPlan sh_plan;
+    __shared__ int sh_br[kWG];
     const int r = blockIdx.x;
     const int M = p.n_mol;
     const int64_t step = p.steps[r];
+    // the next proposal's molecule and move kind follow from its random
+    // words alone (src/Molecule.cpp:451-466, src/Replica.cpp:309-334)
+    const int64_t next = propose_only ? step : step + 1;
+    const Proposal nx = draw_proposal(p.seed, r, next, M);
+    const int kind2 = p.n_flex && p.mol_flex[nx.mol] ? (M > 1 ? mulhi_n(nx.kind_word, 4) : 1 + mulhi_n(nx.kind_word, 3))
+                                                     : 0;
+    bool planned = false, copied = false;
     if (!propose_only) {
         const int m = p.prop_mol[r];
         const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -1682,9 +1884,19 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
             }
             d += v - mat[(size_t)m * M + J];
         }
-        d = wave_sum(d);
+        d = wave_total(d);
         if (lane == 0) red[wave] = d;
         __syncthreads();
+        // while thread 0 decides, waves 1-3 resolve the topology of the next
+        // step's flexible move (fixed whatever the decision)
+        planned = kind2 >= 2;
+        if (planned && threadIdx.x >= kWave) {
+            FlexPlan fp;
+            plan_flexible(p, r, next, nx.mol, kind2, fp);
+            if (threadIdx.x == kWave) sh_plan = fp;
+            if (kind2 == 2)
+                for (int k = threadIdx.x - kWave; k < min(fp.nb, kWG); k += kWG - kWave) sh_br[k] = p.br[fp.b0 + k];
+        }
         if (threadIdx.x == 0) {
             double delta = 0.0;
             for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
@@ -1743,14 +1955,25 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
                 }
                 if (threadIdx.x < 9) p.frames[((size_t)r * M + m) * 9 + threadIdx.x] = p.prop_frame[(size_t)r * 9 + threadIdx.x];
             }
+        } else if (planned && nx.mol == m) {
+            // rejected, and the next flexible move is of the same molecule:
+            // its proposal starts from the current coordinates
+            const int i_beg = p.mol_off[m], n_m = p.mol_off[m + 1] - i_beg;
+            const float4 *pos = p.pos + (size_t)r * p.stride + i_beg;
+            const float4 *rel = p.rel + (size_t)r * p.stride + i_beg;
+            for (int k = threadIdx.x; k < n_m; k += kWG) {
+                p.prop_pos[(size_t)r * p.max_n + k] = pos[k];
+                p.prop_rel[(size_t)r * p.max_n + k] = rel[k];
+            }
         }
+        // (accepted, the proposal buffers already hold the new coordinates)
+        copied = planned && nx.mol == m;
         __syncthreads();  // every thread read the row before it is cleared
         for (int J = threadIdx.x; J < M; J += kWG) row[J] = 0.0;
         __syncthreads();  // the new centre and relative positions are in place
     }
-    const int64_t next = propose_only ? step : step + 1;
     if (!propose_only && threadIdx.x == 0) p.steps[r] = next;
-    propose(p, r, next);
+    propose(p, r, next, planned ? &sh_plan : nullptr, planned ? sh_br : nullptr, copied);
     if (p.da && !propose_only) {
         __syncthreads();  // the proposal is laid out
         stage_one(p, r);
@@ -1814,7 +2037,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         if (f) v += (double)(long long)f * kFixInv;
         d += v - mat[(size_t)m * M + J];
     }
-    d = wave_sum(d);
+    d = wave_total(d);
     if ((t & (kWave - 1)) == 0) red[t / kWave] = d;
     __syncthreads();
     if (t == 0) {
@@ -1894,7 +2117,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
         p.prop_n[r] = n_2;
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-        record_proposal(p, r, m2, n_2, kMovedNone, 0, p.radius[(size_t)r * M + m2], 0.0);
+        record_proposal(p, r, m2, j_beg, n_2, kMovedNone, 0, p.radius[(size_t)r * M + m2], 0.0);
     }
 }
 
@@ -2385,6 +2608,7 @@ struct cgppd_ctx {
     // device residue order: within each molecule charged residues first, then
     // by type (perm[device index] = caller's index, inv the reverse)
     std::vector<int> perm, inv, type_dev;
+    std::vector<int> run_left;  // device order: residues left in the residue's type run, itself included
     int *d_run_of = nullptr, *d_run_start = nullptr;
     unsigned *d_run_code = nullptr;
     int R = 0;
@@ -2405,6 +2629,7 @@ struct cgppd_ctx {
     int4 *d_prop_moved = nullptr;
     int *d_prop_mol = nullptr, *d_prop_nib = nullptr;
     int *d_prop_n = nullptr, *d_prop_items = nullptr;
+    int4 *d_prop_hdr = nullptr;
     long long *d_counters = nullptr, *d_steps = nullptr, *d_ex_round = nullptr;
     unsigned long long *d_ex_stats = nullptr;
     unsigned long long *d_pair_evals = nullptr;
@@ -2555,6 +2780,7 @@ void free_replicas(cgppd_ctx *ctx)
     dev_free(ctx->d_prop_nib);
     dev_free(ctx->d_prop_n);
     dev_free(ctx->d_prop_items);
+    dev_free(ctx->d_prop_hdr);
     dev_free(ctx->d_counters);
     dev_free(ctx->d_steps);
     dev_free(ctx->d_ex_round);
@@ -2628,6 +2854,14 @@ uint32_t residue_code(const cgppd_ctx *ctx, int t)
     return ((uint32_t)t << kRowShift) | (ctx->charge[t] != 0.0f ? kChargedBit : 0u);
 }
 
+// The word of device residue i: its code and the length of the rest of its
+// type run (the row kernel reads run boundaries from the records).
+uint32_t residue_word(const cgppd_ctx *ctx, int i)
+{
+    const uint32_t left = (uint32_t)std::min<long>(ctx->run_left[i], (long)kRunMax);
+    return residue_code(ctx, ctx->type_dev[i]) | left << kRunShift;
+}
+
 // The residue code travels in the w lane of the float4 coordinates.
 float code_bits(uint32_t c)
 {
@@ -2665,6 +2899,9 @@ int build_order(cgppd_ctx *ctx)
         }
     }
     run_start.push_back(N);
+    ctx->run_left.resize(N);
+    for (size_t k = 0; k + 1 < run_start.size(); ++k)
+        for (int i = run_start[k]; i < run_start[k + 1]; ++i) ctx->run_left[i] = run_start[k + 1] - i;
     ctx->topo_dirty = true;
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_run_of, N))) return rc;
@@ -2705,7 +2942,7 @@ void split_molecule(const cgppd_ctx *ctx, int i0, int n, const float *xyz, float
     cz /= fn;
     for (int k = 0; k < n; ++k) {
         const int d = ctx->inv[i0 + k] - i0;
-        const float w = code_bits(residue_code(ctx, ctx->type[i0 + k]));
+        const float w = code_bits(residue_word(ctx, i0 + d));
         pos[d] = make_float4(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], w);
         rel[d] = make_float4(xyz[3 * k] - cx, xyz[3 * k + 1] - cy, xyz[3 * k + 2] - cz, w);
     }
@@ -2784,9 +3021,13 @@ int sync_topology(cgppd_ctx *ctx)
         crank_off.push_back((int)crank.size());
         flexr_off.push_back((int)flexr.size());
     }
-    std::vector<double> tor(3200);
+    std::vector<double> tor(6400);
     std::copy(ctx->tor_v, ctx->tor_v + 1600, tor.begin());
     std::copy(ctx->tor_d, ctx->tor_d + 1600, tor.begin() + 1600);
+    for (int k = 0; k < 1600; ++k) {
+        tor[3200 + k] = std::cos(ctx->tor_d[k]);
+        tor[4800 + k] = std::sin(ctx->tor_d[k]);
+    }
     int rc;
     if ((rc = upload_vec(ctx, &ctx->d_rtag, rtag)) || (rc = upload_vec(ctx, &ctx->d_mol_flex, mol_flex)) ||
         (rc = upload_vec(ctx, &ctx->d_flex_mols, flex_mols)) ||
@@ -2858,6 +3099,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.prop_nib = ctx->d_prop_nib;
     p.prop_n = ctx->d_prop_n;
     p.prop_items = ctx->d_prop_items;
+    p.prop_hdr = ctx->d_prop_hdr;
     p.row_j = kWave * row_lanes_of(pair_layout_of(ctx));
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
@@ -2892,6 +3134,8 @@ PairParams base_params(cgppd_ctx *ctx)
     p.br = ctx->d_br;
     p.tor_v = ctx->d_tor;
     p.tor_d = ctx->d_tor ? ctx->d_tor + 1600 : nullptr;
+    p.tor_cd = ctx->d_tor ? ctx->d_tor + 3200 : nullptr;
+    p.tor_sd = ctx->d_tor ? ctx->d_tor + 4800 : nullptr;
     p.out_int = ctx->d_out_int;
     p.da = ctx->da ? 1 : 0;
     p.s1 = ctx->d_s1;
@@ -3099,6 +3343,13 @@ int pair_wg_per_cu(const cgppd_ctx *ctx, int layout)
 
 // ev_start/ev_stop (optional) are recorded by the dispatch itself, so their
 // elapsed time is the kernel's execution time without separate event packets.
+// Dynamic LDS of the row kernel for R replicas: R + 1 unit offsets (padded
+// to 16 bytes), then, for R <= 64, R proposal headers and R centres.
+size_t row_lds_bytes(int R)
+{
+    return sizeof(long long) * (size_t)((R + 2) & ~1) + (R <= kWave ? (size_t)R * (sizeof(int4) + sizeof(float4)) : 0);
+}
+
 int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr)
 {
     PairParams p = base_params(ctx);
@@ -3118,10 +3369,11 @@ int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_sto
             hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
     } else {
         auto kern = layout == CGPPD_LAYOUT_JLANES4 ? mc_row_kernel<4> : mc_row_kernel<2>;
+        const size_t lds = row_lds_bytes(ctx->R);
         if (ev_start)
-            hipExtLaunchKernelGGL(kern, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+            hipExtLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, ctx->stream, ev_start, ev_stop, 0, p);
         else
-            hipLaunchKernelGGL(kern, dim3(g), dim3(kWG), 0, ctx->stream, p);
+            hipLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, ctx->stream, p);
     }
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
@@ -3492,7 +3744,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
             hcen[(size_t)r * M + m] = make_float4(c[0], c[1], c[2], 0.0f);
             for (int i = ctx->mol_off[m]; i < ctx->mol_off[m + 1]; ++i) {
                 const float *v = rel + ((size_t)r * N + ctx->perm[i]) * 3;
-                const float w = code_bits(residue_code(ctx, ctx->type_dev[i]));
+                const float w = code_bits(residue_word(ctx, i));
                 hrel[(size_t)r * S + i] = make_float4(v[0], v[1], v[2], w);
                 // src/Molecule.cpp:183-191: position = relative + centre
                 hpos[(size_t)r * S + i] = make_float4(v[0] + c[0], v[1] + c[1], v[2] + c[2], w);
@@ -3547,6 +3799,7 @@ int cgppd_set_replicas(cgppd_ctx *ctx, int n_rep, const float *center, const flo
     if ((rc = dev_alloc(ctx, &ctx->d_prop_nib, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_n, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_prop_items, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_prop_hdr, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_counters, (size_t)3 * R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_steps, R))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_ex_round, 1))) return rc;

@@ -159,6 +159,26 @@ def test_pair_layouts_match_oracle(gpu_engine_factory, layout):
             e.close()
 
 
+@pytest.mark.parametrize("n_rep", [64, 65, 130])
+def test_row_kernel_replica_counts(gpu_engine_factory, n_rep):
+    """The row kernel keeps the proposal headers of up to 64 replicas in LDS
+    and reads them from memory beyond; its unit-offset scan takes 64
+    replicas per pass.  Both sides of each boundary follow the oracle."""
+    spec = small_spec(n_rep)
+    e, st = gpu_and_oracle(gpu_engine_factory, spec, 23)
+    for layout in ("jlanes", "jlanes4"):
+        print(f"n_rep {n_rep} layout {layout}", flush=True)
+        e.set_pair_layout(layout)
+        e.mc_steps(20)
+        st.mc_steps(20)
+        s = e.stats()
+        np.testing.assert_array_equal(s["counters"], st.counters)
+        np.testing.assert_allclose(s["energy"], st.energy, rtol=1e-5, atol=1e-8)
+        for r in (0, 63, n_rep - 1):
+            np.testing.assert_array_equal(e.coordinates(r)[0], st.pos[r])
+    e.close()
+
+
 def test_pair_layouts_agree_on_config1(gpu_engine_factory):
     """40-chain system: the two layouts make the same decisions and their
     running totals agree far below the parity bar."""
