@@ -1864,17 +1864,74 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
                                                      : 0;
     bool planned = false, copied = false;
     if (!propose_only) {
-        const int m = p.prop_mol[r];
-        const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+        // every load of the decision and of applying it is issued first
+        // (the proposal header names the molecule and its residues); they
+        // complete while waves 1-3 resolve the next move's topology
+        const int t = threadIdx.x;
+        const int4 hd = p.prop_hdr[r];
+        const int m = hd.x, i_beg = hd.y, n_m = hd.z;
+        const int lane = t & (kWave - 1), wave = t / kWave;
         double *row = p.rowbuf + (size_t)r * M;
         unsigned long long *fix = p.rowfix + (size_t)r * M;
         double *mat = p.emat + (size_t)r * M * M;
-        // the diagonal entry is the moved molecule's internal energy when it
-        // is flexible (0 in both the row and the matrix otherwise)
-        // (row entries are completed with their fixed-point part here; the
-        // same thread reads entry J again below)
+        float4 *pos = p.pos + (size_t)r * p.stride + i_beg;
+        float4 *rel = p.rel + (size_t)r * p.stride + i_beg;
+        float4 *pp = p.prop_pos + (size_t)r * p.max_n;
+        float4 *pq = p.prop_rel + (size_t)r * p.max_n;
+        double rv = 0.0, mv = 0.0;
+        unsigned long long fv = 0ull;
+        if (t < M) {
+            rv = row[t];
+            fv = fix[t];
+            mv = mat[(size_t)m * M + t];
+        }
+        float4 a_pp = make_float4(0.f, 0.f, 0.f, 0.f), a_pq = a_pp, a_pos = a_pp, a_rel = a_pp;
+        if (t < n_m) {
+            a_pp = pp[t];
+            a_pq = pq[t];
+            a_pos = pos[t];
+            a_rel = rel[t];
+        }
+        long long *cnt = p.counters + 3 * (size_t)r;
+        double temp = 0.0, energy = 0.0, dS = 0.0;
+        long long c0 = 0, c1 = 0, c2 = 0;
+        int s1 = 1, flex_m = 0;
+        if (t == 0) {
+            temp = p.temperature[r];
+            energy = p.energy[r];
+            c0 = cnt[0];
+            c1 = cnt[1];
+            c2 = cnt[2];
+            flex_m = p.n_flex && p.mol_flex[m];
+            if (p.da) {
+                s1 = p.s1[r];
+                dS = p.s_dS[r];
+            }
+        }
+        // waves 1-3 resolve the topology of the next step's flexible move
+        // (fixed whatever the decision)
+        planned = kind2 >= 2;
+        if (planned && t >= kWave) {
+            FlexPlan fp;
+            plan_flexible(p, r, next, nx.mol, kind2, fp);
+            if (t == kWave) sh_plan = fp;
+            if (kind2 == 2)
+                for (int k = t - kWave; k < min(fp.nb, kWG); k += kWG - kWave) sh_br[k] = p.br[fp.b0 + k];
+        }
+        // dE = sum over J of row_J - matrix_mJ; the diagonal entry is the
+        // moved molecule's internal energy when it is flexible (0 in both
+        // the row and the matrix otherwise).  Row entries are completed with
+        // their fixed-point part (the same thread reads entry J again below).
         double d = 0.0;
-        for (int J = threadIdx.x; J < M; J += kWG) {
+        if (t < M) {
+            if (fv) {
+                rv += (double)(long long)fv * kFixInv;
+                row[t] = rv;
+                fix[t] = 0ull;
+            }
+            d = rv - mv;
+        }
+        for (int J = t + kWG; J < M; J += kWG) {
             double v = row[J];
             const unsigned long long f = fix[J];
             if (f) {
@@ -1887,89 +1944,87 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
         d = wave_total(d);
         if (lane == 0) red[wave] = d;
         __syncthreads();
-        // while thread 0 decides, waves 1-3 resolve the topology of the next
-        // step's flexible move (fixed whatever the decision)
-        planned = kind2 >= 2;
-        if (planned && threadIdx.x >= kWave) {
-            FlexPlan fp;
-            plan_flexible(p, r, next, nx.mol, kind2, fp);
-            if (threadIdx.x == kWave) sh_plan = fp;
-            if (kind2 == 2)
-                for (int k = threadIdx.x - kWave; k < min(fp.nb, kWG); k += kWG - kWave) sh_br[k] = p.br[fp.b0 + k];
-        }
-        if (threadIdx.x == 0) {
+        if (t == 0) {
             double delta = 0.0;
             for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
-            const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
-            long long *cnt = p.counters + 3 * (size_t)r;
+            const Proposal pr = draw_proposal(p.seed, r, step, M);
             int acc = 0;
-            if (p.da && !p.s1[r]) {
+            if (p.da && !s1) {
                 // rejected at stage one: no exact row was evaluated
                 p.s1_rej[r] += 1;
             } else {
-                // plain Metropolis, or delayed acceptance stage two with
-                // min(1, exp(-(dE - dS) / kT)) (Christen & Fox 2005)
-                const double d2 = p.da ? delta - p.s_dS[r] : delta;
+                // plain Metropolis (src/Replica.cpp:385-410), or delayed
+                // acceptance stage two with min(1, exp(-(dE - dS) / kT))
+                // (Christen & Fox 2005)
+                const double d2 = p.da ? delta - dS : delta;
                 if (d2 < 0.0) {
                     acc = 1;
-                    cnt[0] += 1;
-                } else if (pr.u < exp(-d2 * 4184.0 / ((double)kRgas * p.temperature[r]))) {
+                    cnt[0] = c0 + 1;
+                } else if (pr.u < exp(-d2 * 4184.0 / ((double)kRgas * temp))) {
                     acc = 1;
-                    cnt[1] += 1;
+                    cnt[1] = c1 + 1;
                 } else {
-                    cnt[2] += 1;
+                    cnt[2] = c2 + 1;
                 }
-                const unsigned long long n_m = (unsigned long long)(p.mol_off[m + 1] - p.mol_off[m]);
-                unsigned long long pairs = n_m * ((unsigned long long)p.n_res - n_m);
-                if (p.n_flex && p.mol_flex[m]) pairs += n_m * (n_m - 1) / 2;
+                const unsigned long long nn = (unsigned long long)n_m;
+                unsigned long long pairs = nn * ((unsigned long long)p.n_res - nn);
+                if (flex_m) pairs += nn * (nn - 1) / 2;
                 atomicAdd(p.pair_evals, pairs);
             }
-            if (acc) p.energy[r] += delta;
+            if (acc) p.energy[r] = energy + delta;
             accept_flag = acc;
         }
         __syncthreads();
         if (accept_flag) {
-            for (int J = threadIdx.x; J < M; J += kWG) {
+            // apply (src/Molecule.cpp:468-504): matrix row and column, the
+            // molecule's coordinates, centre and radius
+            if (t < M) {
+                mat[(size_t)m * M + t] = rv;
+                mat[(size_t)t * M + m] = rv;
+            }
+            for (int J = t + kWG; J < M; J += kWG) {
                 const double v = row[J];
                 mat[(size_t)m * M + J] = v;
                 mat[(size_t)J * M + m] = v;
             }
-            const int i_beg = p.mol_off[m], n_m = p.mol_off[m + 1] - i_beg;
-            float4 *pos = p.pos + (size_t)r * p.stride + i_beg;
-            float4 *rel = p.rel + (size_t)r * p.stride + i_beg;
-            for (int k = threadIdx.x; k < n_m; k += kWG) {
-                pos[k] = p.prop_pos[(size_t)r * p.max_n + k];
-                rel[k] = p.prop_rel[(size_t)r * p.max_n + k];
+            if (t < n_m) {
+                pos[t] = a_pp;
+                rel[t] = a_pq;
             }
-            if (threadIdx.x == 0) {
+            for (int k = t + kWG; k < n_m; k += kWG) {
+                pos[k] = pp[k];
+                rel[k] = pq[k];
+            }
+            if (t == 0) {
                 p.center[(size_t)r * M + m] = p.prop_center[r];
                 p.radius[(size_t)r * M + m] = p.prop_radius[r];
             }
             if (p.da) {
                 float *sm = p.smat + (size_t)r * M * M;
-                for (int J = threadIdx.x; J < M; J += kWG) {
+                for (int J = t; J < M; J += kWG) {
                     if (J == m) continue;
                     const float v = p.srow[(size_t)r * M + J];
                     sm[(size_t)m * M + J] = v;
                     sm[(size_t)J * M + m] = v;
                 }
-                if (threadIdx.x < 9) p.frames[((size_t)r * M + m) * 9 + threadIdx.x] = p.prop_frame[(size_t)r * 9 + threadIdx.x];
+                if (t < 9) p.frames[((size_t)r * M + m) * 9 + t] = p.prop_frame[(size_t)r * 9 + t];
             }
         } else if (planned && nx.mol == m) {
             // rejected, and the next flexible move is of the same molecule:
             // its proposal starts from the current coordinates
-            const int i_beg = p.mol_off[m], n_m = p.mol_off[m + 1] - i_beg;
-            const float4 *pos = p.pos + (size_t)r * p.stride + i_beg;
-            const float4 *rel = p.rel + (size_t)r * p.stride + i_beg;
-            for (int k = threadIdx.x; k < n_m; k += kWG) {
-                p.prop_pos[(size_t)r * p.max_n + k] = pos[k];
-                p.prop_rel[(size_t)r * p.max_n + k] = rel[k];
+            if (t < n_m) {
+                pp[t] = a_pos;
+                pq[t] = a_rel;
+            }
+            for (int k = t + kWG; k < n_m; k += kWG) {
+                pp[k] = pos[k];
+                pq[k] = rel[k];
             }
         }
         // (accepted, the proposal buffers already hold the new coordinates)
         copied = planned && nx.mol == m;
         __syncthreads();  // every thread read the row before it is cleared
-        for (int J = threadIdx.x; J < M; J += kWG) row[J] = 0.0;
+        for (int J = t; J < M; J += kWG) row[J] = 0.0;
         __syncthreads();  // the new centre and relative positions are in place
     }
     if (!propose_only && threadIdx.x == 0) p.steps[r] = next;
