@@ -1238,7 +1238,7 @@ __device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, 
 
 template <int IMG, int JPL>
 __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec,
-                                         const float *__restrict__ qtab, int ia, int ib,
+                                         const float *__restrict__ qtab, uint32_t code0, int ia, int ib,
                                          const LaneJ<JPL> &lj, const char *__restrict__ tab, float (&s)[JPL],
                                          float (&c)[JPL])
 {
@@ -1246,7 +1246,7 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
     while (a < ib) {
         // the run [a, b) of one type: the first record's word gives the
         // type, the charge flag and the rest of the run
-        const uint32_t code = __float_as_uint(scalar_rec(irec, a).w);
+        const uint32_t code = a == ia ? code0 : __float_as_uint(scalar_rec(irec, a).w);
         const int b = min(ib, a + max(1, (int)(code >> kRunShift & kRunMax)));
         RowCoef cf[JPL];
         float qq[JPL];
@@ -1279,33 +1279,51 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     long long *unit_off = row_lds;
     int4 *s_hdr = reinterpret_cast<int4 *>(row_lds + ((R + 2) & ~1));
     float4 *s_cen = reinterpret_cast<float4 *>(s_hdr + (hdr_lds ? R : 0));
-    // entry e = tj * 20 + ti (i = the moved residue) goes to row ti, column tj
-    for (int e = threadIdx.x; e < CGPPD_AA_COUNT * CGPPD_AA_COUNT; e += kWG) {
-        const int tj = e / CGPPD_AA_COUNT, ti = e % CGPPD_AA_COUNT;
-        *reinterpret_cast<RowCoef *>(tab + ((ti << kRowShift) | (tj << kColShift))) = p.row_table[e];
-    }
-    if (threadIdx.x < CGPPD_AA_COUNT) qtab[threadIdx.x] = p.charge[threadIdx.x];
-    // unit offsets of the replicas: an inclusive scan by wave 0, 64
-    // replicas at a time, from the proposal headers
-    if (threadIdx.x < kWave) {
-        const int lane0 = threadIdx.x;
-        long long carry = 0;
-        for (int base = 0; base < R; base += kWave) {
-            const int r = base + lane0;
-            long long c = 0;
-            if (r < R) {
-                const int4 hd = p.prop_hdr[r];
-                if (hdr_lds) {
-                    s_hdr[r] = hd;
-                    s_cen[r] = p.prop_center[r];
-                }
-                if (!(p.da && !p.s1[r])) c = (long long)((p.n_res - hd.z + JPL * kWave - 1) / (JPL * kWave)) * hd.z;
+    // every set-up load is issued before the LDS stores: the table (entry
+    // e = tj * 20 + ti, i = the moved residue, goes to row ti, column tj),
+    // the charges, and the first 64 replicas' headers and centres
+    static_assert(CGPPD_AA_COUNT * CGPPD_AA_COUNT <= 2 * kWG, "two table entries per thread");
+    // (unconditional, so that none waits for another; only wave 0 uses
+    // the headers)
+    const int e0 = threadIdx.x, e1 = threadIdx.x + kWG;
+    const int rr = min((int)threadIdx.x & (kWave - 1), R - 1);
+    const RowCoef t0 = p.row_table[e0];
+    const RowCoef t1 = p.row_table[min(e1, CGPPD_AA_COUNT * CGPPD_AA_COUNT - 1)];
+    const float q0 = p.charge[min((int)threadIdx.x, CGPPD_AA_COUNT - 1)];
+    const int4 hd0 = p.prop_hdr[rr];
+    const float4 cen0 = p.prop_center[rr];
+    constexpr int kEntries = CGPPD_AA_COUNT * CGPPD_AA_COUNT;
+    const int e1c = min(e1, kEntries - 1);  // (threads past the table rewrite its last entry)
+    *reinterpret_cast<RowCoef *>(tab + (((e0 % CGPPD_AA_COUNT) << kRowShift) | ((e0 / CGPPD_AA_COUNT) << kColShift))) = t0;
+    *reinterpret_cast<RowCoef *>(tab + (((e1c % CGPPD_AA_COUNT) << kRowShift) | ((e1c / CGPPD_AA_COUNT) << kColShift))) = t1;
+    qtab[min((int)threadIdx.x, CGPPD_AA_COUNT - 1)] = q0;  // (likewise)
+    // unit offsets of the replicas: an inclusive scan, 64 replicas at a
+    // time, from the proposal headers (the first 64 loaded above).  Every
+    // wave scans (the loads then have an unconditional use and are not
+    // deferred); wave 0 stores.
+    {
+        const int lane0 = threadIdx.x & (kWave - 1);
+        const bool w0 = threadIdx.x < kWave;
+        auto units = [&](int r, int4 hd) -> long long {
+            if (r >= R || (p.da && !p.s1[r])) return 0;
+            return (long long)((p.n_res - hd.z + JPL * kWave - 1) / (JPL * kWave)) * hd.z;
+        };
+        long long c = wave_scan_i64(units(lane0, hd0));
+        if (w0 && lane0 < R) {
+            unit_off[lane0 + 1] = c;
+            if (hdr_lds) {
+                s_hdr[lane0] = hd0;
+                s_cen[lane0] = cen0;
             }
-            c = wave_scan_i64(c);
-            if (r < R) unit_off[r + 1] = carry + c;
+        }
+        long long carry = readlane_b64(c, kWave - 1);
+        for (int base = kWave; base < R; base += kWave) {
+            const int r = base + lane0;
+            c = wave_scan_i64(units(r, p.prop_hdr[min(r, R - 1)]));
+            if (w0 && r < R) unit_off[r + 1] = carry + c;
             carry += readlane_b64(c, kWave - 1);
         }
-        if (lane0 == 0) unit_off[0] = 0;
+        if (threadIdx.x == 0) unit_off[0] = 0;
     }
     __syncthreads();
     const long long total = unit_off[R];
@@ -1363,22 +1381,30 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
             ia = __builtin_amdgcn_readfirstlane((int)(local % (unsigned)n_m));
         }
         const int ib = __builtin_amdgcn_readfirstlane((int)min((long long)n_m, (long long)ia + (P1 - P)));
+        // the first i record's word (its run), requested before the j loads
+        const float4 *irec = p.prop_pos + (size_t)r * p.max_n;
+        const uint32_t code0 = __float_as_uint(scalar_rec(irec, ia).w);
         // this lane's j residues (jb * 64 JPL + 64 h + lane, h < JPL)
         const int nj = p.n_res - n_m;
         LaneJ<JPL> lj;
         bool valid[JPL];
         int J[JPL];
         float4 rj[JPL];
+        // (every load issued before any is used; lanes past the last j
+        // residue read the last residue and are masked)
 #pragma unroll
         for (int h = 0; h < JPL; ++h) {
             const int jp = jb * (JPL * kWave) + h * kWave + lane;
             valid[h] = jp < nj;
-            const int j = jp + (jp >= i0 ? n_m : 0);
-            rj[h] = make_float4(0.f, 0.f, 0.f, 0.f);
-            J[h] = 0x7fffffff;
-            if (valid[h]) {
-                rj[h] = p.pos[(size_t)r * p.stride + j];
-                J[h] = p.mol_of[j];
+            const int j = min(jp + (jp >= i0 ? n_m : 0), p.n_res - 1);
+            rj[h] = p.pos[(size_t)r * p.stride + j];
+            J[h] = p.mol_of[j];
+        }
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) {
+            if (!valid[h]) {
+                rj[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+                J[h] = 0x7fffffff;
             }
             lj.col[h] = lane_col(rj[h]);
             lj.q[h] = qtab[lj.col[h] >> kColShift];
@@ -1416,20 +1442,19 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
             lj.y[h] = shift ? rj[h].y + sh[h][1] : rj[h].y;
             lj.z[h] = shift ? rj[h].z + sh[h][2] : rj[h].z;
         }
-        const float4 *irec = p.prop_pos + (size_t)r * p.max_n;
         float sa[JPL], ca[JPL];
 #pragma unroll
         for (int h = 0; h < JPL; ++h) sa[h] = ca[h] = 0.0f;
         switch (img) {
-        case 0: row_runs<0, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        case 1: row_runs<1, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        case 2: row_runs<2, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        case 3: row_runs<3, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        case 4: row_runs<4, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        case 5: row_runs<5, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        case 6: row_runs<6, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        case 7: row_runs<7, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
-        default: row_runs<kImgFull, JPL>(p, irec, qtab, ia, ib, lj, tab, sa, ca); break;
+        case 0: row_runs<0, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 1: row_runs<1, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 2: row_runs<2, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 3: row_runs<3, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 4: row_runs<4, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 5: row_runs<5, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 6: row_runs<6, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 7: row_runs<7, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        default: row_runs<kImgFull, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         }
         // FP64 reduction per molecule J: across the lanes J[h] is
         // non-decreasing for each h, and every J[h] <= every J[h + 1]
