@@ -2048,7 +2048,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
         }
         // (accepted, the proposal buffers already hold the new coordinates)
         copied = planned && nx.mol == m;
-        __syncthreads();  // every thread read the row before it is cleared
+        // (each row entry is read, applied and cleared by one thread)
         for (int J = t; J < M; J += kWG) row[J] = 0.0;
         __syncthreads();  // the new centre and relative positions are in place
     }
@@ -2166,7 +2166,8 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         }
         if (t == 0) p.center[(size_t)r * M + m] = p.prop_center[r];
     }
-    __syncthreads();  // every thread read the row (and the proposal) before they are replaced
+    // (each row entry, and each proposal record, is read, applied and
+    // replaced by one thread: no barrier before they are cleared)
     for (int J = t; J < M; J += kWG) {
         row[J] = 0.0;
         fix[J] = 0ull;
