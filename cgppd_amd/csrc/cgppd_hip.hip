@@ -2072,7 +2072,6 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
 {
 #pragma clang fp contract(off)
     __shared__ double red[kWavesPerWG];
-    __shared__ int accept_flag;
     const int r = blockIdx.x, t = threadIdx.x;
     const int M = p.n_mol;
     const int64_t step = p.steps[r];
@@ -2103,8 +2102,10 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
     }
     if (t < n_2) n_rel = rel[j_beg + t];
     float4 c2 = p.center[(size_t)r * M + m2];
+    const float4 c_new = p.prop_center[r];
     const float tstep = p.tstep[r];
     const double rcos = p.rcos[r], rsin = p.rsin[r];
+    const double temp = p.temperature[r];
     // Metropolis (src/Replica.cpp:385-410, delta and exponent in FP64)
     double d = 0.0;
     if (t < M) {
@@ -2119,30 +2120,23 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
     }
     d = wave_total(d);
     if ((t & (kWave - 1)) == 0) red[t / kWave] = d;
+    // one barrier: the wave sums are in LDS and every wave has read the
+    // step's inputs (nothing read below is written by another thread)
     __syncthreads();
+    // every thread takes the same decision (same inputs, same operations);
+    // thread 0 records it
+    double delta = 0.0;
+    for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
+    const Proposal pr = draw_proposal(p.seed, r, step, M);
+    const int kind = delta < 0.0 ? 0 : pr.u < exp(-delta * 4184.0 / ((double)kRgas * temp)) ? 1 : 2;
+    const int acc = kind < 2;
     if (t == 0) {
-        double delta = 0.0;
-        for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
-        const Proposal pr = draw_proposal(p.seed, r, step, M);
-        long long *cnt = p.counters + 3 * (size_t)r;
-        int acc = 0;
-        if (delta < 0.0) {
-            acc = 1;
-            cnt[0] += 1;
-        } else if (pr.u < exp(-delta * 4184.0 / ((double)kRgas * p.temperature[r]))) {
-            acc = 1;
-            cnt[1] += 1;
-        } else {
-            cnt[2] += 1;
-        }
+        p.counters[3 * (size_t)r + kind] += 1;
         const unsigned long long nm = (unsigned long long)n_m;
         atomicAdd(p.pair_evals, nm * ((unsigned long long)p.n_res - nm));
         if (acc) p.energy[r] += delta;
-        accept_flag = acc;
         p.steps[r] = step + 1;
     }
-    __syncthreads();
-    const int acc = accept_flag;
     if (acc) {
         // apply (src/Molecule.cpp:468-504) and the matrix row and column
         if (t < M) {
@@ -2164,7 +2158,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
             pos[i_beg + k] = pp[k];
             rel[i_beg + k] = pq[k];
         }
-        if (t == 0) p.center[(size_t)r * M + m] = p.prop_center[r];
+        if (t == 0) p.center[(size_t)r * M + m] = c_new;
     }
     // (each row entry, and each proposal record, is read, applied and
     // replaced by one thread: no barrier before they are cleared)
@@ -2172,11 +2166,13 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         row[J] = 0.0;
         fix[J] = 0ull;
     }
-    // the next proposal (src/Molecule.cpp:146-266)
+    // the next proposal (src/Molecule.cpp:146-266); when it moves the
+    // molecule just accepted, it starts from the applied coordinates, which
+    // this thread holds (residues past the first kWG are re-read below, after
+    // this thread wrote them)
     if (acc && m2 == m) {
-        __syncthreads();  // the applied move is in place
-        if (t < n_2) n_rel = rel[j_beg + t];
-        c2 = p.center[(size_t)r * M + m2];
+        n_rel = a_rel;
+        c2 = c_new;
     }
     const Move mv = make_move(nx, c2, tstep, rcos, rsin, p.box);
     float4 *qp = p.prop_pos + (size_t)r * p.max_n;
