@@ -1269,6 +1269,12 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
 template <int JPL>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4 : 8, 8))) void mc_row_kernel(PairParams p)
 {
+    // Wave priority: the SIMDs issue from the oldest wave first, so a wave
+    // still in its set-up (a chain of dependent loads and a few hundred
+    // instructions) would wait behind the pair loops of older waves and
+    // start its own loops last.  Set-up, visit headers and reductions run
+    // at high priority, the pair loops at normal priority.
+    __builtin_amdgcn_s_setprio(2);
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ float qtab[CGPPD_AA_COUNT];
     // dynamic LDS (row_lds_bytes): the replicas' unit offsets, then for at
@@ -1445,6 +1451,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         float sa[JPL], ca[JPL];
 #pragma unroll
         for (int h = 0; h < JPL; ++h) sa[h] = ca[h] = 0.0f;
+        __builtin_amdgcn_s_setprio(0);
         switch (img) {
         case 0: row_runs<0, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         case 1: row_runs<1, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
@@ -1456,6 +1463,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         case 7: row_runs<7, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         default: row_runs<kImgFull, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         }
+        __builtin_amdgcn_s_setprio(2);
         // FP64 reduction per molecule J: across the lanes J[h] is
         // non-decreasing for each h, and every J[h] <= every J[h + 1]
         double v[JPL];
