@@ -1023,6 +1023,9 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     __shared__ float4 jrec[kJChunk];
     __shared__ int4 jtag[FLEX ? kJChunk : 1];
     __shared__ int item_off[CGPPD_MAX_REPLICAS + 1];
+    // set-up and item staging at high wave priority, the pair loops at
+    // normal priority (see mc_row_kernel)
+    __builtin_amdgcn_s_setprio(2);
     load_table(p.table, tab);
     const int R = p.rep_count, JC = p.jc_count;
     // item offsets of the replicas (counts recorded with each proposal;
@@ -1075,6 +1078,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
             double v;
             // (the bonded terms were added to the diagonal when the proposal
             // was laid out)
+            __builtin_amdgcn_s_setprio(0);
             if (mv.x == kMovedAll) {
                 // the intra-molecular pairs of the proposal
                 float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
@@ -1087,6 +1091,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
                 moved_item(src, p.pos + (size_t)r * p.stride + i0, p.rtag + i0, n, mv.y, l2, p, tab, s0, c0);
                 v = (double)s0 - (double)c0;
             }
+            __builtin_amdgcn_s_setprio(2);
             v = wave_total(v);
             if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + m, v);
             continue;
@@ -1109,6 +1114,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
         __syncthreads();
         int js = jbase + wave * kWave;
         const int je = min(p.n_res, jbase + (wave + 1) * kWave);
+        __builtin_amdgcn_s_setprio(0);
         while (js < je) {
             const int J = p.mol_of[js];
             const int se = min(je, p.mol_off[J + 1]);
@@ -1122,6 +1128,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
             }
             js = se;
         }
+        __builtin_amdgcn_s_setprio(2);
     }
 }
 
