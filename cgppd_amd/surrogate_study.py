@@ -159,6 +159,7 @@ def main(argv=None):
         e.surrogate_collect()
     n_train = e.surrogate_sample_count()
     t_collect = time.perf_counter() - t0
+    print(f"collected {n_train} training pairs in {t_collect:.1f} s", file=sys.stderr, flush=True)
 
     # 2. training and a held-out check
     t0 = time.perf_counter()
@@ -168,6 +169,8 @@ def main(argv=None):
         losses.append(e.surrogate_train(args.train_steps // 10, batch=args.batch, lr=lr, seed=args.seed + k))
     e.synchronize()
     t_train = time.perf_counter() - t0
+    print(f"trained {args.train_steps} steps in {t_train:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g}",
+          file=sys.stderr, flush=True)
     e.run(4, args.collect_every, use_graph=True)
     # held out: the pairs within the surrogate's range (centre distance below
     # r_I + r_J + 15 A, bounding radii of the relative positions)
@@ -214,6 +217,7 @@ def main(argv=None):
             e.set_delayed_acceptance(True)
         runs[name] = record_run(e, spec, args.warmup, args.steps, args.chunk, args.exchange_every, da)
         e.close()
+        print(f"{name}: {runs[name][0]['mc_steps_per_s']:.0f} MC steps/s", file=sys.stderr, flush=True)
     plain, pe, pd = runs["plain"]
     dar, de, dd = runs["delayed_acceptance"]
     result["plain"] = plain
