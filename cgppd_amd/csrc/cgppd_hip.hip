@@ -42,6 +42,11 @@ constexpr int kWave = 64;
 constexpr int kWavesPerWG = 4;
 constexpr int kWG = kWave * kWavesPerWG;  // threads per workgroup
 constexpr int kJChunk = kWG;              // j residues staged per item
+// j residues per intra-molecular item of a flexible molecule (32 per wave):
+// half the i-lane chunk, so that a proposal's intra items spread over twice
+// as many workgroups
+constexpr int kIntraChunk = kWG / 2;
+__host__ __device__ constexpr int intra_chunks(int n) { return (n + kIntraChunk - 1) / kIntraChunk; }
 constexpr int kIBlock = kWave;            // i residues per item (one per lane)
 constexpr int kRowShift = 9;              // LDS table row stride: 32 entries x 16 B
 constexpr int kColShift = 4;              // 16-byte entries
@@ -599,8 +604,8 @@ __device__ __forceinline__ void intra_item(const float4 *__restrict__ src, const
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     __syncthreads();  // previous item finished reading the staging arrays
     {
-        const int j = jc * kJChunk + threadIdx.x;
-        if (j < n) {
+        const int j = jc * kIntraChunk + threadIdx.x;
+        if (threadIdx.x < kIntraChunk && j < n) {
             jrec[threadIdx.x] = src[j];
             jtag[threadIdx.x] = tg[j];
         }
@@ -610,11 +615,13 @@ __device__ __forceinline__ void intra_item(const float4 *__restrict__ src, const
     const float4 pa = src[valid ? a : 0];
     const int4 ta = tg[valid ? a : 0];
     __syncthreads();
-    const int j0 = jc * kJChunk + wave * kWave;
-    const int j1 = min(n, j0 + kWave);
+    constexpr int kPerWave = kIntraChunk / kWavesPerWG;
+    const int j0 = jc * kIntraChunk + wave * kPerWave;
+    // a wave whose j residues all precede its i block has no pair b > a
+    const int j1 = j0 + kPerWave - 1 <= ib * kIBlock ? j0 : min(n, j0 + kPerWave);
     for (int b = j0; b < j1; ++b) {
-        const float4 pb = jrec[b - jc * kJChunk];
-        const int4 tb = jtag[b - jc * kJChunk];
+        const float4 pb = jrec[b - jc * kIntraChunk];
+        const int4 tb = jtag[b - jc * kIntraChunk];
         float lj, dh;
         pair_any(pa, pb, tab, p, lj, dh);
         if (valid && b > a && !intra_excluded(ta, tb)) {
@@ -969,7 +976,7 @@ __global__ __launch_bounds__(kWG) void intra_energy_kernel(PairParams p)
         while (q >= p.flex_item_off[f + 1]) ++f;
         const int m = p.flex_mols[f];
         const int i0 = p.mol_off[m], n = p.mol_off[m + 1] - i0;
-        const int nib = (n + kIBlock - 1) / kIBlock, njc = (n + kJChunk - 1) / kJChunk;
+        const int nib = (n + kIBlock - 1) / kIBlock, njc = intra_chunks(n);
         const float4 *src = p.pos + (size_t)r * p.stride + i0;
         const int local = q - p.flex_item_off[f];
         double *diag = p.emat + r * mm + (size_t)m * p.n_mol + m;
@@ -1071,7 +1078,7 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
         if (FLEX && local >= p.prop_nib[r] * JC) {
             const int m = p.prop_mol[r];
             const int i0 = p.mol_off[m], n = p.mol_off[m + 1] - i0;
-            const int nib = p.prop_nib[r], njc = (n + kJChunk - 1) / kJChunk;
+            const int nib = p.prop_nib[r], njc = intra_chunks(n);
             const int l2 = local - nib * JC;
             const float4 *src = p.prop_pos + (size_t)r * p.max_n;
             const int4 mv = p.prop_moved[r];
@@ -1520,10 +1527,7 @@ __device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, in
 {
     const int nib = (n + kIBlock - 1) / kIBlock;
     int items = nib * p.jc_count;
-    if (p.n_flex && p.mol_flex[m]) {
-        const int njc = (n + kJChunk - 1) / kJChunk;
-        items += kind == kMovedOne ? njc : nib * njc;
-    }
+    if (p.n_flex && p.mol_flex[m]) items += kind == kMovedOne ? (n + kJChunk - 1) / kJChunk : nib * intra_chunks(n);
     return items;
 }
 
@@ -3083,8 +3087,8 @@ int sync_topology(cgppd_ctx *ctx)
         if (flex) {
             mol_flex[m] = 1;
             flex_mols.push_back(m);
-            const int nib = (n + kIBlock - 1) / kIBlock, njc = (n + kJChunk - 1) / kJChunk;
-            flex_item_off.push_back(flex_item_off.back() + nib * njc + 1);
+            const int nib = (n + kIBlock - 1) / kIBlock;
+            flex_item_off.push_back(flex_item_off.back() + nib * intra_chunks(n) + 1);
             auto d = [&](int v) { return inv[i0 + v]; };
             for (const auto &b : t.bonds) bonds.push_back(make_int2(d(b[0]), d(b[1])));
             for (const auto &a : t.angles) angles.push_back(make_int4(d(a[0]), d(a[1]), d(a[2]), 0));
