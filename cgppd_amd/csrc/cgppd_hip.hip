@@ -1033,26 +1033,31 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     // set-up and item staging at high wave priority, the pair loops at
     // normal priority (see mc_row_kernel)
     __builtin_amdgcn_s_setprio(2);
-    load_table(p.table, tab);
     const int R = p.rep_count, JC = p.jc_count;
-    // item offsets of the replicas (counts recorded with each proposal;
-    // zero for a proposal rejected by the surrogate's stage one): an
-    // inclusive scan by wave 0, 64 replicas at a time
-    if (threadIdx.x < kWave) {
-        const int lane0 = threadIdx.x;
-        int carry = 0;
-        for (int base = 0; base < R; base += kWave) {
+    // the set-up's loads together: the table (two entries per thread) and
+    // the first 64 replicas' item counts (recorded with each proposal; zero
+    // for a proposal rejected by the surrogate's stage one)
+    const int lane0 = threadIdx.x & (kWave - 1);
+    const int e0 = threadIdx.x, e1 = min((int)threadIdx.x + kWG, CGPPD_AA_COUNT * CGPPD_AA_COUNT - 1);
+    const Coef g0 = p.table[e0], g1 = p.table[e1];
+    const int it0 = p.prop_items[min(lane0, R - 1)];
+    *reinterpret_cast<Coef *>(tab + (((e0 / CGPPD_AA_COUNT) << kRowShift) | ((e0 % CGPPD_AA_COUNT) << kColShift))) = g0;
+    *reinterpret_cast<Coef *>(tab + (((e1 / CGPPD_AA_COUNT) << kRowShift) | ((e1 % CGPPD_AA_COUNT) << kColShift))) = g1;
+    // item offsets of the replicas: an inclusive DPP scan, 64 replicas at a
+    // time; every wave scans (an unconditional use of the loads), wave 0
+    // stores
+    {
+        auto items = [&](int r, int c) -> long long { return (r < R && !(p.da && !p.s1[r])) ? c : 0; };
+        long long c = wave_scan_i64(items(lane0, it0));
+        if (threadIdx.x < kWave && lane0 < R) item_off[lane0 + 1] = (int)c;
+        long long carry = readlane_b64(c, kWave - 1);
+        for (int base = kWave; base < R; base += kWave) {
             const int r = base + lane0;
-            int c = (r < R && !(p.da && !p.s1[r])) ? p.prop_items[r] : 0;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int t = __shfl_up(c, o, kWave);
-                if (lane0 >= o) c += t;
-            }
-            if (r < R) item_off[r + 1] = carry + c;
-            carry += __shfl(c, kWave - 1, kWave);
+            c = wave_scan_i64(items(r, p.prop_items[min(r, R - 1)]));
+            if (threadIdx.x < kWave && r < R) item_off[r + 1] = (int)(carry + c);
+            carry += readlane_b64(c, kWave - 1);
         }
-        if (lane0 == 0) item_off[0] = 0;
+        if (threadIdx.x == 0) item_off[0] = 0;
     }
     __syncthreads();
     const int total = item_off[R];
@@ -1062,15 +1067,16 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    // replica of the first item (binary search), then advance
+    // replica of the first item: the number of replicas that end at or
+    // before it (item_off is non-decreasing), 64 at a time; then advance
     int r = 0;
     if (k0 < k1) {
-        int lo = 0, hi = R;  // item_off[lo] <= k0 < item_off[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (item_off[mid] <= k0) lo = mid; else hi = mid;
+        for (int base = 0; base < R; base += kWave) {
+            const int k = base + lane;
+            const unsigned long long before = __ballot(k < R && item_off[k + 1] <= k0);
+            r = base + __popcll(before);
+            if (before != ~0ull) break;
         }
-        r = lo;
     }
     for (int k = k0; k < k1; ++k) {
         while (k >= item_off[r + 1]) ++r;
