@@ -387,6 +387,21 @@ __device__ __forceinline__ double dpp_f64(double v)
     return __longlong_as_double(dpp_b64<CTRL, ROW_MASK>(__double_as_longlong(v)));
 }
 
+// Maximum over the 64 lanes of non-negative values (every lane active),
+// returned in every lane: the same DPP steps with max.
+__device__ __forceinline__ float wave_max_nonneg(float v)
+{
+#define CG_DPP_MAX(CTRL, RM) v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xf, false)))
+    CG_DPP_MAX(0x111, 0xf);
+    CG_DPP_MAX(0x112, 0xf);
+    CG_DPP_MAX(0x114, 0xf);
+    CG_DPP_MAX(0x118, 0xf);
+    CG_DPP_MAX(0x142, 0xa);
+    CG_DPP_MAX(0x143, 0xc);
+#undef CG_DPP_MAX
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), kWave - 1));
+}
+
 // Sum over the 64 lanes (every lane active), returned in every lane: the
 // DPP scan's last lane, read back.  A fixed summation order, so the result
 // is reproducible.
@@ -1646,6 +1661,13 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
     float4 *pq = p.prop_rel + (size_t)r * p.max_n;
     __shared__ int sh_mone, sh_wrap;
     __shared__ float sh_r2[kWavesPerWG];
+    // the molecule's current centre and radius, for thread 0 below
+    float4 c_cur = make_float4(0.f, 0.f, 0.f, 0.f);
+    float rad_cur = 0.0f;
+    if (threadIdx.x == 0) {
+        c_cur = p.center[(size_t)r * p.n_mol + m];
+        rad_cur = p.radius[(size_t)r * p.n_mol + m];
+    }
     // the proposal starts as a copy of the current coordinates (already in
     // place when copied)
     if (!copied)
@@ -1773,7 +1795,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
         }
     }
     if (threadIdx.x == 0) {
-        const float4 c = p.center[(size_t)r * p.n_mol + m];
+        const float4 c = c_cur;
         const float fn = (float)n;
         for (int k = 0; k < 3; ++k) sh_vf[k] = moved ? acc[k] / fn : 0.0f;
         sh_nc[0] = c.x + sh_vf[0];
@@ -1805,14 +1827,13 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
         if (threadIdx.x == 0) sh_wrap = wrap;
     }
     // bounding radius of the proposed molecule (block maximum)
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) r2max = fmaxf(r2max, __shfl_xor(r2max, o, kWave));
+    r2max = wave_max_nonneg(r2max);
     if ((threadIdx.x & (kWave - 1)) == 0) sh_r2[threadIdx.x / kWave] = r2max;
     __syncthreads();
     const bool one = sh_moved && sh_mone >= 0 && !sh_wrap;
     const double bonded = proposal_bonded(p, r, m, one ? sh_mone : -1);
     if (threadIdx.x == 0) {
-        float rad = p.radius[(size_t)r * p.n_mol + m];
+        float rad = rad_cur;
         if (sh_moved) {
             float q2 = 0.0f;
             for (int w = 0; w < kWavesPerWG; ++w) q2 = fmaxf(q2, sh_r2[w]);
