@@ -159,6 +159,30 @@ def test_pair_layouts_match_oracle(gpu_engine_factory, layout):
             e.close()
 
 
+@pytest.mark.parametrize("layout", ["ilanes", "jlanes", "jlanes4"])
+def test_tight_box_image_paths(gpu_engine_factory, layout):
+    """A 46 A box holding a 120-residue molecule of bounding radius 21 A
+    (above 0.45 L, so its rows take the per-pair rint image) and three small
+    ones, whose rows need the per-axis minimum image on most visits: the
+    oracle's trajectory."""
+    from cgppd_amd import synth
+    spec = synth.make_system("tightmc", [120, 8, 5, 3], 46.0, 6, 288.0, 450.0, 11, 2.0, 0.2)
+    e, st = gpu_and_oracle(gpu_engine_factory, spec, 5)
+    e.set_pair_layout(layout)
+    for chunk in range(2):
+        e.mc_steps(40)
+        st.mc_steps(40)
+        s = e.stats()
+        c, o = s["counters"], st.counters
+        print(f"{layout} after {40 * (chunk + 1)} steps: gpu {c.sum(0)} oracle {o.sum(0)}", flush=True)
+        np.testing.assert_array_equal(c[:, 0] + c[:, 1], o[:, 0] + o[:, 1])
+        np.testing.assert_array_equal(c[:, 2], o[:, 2])
+        np.testing.assert_allclose(s["energy"], st.energy, rtol=1e-5, atol=1e-6)
+        for r in range(spec.n_rep):
+            np.testing.assert_array_equal(e.coordinates(r)[0], st.pos[r])
+    e.close()
+
+
 @pytest.mark.parametrize("n_rep", [64, 65, 130])
 def test_row_kernel_replica_counts(gpu_engine_factory, n_rep):
     """The row kernel keeps the proposal headers of up to 64 replicas in LDS
