@@ -253,7 +253,8 @@ struct PairParams {
     // topology
     int n_mol, n_res, stride;
     int n_active;       // molecules taking part (n_mol, or n_noncrowder)
-    int ib_max;         // i-blocks of the largest molecule
+    int crowd_mol;      // repulsive crowding: first crowder molecule (n_mol when off)
+    int ib_max;        // i-blocks of the largest molecule
     int jc_count;       // j-chunks over n_res
     const int *mol_off;  // n_mol + 1
     const int *mol_of;   // n_res
@@ -560,14 +561,54 @@ __device__ __forceinline__ bool segment_shift(float4 ci, float4 cj, float rsum, 
     return fabsf(dx) < lim && fabsf(dy) < lim && fabsf(dz) < lim;
 }
 
+// Repulsive crowding (the reference's compile-time REPULSIVE_CROWDING,
+// src/definitions.h:18; src/Replica.cpp:416-483, src/CudaFunctions.cu:335-338,
+// 492-500): a residue pair with a crowder molecule contributes (6/r)^12 to LJ
+// for r < 15 A (src/definitions.h:38) and no Debye-Hueckel term.  With the
+// minimum-image differences (dx, dy, dz); raw LJ units.
+constexpr float kCrowdCut2 = 225.0f;
+constexpr float kSix12 = 2176782336.0f;  // 6^12, exact in float
+__device__ __forceinline__ float crowd_term(float dx, float dy, float dz)
+{
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float i2 = __builtin_amdgcn_rcpf(r2);
+    const float u = i2 * i2 * i2;
+    return r2 < kCrowdCut2 ? kSix12 * (u * u) : 0.0f;
+}
+
+__device__ __forceinline__ float min_image(float d, float box, float inv_box)
+{
+    return fmaf(-box, rintf(d * inv_box), d);
+}
+
+// A segment whose pairs all involve a crowder: this lane's residue pi
+// against the j residues [js, je), LJ only (kcal/mol unless SEPARATE).
+template <bool SEPARATE>
+__device__ __forceinline__ void crowd_segment(const float4 *__restrict__ jrec, int js, int je, float4 pi,
+                                              const PairParams &p, float &s0, float &c0)
+{
+    const float scale = SEPARATE ? 1.0f : p.rt_to_kcal;
+    for (int j = js; j < je; ++j) {
+        const float4 rj = jrec[j];
+        const float e = crowd_term(min_image(pi.x - rj.x, p.box, p.inv_box), min_image(pi.y - rj.y, p.box, p.inv_box),
+                                   min_image(pi.z - rj.z, p.box, p.inv_box));
+        kahan_add(s0, c0, e * scale);
+    }
+}
+
 // A segment (i-block of one molecule against the j residues [js, je) of
 // molecule J): the shifted fast path when the bounding spheres allow it.
+// crowd: either molecule is a crowder under repulsive crowding.
 template <bool SEPARATE>
 __device__ __forceinline__ void segment(const float4 *__restrict__ jrec, int js, int je, int jbase, float4 pi,
                                         uint32_t lane_off, const char *__restrict__ tab, const PairParams &p,
-                                        float4 ci, float ri, float4 cj, float rj, float &s0, float &c0, float &s1,
-                                        float &c1)
+                                        float4 ci, float ri, float4 cj, float rj, bool crowd, float &s0, float &c0,
+                                        float &s1, float &c1)
 {
+    if (crowd) {
+        crowd_segment<SEPARATE>(jrec, js, je, pi, p, s0, c0);
+        return;
+    }
     float sx, sy, sz;
     if (p.fast_image && segment_shift(ci, cj, ri + rj, p.box, p.inv_box, sx, sy, sz))
         pair_segment<SEPARATE, false>(jrec, js, je, jbase, p, pi.x - sx, pi.y - sy, pi.z - sz, lane_off, tab, s0, c0,
@@ -941,7 +982,8 @@ __global__ __launch_bounds__(kWG) void full_energy_kernel(PairParams p)
             const int se = min(je, p.mol_off[J + 1]);
             float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
             segment<!SCALED>(jrec, js - jc * kJChunk, se - jc * kJChunk, jc * kJChunk, pi, lane_off, tab, p, ci, ri,
-                             p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
+                             p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J],
+                             J >= p.crowd_mol, s0, c0, s1, c1);
             double v0 = valid ? (double)s0 - (double)c0 : 0.0;
             double v1 = valid ? (double)s1 - (double)c1 : 0.0;
             v0 = wave_sum(v0);
@@ -1149,7 +1191,8 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
             if (J != m) {
                 float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
                 segment<false>(jrec, js - jbase, se - jbase, jbase, pn, lane_off, tab, p, ci, ri,
-                               p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J], s0, c0, s1, c1);
+                               p.center[(size_t)r * p.n_mol + J], p.radius[(size_t)r * p.n_mol + J],
+                               m >= p.crowd_mol || J >= p.crowd_mol, s0, c0, s1, c1);
                 double v = valid ? (double)s0 - (double)c0 : 0.0;
                 v = wave_total(v);
                 if (lane == 0) unsafeAtomicAdd(p.rowbuf + (size_t)r * p.n_mol + J, v);
@@ -1304,7 +1347,10 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
     }
 }
 
-template <int JPL>
+// CROWD: the system has crowders under repulsive crowding (a separate
+// instantiation, so that systems without them keep the registers of the
+// plain kernel)
+template <int JPL, bool CROWD>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4 : 8, 8))) void mc_row_kernel(PairParams p)
 {
     // Wave priority: the SIMDs issue from the oldest wave first, so a wave
@@ -1453,6 +1499,18 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
             lj.col[h] = lane_col(rj[h]);
             lj.q[h] = qtab[lj.col[h] >> kColShift];
         }
+        // repulsive crowding: a visit pairing a crowder with anything (the
+        // moved molecule is one, or some lane's molecule is) adds the
+        // crowder pass below, whose sums replace the crowder lanes' own
+        const bool m_crowd = CROWD && __builtin_amdgcn_readfirstlane(hd.x) >= p.crowd_mol;
+        bool cl[JPL];
+        bool any_c = false;
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) {
+            cl[h] = CROWD && valid[h] && (m_crowd || J[h] >= p.crowd_mol);
+            any_c |= cl[h];
+        }
+        const bool crowd_visit = CROWD && __any(any_c);
         // image handling of this visit (see image_axis): each j residue is
         // moved to its image nearest the proposal's centre cm; every moved
         // residue i lies within rm of cm, so on an axis where
@@ -1490,7 +1548,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
 #pragma unroll
         for (int h = 0; h < JPL; ++h) sa[h] = ca[h] = 0.0f;
         __builtin_amdgcn_s_setprio(0);
-        switch (img) {
+        switch (m_crowd ? -1 : img) {
+        case -1: break;  // every pair of the visit is a crowder pair
         case 0: row_runs<0, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         case 1: row_runs<1, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         case 2: row_runs<2, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
@@ -1500,6 +1559,35 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         case 6: row_runs<6, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         case 7: row_runs<7, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         default: row_runs<kImgFull, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        }
+        if (crowd_visit) {
+            // crowder pairs: the stored j coordinates (reloaded: the
+            // common visit keeps no copy), per-pair rint image, kcal/mol
+            float4 q[JPL];
+#pragma unroll
+            for (int h = 0; h < JPL; ++h) {
+                const int jp = jb * (JPL * kWave) + h * kWave + lane;
+                q[h] = p.pos[(size_t)r * p.stride + min(jp + (jp >= i0 ? n_m : 0), p.n_res - 1)];
+            }
+            float sc[JPL], cc[JPL];
+#pragma unroll
+            for (int h = 0; h < JPL; ++h) sc[h] = cc[h] = 0.0f;
+            for (int i = ia; i < ib; ++i) {
+                const f4_t ri = scalar_rec(irec, i);
+#pragma unroll
+                for (int h = 0; h < JPL; ++h) {
+                    const float e = crowd_term(min_image(ri.x - q[h].x, p.box, p.inv_box),
+                                               min_image(ri.y - q[h].y, p.box, p.inv_box),
+                                               min_image(ri.z - q[h].z, p.box, p.inv_box));
+                    kahan_add(sc[h], cc[h], e * p.rt_to_kcal);
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < JPL; ++h)
+                if (cl[h]) {
+                    sa[h] = sc[h];
+                    ca[h] = cc[h];
+                }
         }
         __builtin_amdgcn_s_setprio(2);
         // FP64 reduction per molecule J: across the lanes J[h] is
@@ -2746,6 +2834,7 @@ struct cgppd_ctx {
     float4 *d_pos = nullptr, *d_rel = nullptr, *d_center = nullptr;
     float *d_radius = nullptr;
     bool fast_image = true;
+    bool rep_crowd = false;  // repulsive crowding (cgppd_set_repulsive_crowding)
     double *d_rowbuf = nullptr, *d_emat = nullptr, *d_energy = nullptr, *d_temp = nullptr;
     double *d_rcos = nullptr, *d_rsin = nullptr;
     float *d_tstep = nullptr;
@@ -2772,6 +2861,7 @@ struct cgppd_ctx {
     double tor_v[1600] = {0.0}, tor_d[1600] = {0.0};
     bool topo_dirty = true;
     int n_flex = 0;
+    std::vector<int> mol_flex_host;  // M: 1 = flexible
     int *d_flex_mols = nullptr, *d_flex_item_off = nullptr, *d_mol_flex = nullptr;
     int4 *d_rtag = nullptr;
     int *d_bond_off = nullptr, *d_angle_off = nullptr, *d_tors_off = nullptr, *d_ind_off = nullptr;
@@ -3169,6 +3259,7 @@ int sync_topology(cgppd_ctx *ctx)
         (rc = upload_vec(ctx, &ctx->d_br, br)) || (rc = upload_vec(ctx, &ctx->d_tor, tor)))
         return rc;
     ctx->n_flex = (int)flex_mols.size();
+    ctx->mol_flex_host = mol_flex;
     ctx->topo_dirty = false;
     destroy_graph(ctx);
     return CGPPD_OK;
@@ -3185,6 +3276,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.n_res = ctx->N;
     p.stride = ctx->stride;
     p.n_active = ctx->M;
+    p.crowd_mol = ctx->rep_crowd ? ctx->n_nc : ctx->M;
     p.ib_max = (ctx->max_n + kIBlock - 1) / kIBlock;
     p.jc_count = (ctx->N + kJChunk - 1) / kJChunk;
     p.mol_off = ctx->d_mol_off;
@@ -3494,7 +3586,9 @@ int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_sto
         else
             hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
     } else {
-        auto kern = layout == CGPPD_LAYOUT_JLANES4 ? mc_row_kernel<4> : mc_row_kernel<2>;
+        const bool crowd = p.crowd_mol < p.n_mol;
+        auto kern = layout == CGPPD_LAYOUT_JLANES4 ? (crowd ? mc_row_kernel<4, true> : mc_row_kernel<4, false>)
+                                                   : (crowd ? mc_row_kernel<2, true> : mc_row_kernel<2, false>);
         const size_t lds = row_lds_bytes(ctx->R);
         if (ev_start)
             hipExtLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, ctx->stream, ev_start, ev_stop, 0, p);
@@ -4004,6 +4098,17 @@ int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float
     if (replica < 0 || replica >= ctx->R || molecule < 0 || molecule >= ctx->M || !xyz)
         return fail(ctx, CGPPD_E_ARG, "bad replica/molecule or null coordinates");
     return upload_molecules(ctx, replica, molecule, molecule + 1, xyz);
+}
+
+int cgppd_set_repulsive_crowding(cgppd_ctx *ctx, int enable)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (enable && ctx->n_flex)
+        for (int m = ctx->n_nc; m < ctx->M; ++m)
+            if (ctx->mol_flex_host[m]) return fail(ctx, CGPPD_E_ARG, "repulsive crowding needs rigid crowders");
+    ctx->rep_crowd = enable != 0;
+    destroy_graph(ctx);
+    return CGPPD_OK;
 }
 
 int cgppd_set_image_shift(cgppd_ctx *ctx, int enable)

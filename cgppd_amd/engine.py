@@ -109,13 +109,14 @@ class Engine:
                                                 C.c_uint64(int(seed))), "set_replicas")
         self.n_rep = R
 
-    def load_spec(self, spec, n_rep=None, seed=0):
+    def load_spec(self, spec, n_rep=None, seed=0, n_noncrowder=None):
         """Force field, box, topology and R identical replicas of a
-        cgppd_amd.synth.SystemSpec (temperatures and steps from the spec)."""
+        cgppd_amd.synth.SystemSpec (temperatures and steps from the spec);
+        molecules from n_noncrowder on are crowders."""
         R = spec.n_rep if n_rep is None else n_rep
         self.set_force_field(spec.ff.eps, spec.ff.sigma, spec.ff.charge)
         self.set_box(spec.box)
-        self.set_system(spec.mol_off, spec.types)
+        self.set_system(spec.mol_off, spec.types, n_noncrowder)
         if getattr(spec, "flexible", False):
             self.set_flexible_spec(spec)
         temps = spec.temperatures() if R == spec.n_rep else np.linspace(spec.t_min, spec.t_max, R)
@@ -325,6 +326,11 @@ class Engine:
                    for i, lay in enumerate(("jlanes", "jlanes4", "ilanes")) for w in range(2, 9)
                    if table[i * 7 + w - 2] > 0}
         return {"layout": names[lay.value], "wg_per_cu": wg.value, "ms": ms.value, "us_by_candidate": timings}
+
+    def set_repulsive_crowding(self, on):
+        """(6/r)^12 below 15 A and no DH for pairs with a crowder molecule
+        (the reference's REPULSIVE_CROWDING, src/Replica.cpp:416-483)."""
+        self._check(self.lib.cgppd_set_repulsive_crowding(self.h, int(bool(on))), "set_repulsive_crowding")
 
     def set_image_shift(self, on):
         self._check(self.lib.cgppd_set_image_shift(self.h, int(bool(on))), "set_image_shift")

@@ -143,6 +143,17 @@ int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float
  * minimum image everywhere (as src/Residue.cpp:68-73). */
 int cgppd_set_image_shift(cgppd_ctx *ctx, int enable);
 
+/* Repulsive crowding (default off).  Replaces the reference's compile-time
+ * switch REPULSIVE_CROWDING (src/definitions.h:18) in Replica::E
+ * (src/Replica.cpp:416-483) and the GPU kernel (src/CudaFunctions.cu:335-338,
+ * 492-500).  When on, every residue pair with a crowder molecule (index >=
+ * n_noncrowder of cgppd_set_system) contributes (6/r)^12 to LJ for r < 15 A
+ * and no Debye-Hueckel term; the energies, the MC rows and the non-crowder
+ * energy follow.  Crowders must be rigid: CGPPD_E_ARG when the flexible
+ * topology already set makes one flexible (set that topology first).  Call
+ * before cgppd_mc_prepare. */
+int cgppd_set_repulsive_crowding(cgppd_ctx *ctx, int enable);
+
 /* Full inter-molecular energy of one replica, synchronous.  Replaces
  * CUDA_EonDevice (src/cudaExterns.h:29) behind Replica::EonDevice
  * (src/Replica.cpp:903-918).  lj and dh are the raw sums (Potential::LJ,

@@ -74,6 +74,8 @@ def lib():
         L.orc_energy_pair_ref.argtypes = [_i32p, C.c_int, C.c_int, _i32p, _f32p, _f32p, _f32p, _f32p, C.c_float,
                                           _f64p, _f64p]
         L.orc_energy_matrix_ref.argtypes = sysargs + [_f64p]
+        L.orc_set_repulsive_crowding.argtypes = [C.c_int]
+        L.orc_set_repulsive_crowding.restype = None
         for name in ("orc_rt_to_kcal", "orc_dh_factor", "orc_r0_factor"):
             getattr(L, name).restype = C.c_float
             getattr(L, name).argtypes = []
@@ -156,6 +158,14 @@ def energy_pair_ref(mol_off, a, b, types, charge, sigma, eps, pos, box):
     lib().orc_energy_pair_ref(_p(mo, C.c_int), int(a), int(b), _p(ty, C.c_int), _p(q, C.c_float),
                               _p(s, C.c_float), _p(e, C.c_float), _p(x, C.c_float), L, C.byref(lj), C.byref(dh))
     return lj.value, dh.value
+
+
+def set_repulsive_crowding(first_crowder_residue):
+    """Repulsive crowding for every energy function and the MC oracle: pairs
+    with a residue at or past first_crowder_residue (the crowder molecules)
+    take (6/r)^12 of LJ below 15 A and no DH (src/Replica.cpp:416-483).
+    None switches it off, the reference's default (src/definitions.h:18)."""
+    lib().orc_set_repulsive_crowding(-1 if first_crowder_residue is None else int(first_crowder_residue))
 
 
 def energy_fp64(mol_off, types, charge, sigma, eps, pos, box):

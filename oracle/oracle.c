@@ -104,6 +104,19 @@ static inline double ref_dh(int ti, int tj, const float *charge, double r)
     return (double)(charge[ti] * charge[tj] * e) / r;
 }
 
+/* Repulsive crowding (the reference's compile-time REPULSIVE_CROWDING,
+ * src/definitions.h:18; src/Replica.cpp:416-483): a pair with a residue of
+ * a crowder molecule contributes crowderPairPotential(r) = (6/r)^12 (float
+ * powf) to LJ for r < const_repulsive_cutoff = 15 A (src/definitions.h:38)
+ * and no Debye-Hueckel term.  Crowders are the molecules from the first
+ * crowder residue on; off (the reference's default) when negative. */
+static int orc_crowd_res = 0x7fffffff;
+void orc_set_repulsive_crowding(int first_crowder_residue)
+{
+    orc_crowd_res = first_crowder_residue < 0 ? 0x7fffffff : first_crowder_residue;
+}
+#define ORC_REPULSIVE_CUTOFF 15.0f
+
 static void ref_block(int i0, int i1, int j0, int j1, const int *type, const float *charge,
                       const float *sigma, const float *eps, const float *pos, float box,
                       double *lj, double *dh)
@@ -113,6 +126,10 @@ static void ref_block(int i0, int i1, int j0, int j1, const int *type, const flo
     for (int i = i0; i < i1; ++i) {
         for (int j = j0; j < j1; ++j) {
             double r = ref_distance(pos + 3 * i, pos + 3 * j, box);
+            if (i >= orc_crowd_res || j >= orc_crowd_res) {
+                if (r < ORC_REPULSIVE_CUTOFF) sl += (double)powf(6.0f / (float)r, 12.0f);
+                continue;
+            }
             sl += ref_lj(type[i], type[j], sigma, eps, r, r0f);
             sd += ref_dh(type[i], type[j], charge, r);
         }
@@ -205,6 +222,10 @@ void orc_energy_fp64(int n_mol, const int *mol_off, const int *type, const float
                         if (L != 0.0) d[k] -= L * round(d[k] / L);
                     }
                     double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + 1e-38;
+                    if (i >= orc_crowd_res || j >= orc_crowd_res) {
+                        if (r < (double)ORC_REPULSIVE_CUTOFF) nsum_add(&sl, pow(6.0 / r, 12.0));
+                        continue;
+                    }
                     int ti = type[i], tj = type[j];
                     double Eij = (double)ORC_LAMBDA * ((double)eps[ti * 20 + tj] - (double)ORC_E0);
                     double sij = 0.5 * ((double)sigma[ti] + (double)sigma[tj]);

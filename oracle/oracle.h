@@ -66,6 +66,11 @@ void orc_energy_matrix_ref(int n_mol, const int *mol_off, const int *type,
 /* FP64 oracle of the same model: every operation in double (positions and
  * table values are the float inputs), Neumaier-compensated sums.  Pinned by
  * closed-form two-residue cases and agreement with orc_energy_ref. */
+/* Repulsive crowding (src/definitions.h:18, src/Replica.cpp:416-483) for
+ * every energy function here: pairs with a residue at or past
+ * first_crowder_residue take (6/r)^12 below 15 A and no DH; negative: off. */
+void orc_set_repulsive_crowding(int first_crowder_residue);
+
 void orc_energy_fp64(int n_mol, const int *mol_off, const int *type,
                      const float *charge, const float *sigma, const float *eps,
                      const float *pos, float box, double *lj, double *dh);
