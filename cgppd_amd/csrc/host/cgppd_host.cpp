@@ -937,6 +937,7 @@ void Simulation::loadArgsFromFile()
                 parameters.title = rest;
             }
             else if (key == "seed") parameters.seed = std::strtoull(value.c_str(), nullptr, 10);
+            else if (key == "repulsivecrowding") parameters.repulsiveCrowding = std::atoi(value.c_str()) != 0;
             else if (key == "aminoacids") parameters.aminoAcidFile = value;
             else if (key == "pairpotentials") parameters.pairPotentialFile = value;
             else if (key == "torsions") parameters.torsionFile = value;
@@ -1027,6 +1028,8 @@ void Simulation::setup_device(int device)
                         "cgppd_set_topology");
         }
     }
+    if (parameters.repulsiveCrowding)  // crowders as u(r) = (6/r)^12 (src/Simulation.cpp:577-584)
+        dev_->check(cgppd_set_repulsive_crowding(dev_->ctx(), 1), "cgppd_set_repulsive_crowding");
     const int R = (int)replica.size(), M = initialReplica.moleculeCount, N = initialReplica.residueCount;
     std::vector<float> centres, rel;
     std::vector<double> temps(R);

@@ -214,6 +214,7 @@ struct argdata {
     bool resume = false;
     uint64_t seed = 0;         // new: keys the device Philox streams
     bool use_graph = true;     // new: replay exchange intervals from a hipGraph
+    bool repulsiveCrowding = false;  // new: run-time REPULSIVE_CROWDING (src/definitions.h:18)
     std::string aminoAcidFile = "data/AminoAcids";
     std::string pairPotentialFile = "data/pair_potentials_MJ1996";
     std::string torsionFile = "data/torsional_pair_potentials";  // src/definitions.h:7
