@@ -228,3 +228,41 @@ def test_cli_runs(tmp_path, gpu_engine_factory):
     lines = idx[0].read_text().splitlines()
     assert lines[0] == "index molecule_file_path crowder(Y/N)"
     assert len(lines) == 1 + spec.n_mol and all(l.split()[-1] == "N" for l in lines[1:])
+
+
+@pytest.mark.gpu
+def test_repulsive_crowding_argfile_key(host, tmp_path, gpu_engine_factory):
+    """The argfile key `repulsivecrowding 1` (the reference's compile-time
+    REPULSIVE_CROWDING, src/definitions.h:18) reaches the device: EonDevice
+    of a system with a crowders section equals the oracle's energy with
+    (6/r)^12 crowder pairs, and differs from the plain model."""
+    from cgppd_amd import synth
+    spec = synth.make_system("crowdmc", [40, 25, 1, 1, 3, 1, 2, 1, 1, 1], 60.0, 2, 288.0, 450.0, 14, 2.0, 0.2)
+    n_nc = 2
+    cfg = write_config(tmp_path, spec, n_rep=2, extra="repulsivecrowding 1")
+    text = cfg.read_text().splitlines()
+    at = next(i for i, line in enumerate(text) if f"mol{n_nc}.pdb" in line)
+    cfg.write_text("\n".join(text[:at] + ["", "crowders"] + text[at:]) + "\n")
+    h = create(host, ["cgppd", "-f", str(cfg)])
+    assert host.cgppd_host_setup_device(h, 0) == 0, host.cgppd_host_last_error()
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    assert host.cgppd_host_replica_EonDevice(h, 0, C.byref(a), C.byref(b), C.byref(c)) == 0, \
+        host.cgppd_host_last_error()
+    n_res = int(spec.mol_off[-1])
+    off = np.zeros(spec.n_mol + 1, np.int32)
+    ty = np.zeros(n_res, np.int32)
+    pos = np.zeros(n_res * 3, np.float32)
+    T, ts, rs = C.c_double(), C.c_float(), C.c_double()
+    host.cgppd_host_replica_state(h, 0, off.ctypes.data_as(C.c_void_p), ty.ctypes.data_as(C.c_void_p),
+                                  pos.ctypes.data_as(C.c_void_p), C.byref(T), C.byref(ts), C.byref(rs))
+    args = (off, ty, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos.reshape(-1, 3), spec.box)
+    plain = oracle.total(*oracle.energy_fp64(*args))
+    try:
+        oracle.set_repulsive_crowding(int(off[n_nc]))
+        want = oracle.total(*oracle.energy_fp64(*args))
+    finally:
+        oracle.set_repulsive_crowding(None)
+    # (force field through its text files: 1e-4)
+    assert abs(a.value - want) <= 1e-4 * abs(want) + 1e-5, (a.value, want)
+    assert abs(a.value - plain) > 1e-2, (a.value, plain)
+    host.cgppd_host_simulation_destroy(h)
