@@ -57,6 +57,7 @@ SIGNATURES = {
     "cgppd_autotune_table": (C.c_int, [_ctxp, _f64p]),
     "cgppd_set_image_shift": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_set_repulsive_crowding": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_set_rigid_potential_only": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_set_timing": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_last_kernel_ms": (C.c_int, [_ctxp, _f64p, _i64p]),
     "cgppd_model_constants": (C.c_int, [_f32p, _f32p, _f32p]),

@@ -307,6 +307,9 @@ struct PairParams {
     // flexible molecules (null / 0 when the system has none).  Residue
     // indices are device indices (global); lists are CSR per molecule.
     int n_flex;                 // number of flexible molecules
+    // 1: flexible molecules carry their internal energy (intra-molecular
+    // pairs and bonded terms); 0 under the rigid-only override
+    int internal;
     const int *flex_mols;       // n_flex: the flexible molecules
     const int *flex_item_off;   // n_flex + 1: full-energy items per replica
     const int *mol_flex;        // M: 1 = flexible
@@ -1639,7 +1642,8 @@ __device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, in
 {
     const int nib = (n + kIBlock - 1) / kIBlock;
     int items = nib * p.jc_count;
-    if (p.n_flex && p.mol_flex[m]) items += kind == kMovedOne ? (n + kJChunk - 1) / kJChunk : nib * intra_chunks(n);
+    if (p.internal && p.n_flex && p.mol_flex[m])
+        items += kind == kMovedOne ? (n + kJChunk - 1) / kJChunk : nib * intra_chunks(n);
     return items;
 }
 
@@ -1652,7 +1656,7 @@ __device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, in
 __device__ __forceinline__ void record_proposal(const PairParams &p, int r, int m, int i0, int n, int kind,
                                                 int moved, float radius_new, double bonded)
 {
-    const bool flex = p.n_flex && p.mol_flex[m];
+    const bool flex = p.internal && p.n_flex && p.mol_flex[m];
     if (!flex) kind = kMovedNone;
     p.prop_hdr[r] = make_int4(m, i0, n, __float_as_int(radius_new));
     p.prop_items[r] = ilane_items(p, m, n, kind);
@@ -2064,7 +2068,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
             c0 = cnt[0];
             c1 = cnt[1];
             c2 = cnt[2];
-            flex_m = p.n_flex && p.mol_flex[m];
+            flex_m = p.internal && p.n_flex && p.mol_flex[m];
             if (p.da) {
                 s1 = p.s1[r];
                 dS = p.s_dS[r];
@@ -2835,6 +2839,7 @@ struct cgppd_ctx {
     float *d_radius = nullptr;
     bool fast_image = true;
     bool rep_crowd = false;  // repulsive crowding (cgppd_set_repulsive_crowding)
+    bool rigid_only = false;  // cgppd_set_rigid_potential_only: no internal energy
     double *d_rowbuf = nullptr, *d_emat = nullptr, *d_energy = nullptr, *d_temp = nullptr;
     double *d_rcos = nullptr, *d_rsin = nullptr;
     float *d_tstep = nullptr;
@@ -3183,6 +3188,10 @@ int sync_topology(cgppd_ctx *ctx)
     if (!ctx->topo_dirty) return CGPPD_OK;
     const int M = ctx->M, N = ctx->N;
     if ((int)ctx->topo.size() != M) ctx->topo.assign(M, cgppd_topo::Molecule());
+    if (ctx->rep_crowd)
+        for (int m = ctx->n_nc; m < M; ++m)
+            if (ctx->topo[m].flexible)
+                return fail(ctx, CGPPD_E_ARG, "molecule %d is a crowder: repulsive crowding needs rigid crowders", m);
     if ((int)ctx->chain_index.size() != N) ctx->chain_index.assign(N, 0);
     std::vector<int4> rtag(N);
     std::vector<int> mol_flex(M, 0), flex_mols, flex_item_off{0};
@@ -3329,6 +3338,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.steps = ctx->d_steps;
     p.pair_evals = ctx->d_pair_evals;
     p.n_flex = ctx->n_flex;
+    p.internal = ctx->rigid_only ? 0 : 1;
     p.flex_mols = ctx->d_flex_mols;
     p.flex_item_off = ctx->d_flex_item_off;
     p.mol_flex = ctx->d_mol_flex;
@@ -3462,6 +3472,14 @@ int grid_for(const cgppd_ctx *ctx, long long items, bool xcd)
     return (int)g;
 }
 
+// The first crowder molecule whose stored topology is flexible, or -1.
+int flexible_crowder(const cgppd_ctx *ctx)
+{
+    for (int m = ctx->n_nc; m < (int)ctx->topo.size(); ++m)
+        if (ctx->topo[m].flexible) return m;
+    return -1;
+}
+
 int check_ready(cgppd_ctx *ctx, bool need_rep)
 {
     if (!ctx) return CGPPD_E_ARG;
@@ -3502,7 +3520,7 @@ int launch_full(cgppd_ctx *ctx, int r_begin, int r_count, int n_active, bool sca
 int launch_intra(cgppd_ctx *ctx, int r_begin, int r_count, bool scaled, double *mat, double *out_lj,
                  double *out_dh)
 {
-    if (ctx->n_flex == 0) return CGPPD_OK;
+    if (ctx->n_flex == 0 || ctx->rigid_only) return CGPPD_OK;
     PairParams p = base_params(ctx);
     p.rep_begin = r_begin;
     p.rep_count = r_count;
@@ -3900,6 +3918,8 @@ int cgppd_set_topology(cgppd_ctx *ctx, int molecule, const int *chain_code, cons
     cgppd_topo::Molecule t;
     if (!cgppd_topo::build(n, chain_code, all_flexible != 0, segs, nchains, t, err))
         return fail(ctx, CGPPD_E_ARG, "%s", err.c_str());
+    if (ctx->rep_crowd && molecule >= ctx->n_nc && t.flexible)
+        return fail(ctx, CGPPD_E_ARG, "molecule %d is a crowder: repulsive crowding needs rigid crowders", molecule);
     if ((int)ctx->topo.size() != ctx->M) ctx->topo.assign(ctx->M, cgppd_topo::Molecule());
     ctx->topo[molecule] = std::move(t);
     std::copy(chain_index, chain_index + n, ctx->chain_index.begin() + i0);
@@ -4103,10 +4123,28 @@ int cgppd_update_molecule(cgppd_ctx *ctx, int replica, int molecule, const float
 int cgppd_set_repulsive_crowding(cgppd_ctx *ctx, int enable)
 {
     if (!ctx) return CGPPD_E_ARG;
-    if (enable && ctx->n_flex)
-        for (int m = ctx->n_nc; m < ctx->M; ++m)
-            if (ctx->mol_flex_host[m]) return fail(ctx, CGPPD_E_ARG, "repulsive crowding needs rigid crowders");
-    ctx->rep_crowd = enable != 0;
+    // (the topology each cgppd_set_topology call stores, not the device
+    // lists, which are rebuilt lazily)
+    if (enable && flexible_crowder(ctx) >= 0)
+        return fail(ctx, CGPPD_E_ARG, "repulsive crowding needs rigid crowders (molecule %d is flexible)",
+                    flexible_crowder(ctx));
+    if (ctx->rep_crowd != (enable != 0)) {
+        // the energy model changed: the MC matrix and totals must be rebuilt
+        ctx->rep_crowd = enable != 0;
+        ctx->mc_ready = false;
+        ctx->smat_dirty = true;
+    }
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_set_rigid_potential_only(cgppd_ctx *ctx, int enable)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (ctx->rigid_only != (enable != 0)) {
+        ctx->rigid_only = enable != 0;
+        ctx->mc_ready = false;
+    }
     destroy_graph(ctx);
     return CGPPD_OK;
 }
@@ -4141,7 +4179,7 @@ int cgppd_energy_internal(cgppd_ctx *ctx, int replica, double *out)
     if (rc) return rc;
     if (replica < 0 || replica >= ctx->R || !out) return fail(ctx, CGPPD_E_ARG, "bad replica or null output");
     for (int k = 0; k < 5; ++k) out[k] = 0.0;
-    if (ctx->n_flex == 0) return CGPPD_OK;
+    if (ctx->n_flex == 0 || ctx->rigid_only) return CGPPD_OK;
     CG_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = zero_dev(ctx, ctx->d_out_int, (size_t)5 * ctx->R))) return rc;
     if ((rc = launch_intra(ctx, replica, 1, false, nullptr, ctx->d_out_scratch, ctx->d_out_scratch + ctx->R)))

@@ -527,6 +527,7 @@ void Replica::init_first_replica(const argdata &parameters, const AminoAcids &am
 {
     aminoAcids = amino_acid_data;
     boundingValue = parameters.bound;
+    repulsiveCrowding = parameters.repulsiveCrowding;
     molecules.clear();
     moleculeCount = residueCount = nonCrowderCount = nonCrowderResidues = max_residue_count = 0;
     bool flexible = false;
@@ -588,6 +589,8 @@ void Replica::copy(const Replica &r)
     label = r.label;
     temperature = r.temperature;
     potential = r.potential;
+    repulsiveCrowding = r.repulsiveCrowding;
+    calculate_rigid_potential_only = r.calculate_rigid_potential_only;
 }
 
 void Replica::init_child_replica(const Replica &ir, int index, double geometricTemperature, double geometricRotation,
@@ -633,16 +636,25 @@ Potential Replica::E(const Molecule *a, const Molecule *b) const
 
 Potential Replica::E() const
 {
-    // src/Replica.cpp:432-493 (one running Potential over all pairs)
+    // src/Replica.cpp:432-493 (one running Potential over all pairs); under
+    // repulsive crowding a pair with a crowder adds (6/r)^12 [RT] below the
+    // 15 A cutoff and no DH (src/Replica.cpp:416-419,436-470,
+    // src/definitions.h:38)
     Potential p;
     for (int I = 0; I < moleculeCount; ++I)
-        for (int J = I + 1; J < moleculeCount; ++J)
+        for (int J = I + 1; J < moleculeCount; ++J) {
+            const bool crowd = repulsiveCrowding && (molecules[I].crowder || molecules[J].crowder);
             for (const Residue &ri : molecules[I].Residues)
                 for (const Residue &rj : molecules[J].Residues) {
                     const double r = ri.distance(rj, boundingValue) + (double)EPS;
+                    if (crowd) {
+                        if (r < 15.0f) p.increment_LJ(powf(6.0f / (float)r, 12.0f));
+                        continue;
+                    }
                     p.increment_LJ(calculate_LJ(ri.aminoAcidIndex, rj.aminoAcidIndex, r, aminoAcids));
                     p.increment_DH(calculate_DH(ri.electrostaticCharge, rj.electrostaticCharge, r));
                 }
+        }
     return p;
 }
 
@@ -694,6 +706,10 @@ void Replica::setup_device(DeviceContext *dev, int slot)
 void Replica::ReplicaDataToDevice()
 {
     if (!dev_) throw error("Replica::ReplicaDataToDevice: no device context (call setup_device)");
+    // src/Replica.cpp:753-757: the rigid-only override travels with the data
+    // (the device context holds one flag for all of its replicas)
+    dev_->check(cgppd_set_rigid_potential_only(dev_->ctx(), calculate_rigid_potential_only ? 1 : 0),
+                "cgppd_set_rigid_potential_only");
     const std::vector<float> x = positions();
     dev_->check(cgppd_upload_positions(dev_->ctx(), slot_, x.data()), "cgppd_upload_positions");
 }

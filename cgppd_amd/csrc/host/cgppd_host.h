@@ -261,6 +261,12 @@ public:
     int boundSamples = 0, samplesSinceLastExchange = 0, totalBoundSamples = 0, totalSamples = 0;
     float fractionBound = 0.f, accumulativeFractionBound = 0.f;
     double nonCrowderPotential = 0.0;
+    // the reference's compile-time REPULSIVE_CROWDING as a run-time flag:
+    // crowder pairs contribute (6/r)^12 below 15 A and no Debye-Hueckel term
+    bool repulsiveCrowding = false;
+    // src/Replica.h:55: tests skip the internal potential; every residue of
+    // a molecule counts as one rigid domain on the device (src/Replica.cpp:753-757)
+    bool calculate_rigid_potential_only = false;
 
     // src/Replica.cpp:19-98
     void init_first_replica(const argdata &parameters, const AminoAcids &amino_acid_data, int initial_molecule_array_size);

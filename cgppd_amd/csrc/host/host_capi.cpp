@@ -139,6 +139,13 @@ int cgppd_host_replica_EonDevice(void *h, int r, double *total, double *total_nc
     });
 }
 
+// Replica::calculate_rigid_potential_only of replica r (takes effect at the
+// next ReplicaDataToDevice, as in the reference)
+int cgppd_host_replica_set_rigid_only(void *h, int r, int flag)
+{
+    return guarded([&] { static_cast<cgppd::Simulation *>(h)->replica.at(r).calculate_rigid_potential_only = flag != 0; });
+}
+
 // Simulation::run, then the per-replica running energies and temperatures
 int cgppd_host_simulation_run(void *h, double *energy, double *temperature)
 {

@@ -332,6 +332,11 @@ class Engine:
         (the reference's REPULSIVE_CROWDING, src/Replica.cpp:416-483)."""
         self._check(self.lib.cgppd_set_repulsive_crowding(self.h, int(bool(on))), "set_repulsive_crowding")
 
+    def set_rigid_potential_only(self, on):
+        """Replica::calculate_rigid_potential_only (src/Replica.cpp:753-757):
+        flexible molecules carry no internal energy."""
+        self._check(self.lib.cgppd_set_rigid_potential_only(self.h, int(bool(on))), "set_rigid_potential_only")
+
     def set_image_shift(self, on):
         self._check(self.lib.cgppd_set_image_shift(self.h, int(bool(on))), "set_image_shift")
 

@@ -149,10 +149,22 @@ int cgppd_set_image_shift(cgppd_ctx *ctx, int enable);
  * 492-500).  When on, every residue pair with a crowder molecule (index >=
  * n_noncrowder of cgppd_set_system) contributes (6/r)^12 to LJ for r < 15 A
  * and no Debye-Hueckel term; the energies, the MC rows and the non-crowder
- * energy follow.  Crowders must be rigid: CGPPD_E_ARG when the flexible
- * topology already set makes one flexible (set that topology first).  Call
- * before cgppd_mc_prepare. */
+ * energy follow.  Crowders must be rigid: CGPPD_E_ARG when a crowder's
+ * topology (cgppd_set_topology) is flexible, whichever of the two calls
+ * comes first.  Changing the switch invalidates the MC state: MC calls
+ * return CGPPD_E_STATE until cgppd_mc_prepare rebuilds the matrix. */
 int cgppd_set_repulsive_crowding(cgppd_ctx *ctx, int enable);
+
+/* Rigid-only override (default off).  Replaces the flag
+ * Replica::calculate_rigid_potential_only (src/Replica.h:55) that tests set
+ * before ReplicaDataToDevice, which then packs every residue of a molecule
+ * into one rigid domain (src/Replica.cpp:753-757): with it on, flexible
+ * molecules contribute no intra-molecular pairs to cgppd_energy /
+ * cgppd_energy_async / cgppd_energy_all, cgppd_energy_internal returns
+ * zeros, and MC steps move flexible molecules as usual but evaluate only
+ * the inter-molecular energy (src/Replica.cpp:366-380).  Applies to every
+ * replica of the context; changing it requires cgppd_mc_prepare again. */
+int cgppd_set_rigid_potential_only(cgppd_ctx *ctx, int enable);
 
 /* Full inter-molecular energy of one replica, synchronous.  Replaces
  * CUDA_EonDevice (src/cudaExterns.h:29) behind Replica::EonDevice

@@ -11,7 +11,12 @@ Parity pins (see tests/test_oracle.py):
   orc_philox4x32_10    Random123 known-answer vectors
   moves                bit-exact vs oracle/_ref (reference Quaternion/Vector3f),
                        rigid-body invariants
-  Metropolis/exchange  closed form (acceptance frequencies), invariants
+  orc_metropolis       acceptance frequencies on a grid of uniforms against
+                       min(1, exp(-dE kcal / (R T))) with R and kcal from
+                       scipy.constants; exact stationary distributions of
+                       small state spaces equal to their Boltzmann weights
+  orc_remc_exchange    acceptance frequency against min(1, exp(delta)),
+                       swaps temperatures only
   flexible linkers     topology (oracle/topology.py), bonded and intra-molecular
                        energies against oracle/_ref/ref_flex_driver and
                        tests/testMolecule.cpp / tests/testReplica.cpp values;
@@ -88,6 +93,10 @@ def lib():
         L.orc_remc_exchange.argtypes = [C.POINTER(RemcState)]
         L.orc_remc_reference_run.argtypes = [C.POINTER(RemcState), C.c_int, C.c_int]
         L.orc_remc_reference_run.restype = C.c_double
+        L.orc_metropolis.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_metropolis.restype = C.c_int
+        L.orc_metropolis_ref.argtypes = [C.c_float, C.c_float, C.c_double]
+        L.orc_metropolis_ref.restype = C.c_int
         for name in ("orc_internal_ref", "orc_internal_fp64"):
             getattr(L, name).argtypes = [C.POINTER(FlexState), C.c_int, _i32p, _i32p, _f32p, _f32p, _f32p, _f32p,
                                          C.c_float, C.c_int, _f64p]
@@ -191,6 +200,13 @@ def philox(ctr, key):
     out = (C.c_uint32 * 4)()
     lib().orc_philox4x32_10(c, k, out)
     return list(out)
+
+
+def metropolis(delta, temperature, u, reference=False):
+    """The oracle's Metropolis decision (0 downhill, 1 accepted by the
+    Boltzmann test, 2 rejected); reference=True: float delta and exponent."""
+    f = lib().orc_metropolis_ref if reference else lib().orc_metropolis
+    return int(f(float(delta), float(temperature), float(u)))
 
 
 def proposal(seed, r, step, n_mol):

@@ -775,6 +775,25 @@ void orc_remc_init(orc_remc *s)
     }
 }
 
+/* src/Replica.cpp:385-410: downhill moves are accepted, uphill ones when
+ * the uniform falls below the Boltzmann factor of delta converted from
+ * kcal/mol to J/mol */
+int orc_metropolis(double delta, double temperature, double u)
+{
+    if (delta < 0.0) return 0;
+    if (u < exp(-delta * 4184.0 / ((double)ORC_RGAS * temperature))) return 1;
+    return 2;
+}
+
+/* the same with the reference's float delta and float exponent
+ * (src/Replica.cpp:385, `float delta = newPotential - potential`) */
+int orc_metropolis_ref(float delta, float temperature, double u)
+{
+    if (delta < 0.0f) return 0;
+    if (u < (double)expf(-(delta * ORC_KCAL) / (ORC_RGAS * temperature))) return 1;
+    return 2;
+}
+
 static void mc_replica_steps(orc_remc *s, int r, int nsteps, float *scratch_rel, float *scratch_pos,
                              double *row)
 {
@@ -815,11 +834,9 @@ static void mc_replica_steps(orc_remc *s, int r, int nsteps, float *scratch_rel,
             delta += row[J] - mat[m * M + J];
         }
         int64_t *cnt = s->counters + 3 * r;
-        int accept = 0;
-        if (delta < 0.0) { accept = 1; cnt[0]++; }
-        else if (u < exp(-delta * 4184.0 / ((double)ORC_RGAS * s->temperature[r]))) { accept = 1; cnt[1]++; }
-        else cnt[2]++;
-        if (accept) {
+        const int decision = orc_metropolis(delta, s->temperature[r], u);
+        cnt[decision]++;
+        if (decision < 2) {
             for (int q = 0; q < 3; ++q) center[3 * m + q] = cnew[q];
             memcpy(rel + 3 * i0, scratch_rel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
             memcpy(pos + 3 * i0, scratch_pos + 3 * i0, sizeof(float) * 3 * (i1 - i0));
@@ -928,13 +945,12 @@ static void *ref_worker(void *arg)
                         enew += orc_internal_total(v);
                     }
             /* src/Replica.cpp:385-410 with float delta and float exponent */
-            float delta = (float)(enew - s->energy[r]);
-            float T = (float)s->temperature[r];
+            const int decision = orc_metropolis_ref((float)(enew - s->energy[r]), (float)s->temperature[r], u);
             int64_t *cnt = s->counters + 3 * r;
-            if (delta < 0.0f) { s->energy[r] = enew; cnt[0]++; }
-            else if (u < (double)expf(-(delta * ORC_KCAL) / (ORC_RGAS * T))) { s->energy[r] = enew; cnt[1]++; }
-            else {
-                cnt[2]++;
+            cnt[decision]++;
+            if (decision < 2) {
+                s->energy[r] = enew;
+            } else {
                 for (int q = 0; q < 3; ++q) center[3 * m + q] = bc[q];
                 memcpy(rel + 3 * i0, brel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
                 memcpy(pos + 3 * i0, bpos + 3 * i0, sizeof(float) * 3 * (i1 - i0));

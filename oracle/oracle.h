@@ -202,6 +202,16 @@ void orc_apply_move(int is_rotate, const double axis_raw[3], float tstep, double
                     float box, const float c_in[3], float c_out[3], int n, const float *rel_in,
                     float *rel_out, float *pos_out);
 
+/* The Metropolis decision of src/Replica.cpp:385-410 on an energy change
+ * delta (kcal/mol) at temperature T (K) with the uniform u in [0, 1):
+ * 0 downhill (delta < 0, the reference's `accept`), 1 accepted by the
+ * Boltzmann test u < exp(-delta kcal / (Rgas T)) (`acceptA`), 2 rejected
+ * (`reject`).  orc_metropolis takes an FP64 delta and exponent (the delta-E
+ * loop and the device); orc_metropolis_ref the reference's float delta and
+ * float exponent (the CPU baseline). */
+int orc_metropolis(double delta, double temperature, double u);
+int orc_metropolis_ref(float delta, float temperature, double u);
+
 /* nsteps MC steps on every replica (delta-E rows against the matrix). */
 void orc_remc_mc_steps(orc_remc *s, int nsteps);
 

@@ -23,7 +23,7 @@ def host():
     lib.cgppd_host_simulation_create.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_void_p)]
     for n in ("cgppd_host_simulation_destroy", "cgppd_host_simulation_sizes", "cgppd_host_replica_state",
               "cgppd_host_replica_E", "cgppd_host_setup_device", "cgppd_host_replica_EonDevice",
-              "cgppd_host_simulation_run", "cgppd_host_replica_internal"):
+              "cgppd_host_simulation_run", "cgppd_host_replica_internal", "cgppd_host_replica_set_rigid_only"):
         getattr(lib, n).argtypes = None
     return lib
 
@@ -177,6 +177,34 @@ def test_flexible_device_entry_points(host, tmp_path, gpu_engine_factory):
 
 
 @pytest.mark.gpu
+def test_rigid_potential_only_override(host, tmp_path, gpu_engine_factory):
+    """tests/testReplica.cpp:96-111: with calculate_rigid_potential_only set
+    and the data sent again, E(), EonDevice(), EonDeviceNC() and the
+    asynchronous path all give the rigid totals (LJ -0.079673, DH -0.212202);
+    cleared again, EonDevice counts the flexible residues' pairs once more."""
+    h = create(host, ["cgppd", "-f", str(flexible_conf1_config(tmp_path))])
+    assert host.cgppd_host_setup_device(h, 0) == 0, host.cgppd_host_last_error()
+    rigid = -0.079673 - 0.212202
+    approx = lambda x, y: abs(x - y) < 100 * np.finfo(np.float32).eps * (1 + max(abs(x), abs(y))) + 5e-6  # noqa: E731
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    assert host.cgppd_host_replica_EonDevice(h, 0, C.byref(a), C.byref(b), C.byref(c)) == 0
+    flex_pairs = a.value
+    assert not approx(flex_pairs, rigid)  # the flexible tail's own pairs count
+    assert host.cgppd_host_replica_set_rigid_only(h, 0, 1) == 0
+    assert host.cgppd_host_replica_EonDevice(h, 0, C.byref(a), C.byref(b), C.byref(c)) == 0, \
+        host.cgppd_host_last_error()
+    lj, dh, tot = C.c_double(), C.c_double(), C.c_double()
+    host.cgppd_host_replica_E(h, 0, C.byref(lj), C.byref(dh), C.byref(tot))
+    assert approx(tot.value, rigid)
+    for v in (a.value, b.value, c.value):
+        assert approx(v, rigid), (a.value, b.value, c.value)
+    assert host.cgppd_host_replica_set_rigid_only(h, 0, 0) == 0
+    assert host.cgppd_host_replica_EonDevice(h, 0, C.byref(a), C.byref(b), C.byref(c)) == 0
+    assert abs(a.value - flex_pairs) <= 1e-9 * abs(flex_pairs) + 1e-12
+    host.cgppd_host_simulation_destroy(h)
+
+
+@pytest.mark.gpu
 def test_device_entry_points_match_E(host, tmp_path, gpu_engine_factory):
     from cgppd_amd import synth
     spec = synth.make_system("hostgpu", [20, 35, 28, 60], 90.0, 4, 288.0, 450.0, 43)
@@ -230,12 +258,9 @@ def test_cli_runs(tmp_path, gpu_engine_factory):
     assert len(lines) == 1 + spec.n_mol and all(l.split()[-1] == "N" for l in lines[1:])
 
 
-@pytest.mark.gpu
-def test_repulsive_crowding_argfile_key(host, tmp_path, gpu_engine_factory):
-    """The argfile key `repulsivecrowding 1` (the reference's compile-time
-    REPULSIVE_CROWDING, src/definitions.h:18) reaches the device: EonDevice
-    of a system with a crowders section equals the oracle's energy with
-    (6/r)^12 crowder pairs, and differs from the plain model."""
+def crowded_config(tmp_path):
+    """Two chains and eight small crowders (a `crowders` section) with the
+    argfile key `repulsivecrowding 1`."""
     from cgppd_amd import synth
     spec = synth.make_system("crowdmc", [40, 25, 1, 1, 3, 1, 2, 1, 1, 1], 60.0, 2, 288.0, 450.0, 14, 2.0, 0.2)
     n_nc = 2
@@ -243,19 +268,58 @@ def test_repulsive_crowding_argfile_key(host, tmp_path, gpu_engine_factory):
     text = cfg.read_text().splitlines()
     at = next(i for i, line in enumerate(text) if f"mol{n_nc}.pdb" in line)
     cfg.write_text("\n".join(text[:at] + ["", "crowders"] + text[at:]) + "\n")
-    h = create(host, ["cgppd", "-f", str(cfg)])
-    assert host.cgppd_host_setup_device(h, 0) == 0, host.cgppd_host_last_error()
-    a, b, c = C.c_double(), C.c_double(), C.c_double()
-    assert host.cgppd_host_replica_EonDevice(h, 0, C.byref(a), C.byref(b), C.byref(c)) == 0, \
-        host.cgppd_host_last_error()
+    return spec, n_nc, cfg
+
+
+def host_replica_system(host, h, spec, r=0):
     n_res = int(spec.mol_off[-1])
     off = np.zeros(spec.n_mol + 1, np.int32)
     ty = np.zeros(n_res, np.int32)
     pos = np.zeros(n_res * 3, np.float32)
     T, ts, rs = C.c_double(), C.c_float(), C.c_double()
-    host.cgppd_host_replica_state(h, 0, off.ctypes.data_as(C.c_void_p), ty.ctypes.data_as(C.c_void_p),
+    host.cgppd_host_replica_state(h, r, off.ctypes.data_as(C.c_void_p), ty.ctypes.data_as(C.c_void_p),
                                   pos.ctypes.data_as(C.c_void_p), C.byref(T), C.byref(ts), C.byref(rs))
-    args = (off, ty, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos.reshape(-1, 3), spec.box)
+    return off, ty, pos.reshape(-1, 3)
+
+
+def test_repulsive_crowding_host_E(host, tmp_path):
+    """The CPU Replica::E under `repulsivecrowding 1` (src/Replica.cpp:432-483):
+    crowder pairs as (6/r)^12 below 15 A without DH, as the oracle's crowding
+    model gives on the same coordinates."""
+    spec, n_nc, cfg = crowded_config(tmp_path)
+    h = create(host, ["cgppd", "-f", str(cfg)])
+    off, ty, pos = host_replica_system(host, h, spec)
+    args = (off, ty, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos, spec.box)
+    plain = oracle.total(*oracle.energy_ref(*args))
+    try:
+        oracle.set_repulsive_crowding(int(off[n_nc]))
+        lj_w, dh_w = oracle.energy_ref(*args)
+    finally:
+        oracle.set_repulsive_crowding(None)
+    lj, dh, tot = C.c_double(), C.c_double(), C.c_double()
+    host.cgppd_host_replica_E(h, 0, C.byref(lj), C.byref(dh), C.byref(tot))
+    # the same reference-semantics arithmetic on both sides, summed in a
+    # different order (molecule pairs vs residue runs): 1e-9 relative
+    assert abs(lj.value - lj_w) <= 1e-9 * abs(lj_w) + 1e-12, (lj.value, lj_w)
+    assert abs(dh.value - dh_w) <= 1e-9 * abs(dh_w) + 1e-12, (dh.value, dh_w)
+    assert abs(tot.value - plain) > 1e-2
+    host.cgppd_host_simulation_destroy(h)
+
+
+@pytest.mark.gpu
+def test_repulsive_crowding_argfile_key(host, tmp_path, gpu_engine_factory):
+    """The argfile key `repulsivecrowding 1` (the reference's compile-time
+    REPULSIVE_CROWDING, src/definitions.h:18) reaches the device: EonDevice
+    of a system with a crowders section equals the oracle's energy with
+    (6/r)^12 crowder pairs, and differs from the plain model."""
+    spec, n_nc, cfg = crowded_config(tmp_path)
+    h = create(host, ["cgppd", "-f", str(cfg)])
+    assert host.cgppd_host_setup_device(h, 0) == 0, host.cgppd_host_last_error()
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    assert host.cgppd_host_replica_EonDevice(h, 0, C.byref(a), C.byref(b), C.byref(c)) == 0, \
+        host.cgppd_host_last_error()
+    off, ty, pos = host_replica_system(host, h, spec)
+    args = (off, ty, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos, spec.box)
     plain = oracle.total(*oracle.energy_fp64(*args))
     try:
         oracle.set_repulsive_crowding(int(off[n_nc]))
@@ -265,4 +329,8 @@ def test_repulsive_crowding_argfile_key(host, tmp_path, gpu_engine_factory):
     # (force field through its text files: 1e-4)
     assert abs(a.value - want) <= 1e-4 * abs(want) + 1e-5, (a.value, want)
     assert abs(a.value - plain) > 1e-2, (a.value, plain)
+    # the CPU Replica::E follows the same switch (src/Replica.cpp:432-483)
+    lj, dh, tot = C.c_double(), C.c_double(), C.c_double()
+    host.cgppd_host_replica_E(h, 0, C.byref(lj), C.byref(dh), C.byref(tot))
+    assert abs(tot.value - want) <= 1e-4 * abs(want) + 1e-5, (tot.value, want)
     host.cgppd_host_simulation_destroy(h)
