@@ -75,8 +75,26 @@ def compact_chain(n: int, rng, bond: float = 3.8, min_dist: float = 4.0, rg76: f
     for the target radius of gyration; returns (n, 3) float64 centred at 0."""
     rg = rg76 * (n / 76.0) ** 0.38
     radius = max(rg * np.sqrt(5.0 / 3.0), bond * 1.5)
+    # Placed residues are binned in cubic cells wider than min_dist.  Every
+    # candidate lies within `bond` of the previous residue, so only residues
+    # within bond + min_dist of it can be closer than min_dist to one: the
+    # cells covering that cube hold all of them, and the test below on that
+    # subset takes the same decisions as the test against every residue.
+    cell = min_dist * 1.125
+    reach = int(np.ceil((bond + min_dist) / cell))
     for _attempt in range(200):
         pts = np.zeros((n, 3))
+        grid: dict = {}
+        cell_of = [None] * n
+
+        def place(i, p):
+            if cell_of[i] is not None:
+                grid[cell_of[i]].discard(i)
+            c = tuple(np.floor(p / cell).astype(np.int64))
+            grid.setdefault(c, set()).add(i)
+            cell_of[i] = c
+
+        place(0, pts[0])
         k = 1
         stuck = 0
         while k < n:
@@ -85,11 +103,16 @@ def compact_chain(n: int, rng, bond: float = 3.8, min_dist: float = 4.0, rg76: f
             cand = pts[k - 1] + bond * dirs
             ok = np.linalg.norm(cand, axis=1) < radius
             if k > 1:
-                d = np.linalg.norm(cand[:, None, :] - pts[None, : k - 1, :], axis=2)
-                ok &= (d >= min_dist).all(axis=1)
+                cx, cy, cz = np.floor(pts[k - 1] / cell).astype(np.int64)
+                near = [i for a in range(cx - reach, cx + reach + 1) for b in range(cy - reach, cy + reach + 1)
+                        for c in range(cz - reach, cz + reach + 1) for i in grid.get((a, b, c), ()) if i < k - 1]
+                if near:
+                    d = np.linalg.norm(cand[:, None, :] - pts[None, near, :], axis=2)
+                    ok &= (d >= min_dist).all(axis=1)
             idx = np.flatnonzero(ok)
             if idx.size:
                 pts[k] = cand[idx[0]]
+                place(k, pts[k])
                 k += 1
                 stuck = 0
             else:

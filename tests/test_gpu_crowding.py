@@ -99,3 +99,67 @@ def test_crowded_blocks_keep_totals_consistent(gpu_engine_factory, crowding, lay
         assert abs(full[r] - want) <= 1e-5 * abs(want) + 1e-6, r
     assert s["counters"].sum() == 300 * spec.n_rep
     e.close()
+
+
+def test_flexible_crowder_rejected_in_either_order(gpu_engine_factory):
+    """Repulsive crowding treats crowders as rigid bodies: a crowder made
+    flexible is refused whether its topology or the switch comes first; a
+    flexible non-crowder is accepted."""
+    from cgppd_amd import engine, synth
+    spec = synth.make_linked_system("crowdflex", 2, 30, 6, 0.2, 120.0, 2, 288.0, 450.0, 41, extra_rigid=(25, 40))
+    m0 = slice(int(spec.mol_off[0]), int(spec.mol_off[1]))
+    topo = (spec.chain_code[m0], spec.chain_index[m0], False, spec.segments[0])
+
+    def fresh(n_nc):
+        e = gpu_engine_factory()
+        e.set_force_field(spec.ff.eps, spec.ff.sigma, spec.ff.charge)
+        e.set_box(spec.box)
+        e.set_system(spec.mol_off, spec.types, n_nc)
+        e.set_torsion_table(spec.torsion_v, spec.torsion_delta)
+        return e
+
+    # molecule 0 (flexible) is a crowder when n_noncrowder = 0
+    e = fresh(0)
+    e.set_topology(0, *topo)
+    with pytest.raises(engine.CgppdError, match=r"\(-1\).*rigid crowders"):
+        e.set_repulsive_crowding(True)
+    e.close()
+    e = fresh(0)
+    e.set_repulsive_crowding(True)
+    with pytest.raises(engine.CgppdError, match=r"\(-1\).*rigid crowders"):
+        e.set_topology(0, *topo)
+    e.close()
+    # a flexible non-crowder with rigid crowders is fine in both orders
+    for first_topology in (True, False):
+        e = fresh(1)
+        if first_topology:
+            e.set_topology(0, *topo)
+        e.set_repulsive_crowding(True)
+        if not first_topology:
+            e.set_topology(0, *topo)
+        e.close()
+
+
+def test_crowding_switch_invalidates_mc_state(gpu_engine_factory, crowding):
+    """Toggling the switch after cgppd_mc_prepare would mix two energy
+    models in the rows and the matrix: MC calls then fail until the matrix
+    is rebuilt, after which the totals follow the new model."""
+    from cgppd_amd import engine
+    spec, n_nc = crowded_spec()
+    e = gpu_engine_factory()
+    e.load_spec(spec, seed=4, n_noncrowder=n_nc)
+    e.mc_prepare()
+    e.mc_steps(5)
+    e.set_repulsive_crowding(True)
+    with pytest.raises(engine.CgppdError, match=r"\(-4\)"):
+        e.mc_steps(1)
+    with pytest.raises(engine.CgppdError, match=r"\(-4\)"):
+        e.run(1, 10)
+    crowding(int(spec.mol_off[n_nc]))
+    e.mc_prepare()
+    e.mc_steps(20)
+    s = e.stats()
+    for r in range(spec.n_rep):
+        want = fp64_total(spec, e.coordinates(r)[0])
+        assert abs(s["energy"][r] - want) <= 1e-5 * abs(want) + 1e-6, r
+    e.close()
