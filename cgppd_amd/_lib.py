@@ -41,6 +41,7 @@ SIGNATURES = {
     "cgppd_energy_async": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_energy_wait": (C.c_int, [_ctxp, C.c_int, _f64p, _f64p, _f64p]),
     "cgppd_energy_all": (C.c_int, [_ctxp, _f64p, _f64p, _f64p, _f64p]),
+    "cgppd_energy_all_noalloc": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cgppd_mc_prepare": (C.c_int, [_ctxp]),
     "cgppd_mc_steps": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_exchange": (C.c_int, [_ctxp]),

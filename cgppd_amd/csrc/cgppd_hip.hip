@@ -29,6 +29,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2436,6 +2437,29 @@ __global__ __launch_bounds__(kWave) void exchange_kernel(ExchangeParams e)
     if (threadIdx.x == 0) *e.round = k + 1;
 }
 
+// Outputs of cgppd_energy_all_noalloc: per replica the raw LJ and DH sums
+// (acc[r], acc[R + r]) and their kcal/mol total, and, with mat non-null, the
+// lower triangle of every replica's molecule-pair matrix mirrored from the
+// upper one the full-energy kernel filled.
+__global__ void noalloc_energy_finish_kernel(const double *acc, int R, int M, float rt_to_kcal, float dh_factor,
+                                             double *lj, double *dh, double *total, double *mat)
+{
+    const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k < (size_t)R) {
+        const double l = acc[k], d = acc[R + k];
+        if (lj) lj[k] = l;
+        if (dh) dh[k] = d;
+        // src/Potential.cpp:219
+        if (total) total[k] = d * (double)dh_factor + l * (double)rt_to_kcal;
+    }
+    const size_t mm = (size_t)M * M;
+    if (mat && k < (size_t)R * mm) {
+        const size_t q = k % mm;
+        const int I = (int)(q / M), J = (int)(q % M);
+        if (I > J) mat[k] = mat[k - q + (size_t)J * M + I];
+    }
+}
+
 __global__ void fill_zero_kernel(double *p, size_t n)
 {
     for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
@@ -2805,6 +2829,46 @@ __global__ void mlp_pack_kernel(const float *master, unsigned short *wb)
     }
 }
 
+}  // namespace
+
+// ===========================================================================
+// Scratch of the allocation-free entry point
+// ===========================================================================
+// cgppd_energy_all_noalloc accumulates its per-replica raw LJ and DH sums in
+// this file-scope device array instead of context buffers: 2 doubles per
+// replica, at most CGPPD_NOALLOC_MAX_REPLICAS replicas (16 KiB per GPU,
+// held until the process ends).  Every call on one GPU shares the GPU's
+// copy, so calls that overlap on one GPU race.  Host code never passes the
+// array's name to a kernel (in host code it is a stand-in, not a device
+// address): it uses the address hipGetSymbolAddress gives for each device.
+#define CGPPD_NOALLOC_SCRATCH_DOUBLES (2 * CGPPD_NOALLOC_MAX_REPLICAS)
+static __device__ __align__(256) double scratch[CGPPD_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each device
+
+// Called by cgppd_create with `device` current: looks up this device's
+// address of the scratch array once.
+hipError_t resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+    return hipSuccess;
+}
+
+double *scratch_of(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
 }  // namespace
 
 // ===========================================================================
@@ -3494,13 +3558,18 @@ int check_ready(cgppd_ctx *ctx, bool need_rep)
     return CGPPD_OK;
 }
 
-int launch_full(cgppd_ctx *ctx, int r_begin, int r_count, int n_active, bool scaled, double *mat)
+// out_lj / out_dh: where the raw sums go (unscaled launches; null: the
+// context's output buffer)
+int launch_full(cgppd_ctx *ctx, int r_begin, int r_count, int n_active, bool scaled, double *mat,
+                double *out_lj = nullptr, double *out_dh = nullptr)
 {
     PairParams p = base_params(ctx);
     p.rep_begin = r_begin;
     p.rep_count = r_count;
     p.n_active = n_active;
     p.out_mat = mat;
+    if (out_lj) p.out_lj = out_lj;
+    if (out_dh) p.out_dh = out_dh;
     p.table = scaled ? ctx->d_table_scaled : ctx->d_table_raw;
     p.xcd_map = (r_begin == 0 && r_count % 8 == 0) ? 1 : 0;
     const long long items = (long long)r_count * n_active * p.ib_max * p.jc_count;
@@ -3716,6 +3785,12 @@ int cgppd_create(cgppd_ctx **out, int device)
     cgppd_ctx *ctx = new cgppd_ctx();
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return CGPPD_E_HIP;
+    }
+    // the allocation-free entry point's scratch address on this device
+    if (resolve_scratch(device) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return CGPPD_E_HIP;
     }
@@ -4249,6 +4324,36 @@ int cgppd_energy_all(cgppd_ctx *ctx, double *lj, double *dh, double *total, doub
             for (int I = 0; I < M; ++I)
                 for (int J = I + 1; J < M; ++J) m[(size_t)J * M + I] = m[(size_t)I * M + J];
         }
+    return CGPPD_OK;
+}
+
+int cgppd_energy_all_noalloc(cgppd_ctx *ctx, double *lj, double *dh, double *total, double *mat)
+{
+    // nothing here may allocate: no lazy topology upload (check_ready would
+    // do one), no context buffers created on first use, no host copies
+    if (!ctx) return CGPPD_E_ARG;
+    if (!ctx->have_ff || !ctx->have_box || !ctx->have_sys || !ctx->have_rep)
+        return fail(ctx, CGPPD_E_STATE, "force field, box, system and replicas must be set");
+    if (ctx->topo_dirty)
+        return fail(ctx, CGPPD_E_STATE, "topology changed since the replicas were set: call cgppd_mc_prepare or "
+                                        "cgppd_energy once first");
+    const int R = ctx->R, M = ctx->M;
+    if (R > CGPPD_NOALLOC_MAX_REPLICAS)
+        return fail(ctx, CGPPD_E_ARG, "cgppd_energy_all_noalloc takes at most %d replicas (context has %d)",
+                    CGPPD_NOALLOC_MAX_REPLICAS, R);
+    double *acc = scratch_of(ctx->device);
+    if (!acc) return fail(ctx, CGPPD_E_STATE, "no scratch address for device %d", ctx->device);
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    int rc;
+    // the scratch keeps the last call's sums, and the matrix is summed into
+    if ((rc = zero_dev(ctx, acc, (size_t)2 * R))) return rc;
+    if (mat && (rc = zero_dev(ctx, mat, (size_t)R * M * M))) return rc;
+    if ((rc = launch_full(ctx, 0, R, M, false, mat, acc, acc + R))) return rc;
+    if ((rc = launch_intra(ctx, 0, R, false, mat, acc, acc + R))) return rc;
+    const size_t n = std::max<size_t>((size_t)R, mat ? (size_t)R * M * M : 0);
+    hipLaunchKernelGGL(noalloc_energy_finish_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, acc,
+                       R, M, ctx->rt_to_kcal, ctx->dh_factor, lj, dh, total, mat);
+    CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
 

@@ -190,6 +190,23 @@ int cgppd_energy_wait(cgppd_ctx *ctx, int replica, double *lj, double *dh, doubl
  * (R*M*M, kcal/mol).  New: the reference evaluates one replica per call. */
 int cgppd_energy_all(cgppd_ctx *ctx, double *lj, double *dh, double *total, double *mat);
 
+/* Allocation-free variant of cgppd_energy_all.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing and copies nothing to the host: the
+ * full energies of all replicas are enqueued on the context stream and the
+ * call returns; lj, dh, total (R doubles each) and mat (R*M*M doubles, the
+ * symmetric molecule-pair matrices in kcal/mol) are DEVICE pointers owned
+ * by the caller, any of them NULL to skip; cgppd_synchronize waits.  The
+ * raw LJ/DH sums go through a scratch array of the library on each GPU
+ * (2 doubles per replica, at most CGPPD_NOALLOC_MAX_REPLICAS replicas):
+ * CGPPD_E_ARG beyond that, CGPPD_E_STATE when the context needs a set-up
+ * upload first (a topology set after the replicas: call cgppd_energy or
+ * cgppd_mc_prepare once).  Calls that overlap on one GPU (two contexts,
+ * two host threads) share that scratch and race. */
+#define CGPPD_NOALLOC_MAX_REPLICAS 1024
+int cgppd_energy_all_noalloc(cgppd_ctx *ctx, double *lj, double *dh, double *total, double *mat);
+
 /* Recompute the device molecule-pair energy matrix and running totals of
  * every replica from their coordinates.  Required once after
  * cgppd_set_replicas and before cgppd_mc_steps; may be called at any time
