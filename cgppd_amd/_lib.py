@@ -53,6 +53,8 @@ SIGNATURES = {
     "cgppd_get_energy_matrix": (C.c_int, [_ctxp, C.c_int, _f64p]),
     "cgppd_set_launch": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_set_pair_layout": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_set_replica_groups": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_get_replica_groups": (C.c_int, [_ctxp, _i32p]),
     "cgppd_autotune": (C.c_int, [_ctxp, C.c_int, _i32p, _i32p, _f64p]),
     "cgppd_get_pair_launch": (C.c_int, [_ctxp, _i32p, _i32p]),
     "cgppd_autotune_table": (C.c_int, [_ctxp, _f64p]),

@@ -264,6 +264,11 @@ struct PairParams {
     const unsigned *run_code;  // n_runs: the run's residue code (table row | charged flag)
     // replicas
     int rep_begin, rep_count;
+    // MC kernels of one replica group see its replicas as 0..rep_count-1
+    // (the per-replica pointers start at the group's first replica);
+    // rep_rng is that replica's index in the context, which keys the
+    // Philox streams so that the grouping leaves every trajectory unchanged
+    int rep_rng;
     int xcd_map;  // items of replica r stay on XCD r % 8
     const Coef *table;   // 20 x 20 entries, row = j type
     const Coef *table_scaled;  // the same in kcal/mol (finish kernels' bonded terms)
@@ -1706,7 +1711,7 @@ struct FlexPlan {
 
 __device__ __forceinline__ void plan_flexible(const PairParams &p, int r, int64_t step, int m, int kind, FlexPlan &fp)
 {
-    philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, kTagMC | 2u, (uint32_t)p.seed,
+    philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)(r + p.rep_rng), kTagMC | 2u, (uint32_t)p.seed,
            (uint32_t)(p.seed >> 32), fp.x);
     fp.pivot = -1;
     fp.b0 = fp.nb = 0;
@@ -1955,7 +1960,7 @@ __device__ void propose_flexible(const PairParams &p, int r, int64_t step, const
 __device__ void propose(const PairParams &p, int r, int64_t step, const FlexPlan *plan = nullptr,
                         const int *plan_br = nullptr, bool copied = false)
 {
-    const Proposal pr = draw_proposal(p.seed, r, step, p.n_mol);
+    const Proposal pr = draw_proposal(p.seed, r + p.rep_rng, step, p.n_mol);
     if (p.n_flex && p.mol_flex[pr.mol]) {
         const int kind = p.n_mol > 1 ? mulhi_n(pr.kind_word, 4) : 1 + mulhi_n(pr.kind_word, 3);
         if (kind >= 2) {
@@ -2026,7 +2031,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
     // the next proposal's molecule and move kind follow from its random
     // words alone (src/Molecule.cpp:451-466, src/Replica.cpp:309-334)
     const int64_t next = propose_only ? step : step + 1;
-    const Proposal nx = draw_proposal(p.seed, r, next, M);
+    const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, next, M);
     const int kind2 = p.n_flex && p.mol_flex[nx.mol] ? (M > 1 ? mulhi_n(nx.kind_word, 4) : 1 + mulhi_n(nx.kind_word, 3))
                                                      : 0;
     bool planned = false, copied = false;
@@ -2114,7 +2119,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
         if (t == 0) {
             double delta = 0.0;
             for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
-            const Proposal pr = draw_proposal(p.seed, r, step, M);
+            const Proposal pr = draw_proposal(p.seed, r + p.rep_rng, step, M);
             int acc = 0;
             if (p.da && !s1) {
                 // rejected at stage one: no exact row was evaluated
@@ -2219,7 +2224,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
     const int64_t step = p.steps[r];
     const int m = p.prop_mol[r];
     const int i_beg = p.mol_off[m], n_m = p.mol_off[m + 1] - i_beg;
-    const Proposal nx = draw_proposal(p.seed, r, step + 1, M);
+    const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, step + 1, M);
     const int m2 = nx.mol;
     const int j_beg = p.mol_off[m2], n_2 = p.mol_off[m2 + 1] - j_beg;
     double *row = p.rowbuf + (size_t)r * M;
@@ -2269,7 +2274,7 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
     // thread 0 records it
     double delta = 0.0;
     for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
-    const Proposal pr = draw_proposal(p.seed, r, step, M);
+    const Proposal pr = draw_proposal(p.seed, r + p.rep_rng, step, M);
     const int kind = delta < 0.0 ? 0 : pr.u < exp(-delta * 4184.0 / ((double)kRgas * temp)) ? 1 : 2;
     const int acc = kind < 2;
     if (t == 0) {
@@ -2803,8 +2808,8 @@ __device__ void stage_one(const PairParams &p, int r)
         for (int w = 0; w < mlp::kThreads / 64; ++w) dS += red[w];
         const uint64_t step = (uint64_t)p.steps[r];
         uint32_t w4[4];
-        philox((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)r, kTagDA, (uint32_t)p.seed, (uint32_t)(p.seed >> 32),
-               w4);
+        philox((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)(r + p.rep_rng), kTagDA, (uint32_t)p.seed,
+               (uint32_t)(p.seed >> 32), w4);
         const double u1 = unit53(w4[0], w4[1]);
         const double beta = 4184.0 / ((double)kRgas * p.temperature[r]);
         p.s1[r] = (dS < 0.0 || u1 < exp(-dS * beta)) ? 1 : 0;
@@ -2874,6 +2879,10 @@ double *scratch_of(int device)
 // ===========================================================================
 // Host side
 // ===========================================================================
+// At most this many replica groups (streams) per context: the box's GPUs
+// serve 4 hardware queues per process.
+constexpr int kMaxGroups = 4;
+
 struct cgppd_ctx {
     int device = 0;
     int n_cu = 256;
@@ -2969,6 +2978,14 @@ struct cgppd_ctx {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
+    int graph_groups = 0;
+
+    // replica groups: the MC steps of each group run on a stream of their
+    // own (group 0 on the context stream), so that one group's finish
+    // kernel and launch gaps overlap another group's pair kernel
+    int replica_groups = 0;  // caller's choice (0: automatic)
+    hipStream_t gstream[kMaxGroups - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxGroups - 1] = {};
 
     int wg_per_cu = 0;
     int pair_layout = CGPPD_LAYOUT_AUTO;  // caller's choice
@@ -3359,6 +3376,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.run_code = ctx->d_run_code;
     p.rep_begin = 0;
     p.rep_count = ctx->R;
+    p.rep_rng = 0;
     p.xcd_map = 0;
     p.table = ctx->d_table_raw;
     p.table_scaled = ctx->d_table_scaled;
@@ -3655,46 +3673,124 @@ size_t row_lds_bytes(int R)
     return sizeof(long long) * (size_t)((R + 2) & ~1) + (R <= kWave ? (size_t)R * (sizeof(int4) + sizeof(float4)) : 0);
 }
 
-int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr)
+// Number of replica groups the MC steps use: the caller's choice, else 1;
+// never more groups than replicas.
+int groups_of(const cgppd_ctx *ctx)
+{
+    const int g = ctx->replica_groups > 0 ? ctx->replica_groups : 1;
+    return std::max(1, std::min(std::min(g, kMaxGroups), ctx->R));
+}
+
+// Kernel parameters of replica group g of G: the replicas
+// [g R / G, (g + 1) R / G), seen by the MC kernels as 0..n-1 (every
+// per-replica pointer starts at the group's first replica).
+PairParams group_params(cgppd_ctx *ctx, int g, int G)
 {
     PairParams p = base_params(ctx);
+    if (G <= 1) return p;
+    const int R = ctx->R;
+    const int r0 = (int)((long long)g * R / G), r1 = (int)((long long)(g + 1) * R / G);
+    const size_t M = ctx->M, S = ctx->stride, mx = ctx->max_n, q = r0;
+    p.rep_count = r1 - r0;
+    p.rep_rng = r0;
+    p.pos += q * S;
+    p.rel += q * S;
+    p.center += q * M;
+    p.radius += q * M;
+    p.rowbuf += q * M;
+    p.rowfix += q * M;
+    p.done += q;
+    p.emat += q * M * M;
+    p.energy += q;
+    p.temperature += q;
+    p.tstep += q;
+    p.rcos += q;
+    p.rsin += q;
+    p.counters += 3 * q;
+    p.steps += q;
+    p.prop_pos += q * mx;
+    p.prop_rel += q * mx;
+    p.prop_center += q;
+    p.prop_radius += q;
+    p.prop_moved += q;
+    p.prop_mol += q;
+    p.prop_nib += q;
+    p.prop_n += q;
+    p.prop_items += q;
+    p.prop_hdr += q;
+    if (p.s1) p.s1 += q;
+    if (p.s_dS) p.s_dS += q;
+    if (p.smat) p.smat += q * M * M;
+    if (p.srow) p.srow += q * M;
+    if (p.frames) p.frames += q * M * 9;
+    if (p.prop_frame) p.prop_frame += q * 9;
+    if (p.s1_rej) p.s1_rej += q;
+    p.out_lj = p.out_dh = p.out_mat = p.out_int = nullptr;
+    return p;
+}
+
+hipStream_t group_stream(const cgppd_ctx *ctx, int g) { return g == 0 ? ctx->stream : ctx->gstream[g - 1]; }
+
+// Streams and fork/join events of G groups (created once, default priority).
+int ensure_group_streams(cgppd_ctx *ctx, int G)
+{
+    if (G > 1 && !ctx->ev_fork) CG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    for (int g = 1; g < G; ++g) {
+        if (!ctx->gstream[g - 1]) CG_HIP(ctx, hipStreamCreateWithFlags(&ctx->gstream[g - 1], hipStreamNonBlocking));
+        if (!ctx->ev_join[g - 1]) CG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join[g - 1], hipEventDisableTiming));
+    }
+    return CGPPD_OK;
+}
+
+int launch_pair_p(cgppd_ctx *ctx, PairParams p, hipStream_t stream, hipEvent_t ev_start = nullptr,
+                  hipEvent_t ev_stop = nullptr)
+{
     p.table = ctx->d_table_scaled;
     p.xcd_map = 1;
     const int layout = pair_layout_of(ctx);
     const int g = std::max(8, ctx->n_cu * pair_wg_per_cu(ctx, layout) / 8 * 8);
     if (ctx->n_flex) {
         if (ev_start)
-            hipExtLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+            hipExtLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, stream, ev_start, ev_stop, 0, p);
         else
-            hipLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+            hipLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, stream, p);
     } else if (layout == CGPPD_LAYOUT_ILANES) {
         if (ev_start)
-            hipExtLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, ev_start, ev_stop, 0, p);
+            hipExtLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, stream, ev_start, ev_stop, 0, p);
         else
-            hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, ctx->stream, p);
+            hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, stream, p);
     } else {
         const bool crowd = p.crowd_mol < p.n_mol;
         auto kern = layout == CGPPD_LAYOUT_JLANES4 ? (crowd ? mc_row_kernel<4, true> : mc_row_kernel<4, false>)
                                                    : (crowd ? mc_row_kernel<2, true> : mc_row_kernel<2, false>);
-        const size_t lds = row_lds_bytes(ctx->R);
+        const size_t lds = row_lds_bytes(p.rep_count);
         if (ev_start)
-            hipExtLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, ctx->stream, ev_start, ev_stop, 0, p);
+            hipExtLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, stream, ev_start, ev_stop, 0, p);
         else
-            hipLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, ctx->stream, p);
+            hipLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, stream, p);
     }
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+int launch_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr)
+{
+    return launch_pair_p(ctx, base_params(ctx), ctx->stream, ev_start, ev_stop);
+}
+
+int launch_finish_p(cgppd_ctx *ctx, const PairParams &p, hipStream_t stream, int propose_only)
+{
+    if (!propose_only && !ctx->n_flex && !ctx->da)
+        hipLaunchKernelGGL(mc_finish_rigid_kernel, dim3(p.rep_count), dim3(kWG), 0, stream, p);
+    else
+        hipLaunchKernelGGL(mc_finish_kernel, dim3(p.rep_count), dim3(kWG), 0, stream, p, propose_only);
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
 
 int launch_finish(cgppd_ctx *ctx, int propose_only)
 {
-    PairParams p = base_params(ctx);
-    if (!propose_only && !ctx->n_flex && !ctx->da)
-        hipLaunchKernelGGL(mc_finish_rigid_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p);
-    else
-        hipLaunchKernelGGL(mc_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, propose_only);
-    CG_HIP(ctx, hipGetLastError());
-    return CGPPD_OK;
+    return launch_finish_p(ctx, base_params(ctx), ctx->stream, propose_only);
 }
 
 int launch_stage1(cgppd_ctx *ctx)
@@ -3726,6 +3822,37 @@ int launch_mc_step(cgppd_ctx *ctx)
     int rc = launch_pair(ctx);
     if (rc) return rc;
     return launch_finish(ctx, 0);
+}
+
+// n_steps MC steps of every replica.  With G > 1 replica groups the context
+// stream forks into the groups' streams and joins them again, so the work
+// after the call (an exchange, a read-back) still follows every group's
+// steps; the same calls are recorded into a graph under stream capture.
+int launch_mc_steps(cgppd_ctx *ctx, int n_steps)
+{
+    const int G = groups_of(ctx);
+    int rc;
+    if (G == 1) {
+        for (int k = 0; k < n_steps; ++k)
+            if ((rc = launch_mc_step(ctx))) return rc;
+        return CGPPD_OK;
+    }
+    if ((rc = ensure_group_streams(ctx, G))) return rc;
+    CG_HIP(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+    for (int g = 1; g < G; ++g) CG_HIP(ctx, hipStreamWaitEvent(ctx->gstream[g - 1], ctx->ev_fork, 0));
+    for (int g = 0; g < G; ++g) {
+        const PairParams p = group_params(ctx, g, G);
+        const hipStream_t st = group_stream(ctx, g);
+        for (int k = 0; k < n_steps; ++k) {
+            if ((rc = launch_pair_p(ctx, p, st))) return rc;
+            if ((rc = launch_finish_p(ctx, p, st, 0))) return rc;
+        }
+    }
+    for (int g = 1; g < G; ++g) {
+        CG_HIP(ctx, hipEventRecord(ctx->ev_join[g - 1], ctx->gstream[g - 1]));
+        CG_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[g - 1], 0));
+    }
+    return CGPPD_OK;
 }
 
 int launch_exchange(cgppd_ctx *ctx)
@@ -3846,6 +3973,11 @@ int cgppd_destroy(cgppd_ctx *ctx)
     dev_free(ctx->d_tor);
     free_surrogate(ctx);
     for (hipEvent_t ev : ctx->events) (void)hipEventDestroy(ev);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (int g = 0; g + 1 < kMaxGroups; ++g) {
+        if (ctx->ev_join[g]) (void)hipEventDestroy(ctx->ev_join[g]);
+        if (ctx->gstream[g]) (void)hipStreamDestroy(ctx->gstream[g]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CGPPD_OK;
@@ -4411,9 +4543,7 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
         ctx->t_launches = n_steps;
         return CGPPD_OK;
     }
-    for (int k = 0; k < n_steps; ++k)
-        if ((rc = launch_mc_step(ctx))) return rc;
-    return CGPPD_OK;
+    return launch_mc_steps(ctx, n_steps);
 }
 
 int cgppd_exchange(cgppd_ctx *ctx)
@@ -4443,11 +4573,12 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
         }
         return CGPPD_OK;
     }
-    if (!ctx->graph_exec || ctx->graph_steps != steps_per_interval) {
+    const int G = groups_of(ctx);
+    if (!ctx->graph_exec || ctx->graph_steps != steps_per_interval || ctx->graph_groups != G) {
         destroy_graph(ctx);
+        if ((rc = ensure_group_streams(ctx, G))) return rc;
         CG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int lrc = CGPPD_OK;
-        for (int k = 0; k < steps_per_interval && !lrc; ++k) lrc = launch_mc_step(ctx);
+        int lrc = launch_mc_steps(ctx, steps_per_interval);
         if (!lrc) lrc = launch_exchange(ctx);
         if (!lrc) lrc = launch_stage1(ctx);  // exchanged temperatures change the stage-one test
         hipGraph_t g = nullptr;
@@ -4460,6 +4591,7 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
         ctx->graph = g;
         CG_HIP(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
         ctx->graph_steps = steps_per_interval;
+        ctx->graph_groups = G;
     }
     for (int k = 0; k < n_intervals; ++k) CG_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
     return CGPPD_OK;
@@ -4559,6 +4691,21 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu)
     if (!ctx || wg_per_cu < 0 || wg_per_cu > 64) return CGPPD_E_ARG;
     ctx->wg_per_cu = wg_per_cu;
     destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_set_replica_groups(cgppd_ctx *ctx, int groups)
+{
+    if (!ctx || groups < 0 || groups > kMaxGroups) return CGPPD_E_ARG;
+    ctx->replica_groups = groups;
+    destroy_graph(ctx);
+    return CGPPD_OK;
+}
+
+int cgppd_get_replica_groups(cgppd_ctx *ctx, int *groups)
+{
+    if (!ctx || !groups) return CGPPD_E_ARG;
+    *groups = groups_of(ctx);
     return CGPPD_OK;
 }
 

@@ -293,6 +293,15 @@ class Engine:
     def set_launch(self, wg_per_cu):
         self._check(self.lib.cgppd_set_launch(self.h, int(wg_per_cu)), "set_launch")
 
+    def set_replica_groups(self, groups):
+        """MC steps of `groups` contiguous replica groups on streams of their own."""
+        self._check(self.lib.cgppd_set_replica_groups(self.h, int(groups)), "set_replica_groups")
+
+    def replica_groups(self):
+        g = C.c_int()
+        self._check(self.lib.cgppd_get_replica_groups(self.h, C.byref(g)), "get_replica_groups")
+        return g.value
+
     LAYOUTS = {"auto": 0, "ilanes": 1, "jlanes": 2, "jlanes4": 3}
 
     def set_pair_layout(self, layout):

@@ -279,6 +279,18 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
 #define CGPPD_LAYOUT_JLANES 2
 #define CGPPD_LAYOUT_JLANES4 3
 int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout);
+/* Replica groups of the MC steps (default 1; at most 4).  New: the
+ * reference runs replicas on host threads, each waiting on its own GPU
+ * stream (src/Simulation.cpp:892-926, src/Replica.cpp:110); here the
+ * replicas of one context are split into `groups` contiguous groups whose
+ * MC steps (pair kernel, then finish kernel, per step) run on streams of
+ * their own, so that one group's finish kernel and launch gaps overlap
+ * another group's pair kernel.  Exchanges and read-backs wait for every
+ * group.  The trajectories do not depend on the grouping beyond the FP32
+ * summation order of the rows (the move streams are keyed by replica). */
+int cgppd_set_replica_groups(cgppd_ctx *ctx, int groups);
+int cgppd_get_replica_groups(cgppd_ctx *ctx, int *groups);
+
 /* The layout and workgroups per CU the next pair-kernel launch uses. */
 int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu);
 

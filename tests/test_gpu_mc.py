@@ -267,3 +267,39 @@ def test_autotune_keeps_state(gpu_engine_factory):
         np.testing.assert_array_equal(a.coordinates(r)[0], st.pos[r])
     a.close()
 
+
+
+@pytest.mark.parametrize("groups,layout", [(2, "jlanes"), (3, "ilanes"), (4, "jlanes4")])
+def test_replica_groups_match_oracle(gpu_engine_factory, groups, layout):
+    """Replica groups on streams of their own: each replica keeps the
+    oracle's trajectory (moves keyed by the replica's index in the
+    context), exchanges see every group's energies, and a replayed graph
+    (the groups' streams forked and joined inside it) equals direct launches."""
+    spec = small_spec(n_rep=9)
+    e, st = gpu_and_oracle(gpu_engine_factory, spec, 101)
+    e.set_pair_layout(layout)
+    e.set_replica_groups(groups)
+    assert e.replica_groups() == groups
+    b = gpu_engine_factory()
+    b.load_spec(spec, seed=101)
+    b.mc_prepare()
+    b.set_pair_layout(layout)
+    b.set_replica_groups(groups)
+    for k in range(4):
+        e.run(1, 40, use_graph=False)
+        b.run(1, 40, use_graph=True)
+        st.mc_steps(40)
+        st.exchange()
+        s = e.stats()
+        print(f"{groups} groups, interval {k}: gpu {s['counters'].sum(0)} oracle {st.counters.sum(0)}", flush=True)
+        np.testing.assert_array_equal(s["counters"], st.counters)
+        np.testing.assert_array_equal(s["temperature"], st.temperature)
+        np.testing.assert_allclose(s["energy"], st.energy, rtol=1e-5, atol=1e-8)
+    sb = b.stats()
+    for key in ("energy", "temperature", "label", "counters", "steps", "exchange"):
+        np.testing.assert_array_equal(s[key], sb[key])
+    for r in range(spec.n_rep):
+        np.testing.assert_array_equal(e.coordinates(r)[0], st.pos[r])
+        np.testing.assert_array_equal(b.coordinates(r)[0], st.pos[r])
+    e.close()
+    b.close()
