@@ -38,6 +38,7 @@ SIGNATURES = {
     "cgppd_update_molecule": (C.c_int, [_ctxp, C.c_int, C.c_int, _f32p]),
     "cgppd_energy": (C.c_int, [_ctxp, C.c_int, _f64p, _f64p, _f64p]),
     "cgppd_energy_nc": (C.c_int, [_ctxp, C.c_int, _f64p, _f64p, _f64p]),
+    "cgppd_energy_nc_all": (C.c_int, [_ctxp, _f64p]),
     "cgppd_energy_async": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_energy_wait": (C.c_int, [_ctxp, C.c_int, _f64p, _f64p, _f64p]),
     "cgppd_energy_all": (C.c_int, [_ctxp, _f64p, _f64p, _f64p, _f64p]),

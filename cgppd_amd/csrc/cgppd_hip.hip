@@ -4418,6 +4418,22 @@ int cgppd_energy_nc(cgppd_ctx *ctx, int replica, double *lj, double *dh, double 
     return energy_outputs(ctx, replica, lj, dh, total);
 }
 
+int cgppd_energy_nc_all(cgppd_ctx *ctx, double *total)
+{
+    int rc = check_ready(ctx, true);
+    if (rc) return rc;
+    if (!total) return fail(ctx, CGPPD_E_ARG, "null output");
+    const int R = ctx->R;
+    CG_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = zero_dev(ctx, ctx->d_out, (size_t)2 * R))) return rc;
+    if ((rc = launch_full(ctx, 0, R, ctx->n_nc, false, nullptr))) return rc;
+    CG_HIP(ctx, hipMemcpyAsync(ctx->h_out, ctx->d_out, sizeof(double) * 2 * R, hipMemcpyDeviceToHost, ctx->stream));
+    CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // src/Potential.cpp:219
+    for (int r = 0; r < R; ++r) total[r] = ctx->h_out[R + r] * (double)ctx->dh_factor + ctx->h_out[r] * (double)ctx->rt_to_kcal;
+    return CGPPD_OK;
+}
+
 int cgppd_energy_async(cgppd_ctx *ctx, int replica) { return energy_common(ctx, replica, ctx ? ctx->M : 0, true); }
 
 int cgppd_energy_wait(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total)

@@ -755,6 +755,7 @@ double Replica::SumGridResults()
 
 void Replica::fraction_bound(FILE *f)
 {
+    // src/Replica.cpp:1095-1107
     totalBoundSamples += boundSamples;
     totalSamples += samplesSinceLastExchange;
     fractionBound = float(boundSamples) / std::max(1.0f, float(samplesSinceLastExchange));
@@ -766,35 +767,62 @@ void Replica::fraction_bound(FILE *f)
 
 void Replica::acceptance_ratio(FILE *f)
 {
+    // src/Replica.cpp:1109-1122 (no guard against an empty interval: 0/0
+    // prints as nan there too)
     totalAccept += acceptA + accept;
     totalAcceptReject += acceptA + accept + reject;
-    acceptanceRatio = float(acceptA + accept) / std::max(1.0f, float(acceptA + accept + reject));
-    accumulativeAcceptanceRatio = float(totalAccept) / std::max(1.0f, float(totalAcceptReject));
+    acceptanceRatio = float(acceptA + accept) / float(acceptA + accept + reject);
+    accumulativeAcceptanceRatio = float(totalAccept) / float(totalAcceptReject);
     if (f) std::fprintf(f, "| %6.4f %6.4f ", acceptanceRatio, accumulativeAcceptanceRatio);
     acceptA = accept = reject = 0;
 }
 
-void Replica::saveAsSinglePDB(const char *filename, const char *prefix, int sample) const
+std::string Replica::sample(const char *prefix, int current_step, float boundEnergyThreshHold)
 {
-    // src/Replica.cpp:1124-1168
+    // src/Replica.cpp:1037-1093; the reference records every sample
+    // (bool_recordAllSamples, src/definitions.h:36)
+    ++samplesSinceLastExchange;
+    if (nonCrowderPotential < boundEnergyThreshHold) ++boundSamples;
+    char savename[512];
+    std::snprintf(savename, sizeof(savename), "output/%s/pdb/sample_%d_%0.1fK_%5.2f.pdb", prefix, current_step,
+                  temperature, nonCrowderPotential);
+    saveAsSinglePDB(savename, prefix, current_step);
+    char record[1024];
+    std::snprintf(record, sizeof(record), "%d; %0.5f (%0.5f); %0.1f\n%s\n", current_step, nonCrowderPotential,
+                  potential, temperature, savename);
+    return record;
+}
+
+void Replica::saveAsSinglePDB(const char *filename, const char *prefix, int sample, bool skip_crowders) const
+{
+    // src/Replica.cpp:1124-1168: chain identifiers from the residues' chain
+    // UIDs ('@' + UID), a TER record where the chain changes and after
+    // every molecule
     FILE *out = std::fopen(filename, "w");
     if (!out) throw error(std::string("cannot write ") + filename);
     std::fprintf(out, "REMARK simulation: %s \n", prefix);
     std::fprintf(out, "REMARK temperature: %5.1f \n", temperature);
     std::fprintf(out, "REMARK sample: %d \n", sample);
     std::fprintf(out, "REMARK potential: %0.10f \n", float(potential));
-    int item = 0;
-    for (int m = 0; m < nonCrowderCount; ++m) {
+    int item = 0, last_seq = 0;
+    char chain = 0, last_chain = 0;
+    for (int m = 0; m < moleculeCount; ++m) {
+        if (skip_crowders && m >= nonCrowderCount) break;
         const Molecule &mol = molecules[m];
-        const char chain = (char)('A' + m);
-        for (const Residue &r : mol.Residues) {
+        std::string last_name;
+        for (size_t i = 0; i < mol.Residues.size(); ++i) {
+            const Residue &r = mol.Residues[i];
+            chain = (char)(64 + r.chain_UID);
+            if (i > 0 && chain != last_chain)
+                std::fprintf(out, "TER   %5d      %3s %c%4d \n", item, last_name.c_str(), last_chain, last_seq);
             ++item;
-            std::fprintf(out, "ATOM  %5d CA   %3s %c%4d    %8.3f%8.3f%8.3f%6.2f%6.2f\n", item,
-                         aminoAcids.get(r.aminoAcidIndex).sname.c_str(), chain, r.resSeq, r.position.x, r.position.y,
-                         r.position.z, 0.0f, 0.0f);
+            last_name = aminoAcids.get(r.aminoAcidIndex).getSNAME();
+            std::fprintf(out, "ATOM  %5d CA   %3s %c%4d    %8.3f%8.3f%8.3f%6.2f%6.2f\n", item, last_name.c_str(),
+                         chain, r.resSeq, r.position.x, r.position.y, r.position.z, 0.0f, 0.0f);
+            last_seq = r.resSeq;
+            last_chain = chain;
         }
-        std::fprintf(out, "TER   %5d      %3s %c%4d \n", item,
-                     aminoAcids.get(mol.Residues.back().aminoAcidIndex).sname.c_str(), chain, mol.Residues.back().resSeq);
+        std::fprintf(out, "TER   %5d      %3s %c%4d \n", item, last_name.c_str(), chain, last_seq);
     }
     std::fprintf(out, "END \n");
     std::fclose(out);
@@ -814,7 +842,7 @@ void Simulation::printHelp() const
         "\t-s|--streams n (accepted, unused)\n\t-g|--gpus n (one GPU)\n\t-z|--gpuoffset n (device id)\n"
         "\t-m|--mcsteps n\n\t-a|--sampleafter n\n\t-e|--resteps n\n\t-r|--replicas n\n\t-o|--output prefix\n"
         "\t-b|--boundary L\n\t-x|--tmax T\n\t-n|--tmin T\n\t-d|--blockdim n (accepted, unused)\n"
-        "\t-S|--seed n (device Philox seed)\n\t-G|--nograph (launch intervals without a hipGraph)\n");
+        "\t-S|--seed n (device Philox seed)\n\t-G|--nograph (accepted, unused)\n");
 }
 
 void Simulation::getArgs(int argc, char **argv, bool first_pass)
@@ -1070,23 +1098,44 @@ void Simulation::setup_device(int device)
 
 void Simulation::initSamplingFiles()
 {
+    // src/Simulation.cpp:1100-1135: output/<prefix>/ with its four files
+    // (appended to when present) and the sample PDBs under pdb/
     if (parameters.prefix.empty()) parameters.prefix = "cgppd_" + std::to_string(parameters.pid);
     const std::string dir = "output/" + parameters.prefix;
     ::mkdir("output", 0755);
     ::mkdir(dir.c_str(), 0755);
-    fractionBoundFile = std::fopen((dir + "/fractionBound").c_str(), "w");
-    acceptanceRatioFile = std::fopen((dir + "/acceptance_ratios").c_str(), "w");
-    exchangeFrequencyFile = std::fopen((dir + "/exchange_freq").c_str(), "w");
-    boundConformationsFile = std::fopen((dir + "/boundconformations").c_str(), "w");
-    if (!fractionBoundFile || !acceptanceRatioFile || !exchangeFrequencyFile || !boundConformationsFile)
-        throw error("cannot create output files under " + dir);
-    // src/Simulation.cpp:1114-1194 headers
-    for (FILE *f : {fractionBoundFile, acceptanceRatioFile}) {
-        std::fprintf(f, "# Temperature: ");
-        for (auto &t : temperature) std::fprintf(f, "%0.1fKi ", t.second);
-        std::fprintf(f, "\n");
+    ::mkdir((dir + "/pdb").c_str(), 0755);
+    auto open = [&](const char *name) {
+        const std::string path = dir + "/" + name;
+        FILE *f = std::fopen(path.c_str(), "a+");
+        if (!f) f = std::fopen(path.c_str(), "wt");
+        if (!f) throw error("Could not open / create sampling file " + path);
+        return f;
+    };
+    fractionBoundFile = open("fractionBound");
+    boundConformationsFile = open("boundconformations");
+    acceptanceRatioFile = open("acceptance_ratios");
+    exchangeFrequencyFile = open("exchange_freq");
+}
+
+void Simulation::writeSamplingFileHeaders()
+{
+    // src/Simulation.cpp:1137-1162
+    std::fprintf(fractionBoundFile, "# Fraction Bound\n#Iteration InstantaneousAve CumulativeAve\n");
+    std::fprintf(boundConformationsFile, "# iteration; complex free energy (replica free energy); temperature;\n# "
+                                         "molecule: pdb file; 1 line per other molecule in the bound state\n");
+    std::fprintf(acceptanceRatioFile, "# Iteration AcceptanceRatio\n");
+    std::fprintf(exchangeFrequencyFile, "# Iteration continous_ratio, inst_ratio\n");
+    std::fprintf(fractionBoundFile, "Iteration:  ");
+    std::fprintf(acceptanceRatioFile, "Iteration:  ");
+    for (const auto &t : position_of_temperature) {
+        std::fprintf(fractionBoundFile, "%0.1fKi %0.1fKc ", t.first, t.first);
+        std::fprintf(acceptanceRatioFile, "%0.1fKi %0.1fKc ", t.first, t.first);
     }
-    std::fprintf(exchangeFrequencyFile, "# step exchange_fraction accumulative_fraction\n");
+    std::fprintf(fractionBoundFile, " \n");
+    std::fprintf(acceptanceRatioFile, "\n");
+    for (FILE *f : {fractionBoundFile, boundConformationsFile, acceptanceRatioFile, exchangeFrequencyFile})
+        std::fflush(f);
 }
 
 // src/Simulation.cpp:1181-1194: the molecule files of the run, in order
@@ -1145,51 +1194,160 @@ void Simulation::sync_from_device()
     totalExchanges = (int)ex[1];
 }
 
+void Simulation::sync_counters()
+{
+    // the device counts accept / acceptA / reject per replica since the
+    // start (swapped with the temperatures on exchange); the replicas hold
+    // the counts since their last acceptance_ratio line, as in the reference
+    const int R = (int)replica.size();
+    std::vector<int64_t> cnt((size_t)3 * R);
+    dev_->check(cgppd_get_replica_stats(dev_->ctx(), nullptr, nullptr, nullptr, cnt.data(), nullptr, nullptr),
+                "cgppd_get_replica_stats");
+    if (seen_counters_.size() != cnt.size()) seen_counters_.assign(cnt.size(), 0);
+    for (int r = 0; r < R; ++r) {
+        replica[r].accept += (long)(cnt[3 * r] - seen_counters_[3 * r]);
+        replica[r].acceptA += (long)(cnt[3 * r + 1] - seen_counters_[3 * r + 1]);
+        replica[r].reject += (long)(cnt[3 * r + 2] - seen_counters_[3 * r + 2]);
+    }
+    seen_counters_ = cnt;
+}
+
+void Simulation::sample_block(int step)
+{
+    // Replica::sample of every replica (src/Replica.cpp:1037-1093): the
+    // non-crowder energy is 0 for a lone molecule, EonDeviceNC with
+    // crowders or flexible molecules (unless rigid-only), else the
+    // replica's running energy
+    const int R = (int)replica.size(), M = initialReplica.moleculeCount, N = initialReplica.residueCount;
+    std::vector<double> e(R), nc(R);
+    dev_->check(cgppd_get_replica_stats(dev_->ctx(), e.data(), nullptr, nullptr, nullptr, nullptr, nullptr),
+                "cgppd_get_replica_stats");
+    const bool use_nc = M != initialReplica.nonCrowderCount ||
+                        (initialReplica.has_flexible() && !initialReplica.calculate_rigid_potential_only);
+    if (M > 1 && use_nc) dev_->check(cgppd_energy_nc_all(dev_->ctx(), nc.data()), "cgppd_energy_nc_all");
+    std::vector<float> pos((size_t)N * 3);
+    for (int r = 0; r < R; ++r) {
+        Replica &rp = replica[r];
+        rp.potential = e[r];
+        rp.nonCrowderPotential = M == 1 ? 0.0f : (float)(use_nc ? nc[r] : e[r]);
+        dev_->check(cgppd_get_coordinates(dev_->ctx(), r, pos.data(), nullptr, nullptr), "cgppd_get_coordinates");
+        int k = 0;
+        for (Molecule &mol : rp.molecules)
+            for (Residue &res : mol.Residues) {
+                res.position = Vector3f{pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
+                ++k;
+            }
+        const std::string record = rp.sample(parameters.prefix.c_str(), step, BOUND_ENERGY_VALUE);
+        std::fputs(record.c_str(), boundConformationsFile);
+    }
+    std::fflush(boundConformationsFile);
+}
+
+void Simulation::print_interval_statistics(int iteration)
+{
+    // src/Simulation.cpp:749-768 and 837-850: one column pair per
+    // temperature, ascending
+    sync_counters();
+    std::fprintf(fractionBoundFile, "%9d: ", iteration);
+    std::fprintf(acceptanceRatioFile, "%9d: ", iteration);
+    for (const auto &t : temperature) {
+        Replica &rp = replica[position_of_temperature[t.second]];
+        rp.fraction_bound(fractionBoundFile);
+        rp.acceptance_ratio(acceptanceRatioFile);
+    }
+    std::fprintf(fractionBoundFile, "\n");
+    std::fflush(fractionBoundFile);
+    std::fprintf(acceptanceRatioFile, "\n");
+    std::fflush(acceptanceRatioFile);
+}
+
+void Simulation::exchange_on_device()
+{
+    // one exchange round on the device (src/Simulation.cpp:770-795); the
+    // replicas whose temperatures it swapped swap their bookkeeping on the
+    // host as Replica::exchangeReplicas does (src/Replica.cpp:188-210)
+    const int R = (int)replica.size();
+    sync_counters();
+    std::vector<int64_t> before(2), after(2);
+    dev_->check(cgppd_get_replica_stats(dev_->ctx(), nullptr, nullptr, nullptr, nullptr, nullptr, before.data()),
+                "cgppd_get_replica_stats");
+    dev_->check(cgppd_exchange(dev_->ctx()), "cgppd_exchange");
+    std::vector<double> t(R);
+    dev_->check(cgppd_get_replica_stats(dev_->ctx(), nullptr, t.data(), nullptr, nullptr, nullptr, after.data()),
+                "cgppd_get_replica_stats");
+    for (int i = 0; i < R; ++i) {
+        if ((float)t[i] == replica[i].temperature) continue;
+        for (int j = i + 1; j < R; ++j)
+            if ((float)t[j] == replica[i].temperature && (float)t[i] == replica[j].temperature) {
+                replica[i].exchangeReplicas(replica[j]);
+                for (int c = 0; c < 3; ++c) std::swap(seen_counters_[3 * i + c], seen_counters_[3 * j + c]);
+                position_of_temperature[replica[i].temperature] = i;
+                position_of_temperature[replica[j].temperature] = j;
+                break;
+            }
+    }
+    tests = (int)(after[0] - before[0]);
+    exchanges = (int)(after[1] - before[1]);
+}
+
+void Simulation::exchange_frequency()
+{
+    // src/Simulation.cpp:876-888
+    totalExchanges += exchanges;
+    totalTests += tests;
+    exchangeFraction = float(exchanges) / float(tests);
+    accumulativeExchangeFraction = float(totalExchanges) / float(totalTests);
+    std::fprintf(exchangeFrequencyFile, "%10d %6.4f %6.4f\n", steps, accumulativeExchangeFraction, exchangeFraction);
+    std::fflush(exchangeFrequencyFile);
+    tests = 0;
+    exchanges = 0;
+}
+
 void Simulation::run()
 {
+    // src/Simulation.cpp:709-873 with the MC loop of the reference's default
+    // GPU build (CUDA with streams, src/Simulation.cpp:1041-1094): every
+    // replica takes MCsteps / REsteps steps per exchange interval; after a
+    // step t with t % samplefrequency == samplefrequency - 1 and
+    // t >= sampleafter every replica is sampled, labelled
+    // t - samplefrequency + 1; between intervals the fraction-bound and
+    // acceptance lines are written and one exchange round is attempted
+    // (REsteps - 1 of them); the last lines are written at MCsteps.  The MC
+    // steps and the exchanges run on the device.
     if (!dev_) setup_device(parameters.gpuoffset);
     initSamplingFiles();
     writeFileIndex();
+    writeSamplingFileHeaders();
     dev_->check(cgppd_mc_prepare(dev_->ctx()), "cgppd_mc_prepare");
-    const int per = std::max(1, parameters.MCsteps / parameters.REsteps);
     const int R = (int)replica.size();
-    int prev_tests = 0, prev_ex = 0;
-    // src/Simulation.cpp:737: REsteps - 1 exchange rounds, then the final
-    // MC interval of the threads
-    for (steps = 1; steps <= parameters.REsteps; ++steps) {
-        const bool exchange = steps < parameters.REsteps;
-        if (exchange) {
-            dev_->check(cgppd_run(dev_->ctx(), 1, per, parameters.use_graph ? 1 : 0), "cgppd_run");
-        } else {
-            dev_->check(cgppd_mc_steps(dev_->ctx(), per), "cgppd_mc_steps");
+    const int per = parameters.MCsteps / std::max(1, parameters.REsteps);
+    const int sf = parameters.sampleFrequency;
+    seen_counters_.assign((size_t)3 * R, 0);
+    totalExchanges = totalTests = exchanges = tests = 0;
+    long mcstep = 0;
+    steps = 0;
+    for (int interval = 0; interval < parameters.REsteps; ++interval) {
+        const long end = mcstep + per;
+        while (mcstep < end) {
+            // the next step after which the replicas are sampled
+            const long next = sf > 0 ? mcstep + ((sf - 1 - mcstep % sf) % sf) : end;
+            if (next >= end) {
+                dev_->check(cgppd_mc_steps(dev_->ctx(), (int)(end - mcstep)), "cgppd_mc_steps");
+                mcstep = end;
+                break;
+            }
+            dev_->check(cgppd_mc_steps(dev_->ctx(), (int)(next + 1 - mcstep)), "cgppd_mc_steps");
+            mcstep = next + 1;
+            if (next >= parameters.sampleStartsAfter) sample_block((int)(next - sf + 1));
         }
-        // interval statistics (src/Simulation.cpp:749-768, 876-890)
-        std::vector<double> e(R), t(R);
-        std::vector<int> lab(R);
-        std::vector<int64_t> cnt((size_t)3 * R), st(R), ex(2);
-        dev_->check(cgppd_get_replica_stats(dev_->ctx(), e.data(), t.data(), lab.data(), cnt.data(), st.data(), ex.data()),
-                    "cgppd_get_replica_stats");
-        const int step_no = per * steps;
-        std::fprintf(acceptanceRatioFile, "%9d: ", step_no);
-        std::vector<int> order(R);
-        for (int r = 0; r < R; ++r) order[r] = r;
-        std::sort(order.begin(), order.end(), [&](int a, int b) { return t[a] < t[b]; });
-        for (int r : order) {
-            const int64_t a = cnt[3 * r] + cnt[3 * r + 1], all = a + cnt[3 * r + 2];
-            std::fprintf(acceptanceRatioFile, "| %6.4f ", all ? double(a) / double(all) : 0.0);
-            std::fprintf(boundConformationsFile, "%d; %0.5f (%0.5f); %0.1f\n", step_no, e[r], e[r], t[r]);
+        if (interval + 1 < parameters.REsteps) {
+            ++steps;
+            if (per * (steps - 1) >= parameters.sampleStartsAfter) print_interval_statistics(per * steps);
+            exchange_on_device();
+            exchange_frequency();
         }
-        std::fprintf(acceptanceRatioFile, "\n");
-        exchanges = (int)ex[1] - prev_ex;
-        tests = (int)ex[0] - prev_tests;
-        prev_ex = (int)ex[1];
-        prev_tests = (int)ex[0];
-        exchangeFraction = tests ? float(exchanges) / float(tests) : 0.f;
-        accumulativeExchangeFraction = ex[0] ? float(ex[1]) / float(ex[0]) : 0.f;
-        if (exchange)
-            std::fprintf(exchangeFrequencyFile, "%10d %6.4f %6.4f\n", step_no, exchangeFraction,
-                         accumulativeExchangeFraction);
     }
+    print_interval_statistics(parameters.MCsteps);
     sync_from_device();
     closeSamplingFiles();
 }

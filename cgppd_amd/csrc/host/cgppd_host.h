@@ -15,6 +15,7 @@
 #ifndef CGPPD_HOST_H
 #define CGPPD_HOST_H
 
+#include <cctype>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -54,6 +55,13 @@ struct AminoAcid {
     float vanderWaalRadius = 0.f;
     float electrostaticCharge = 0.f;
     int index = 0;
+    // src/AminoAcid.cpp:19-25: the three-letter name in upper case
+    std::string getSNAME() const
+    {
+        std::string u = sname;
+        for (char &c : u) c = (char)std::toupper((unsigned char)c);
+        return u;
+    }
 };
 
 class AminoAcids {
@@ -260,7 +268,7 @@ public:
     float acceptanceRatio = 0.f, accumulativeAcceptanceRatio = 0.f;
     int boundSamples = 0, samplesSinceLastExchange = 0, totalBoundSamples = 0, totalSamples = 0;
     float fractionBound = 0.f, accumulativeFractionBound = 0.f;
-    double nonCrowderPotential = 0.0;
+    float nonCrowderPotential = 0.0f;  // src/Replica.h:106
     // the reference's compile-time REPULSIVE_CROWDING as a run-time flag:
     // crowder pairs contribute (6/r)^12 below 15 A and no Debye-Hueckel term
     bool repulsiveCrowding = false;
@@ -299,10 +307,19 @@ public:
     std::vector<float> relative_positions() const;
     std::vector<float> centres() const;
 
+    // src/Replica.cpp:1037-1093: one sample of this replica at
+    // current_step, its non-crowder energy already in nonCrowderPotential
+    // (the Simulation evaluates every replica's at once, cgppd_energy_nc_all);
+    // counts it (bound below boundEnergyThreshHold) and, as the reference
+    // records every sample, writes its PDB under output/<prefix>/pdb and
+    // returns the boundconformations record.
+    std::string sample(const char *prefix, int current_step, float boundEnergyThreshHold);
     // src/Replica.cpp:1095-1122
     void fraction_bound(FILE *f);
     void acceptance_ratio(FILE *f);
-    void saveAsSinglePDB(const char *filename, const char *prefix, int sample) const;
+    // src/Replica.cpp:1124-1168
+    void saveAsSinglePDB(const char *filename, const char *prefix, int sample, bool skip_crowders = true) const;
+    bool flexible_molecule_exists() const { return has_flexible(); }
 
 private:
     DeviceContext *dev_ = nullptr;
@@ -343,9 +360,18 @@ private:
     FILE *fractionBoundFile = nullptr, *acceptanceRatioFile = nullptr, *exchangeFrequencyFile = nullptr,
          *boundConformationsFile = nullptr;
     void initSamplingFiles();
+    void writeSamplingFileHeaders();
     void closeSamplingFiles();
     void writeFileIndex();
     void sync_from_device();
+    // the device's move counters at the last sync, per replica (they travel
+    // with the temperatures on exchange, as the replicas' counters do)
+    std::vector<int64_t> seen_counters_;
+    void sync_counters();
+    void sample_block(int step);
+    void print_interval_statistics(int iteration);
+    void exchange_on_device();
+    void exchange_frequency();
 };
 
 }  // namespace cgppd

@@ -146,6 +146,11 @@ class Engine:
     def energy_nc(self, r=0):
         return self._energy3(self.lib.cgppd_energy_nc, r, "energy_nc")
 
+    def energy_nc_all(self):
+        out = np.zeros(self.n_rep)
+        self._check(self.lib.cgppd_energy_nc_all(self.h, ptr(out, C.c_double)), "energy_nc_all")
+        return out
+
     def energy_async(self, r=0):
         self._check(self.lib.cgppd_energy_async(self.h, int(r)), "energy_async")
 

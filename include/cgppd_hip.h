@@ -178,6 +178,12 @@ int cgppd_energy(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *to
  * (src/Replica.cpp:920-935). */
 int cgppd_energy_nc(cgppd_ctx *ctx, int replica, double *lj, double *dh, double *total);
 
+/* The same for every replica in one launch: total[R] in kcal/mol.  New:
+ * Replica::sample calls EonDeviceNC per replica and sample
+ * (src/Replica.cpp:1037-1093); the C++ Simulation samples all replicas of a
+ * block at once. */
+int cgppd_energy_nc_all(cgppd_ctx *ctx, double *total);
+
 /* Asynchronous variant: enqueue on the context stream, then wait.  Replaces
  * CUDA_EonDevice_async (src/cudaExterns.h:42) + the D2H copy in
  * Replica::EonDeviceAsync (src/Replica.cpp:883-901) and
