@@ -17,6 +17,8 @@ Parity pins (see tests/test_oracle.py):
                        small state spaces equal to their Boltzmann weights
   orc_remc_exchange    acceptance frequency against min(1, exp(delta)),
                        swaps temperatures only
+  orc_remc_mc_steps_fast  energies against orc_energy_fp64; the decisions of
+                       orc_remc_mc_steps on the same streams
   flexible linkers     topology (oracle/topology.py), bonded and intra-molecular
                        energies against oracle/_ref/ref_flex_driver and
                        tests/testMolecule.cpp / tests/testReplica.cpp values;
@@ -93,6 +95,10 @@ def lib():
         L.orc_remc_exchange.argtypes = [C.POINTER(RemcState)]
         L.orc_remc_reference_run.argtypes = [C.POINTER(RemcState), C.c_int, C.c_int]
         L.orc_remc_reference_run.restype = C.c_double
+        L.orc_remc_refresh_fast.argtypes = [C.POINTER(RemcState)]
+        L.orc_remc_mc_steps_fast.argtypes = [C.POINTER(RemcState), C.c_int, C.c_int]
+        L.orc_energy_fast.argtypes = [C.POINTER(RemcState), C.c_int]
+        L.orc_energy_fast.restype = C.c_double
         L.orc_metropolis.argtypes = [C.c_double, C.c_double, C.c_double]
         L.orc_metropolis.restype = C.c_int
         L.orc_metropolis_ref.argtypes = [C.c_float, C.c_float, C.c_double]
@@ -402,6 +408,15 @@ class Remc:
 
     def exchange(self):
         lib().orc_remc_exchange(C.byref(self.st))
+
+    def refresh_fast(self):
+        lib().orc_remc_refresh_fast(C.byref(self.st))
+
+    def mc_steps_fast(self, n, threads=1):
+        lib().orc_remc_mc_steps_fast(C.byref(self.st), int(n), int(threads))
+
+    def energy_fast(self, r):
+        return lib().orc_energy_fast(C.byref(self.st), int(r))
 
     def reference_run(self, n, threads):
         return lib().orc_remc_reference_run(C.byref(self.st), int(n), int(threads))

@@ -899,6 +899,208 @@ void orc_remc_exchange(orc_remc *s)
 }
 
 /* ------------------------------------------------------------------ */
+/* Lean FP64 sampler for long CPU REMC runs                             */
+/* ------------------------------------------------------------------ */
+/* The Markov chain of orc_remc_mc_steps (same proposals, moves and
+ * orc_metropolis decisions, delta-E rows against the molecule-pair
+ * matrix) with the pair term in FP64 written for speed: a table per type
+ * pair of -4E, sigma^2, the squared repulsive cutoff (or -1) and 2E,
+ * (sigma/r)^6 from sigma^2 / r^2, the minimum image by rounding, and the
+ * Debye-Hueckel term only over the charged residues.  Rigid molecules,
+ * crowding off.  Pinned against orc_energy_fp64. */
+typedef struct {
+    double a[400], s2[400], rc2[400], e2[400];
+    int *charged;     /* the charged residues, ascending */
+    int *charged_at;  /* n_res + 1: charged residues before each residue */
+} fast_tab;
+
+static void fast_table(const orc_remc *s, fast_tab *t)
+{
+    const double r0 = pow(2.0, 1.0 / 6.0);
+    for (int ti = 0; ti < 20; ++ti)
+        for (int tj = 0; tj < 20; ++tj) {
+            const int k = ti * 20 + tj;
+            const double E = (double)ORC_LAMBDA * ((double)s->eps[k] - (double)ORC_E0);
+            const double sij = 0.5 * ((double)s->sigma[ti] + (double)s->sigma[tj]);
+            t->a[k] = -4.0 * E;
+            t->s2[k] = sij * sij;
+            t->rc2[k] = E > 0.0 ? (sij * r0) * (sij * r0) : -1.0;
+            t->e2[k] = 2.0 * E;
+        }
+    t->charged = (int *)malloc(sizeof(int) * (s->n_res + 1));
+    t->charged_at = (int *)malloc(sizeof(int) * (s->n_res + 1));
+    int c = 0;
+    for (int i = 0; i < s->n_res; ++i) {
+        t->charged_at[i] = c;
+        if (s->charge[s->type[i]] != 0.0f) t->charged[c++] = i;
+    }
+    t->charged_at[s->n_res] = c;
+}
+
+static void fast_table_free(fast_tab *t)
+{
+    free(t->charged);
+    free(t->charged_at);
+}
+
+/* round() of the small multiples of the box that occur here, by
+ * truncation (no library call) */
+static inline double fast_round(double x) { return (double)(long long)(x + (x >= 0.0 ? 0.5 : -0.5)); }
+
+/* kcal/mol of the residues [i0, i1) against [j0, j1) */
+static double fast_block(const orc_remc *s, const fast_tab *t, int i0, int i1, int j0, int j1, const float *pos)
+{
+    const double L = (double)s->box, iL = 1.0 / L;
+    double lj = 0.0, dh = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+        const int row = s->type[i] * 20;
+        for (int j = j0; j < j1; ++j) {
+            double dx = xi - (double)pos[3 * j], dy = yi - (double)pos[3 * j + 1], dz = zi - (double)pos[3 * j + 2];
+            dx -= L * fast_round(dx * iL);
+            dy -= L * fast_round(dy * iL);
+            dz -= L * fast_round(dz * iL);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const int k = row + s->type[j];
+            const double q = t->s2[k] / r2, s6 = q * q * q;
+            const double u = t->a[k] * s6 * (s6 - 1.0);
+            lj += r2 < t->rc2[k] ? t->e2[k] - u : u;
+        }
+    }
+    /* Debye-Hueckel: charged pairs only */
+    for (int a = t->charged_at[i0]; a < t->charged_at[i1]; ++a) {
+        const int i = t->charged[a];
+        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+        const double qi = (double)s->charge[s->type[i]];
+        for (int b = t->charged_at[j0]; b < t->charged_at[j1]; ++b) {
+            const int j = t->charged[b];
+            double dx = xi - (double)pos[3 * j], dy = yi - (double)pos[3 * j + 1], dz = zi - (double)pos[3 * j + 2];
+            dx -= L * fast_round(dx * iL);
+            dy -= L * fast_round(dy * iL);
+            dz -= L * fast_round(dz * iL);
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            dh += qi * (double)s->charge[s->type[j]] * exp(-r / (double)ORC_XI) / r;
+        }
+    }
+    return pair_total(lj, dh);
+}
+
+double orc_energy_fast(const orc_remc *s, int r)
+{
+    fast_tab t;
+    fast_table(s, &t);
+    const int M = s->n_mol;
+    const float *pos = s->pos + (size_t)r * s->n_res * 3;
+    double e = 0.0;
+    for (int I = 0; I < M; ++I)
+        for (int J = I + 1; J < M; ++J)
+            e += fast_block(s, &t, s->mol_off[I], s->mol_off[I + 1], s->mol_off[J], s->mol_off[J + 1], pos);
+    fast_table_free(&t);
+    return e;
+}
+
+void orc_remc_refresh_fast(orc_remc *s)
+{
+    fast_tab t;
+    fast_table(s, &t);
+    const int M = s->n_mol;
+    for (int r = 0; r < s->n_rep; ++r) {
+        const float *pos = s->pos + (size_t)r * s->n_res * 3;
+        double *mat = s->emat + (size_t)r * M * M;
+        double e = 0.0;
+        for (int I = 0; I < M; ++I) {
+            mat[I * M + I] = 0.0;
+            for (int J = I + 1; J < M; ++J) {
+                const double v = fast_block(s, &t, s->mol_off[I], s->mol_off[I + 1], s->mol_off[J],
+                                            s->mol_off[J + 1], pos);
+                mat[I * M + J] = mat[J * M + I] = v;
+                e += v;
+            }
+        }
+        s->energy[r] = e;
+    }
+    fast_table_free(&t);
+}
+
+typedef struct {
+    orc_remc *s;
+    const fast_tab *t;
+    int r0, r1, nsteps;
+} fast_job;
+
+static void *fast_worker(void *arg)
+{
+    fast_job *job = (fast_job *)arg;
+    orc_remc *s = job->s;
+    const int M = s->n_mol, N = s->n_res;
+    float *spos = (float *)malloc(sizeof(float) * 3 * N);
+    float *srel = (float *)malloc(sizeof(float) * 3 * N);
+    double *row = (double *)malloc(sizeof(double) * M);
+    for (int r = job->r0; r < job->r1; ++r) {
+        float *center = s->center + (size_t)r * M * 3;
+        float *rel = s->rel + (size_t)r * N * 3;
+        float *pos = s->pos + (size_t)r * N * 3;
+        double *mat = s->emat + (size_t)r * M * M;
+        const double rs = (double)s->rotate_step[r];
+        const double rcos = cos(rs / 2.0), rsin = sin(rs / 2.0);
+        memcpy(spos, pos, sizeof(float) * 3 * N);
+        for (int k = 0; k < job->nsteps; ++k) {
+            const int64_t step = s->step[r];
+            int m, kind;
+            double axis[3], u;
+            uint32_t extra[4];
+            orc_mc_proposal_flex(s->seed, r, step, M, NULL, &m, &kind, axis, &u, extra);
+            const int i0 = s->mol_off[m], i1 = s->mol_off[m + 1];
+            float cnew[3] = {center[3 * m], center[3 * m + 1], center[3 * m + 2]};
+            memcpy(srel + 3 * i0, rel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+            apply_proposal(s, r, m, kind, axis, extra, rcos, rsin, spos, srel, cnew);
+            double delta = 0.0;
+            for (int J = 0; J < M; ++J) {
+                if (J == m) { row[J] = 0.0; continue; }
+                row[J] = fast_block(s, job->t, i0, i1, s->mol_off[J], s->mol_off[J + 1], spos);
+                delta += row[J] - mat[m * M + J];
+            }
+            const int decision = orc_metropolis(delta, s->temperature[r], u);
+            s->counters[3 * r + decision]++;
+            if (decision < 2) {
+                for (int q = 0; q < 3; ++q) center[3 * m + q] = cnew[q];
+                memcpy(rel + 3 * i0, srel + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+                memcpy(pos + 3 * i0, spos + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+                for (int J = 0; J < M; ++J)
+                    if (J != m) mat[m * M + J] = mat[J * M + m] = row[J];
+                s->energy[r] += delta;
+            } else {
+                memcpy(spos + 3 * i0, pos + 3 * i0, sizeof(float) * 3 * (i1 - i0));
+            }
+            s->step[r] = step + 1;
+        }
+    }
+    free(spos); free(srel); free(row);
+    return NULL;
+}
+
+void orc_remc_mc_steps_fast(orc_remc *s, int nsteps, int threads)
+{
+    fast_tab t;
+    fast_table(s, &t);
+    if (threads < 1) threads = 1;
+    if (threads > s->n_rep) threads = s->n_rep;
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    fast_job *jobs = (fast_job *)malloc(sizeof(fast_job) * threads);
+    int used = 0;
+    for (int k = 0; k < threads; ++k) {
+        const int r0 = (int)((long)k * s->n_rep / threads), r1 = (int)((long)(k + 1) * s->n_rep / threads);
+        if (r0 == r1) continue;
+        jobs[used].s = s; jobs[used].t = &t; jobs[used].r0 = r0; jobs[used].r1 = r1; jobs[used].nsteps = nsteps;
+        pthread_create(&tid[used], NULL, fast_worker, &jobs[used]);
+        ++used;
+    }
+    for (int k = 0; k < used; ++k) pthread_join(tid[k], NULL);
+    free(tid); free(jobs);
+    fast_table_free(&t);
+}
+
+/* ------------------------------------------------------------------ */
 /* CPU baseline: the reference's CPU loop                               */
 /* ------------------------------------------------------------------ */
 typedef struct {

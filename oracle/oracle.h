@@ -218,6 +218,15 @@ void orc_remc_mc_steps(orc_remc *s, int nsteps);
 /* One exchange round over adjacent temperature ranks. */
 void orc_remc_exchange(orc_remc *s);
 
+/* Lean FP64 sampler (rigid molecules, crowding off): the Markov chain of
+ * orc_remc_mc_steps with an FP64 pair term written for speed, replicas over
+ * threads; for CPU REMC runs long enough for statistical comparisons.
+ * orc_remc_refresh_fast rebuilds the matrices and totals with the same pair
+ * term; orc_energy_fast is replica r's total (kcal/mol). */
+void orc_remc_refresh_fast(orc_remc *s);
+void orc_remc_mc_steps_fast(orc_remc *s, int nsteps, int threads);
+double orc_energy_fast(const orc_remc *s, int r);
+
 /* ---------------------------------------------------------------------
  * CPU baseline: the reference's CPU REMC loop (src/Replica.cpp:350-414,
  * src/Simulation.cpp:996-1039): full Replica::E() recompute every MC step,

@@ -275,3 +275,40 @@ def test_exchange_swaps_temperature_not_coordinates():
     assert sorted(st.temperature) == sorted(spec.temperatures())
     assert st.exchange_stats[0] == 2
     assert list(st.rank_of[st.rep_at_rank]) == [0, 1, 2, 3]
+
+
+def test_lean_sampler_energies_match_fp64(golden):
+    """orc_energy_fast (the lean FP64 pair term of the long-run sampler)
+    equals the FP64 oracle on the golden systems, wrapped ones included."""
+    from cgppd_amd import formats, synth
+    for name in GOLDEN_SYSTEMS:
+        g = golden_system(golden, name)
+        M = len(g["mol_off"]) - 1
+        spec = synth.make_system("lean", np.diff(g["mol_off"]), g["box"], 1, 300.0, 300.0, 3)
+        spec.types, spec.mol_off = g["types"].astype(np.int32), g["mol_off"].astype(np.int32)
+        spec.ff = formats.ForceField([formats.AminoAcid(a.name, a.sname, a.symbol, float(sg), float(q))
+                                      for a, sg, q in zip(spec.ff.acids, g["sigma"], g["charge"])],
+                                     g["eps"].astype(np.float32))
+        spec.centers = np.zeros((M, 3), np.float32)
+        spec.rel = g["pos"].astype(np.float32)
+        st = oracle.Remc(spec, seed=1)
+        want = oracle.total(*oracle.energy_fp64(g["mol_off"], g["types"], g["charge"], g["sigma"], g["eps"],
+                                                g["pos"], g["box"]))
+        assert st.energy_fast(0) == pytest.approx(want, rel=1e-12, abs=1e-14), name
+
+
+def test_lean_sampler_takes_the_oracle_decisions():
+    """The lean sampler is the Markov chain of orc_remc_mc_steps: on the
+    same streams it takes the same decisions and reaches the same
+    coordinates; its running totals equal its own fresh evaluations."""
+    spec, a = _tiny_remc(n_rep=4, seed=31)
+    _, b = _tiny_remc(n_rep=4, seed=31)
+    b.refresh_fast()
+    a.mc_steps(300)
+    b.mc_steps_fast(300, 3)
+    np.testing.assert_array_equal(a.counters, b.counters)
+    np.testing.assert_array_equal(a.pos, b.pos)
+    # (reference-semantics float distances against FP64 ones)
+    np.testing.assert_allclose(a.energy, b.energy, rtol=1e-5, atol=1e-6)
+    for r in range(spec.n_rep):
+        assert b.energy[r] == pytest.approx(b.energy_fast(r), rel=1e-9, abs=1e-12)
