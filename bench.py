@@ -24,6 +24,10 @@ WRITE_SIZE), scaled by what the same counters read for a plain device copy
 of known size (--copy-probe-mib), and is read from the committed summary in
 profiles/ when present.
 
+The delayed_acceptance object compares plain REMC with the learned
+surrogate's delayed acceptance on the same workload from the same state
+(effective samples/s, stage-one acceptance; see da_leg).
+
 The cpu_baseline leg times the reference's CPU loop (full Replica::E per MC
 step, pthreads over replicas) as restated in oracle/, on a bounded sample,
 with one thread per core of the process's CPU share (at most 16, the GPU
@@ -65,6 +69,12 @@ def parse():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
+    ap.add_argument("--no-da", action="store_true", help="leave out the delayed-acceptance leg")
+    ap.add_argument("--da-equilibrate", type=int, default=400000,
+                    help="plain MC steps before the delayed-acceptance comparison (training samples are "
+                         "collected along them)")
+    ap.add_argument("--da-steps", type=int, default=1200000, help="MC steps of each leg of the comparison")
+    ap.add_argument("--da-train-steps", type=int, default=3000)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run this many direct MC launches (for rocprofv3), no JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -158,6 +168,112 @@ def cpu_baseline(spec, seed, target_s):
         "pairs_per_s": steps * spec.pairs_full() / t,
         "one_thread": {"value": n1 / t_one, "unit": "MC steps/s (all replicas)",
                        "sample": f"{n1} MC steps of one replica of the same workload, 1 thread, {t_one:.1f} s"},
+    }
+
+
+def tau_int(x, c=5.0):
+    """Integrated autocorrelation time, Sokal's automatic window (c = 5)."""
+    from cgppd_amd.surrogate_study import tau_int as t
+    return t(x, c)
+
+
+def da_leg(args, layout, wg_per_cu):
+    """Plain and delayed-acceptance REMC on the 40-chain box from the same
+    state: after --da-equilibrate plain steps (training pairs collected
+    along them every 250 steps) and --da-train-steps Adam steps, both legs
+    run --da-steps steps from that state, recording every replica's energy
+    after each 1000-step exchange interval.  Effective samples/s = sum over
+    the temperatures of n / tau_int of the energy at that temperature,
+    divided by the leg's wall time."""
+    from cgppd_amd import engine, synth
+
+    spec = synth.config(1)
+    per = 1000
+    e = engine.Engine(0)
+    e.load_spec(spec, seed=args.seed + 7)
+    e.set_pair_layout(layout)
+    e.set_launch(wg_per_cu)
+    e.mc_prepare()
+    e.surrogate_init(seed=args.seed)
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.da_equilibrate // per)):
+        for _ in range(4):
+            e.mc_steps(per // 4)
+            e.surrogate_collect()
+        e.exchange()
+    n_train = e.surrogate_sample_count()
+    t_eq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = [e.surrogate_train(args.da_train_steps // 10, batch=1024, lr=1e-3 if k < 7 else 5e-4, seed=args.seed + k)
+              for k in range(10)]
+    e.synchronize()
+    t_train = time.perf_counter() - t0
+    params = e.surrogate_params()
+    temps = e.stats()["temperature"]
+    labels = e.stats()["label"]
+    cen = np.zeros((spec.n_rep, spec.n_mol, 3), np.float32)
+    rel = np.zeros((spec.n_rep, spec.n_res, 3), np.float32)
+    for r in range(spec.n_rep):
+        _, rel[r], cen[r] = e.coordinates(r)
+    e.close()
+    print(f"delayed acceptance: {n_train} training pairs over {t_eq:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g} "
+          f"in {t_train:.1f} s", file=sys.stderr, flush=True)
+
+    legs = {}
+    for name, da in (("plain", False), ("delayed_acceptance", True)):
+        f = engine.Engine(0)
+        f.load_spec(spec, seed=args.seed + 7)
+        ts, rs = spec.step_sizes()
+        f.set_replicas(cen, rel, temps, ts, rs, labels, seed=args.seed + 11)
+        f.set_pair_layout(layout)
+        f.set_launch(wg_per_cu)
+        f.mc_prepare()
+        if da:
+            f.surrogate_set_params(params)
+            f.set_delayed_acceptance(True)
+        f.run(0, per, use_graph=True)
+        s0 = f.stats()
+        rej0 = f.stage1_rejections().sum()
+        n = max(1, args.da_steps // per)
+        energies = []
+        f.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            f.run(1, per, use_graph=True)
+            s = f.stats()
+            energies.append(s["energy"][np.argsort(s["temperature"], kind="stable")])
+        wall = time.perf_counter() - t0
+        energies = np.array(energies)
+        taus = np.array([tau_int(energies[:, k]) for k in range(spec.n_rep)])
+        ess = float((n / taus).sum())
+        acc = s["counters"].sum(0) - s0["counters"].sum(0)
+        rej = int(f.stage1_rejections().sum() - rej0)
+        decided = int(acc.sum()) + rej
+        legs[name] = {
+            "mc_steps_per_s": spec.n_rep * n * per / wall,
+            "acceptance": float((acc[0] + acc[1]) / max(1, decided)),
+            "stage1_acceptance": float(1.0 - rej / max(1, decided)) if da else None,
+            "tau_int_intervals": {"min": float(taus.min()), "median": float(np.median(taus)),
+                                  "max": float(taus.max())},
+            "runs_cover_20_tau": bool(n >= 20 * taus.max()),
+            "effective_samples": ess,
+            "effective_samples_per_s": ess / wall,
+            "wall_s": wall,
+        }
+        f.close()
+        print(f"{name}: {legs[name]['mc_steps_per_s']:.0f} MC steps/s, ESS/s {legs[name]['effective_samples_per_s']:.3g}",
+              file=sys.stderr, flush=True)
+    p, d = legs["plain"], legs["delayed_acceptance"]
+    return {
+        "workload": spec.name,
+        "steps_per_leg": (args.da_steps // per) * per,
+        "equilibration_steps": max(1, args.da_equilibrate // per) * per,
+        "training": {"pairs": int(n_train), "adam_steps": args.da_train_steps, "loss_first": losses[0],
+                     "loss_last": losses[-1], "train_s": t_train},
+        "plain": p,
+        "delayed_acceptance": d,
+        "ess_per_s_ratio": d["effective_samples_per_s"] / max(p["effective_samples_per_s"], 1e-30),
+        "mc_steps_per_s_ratio": d["mc_steps_per_s"] / max(p["mc_steps_per_s"], 1e-30),
     }
 
 
@@ -311,6 +427,8 @@ def main():
             line["cpu_baseline_skipped"] = (f"one full evaluation of every replica would take about {est_s:.0f} s "
                                             f"on {threads} threads")
     e.close()
+    if not args.no_da and args.config == 1:
+        line["delayed_acceptance"] = da_leg(args, layout, line["config"]["wg_per_cu"])
     print(json.dumps(line), flush=True)
     return 0
 
