@@ -2345,6 +2345,224 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Resident MC steps for small rigid systems (every residue of a replica in
+// LDS): one workgroup per replica runs many MC steps in one launch.  The
+// replica's positions, relative positions, centres and pending proposal
+// live in LDS for the whole launch; a step evaluates the proposal's row
+// against every other residue, decides, applies and lays out the next
+// proposal with workgroup barriers only, so a step costs LDS latencies and
+// a few thousand pair terms instead of two kernel launches.  Systems: no
+// flexible molecules, no delayed acceptance, no repulsive crowding, at
+// most kResidentRes residues and kResidentMol molecules.
+//
+// The row: a thread takes a j residue outside the moved molecule m against
+// a range of m's proposed residues (the residues split into as many ranges
+// as the workgroup's threads can take side by side), summing in Kahan FP32
+// with the per-pair minimum image; each molecule J's lanes are summed in
+// FP64 (a fixed DPP order) into the wave's slot for J, and the four waves'
+// slots are added in wave order: the row does not depend on timing.  The decision,
+// apply and next proposal are those of mc_finish_rigid_kernel.
+constexpr int kResidentRes = 1024;
+constexpr int kResidentMol = 256;
+
+struct ResidentParams {
+    int steps;  // MC steps of this launch
+};
+
+// Dynamic LDS of the resident kernel for n residues, m molecules and
+// proposals of at most max_n residues.
+__host__ __device__ inline size_t resident_lds_bytes(int n, int m, int max_n)
+{
+    return (size_t)16 * (2 * (size_t)n + 2 * (size_t)max_n + (size_t)m) + (size_t)8 * (kWavesPerWG + 1) * m +
+           (size_t)4 * ((size_t)n + (size_t)m + 1);
+}
+
+__global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, ResidentParams rp)
+{
+#pragma clang fp contract(off)
+    __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
+    __shared__ double red[kWavesPerWG];
+    __shared__ int s_acc;
+    extern __shared__ __attribute__((aligned(16))) float4 res_lds[];
+    const int r = blockIdx.x, t = threadIdx.x;
+    const int lane = t & (kWave - 1), wave = t / kWave;
+    const int N = p.n_res, M = p.n_mol;
+    float4 *s_pos = res_lds;                  // N
+    float4 *s_rel = s_pos + N;                // N
+    float4 *s_prop = s_rel + N;               // max_n: proposed positions
+    float4 *s_prel = s_prop + p.max_n;        // max_n: proposed relative positions
+    float4 *s_cen = s_prel + p.max_n;         // M
+    double *s_part = reinterpret_cast<double *>(s_cen + M);  // kWavesPerWG x M: the waves' row slots
+    double *s_row = s_part + (size_t)kWavesPerWG * M;        // M: the step's row
+    int *s_mol = reinterpret_cast<int *>(s_row + M);         // N: molecule of each residue
+    int *s_off = s_mol + N;                                  // M + 1: molecule offsets
+    // the table (entry e = tj * 20 + ti at row tj, column ti, kcal/mol) and
+    // the replica's state
+    {
+        constexpr int kEntries = CGPPD_AA_COUNT * CGPPD_AA_COUNT;
+        const int e0 = t, e1 = min(t + kWG, kEntries - 1);
+        const Coef g0 = p.table[e0], g1 = p.table[e1];
+        *reinterpret_cast<Coef *>(tab + (((e0 / CGPPD_AA_COUNT) << kRowShift) | ((e0 % CGPPD_AA_COUNT) << kColShift))) = g0;
+        *reinterpret_cast<Coef *>(tab + (((e1 / CGPPD_AA_COUNT) << kRowShift) | ((e1 % CGPPD_AA_COUNT) << kColShift))) = g1;
+    }
+    for (int k = t; k < N; k += kWG) {
+        s_pos[k] = p.pos[(size_t)r * p.stride + k];
+        s_rel[k] = p.rel[(size_t)r * p.stride + k];
+    }
+    for (int k = t; k < M; k += kWG) s_cen[k] = p.center[(size_t)r * M + k];
+    for (int k = t; k < N; k += kWG) s_mol[k] = p.mol_of[k];
+    for (int k = t; k <= M; k += kWG) s_off[k] = p.mol_off[k];
+    const int n_first = p.prop_n[r];
+    for (int k = t; k < n_first; k += kWG) {
+        s_prop[k] = p.prop_pos[(size_t)r * p.max_n + k];
+        s_prel[k] = p.prop_rel[(size_t)r * p.max_n + k];
+    }
+    float4 pc = p.prop_center[r];  // the pending proposal's centre (every thread)
+    int64_t step = p.steps[r];
+    double energy = p.energy[r];
+    long long cnt[3] = {0, 0, 0};
+    unsigned long long pairs = 0;
+    const float tstep = p.tstep[r];
+    const double rcos = p.rcos[r], rsin = p.rsin[r];
+    const double temp = p.temperature[r];
+    double *mat = p.emat + (size_t)r * M * M;
+    __syncthreads();
+    for (int s = 0; s < rp.steps; ++s) {
+        const Proposal pr = draw_proposal(p.seed, r + p.rep_rng, step, M);
+        const int m = pr.mol;
+        const int i0 = s_off[m], n_m = s_off[m + 1] - i0;
+        const int nj = N - n_m;
+        // the matrix row of m, read now and used after the pair loop
+        double mrow = 0.0;
+        if (t < M) mrow = mat[(size_t)m * M + t];
+        // work items q = (range si of the proposal's residues, j residue jp
+        // outside m): S ranges, as many as the threads can take side by side
+        const int S = nj >= kWG || nj == 0 ? 1 : kWG / nj;
+        for (int k = t; k < kWavesPerWG * M; k += kWG) s_part[k] = 0.0;
+        __syncthreads();
+        for (int q0 = wave * kWave; q0 < S * nj; q0 += kWG) {
+            const int q = q0 + lane;
+            const bool valid = q < S * nj;
+            const int jp = valid ? q % nj : 0, si = valid ? q / nj : 0;
+            const int ia = si * n_m / S, ib = (si + 1) * n_m / S;
+            const int j = min(jp + (jp >= i0 ? n_m : 0), N - 1);
+            const float4 rj = s_pos[j];
+            const int J = s_mol[j];
+            const uint32_t jrow = __float_as_uint(rj.w) & kRowMask;
+            float sum = 0.0f, comp = 0.0f;
+            // four proposal residues at a time: their records and table
+            // entries are read before any term is formed
+            int i = valid ? ia : ib;
+            for (; i + 4 <= ib; i += 4) {
+                float4 pi[4];
+                Coef cf[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pi[u] = s_prop[i + u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) cf[u] = *reinterpret_cast<const Coef *>(tab + (jrow | lane_col(pi[u])));
+                float g = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float lj, dh;
+                    pair_term<true, true>(rj, pi[u].x, pi[u].y, pi[u].z, cf[u], p.box, p.inv_box, lj, dh);
+                    g += lj + dh;
+                }
+                kahan_add(sum, comp, g);
+            }
+            for (; i < ib; ++i) {
+                float lj, dh;
+                pair_any(s_prop[i], rj, tab, p, lj, dh);
+                kahan_add(sum, comp, lj + dh);
+            }
+            const double v = valid ? (double)sum - (double)comp : 0.0;
+            // the molecules of the wave's items, one at a time in lane order
+            unsigned long long todo = __ballot(valid);
+            while (todo) {
+                const int Jc = __builtin_amdgcn_readlane(J, __builtin_ctzll(todo));
+                const bool mine = valid && J == Jc;
+                const double tt = wave_total(mine ? v : 0.0);
+                if (lane == 0) s_part[wave * M + Jc] += tt;
+                todo &= ~__ballot(mine);
+            }
+        }
+        __syncthreads();
+        // Metropolis (src/Replica.cpp:385-410, delta and exponent in FP64)
+        double d = 0.0;
+        for (int J = t; J < M; J += kWG) {
+            double e = 0.0;
+            for (int w = 0; w < kWavesPerWG; ++w) e += s_part[w * M + J];
+            s_row[J] = e;
+            d += e - (J == t ? mrow : mat[(size_t)m * M + J]);
+        }
+        d = wave_total(d);
+        if (lane == 0) red[wave] = d;
+        __syncthreads();
+        double delta = 0.0;
+        for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
+        const int kind = delta < 0.0 ? 0 : pr.u < exp(-delta * 4184.0 / ((double)kRgas * temp)) ? 1 : 2;
+        const bool acc = kind < 2;
+        cnt[kind] += 1;
+        pairs += (unsigned long long)n_m * (unsigned long long)nj;
+        if (acc) {
+            // apply (src/Molecule.cpp:468-504) and the matrix row and column
+            energy += delta;
+            for (int J = t; J < M; J += kWG) {
+                mat[(size_t)m * M + J] = s_row[J];
+                mat[(size_t)J * M + m] = s_row[J];
+            }
+            for (int k = t; k < n_m; k += kWG) {
+                s_pos[i0 + k] = s_prop[k];
+                s_rel[i0 + k] = s_prel[k];
+            }
+            if (t == 0) s_cen[m] = pc;
+        }
+        __syncthreads();
+        // the next proposal (src/Molecule.cpp:146-266)
+        ++step;
+        const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, step, M);
+        const int m2 = nx.mol;
+        const int j_beg = s_off[m2], n_2 = s_off[m2 + 1] - j_beg;
+        const Move mv = make_move(nx, s_cen[m2], tstep, rcos, rsin, p.box);
+        for (int k = t; k < n_2; k += kWG) {
+            float4 rn, pn;
+            move_residue(mv, s_rel[j_beg + k], rn, pn);
+            s_prop[k] = pn;
+            s_prel[k] = rn;
+        }
+        pc = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+        __syncthreads();
+    }
+    // the replica's state back to memory, and the pending proposal in the
+    // form the other MC kernels read
+    for (int k = t; k < N; k += kWG) {
+        p.pos[(size_t)r * p.stride + k] = s_pos[k];
+        p.rel[(size_t)r * p.stride + k] = s_rel[k];
+    }
+    for (int k = t; k < M; k += kWG) p.center[(size_t)r * M + k] = s_cen[k];
+    const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, step, M);
+    const int m2 = nx.mol;
+    const int j_beg = p.mol_off[m2], n_2 = p.mol_off[m2 + 1] - j_beg;
+    for (int k = t; k < n_2; k += kWG) {
+        p.prop_pos[(size_t)r * p.max_n + k] = s_prop[k];
+        p.prop_rel[(size_t)r * p.max_n + k] = s_prel[k];
+    }
+    if (t == 0) {
+        p.energy[r] = energy;
+        long long *c = p.counters + 3 * (size_t)r;
+        c[0] += cnt[0];
+        c[1] += cnt[1];
+        c[2] += cnt[2];
+        p.steps[r] = step;
+        atomicAdd(p.pair_evals, pairs);
+        p.prop_mol[r] = m2;
+        p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
+        p.prop_n[r] = n_2;
+        p.prop_center[r] = pc;
+        record_proposal(p, r, m2, j_beg, n_2, kMovedNone, 0, p.radius[(size_t)r * M + m2], 0.0);
+    }
+}
+
 // After the scaled full-energy passes: mirror the upper triangle of each
 // replica's matrix and set the running total to the sum over I <= J (the
 // diagonal holds flexible molecules' internal energies).
@@ -3650,6 +3868,7 @@ int pair_layout_of(const cgppd_ctx *ctx)
     if (ctx->n_flex) return CGPPD_LAYOUT_ILANES;
     // the row kernel counts one replica's units in 32 bits
     if ((long long)((ctx->N + kWave - 1) / kWave) * ctx->max_n >= (1ll << 32)) return CGPPD_LAYOUT_ILANES;
+    if (ctx->pair_layout == CGPPD_LAYOUT_RESIDENT) return CGPPD_LAYOUT_JLANES;  // (its pair-kernel steps)
     if (ctx->pair_layout != CGPPD_LAYOUT_AUTO) return ctx->pair_layout;
     return ctx->tuned_layout ? ctx->tuned_layout : CGPPD_LAYOUT_JLANES;
 }
@@ -3824,6 +4043,33 @@ int launch_mc_step(cgppd_ctx *ctx)
     return launch_finish(ctx, 0);
 }
 
+// Resident MC steps: chosen with CGPPD_LAYOUT_RESIDENT, for systems of
+// rigid molecules without delayed acceptance or repulsive crowding whose
+// replica fits the kernel's LDS, with one replica group; otherwise the
+// pair and finish kernels run.
+bool resident_active(const cgppd_ctx *ctx)
+{
+    return ctx->pair_layout == CGPPD_LAYOUT_RESIDENT && !ctx->n_flex && !ctx->da && !ctx->rep_crowd &&
+           ctx->N <= kResidentRes && ctx->M <= kResidentMol && groups_of(ctx) == 1 && !ctx->timing &&
+           resident_lds_bytes(ctx->N, ctx->M, ctx->max_n) + kTableBytes + 1024 <= 65536;
+}
+
+// n_steps resident MC steps, at most 16384 per launch.
+int launch_resident_steps(cgppd_ctx *ctx, int n_steps)
+{
+    PairParams p = base_params(ctx);
+    p.table = ctx->d_table_scaled;
+    const size_t lds = resident_lds_bytes(ctx->N, ctx->M, ctx->max_n);
+    for (int done = 0; done < n_steps;) {
+        ResidentParams rp;
+        rp.steps = std::min(n_steps - done, 16384);
+        hipLaunchKernelGGL(mc_resident_kernel, dim3(ctx->R), dim3(kWG), lds, ctx->stream, p, rp);
+        CG_HIP(ctx, hipGetLastError());
+        done += rp.steps;
+    }
+    return CGPPD_OK;
+}
+
 // n_steps MC steps of every replica.  With G > 1 replica groups the context
 // stream forks into the groups' streams and joins them again, so the work
 // after the call (an exchange, a read-back) still follows every group's
@@ -3833,6 +4079,7 @@ int launch_mc_steps(cgppd_ctx *ctx, int n_steps)
     const int G = groups_of(ctx);
     int rc;
     if (G == 1) {
+        if (resident_active(ctx)) return n_steps > 0 ? launch_resident_steps(ctx, n_steps) : CGPPD_OK;
         for (int k = 0; k < n_steps; ++k)
             if ((rc = launch_mc_step(ctx))) return rc;
         return CGPPD_OK;
@@ -4727,7 +4974,7 @@ int cgppd_get_replica_groups(cgppd_ctx *ctx, int *groups)
 
 int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout)
 {
-    if (!ctx || layout < CGPPD_LAYOUT_AUTO || layout > CGPPD_LAYOUT_JLANES4) return CGPPD_E_ARG;
+    if (!ctx || layout < CGPPD_LAYOUT_AUTO || layout > CGPPD_LAYOUT_RESIDENT) return CGPPD_E_ARG;
     ctx->pair_layout = layout;
     destroy_graph(ctx);
     return CGPPD_OK;
@@ -4737,7 +4984,7 @@ int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu)
 {
     if (!ctx) return CGPPD_E_ARG;
     const int l = pair_layout_of(ctx);
-    if (layout) *layout = l;
+    if (layout) *layout = ctx->have_sys && resident_active(ctx) ? CGPPD_LAYOUT_RESIDENT : l;
     if (wg_per_cu) *wg_per_cu = pair_wg_per_cu(ctx, l);
     return CGPPD_OK;
 }

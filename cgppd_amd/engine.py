@@ -307,7 +307,7 @@ class Engine:
         self._check(self.lib.cgppd_get_replica_groups(self.h, C.byref(g)), "get_replica_groups")
         return g.value
 
-    LAYOUTS = {"auto": 0, "ilanes": 1, "jlanes": 2, "jlanes4": 3}
+    LAYOUTS = {"auto": 0, "ilanes": 1, "jlanes": 2, "jlanes4": 3, "resident": 4}
 
     def set_pair_layout(self, layout):
         """Pair-kernel layout: "auto", "ilanes" or "jlanes" (or 0 / 1 / 2)."""

@@ -279,11 +279,21 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
  *                       molecule's residues are read as wave-uniform
  *                       records; every wave takes an equal share of the work
  *   CGPPD_LAYOUT_JLANES4 the same with four residues per lane
+ *   CGPPD_LAYOUT_RESIDENT MC steps in one workgroup per replica with the
+ *                       replica held in LDS, many steps per launch
+ *                       (cgppd_mc_steps / cgppd_run): for systems of rigid
+ *                       molecules without delayed acceptance or repulsive
+ *                       crowding, up to 1024 residues and 256 molecules,
+ *                       whose replica fits 64 KB of LDS, with one replica
+ *                       group; other systems, cgppd_autotune and timed
+ *                       launches use the j-lane kernel.  cgppd_get_pair_launch
+ *                       reports CGPPD_LAYOUT_RESIDENT when it applies.
  * Systems with flexible molecules always use the i-lane kernel. */
 #define CGPPD_LAYOUT_AUTO 0
 #define CGPPD_LAYOUT_ILANES 1
 #define CGPPD_LAYOUT_JLANES 2
 #define CGPPD_LAYOUT_JLANES4 3
+#define CGPPD_LAYOUT_RESIDENT 4
 int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout);
 /* Replica groups of the MC steps (default 1; at most 4).  New: the
  * reference runs replicas on host threads, each waiting on its own GPU
