@@ -2374,7 +2374,7 @@ struct ResidentParams {
 // proposals of at most max_n residues.
 __host__ __device__ inline size_t resident_lds_bytes(int n, int m, int max_n)
 {
-    return (size_t)16 * (2 * (size_t)n + 2 * (size_t)max_n + (size_t)m) + (size_t)8 * (kWavesPerWG + 1) * m +
+    return (size_t)16 * (2 * (size_t)n + 4 * (size_t)max_n + (size_t)m) + (size_t)8 * 2 * kWavesPerWG * m +
            (size_t)4 * ((size_t)n + (size_t)m + 1);
 }
 
@@ -2382,21 +2382,18 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
 {
 #pragma clang fp contract(off)
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
-    __shared__ double red[kWavesPerWG];
-    __shared__ int s_acc;
     extern __shared__ __attribute__((aligned(16))) float4 res_lds[];
     const int r = blockIdx.x, t = threadIdx.x;
     const int lane = t & (kWave - 1), wave = t / kWave;
     const int N = p.n_res, M = p.n_mol;
     float4 *s_pos = res_lds;                  // N
     float4 *s_rel = s_pos + N;                // N
-    float4 *s_prop = s_rel + N;               // max_n: proposed positions
-    float4 *s_prel = s_prop + p.max_n;        // max_n: proposed relative positions
-    float4 *s_cen = s_prel + p.max_n;         // M
-    double *s_part = reinterpret_cast<double *>(s_cen + M);  // kWavesPerWG x M: the waves' row slots
-    double *s_row = s_part + (size_t)kWavesPerWG * M;        // M: the step's row
-    int *s_mol = reinterpret_cast<int *>(s_row + M);         // N: molecule of each residue
-    int *s_off = s_mol + N;                                  // M + 1: molecule offsets
+    float4 *s_pa = s_rel + N;                 // 2 x max_n: proposed positions (two slots)
+    float4 *s_ra = s_pa + 2 * p.max_n;        // 2 x max_n: proposed relative positions
+    float4 *s_cen = s_ra + 2 * p.max_n;       // M
+    double *s_part = reinterpret_cast<double *>(s_cen + M);  // 2 x kWavesPerWG x M: the waves' row slots
+    int *s_mol = reinterpret_cast<int *>(s_part + 2 * kWavesPerWG * M);  // N: molecule of each residue
+    int *s_off = s_mol + N;                                              // M + 1: molecule offsets
     // the table (entry e = tj * 20 + ti at row tj, column ti, kcal/mol) and
     // the replica's state
     {
@@ -2413,10 +2410,11 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
     for (int k = t; k < M; k += kWG) s_cen[k] = p.center[(size_t)r * M + k];
     for (int k = t; k < N; k += kWG) s_mol[k] = p.mol_of[k];
     for (int k = t; k <= M; k += kWG) s_off[k] = p.mol_off[k];
+    for (int k = t; k < kWavesPerWG * M; k += kWG) s_part[k] = 0.0;
     const int n_first = p.prop_n[r];
     for (int k = t; k < n_first; k += kWG) {
-        s_prop[k] = p.prop_pos[(size_t)r * p.max_n + k];
-        s_prel[k] = p.prop_rel[(size_t)r * p.max_n + k];
+        s_pa[k] = p.prop_pos[(size_t)r * p.max_n + k];
+        s_ra[k] = p.prop_rel[(size_t)r * p.max_n + k];
     }
     float4 pc = p.prop_center[r];  // the pending proposal's centre (every thread)
     int64_t step = p.steps[r];
@@ -2427,20 +2425,41 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
     const double rcos = p.rcos[r], rsin = p.rsin[r];
     const double temp = p.temperature[r];
     double *mat = p.emat + (size_t)r * M * M;
+    Proposal pr = draw_proposal(p.seed, r + p.rep_rng, step, M);
     __syncthreads();
     for (int s = 0; s < rp.steps; ++s) {
-        const Proposal pr = draw_proposal(p.seed, r + p.rep_rng, step, M);
+        // slot cur holds this step's proposal, the other one the next
+        // step's; the row slots alternate the same way
+        const int cur = s & 1;
+        const float4 *prop = s_pa + cur * p.max_n, *prel = s_ra + cur * p.max_n;
+        float4 *nprop = s_pa + (cur ^ 1) * p.max_n, *nprel = s_ra + (cur ^ 1) * p.max_n;
+        double *part = s_part + cur * kWavesPerWG * M;
         const int m = pr.mol;
         const int i0 = s_off[m], n_m = s_off[m + 1] - i0;
         const int nj = N - n_m;
-        // the matrix row of m, read now and used after the pair loop
-        double mrow = 0.0;
-        if (t < M) mrow = mat[(size_t)m * M + t];
+        // the matrix row of m, read now (before any wave can apply this
+        // step's move to it) and used after the pair loop
+        constexpr int kRowPerLane = kResidentMol / kWave;
+        double mrow[kRowPerLane];
+#pragma unroll
+        for (int h = 0; h < kRowPerLane; ++h)
+            mrow[h] = lane + h * kWave < M ? mat[(size_t)m * M + lane + h * kWave] : 0.0;
+        // the next step's proposal, laid out now from the current state (it
+        // holds unless this step moves the same molecule and is accepted)
+        const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, step + 1, M);
+        const int m2 = nx.mol;
+        const int j_beg = s_off[m2], n_2 = s_off[m2 + 1] - j_beg;
+        const Move mv = make_move(nx, s_cen[m2], tstep, rcos, rsin, p.box);
+        for (int k = t; k < n_2; k += kWG) {
+            float4 rn, pn;
+            move_residue(mv, s_rel[j_beg + k], rn, pn);
+            nprop[k] = pn;
+            nprel[k] = rn;
+        }
+        float4 npc = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
         // work items q = (range si of the proposal's residues, j residue jp
         // outside m): S ranges, as many as the threads can take side by side
         const int S = nj >= kWG || nj == 0 ? 1 : kWG / nj;
-        for (int k = t; k < kWavesPerWG * M; k += kWG) s_part[k] = 0.0;
-        __syncthreads();
         for (int q0 = wave * kWave; q0 < S * nj; q0 += kWG) {
             const int q = q0 + lane;
             const bool valid = q < S * nj;
@@ -2458,7 +2477,7 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
                 float4 pi[4];
                 Coef cf[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) pi[u] = s_prop[i + u];
+                for (int u = 0; u < 4; ++u) pi[u] = prop[i + u];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) cf[u] = *reinterpret_cast<const Coef *>(tab + (jrow | lane_col(pi[u])));
                 float g = 0.0f;
@@ -2472,7 +2491,7 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
             }
             for (; i < ib; ++i) {
                 float lj, dh;
-                pair_any(s_prop[i], rj, tab, p, lj, dh);
+                pair_any(prop[i], rj, tab, p, lj, dh);
                 kahan_add(sum, comp, lj + dh);
             }
             const double v = valid ? (double)sum - (double)comp : 0.0;
@@ -2482,24 +2501,24 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
                 const int Jc = __builtin_amdgcn_readlane(J, __builtin_ctzll(todo));
                 const bool mine = valid && J == Jc;
                 const double tt = wave_total(mine ? v : 0.0);
-                if (lane == 0) s_part[wave * M + Jc] += tt;
+                if (lane == 0) part[wave * M + Jc] += tt;
                 todo &= ~__ballot(mine);
             }
         }
         __syncthreads();
-        // Metropolis (src/Replica.cpp:385-410, delta and exponent in FP64)
+        // Metropolis (src/Replica.cpp:385-410, delta and exponent in FP64),
+        // taken by every wave on the same values in the same order
         double d = 0.0;
-        for (int J = t; J < M; J += kWG) {
-            double e = 0.0;
-            for (int w = 0; w < kWavesPerWG; ++w) e += s_part[w * M + J];
-            s_row[J] = e;
-            d += e - (J == t ? mrow : mat[(size_t)m * M + J]);
+#pragma unroll
+        for (int h = 0; h < kRowPerLane; ++h) {
+            const int J = lane + h * kWave;
+            if (J < M) {
+                double e = 0.0;
+                for (int w = 0; w < kWavesPerWG; ++w) e += part[w * M + J];
+                d += e - mrow[h];
+            }
         }
-        d = wave_total(d);
-        if (lane == 0) red[wave] = d;
-        __syncthreads();
-        double delta = 0.0;
-        for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
+        const double delta = wave_total(d);
         const int kind = delta < 0.0 ? 0 : pr.u < exp(-delta * 4184.0 / ((double)kRgas * temp)) ? 1 : 2;
         const bool acc = kind < 2;
         cnt[kind] += 1;
@@ -2508,44 +2527,48 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
             // apply (src/Molecule.cpp:468-504) and the matrix row and column
             energy += delta;
             for (int J = t; J < M; J += kWG) {
-                mat[(size_t)m * M + J] = s_row[J];
-                mat[(size_t)J * M + m] = s_row[J];
+                double e = 0.0;
+                for (int w = 0; w < kWavesPerWG; ++w) e += part[w * M + J];
+                mat[(size_t)m * M + J] = e;
+                mat[(size_t)J * M + m] = e;
             }
             for (int k = t; k < n_m; k += kWG) {
-                s_pos[i0 + k] = s_prop[k];
-                s_rel[i0 + k] = s_prel[k];
+                s_pos[i0 + k] = prop[k];
+                s_rel[i0 + k] = prel[k];
             }
             if (t == 0) s_cen[m] = pc;
+            if (m2 == m) {
+                // the next move starts from the accepted coordinates
+                const Move mva = make_move(nx, pc, tstep, rcos, rsin, p.box);
+                for (int k = t; k < n_2; k += kWG) {
+                    float4 rn, pn;
+                    move_residue(mva, prel[k], rn, pn);
+                    nprop[k] = pn;
+                    nprel[k] = rn;
+                }
+                npc = make_float4(mva.c[0], mva.c[1], mva.c[2], 0.0f);
+            }
         }
+        // the next step's row slots start at zero
+        for (int k = t; k < kWavesPerWG * M; k += kWG) s_part[(cur ^ 1) * kWavesPerWG * M + k] = 0.0;
         __syncthreads();
-        // the next proposal (src/Molecule.cpp:146-266)
+        pr = nx;
+        pc = npc;
         ++step;
-        const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, step, M);
-        const int m2 = nx.mol;
-        const int j_beg = s_off[m2], n_2 = s_off[m2 + 1] - j_beg;
-        const Move mv = make_move(nx, s_cen[m2], tstep, rcos, rsin, p.box);
-        for (int k = t; k < n_2; k += kWG) {
-            float4 rn, pn;
-            move_residue(mv, s_rel[j_beg + k], rn, pn);
-            s_prop[k] = pn;
-            s_prel[k] = rn;
-        }
-        pc = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-        __syncthreads();
     }
     // the replica's state back to memory, and the pending proposal in the
     // form the other MC kernels read
+    const int last = rp.steps & 1;
     for (int k = t; k < N; k += kWG) {
         p.pos[(size_t)r * p.stride + k] = s_pos[k];
         p.rel[(size_t)r * p.stride + k] = s_rel[k];
     }
     for (int k = t; k < M; k += kWG) p.center[(size_t)r * M + k] = s_cen[k];
-    const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, step, M);
-    const int m2 = nx.mol;
-    const int j_beg = p.mol_off[m2], n_2 = p.mol_off[m2 + 1] - j_beg;
+    const int m2 = pr.mol;
+    const int j_beg = s_off[m2], n_2 = s_off[m2 + 1] - j_beg;
     for (int k = t; k < n_2; k += kWG) {
-        p.prop_pos[(size_t)r * p.max_n + k] = s_prop[k];
-        p.prop_rel[(size_t)r * p.max_n + k] = s_prel[k];
+        p.prop_pos[(size_t)r * p.max_n + k] = s_pa[last * p.max_n + k];
+        p.prop_rel[(size_t)r * p.max_n + k] = s_ra[last * p.max_n + k];
     }
     if (t == 0) {
         p.energy[r] = energy;

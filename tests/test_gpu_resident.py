@@ -149,3 +149,20 @@ def test_resident_falls_back_outside_its_scope(gpu_engine_factory):
     st.mc_steps(10)
     assert_matches(e, st, big)
     e.close()
+
+
+def test_resident_rows_of_more_than_64_molecules(gpu_engine_factory):
+    """90 molecules: the matrix row of the moved molecule spans several
+    entries per lane, read before any wave applies the step."""
+    from cgppd_amd import synth
+    spec = synth.make_system("many", [3, 5, 2, 8] * 22 + [4, 6], 90.0, 6, 288.0, 450.0, 17, 2.0, 0.2)
+    assert spec.n_mol > 64
+    e = resident_engine(gpu_engine_factory, spec, 17)
+    st = oracle.Remc(spec, seed=17)
+    for _ in range(2):
+        e.mc_steps(60)
+        st.mc_steps(60)
+        assert_matches(e, st, spec, labels=False)
+    _, _, full = e.energy_all()
+    np.testing.assert_allclose(e.stats()["energy"], full, rtol=1e-5, atol=1e-8)
+    e.close()
