@@ -4066,15 +4066,29 @@ int launch_mc_step(cgppd_ctx *ctx)
     return launch_finish(ctx, 0);
 }
 
-// Resident MC steps: chosen with CGPPD_LAYOUT_RESIDENT, for systems of
-// rigid molecules without delayed acceptance or repulsive crowding whose
-// replica fits the kernel's LDS, with one replica group; otherwise the
-// pair and finish kernels run.
+// Resident MC steps: chosen with CGPPD_LAYOUT_RESIDENT, or by
+// CGPPD_LAYOUT_AUTO for small systems (at most 16384 residue pairs per
+// replica step on average over the molecules: there a step is a chain of
+// latencies that one workgroup per replica shortens), for systems of rigid
+// molecules without delayed acceptance or repulsive crowding whose replica
+// fits the kernel's LDS, with one replica group; otherwise the pair and
+// finish kernels run.
+bool resident_small(const cgppd_ctx *ctx)
+{
+    double pairs = 0.0;
+    for (int m = 0; m < ctx->M; ++m) {
+        const double n = ctx->mol_off[m + 1] - ctx->mol_off[m];
+        pairs += n * (ctx->N - n) / ctx->M;
+    }
+    return pairs <= 16384.0;
+}
+
 bool resident_active(const cgppd_ctx *ctx)
 {
-    return ctx->pair_layout == CGPPD_LAYOUT_RESIDENT && !ctx->n_flex && !ctx->da && !ctx->rep_crowd &&
-           ctx->N <= kResidentRes && ctx->M <= kResidentMol && groups_of(ctx) == 1 && !ctx->timing &&
-           resident_lds_bytes(ctx->N, ctx->M, ctx->max_n) + kTableBytes + 1024 <= 65536;
+    const bool chosen = ctx->pair_layout == CGPPD_LAYOUT_RESIDENT ||
+                        (ctx->pair_layout == CGPPD_LAYOUT_AUTO && resident_small(ctx));
+    return chosen && !ctx->n_flex && !ctx->da && !ctx->rep_crowd && ctx->N <= kResidentRes && ctx->M <= kResidentMol &&
+           groups_of(ctx) == 1 && !ctx->timing && resident_lds_bytes(ctx->N, ctx->M, ctx->max_n) + kTableBytes + 1024 <= 65536;
 }
 
 // n_steps resident MC steps, at most 16384 per launch.

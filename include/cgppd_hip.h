@@ -272,7 +272,10 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
 /* Layout of the pair kernel for systems without flexible molecules.
  * Replaces the reference's block-size choice (Replica::init_first_replica,
  * src/Replica.cpp:59-68: 32 threads below 1024 residues, else 64, or -d).
- *   CGPPD_LAYOUT_AUTO   the autotuned choice, else j-lanes
+ *   CGPPD_LAYOUT_AUTO   resident MC steps for systems the resident kernel
+ *                       takes with at most 16384 residue pairs per replica
+ *                       step (averaged over the molecules), else the
+ *                       autotuned choice, else j-lanes
  *   CGPPD_LAYOUT_ILANES lanes hold the moved molecule's residues, the other
  *                       residues are staged through LDS in 256-residue chunks
  *   CGPPD_LAYOUT_JLANES lanes hold two of the other residues each, the moved
@@ -288,6 +291,8 @@ int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
  *                       group; other systems, cgppd_autotune and timed
  *                       launches use the j-lane kernel.  cgppd_get_pair_launch
  *                       reports CGPPD_LAYOUT_RESIDENT when it applies.
+ *                       Chosen explicitly for larger systems it is slower
+ *                       than the j-lane kernel (one workgroup per replica). 
  * Systems with flexible molecules always use the i-lane kernel. */
 #define CGPPD_LAYOUT_AUTO 0
 #define CGPPD_LAYOUT_ILANES 1
