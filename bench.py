@@ -352,26 +352,16 @@ def main():
     pairs = e.pair_evaluations() - p0
     s1 = e.stats()
 
-    if layout == "resident":
-        # the resident kernel runs all the steps of a launch: one launch of
-        # n_timed steps, timed by the host around it, per step
-        n_timed = 2000
-        p1 = e.pair_evaluations()
-        e.synchronize()
-        t1 = time.perf_counter()
-        e.mc_steps(n_timed)
-        e.synchronize()
-        kern_ms = (time.perf_counter() - t1) / n_timed * 1e3
-        pairs_timed = e.pair_evaluations() - p1
-    else:
-        # kernel timing pass (direct launches bracketed by HIP events)
-        e.set_timing(True)
-        p1 = e.pair_evaluations()
-        n_timed = 500
-        e.mc_steps(n_timed)
-        kern_ms, _ = e.last_kernel_ms()
-        pairs_timed = e.pair_evaluations() - p1
-        e.set_timing(False)
+    # kernel timing pass: direct launches bracketed by HIP events on the
+    # context stream (the pair kernel: each launch; the resident kernel, which
+    # runs all the steps in one launch: its span per MC step)
+    e.set_timing(True)
+    p1 = e.pair_evaluations()
+    n_timed = 2000 if layout == "resident" else 500
+    e.mc_steps(n_timed)
+    kern_ms, _ = e.last_kernel_ms()
+    pairs_timed = e.pair_evaluations() - p1
+    e.set_timing(False)
     achieved = pairs_timed / n_timed * FLOPS_PER_PAIR / (kern_ms * 1e-3) / 1e12
 
     # acceptance of the timed steps (counters follow the temperatures, so

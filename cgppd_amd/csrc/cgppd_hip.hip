@@ -684,9 +684,35 @@ __device__ __forceinline__ void intra_item(const float4 *__restrict__ src, const
     const int j0 = jc * kIntraChunk + wave * kPerWave;
     // a wave whose j residues all precede its i block has no pair b > a
     const int j1 = j0 + kPerWave - 1 <= ib * kIBlock ? j0 : min(n, j0 + kPerWave);
-    for (int b = j0; b < j1; ++b) {
-        const float4 pb = jrec[b - jc * kIntraChunk];
-        const int4 tb = jtag[b - jc * kIntraChunk];
+    // four j residues at a time: their records, tags and table entries are
+    // read before any term is formed (the terms enter the sums in j order)
+    const int jb = jc * kIntraChunk;
+    const uint32_t col = lane_col(pa);
+    int b = j0;
+    for (; b + 4 <= j1; b += 4) {
+        float4 pb[4];
+        int4 tb[4];
+        Coef cf[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pb[u] = jrec[b + u - jb];
+            tb[u] = jtag[b + u - jb];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cf[u] = *reinterpret_cast<const Coef *>(tab + ((__float_as_uint(pb[u].w) & kRowMask) | col));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float lj, dh;
+            pair_term<true, true>(pb[u], pa.x, pa.y, pa.z, cf[u], p.box, p.inv_box, lj, dh);
+            if (valid && b + u > a && !intra_excluded(ta, tb[u])) {
+                kahan_add(s0, c0, lj);
+                kahan_add(s1, c1, dh);
+            }
+        }
+    }
+    for (; b < j1; ++b) {
+        const float4 pb = jrec[b - jb];
+        const int4 tb = jtag[b - jb];
         float lj, dh;
         pair_any(pa, pb, tab, p, lj, dh);
         if (valid && b > a && !intra_excluded(ta, tb)) {
@@ -1079,6 +1105,124 @@ __global__ __launch_bounds__(kWG) void intra_energy_kernel(PairParams p)
     }
 }
 
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) f4_t const_f4;
+
+// A wave-uniform record of the proposal through the scalar cache (the
+// proposal is not written while the pair kernels run)
+__device__ __forceinline__ f4_t scalar_rec(const float4 *base, int i)
+{
+    return ((const_f4 *)base)[i];
+}
+
+// Wave units of a flexible molecule's intra-molecular pairs in the MC pair
+// kernel: unit (ib, jr) is the i block ib (64 residues, one per lane)
+// against the kUnitJ residues of j range jr.  A range wholly before the i
+// block (jr < kUnitRatio ib) holds no pair b > a and is not listed; the
+// units run i block by i block, j ranges ascending.
+constexpr int kUnitJ = 32;
+constexpr int kUnitRatio = kIBlock / kUnitJ;  // j ranges per i block
+static_assert(kIBlock == kUnitRatio * kUnitJ, "an i block spans whole j ranges");
+
+__host__ __device__ inline int intra_units(int n)
+{
+    const int nib = (n + kIBlock - 1) / kIBlock, nr = (n + kUnitJ - 1) / kUnitJ;
+    int u = 0;
+    for (int ib = 0; ib < nib; ++ib) u += nr - kUnitRatio * ib > 0 ? nr - kUnitRatio * ib : 0;
+    return u;
+}
+
+// (i block, j range) of unit u < intra_units(n)
+__device__ __forceinline__ int2 intra_unit_of(int u, int n)
+{
+    const int nr = (n + kUnitJ - 1) / kUnitJ;
+    int ib = 0;
+    while (u >= nr - kUnitRatio * ib) {
+        u -= nr - kUnitRatio * ib;
+        ++ib;
+    }
+    return make_int2(ib, kUnitRatio * ib + u);
+}
+
+// Part [b0, b1) of a wave unit's j range (records and tags staged at
+// jrec[b - j0], jtag[b - j0]) against this lane's residue a (record pa, tag
+// ta, table column col), pairs b > a outside the exclusions; four j
+// residues are read at a time and the terms enter the sums in b order.
+// MINIMG: the per-pair minimum image.  CHARGED: the j residues carry
+// charge (else the Debye-Hueckel term is zero and skipped).
+template <bool MINIMG, bool CHARGED>
+__device__ __forceinline__ void intra_run(int b0, int b1, int j0, int a, float4 pa, int4 ta, uint32_t col,
+                                          const float4 *jrec, const int4 *jtag, const PairParams &p,
+                                          const char *__restrict__ tab, float &s0, float &c0, float &s1, float &c1)
+{
+    int b = b0;
+    for (; b + 4 <= b1; b += 4) {
+        float4 rb[4];
+        int4 gb[4];
+        Coef cf[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            rb[u] = jrec[b + u - j0];
+            gb[u] = jtag[b + u - j0];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            cf[u] = *reinterpret_cast<const Coef *>(tab + ((__float_as_uint(rb[u].w) & kRowMask) | col));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float lj, dh;
+            pair_term<MINIMG, CHARGED>(rb[u], pa.x, pa.y, pa.z, cf[u], p.box, p.inv_box, lj, dh);
+            if (b + u > a && !intra_excluded(ta, gb[u])) {
+                kahan_add(s0, c0, lj);
+                if (CHARGED) kahan_add(s1, c1, dh);
+            }
+        }
+    }
+    for (; b < b1; ++b) {
+        const float4 rb = jrec[b - j0];
+        const int4 gb = jtag[b - j0];
+        const Coef cf = *reinterpret_cast<const Coef *>(tab + ((__float_as_uint(rb.w) & kRowMask) | col));
+        float lj, dh;
+        pair_term<MINIMG, CHARGED>(rb, pa.x, pa.y, pa.z, cf, p.box, p.inv_box, lj, dh);
+        if (b > a && !intra_excluded(ta, gb)) {
+            kahan_add(s0, c0, lj);
+            if (CHARGED) kahan_add(s1, c1, dh);
+        }
+    }
+}
+
+// One wave unit: this lane's residue a = ib * 64 + lane of a molecule of n
+// residues (molecule-local coordinates src[], tags tg[]) against the
+// residues b > a of j range jr, whose records and tags the caller staged in
+// LDS, the first nq of them charged (the device order puts a molecule's
+// charged residues first).  Kahan-compensated FP32 sums of LJ and DH.
+// MINIMG: the per-pair minimum image; without it the plain differences
+// (the caller's molecule spans less than half the box, so the minimum
+// image is the plain difference).
+template <bool MINIMG>
+__device__ __forceinline__ void intra_unit(const float4 *__restrict__ src, const int4 *__restrict__ tg, int n, int ib,
+                                           int jr, int nq, const float4 *jrec, const int4 *jtag, const PairParams &p,
+                                           const char *__restrict__ tab, float &s0, float &c0, float &s1, float &c1,
+                                           bool &valid)
+{
+    const int a0 = ib * kIBlock + (threadIdx.x & (kWave - 1));
+    valid = a0 < n;
+    const float4 pa = src[valid ? a0 : 0];
+    const int4 ta = tg[valid ? a0 : 0];
+    const int a = valid ? a0 : 0x7fffffff;  // (no b > a for lanes past the molecule)
+    const uint32_t col = lane_col(pa);
+    const int j0 = jr * kUnitJ, j1 = min(n, j0 + kUnitJ), jq = j0 + nq;
+    intra_run<MINIMG, true>(j0, jq, j0, a, pa, ta, col, jrec, jtag, p, tab, s0, c0, s1, c1);
+    intra_run<MINIMG, false>(jq, j1, j0, a, pa, ta, col, jrec, jtag, p, tab, s0, c0, s1, c1);
+}
+
+// Inter-molecular items of the i-lane kernel for a moved molecule of n
+// residues in nib i blocks: none when the molecule is alone in the system.
+__device__ __forceinline__ int ilane_inter_items(const PairParams &p, int nib, int n)
+{
+    return p.n_res > n ? nib * p.jc_count : 0;
+}
+
 // MC step, phase 1 (pair kernel): the delta-E row of every replica's moved
 // molecule against all other molecules, FP32 pair terms only.  The proposed
 // residue positions of the moved molecule were laid out by the previous
@@ -1087,14 +1231,16 @@ __global__ __launch_bounds__(kWG) void intra_energy_kernel(PairParams p)
 // i-block, 256-residue j-chunk) triples, listed by prefix sum over the
 // replicas' i-block counts and split into equal contiguous ranges -- one per
 // workgroup, workgroups ordered by XCD so a replica's items share an L2.
-// FLEX: a flexible moved molecule adds its intra-molecular (i-block,
-// j-chunk) items and one bonded item, summed into the row's diagonal entry.
+// FLEX: a flexible moved molecule adds its intra-molecular items, summed
+// into the row's diagonal entry: after a move of all its residues, four
+// wave units (intra_unit) per item; after a one-residue move, the moved
+// residue's pairs (moved_item).
 template <bool FLEX>
 __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 {
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ float4 jrec[kJChunk];
-    __shared__ int4 jtag[FLEX ? kJChunk : 1];
+    __shared__ int4 jtag[FLEX ? kWavesPerWG * kUnitJ : 1];
     __shared__ int item_off[CGPPD_MAX_REPLICAS + 1];
     // set-up and item staging at high wave priority, the pair loops at
     // normal priority (see mc_row_kernel)
@@ -1147,22 +1293,55 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
     for (int k = k0; k < k1; ++k) {
         while (k >= item_off[r + 1]) ++r;
         const int local = k - item_off[r];
-        if (FLEX && local >= p.prop_nib[r] * JC) {
-            const int m = p.prop_mol[r];
-            const int i0 = p.mol_off[m], n = p.mol_off[m + 1] - i0;
-            const int nib = p.prop_nib[r], njc = intra_chunks(n);
-            const int l2 = local - nib * JC;
+        // (the proposal header holds the moved molecule, its first residue
+        // and its size: one load level)
+        const int4 hd = FLEX ? p.prop_hdr[r] : make_int4(0, 0, 0, 0);
+        const int4 mv = FLEX ? p.prop_moved[r] : make_int4(0, 0, 0, 0);
+        const int n_inter = FLEX ? ilane_inter_items(p, (hd.z + kIBlock - 1) / kIBlock, hd.z) : 0;
+        if (FLEX && local >= n_inter) {
+            const int m = hd.x, i0 = hd.y, n = hd.z;
+            const int l2 = local - n_inter;
             const float4 *src = p.prop_pos + (size_t)r * p.max_n;
-            const int4 mv = p.prop_moved[r];
-            double v;
+            double v = 0.0;
             // (the bonded terms were added to the diagonal when the proposal
             // was laid out)
             __builtin_amdgcn_s_setprio(0);
             if (mv.x == kMovedAll) {
-                // the intra-molecular pairs of the proposal
+                // the intra-molecular pairs of the proposal: this wave's
+                // unit, its j range staged in the wave's part of jrec / jtag
+                const int u = l2 * kWavesPerWG + wave;
+                const bool has = u < intra_units(n);
+                const int2 q = has ? intra_unit_of(u, n) : make_int2(0, 0);
+                float4 *wrec = jrec + wave * kUnitJ;
+                int4 *wtag = jtag + wave * kUnitJ;
+                __builtin_amdgcn_s_setprio(2);
+                __syncthreads();  // the previous item finished reading jrec
+                int nq;
+                {
+                    const int j = q.y * kUnitJ + lane;
+                    const bool mine = has && lane < kUnitJ && j < n;
+                    float4 rj = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (mine) {
+                        rj = src[j];
+                        wrec[lane] = rj;
+                        wtag[lane] = p.rtag[i0 + j];
+                    }
+                    // the range's charged residues (a prefix of it)
+                    nq = __popcll(__ballot(mine && (__float_as_uint(rj.w) & kChargedBit)));
+                }
+                __syncthreads();
+                if (!has) continue;
+                __builtin_amdgcn_s_setprio(0);
                 float s0 = 0.f, c0 = 0.f, s1 = 0.f, c1 = 0.f;
                 bool valid;
-                intra_item(src, p.rtag + i0, n, l2 / njc, l2 % njc, p, tab, jrec, jtag, s0, c0, s1, c1, valid);
+                // every residue lies within the proposal's bounding radius of
+                // its centre: below a quarter box (less a margin for the
+                // rounding of centre plus relative position) no pair needs
+                // an image
+                if (4.0f * p.prop_radius[r] < 0.98f * p.box)
+                    intra_unit<false>(src, p.rtag + i0, n, q.x, q.y, nq, wrec, wtag, p, tab, s0, c0, s1, c1, valid);
+                else
+                    intra_unit<true>(src, p.rtag + i0, n, q.x, q.y, nq, wrec, wtag, p, tab, s0, c0, s1, c1, valid);
                 v = valid ? ((double)s0 - (double)c0) + ((double)s1 - (double)c1) : 0.0;
             } else {
                 // the moved residue's pairs, new minus current
@@ -1228,15 +1407,6 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 // accumulator; at the end of a j-block visit the lanes of each molecule J
 // are reduced in FP64 and added atomically into the row.
 // ---------------------------------------------------------------------------
-typedef float f4_t __attribute__((ext_vector_type(4)));
-typedef const __attribute__((address_space(4))) f4_t const_f4;
-
-// i record of the proposal through the scalar cache (the proposal is not
-// written while the row kernel runs)
-__device__ __forceinline__ f4_t scalar_rec(const float4 *base, int i)
-{
-    return ((const_f4 *)base)[i];
-}
 
 // Image handling of the row kernel, wave-uniform per j-block visit.  The
 // lanes' j coordinates are shifted by their molecule's centre image (the
@@ -1647,9 +1817,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
 __device__ __forceinline__ int ilane_items(const PairParams &p, int m, int n, int kind)
 {
     const int nib = (n + kIBlock - 1) / kIBlock;
-    int items = nib * p.jc_count;
+    int items = p.n_flex ? ilane_inter_items(p, nib, n) : nib * p.jc_count;
     if (p.internal && p.n_flex && p.mol_flex[m])
-        items += kind == kMovedOne ? (n + kJChunk - 1) / kJChunk : nib * intra_chunks(n);
+        items += kind == kMovedOne ? (n + kJChunk - 1) / kJChunk : (intra_units(n) + kWavesPerWG - 1) / kWavesPerWG;
     return items;
 }
 
@@ -2394,14 +2564,14 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
     double *s_part = reinterpret_cast<double *>(s_cen + M);  // 2 x kWavesPerWG x M: the waves' row slots
     int *s_mol = reinterpret_cast<int *>(s_part + 2 * kWavesPerWG * M);  // N: molecule of each residue
     int *s_off = s_mol + N;                                              // M + 1: molecule offsets
-    // the table (entry e = tj * 20 + ti at row tj, column ti, kcal/mol) and
-    // the replica's state
+    // the table (entry e = tj * 20 + ti, i = the proposal's residue, at row
+    // ti, column tj, kcal/mol) and the replica's state
     {
         constexpr int kEntries = CGPPD_AA_COUNT * CGPPD_AA_COUNT;
         const int e0 = t, e1 = min(t + kWG, kEntries - 1);
         const Coef g0 = p.table[e0], g1 = p.table[e1];
-        *reinterpret_cast<Coef *>(tab + (((e0 / CGPPD_AA_COUNT) << kRowShift) | ((e0 % CGPPD_AA_COUNT) << kColShift))) = g0;
-        *reinterpret_cast<Coef *>(tab + (((e1 / CGPPD_AA_COUNT) << kRowShift) | ((e1 % CGPPD_AA_COUNT) << kColShift))) = g1;
+        *reinterpret_cast<Coef *>(tab + (((e0 % CGPPD_AA_COUNT) << kRowShift) | ((e0 / CGPPD_AA_COUNT) << kColShift))) = g0;
+        *reinterpret_cast<Coef *>(tab + (((e1 % CGPPD_AA_COUNT) << kRowShift) | ((e1 / CGPPD_AA_COUNT) << kColShift))) = g1;
     }
     for (int k = t; k < N; k += kWG) {
         s_pos[k] = p.pos[(size_t)r * p.stride + k];
@@ -2468,7 +2638,10 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
             const int j = min(jp + (jp >= i0 ? n_m : 0), N - 1);
             const float4 rj = s_pos[j];
             const int J = s_mol[j];
-            const uint32_t jrow = __float_as_uint(rj.w) & kRowMask;
+            // row = the proposal residue's type (the same for every lane),
+            // column = the j residue's: a wave's lanes read one 320-byte row
+            // and do not collide in LDS banks
+            const uint32_t jcol = lane_col(rj);
             float sum = 0.0f, comp = 0.0f;
             // four proposal residues at a time: their records and table
             // entries are read before any term is formed
@@ -2479,7 +2652,8 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
 #pragma unroll
                 for (int u = 0; u < 4; ++u) pi[u] = prop[i + u];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) cf[u] = *reinterpret_cast<const Coef *>(tab + (jrow | lane_col(pi[u])));
+                for (int u = 0; u < 4; ++u)
+                    cf[u] = *reinterpret_cast<const Coef *>(tab + ((__float_as_uint(pi[u].w) & kRowMask) | jcol));
                 float g = 0.0f;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -2490,8 +2664,10 @@ __global__ __launch_bounds__(kWG) void mc_resident_kernel(PairParams p, Resident
                 kahan_add(sum, comp, g);
             }
             for (; i < ib; ++i) {
+                const float4 pi = prop[i];
+                const Coef cf = *reinterpret_cast<const Coef *>(tab + ((__float_as_uint(pi.w) & kRowMask) | jcol));
                 float lj, dh;
-                pair_any(prop[i], rj, tab, p, lj, dh);
+                pair_term<true, true>(rj, pi.x, pi.y, pi.z, cf, p.box, p.inv_box, lj, dh);
                 kahan_add(sum, comp, lj + dh);
             }
             const double v = valid ? (double)sum - (double)comp : 0.0;
@@ -3181,6 +3357,7 @@ struct cgppd_ctx {
     bool topo_dirty = true;
     int n_flex = 0;
     std::vector<int> mol_flex_host;  // M: 1 = flexible
+    int ilane_max_items = 0;          // most i-lane items one replica's proposal can have (set by mc_prepare)
     int *d_flex_mols = nullptr, *d_flex_item_off = nullptr, *d_mol_flex = nullptr;
     int4 *d_rtag = nullptr;
     int *d_bond_off = nullptr, *d_angle_off = nullptr, *d_tors_off = nullptr, *d_ind_off = nullptr;
@@ -3990,8 +4167,12 @@ int launch_pair_p(cgppd_ctx *ctx, PairParams p, hipStream_t stream, hipEvent_t e
     p.table = ctx->d_table_scaled;
     p.xcd_map = 1;
     const int layout = pair_layout_of(ctx);
-    const int g = std::max(8, ctx->n_cu * pair_wg_per_cu(ctx, layout) / 8 * 8);
+    int g = std::max(8, ctx->n_cu * pair_wg_per_cu(ctx, layout) / 8 * 8);
     if (ctx->n_flex) {
+        // no more workgroups than the replicas' proposals can have items: a
+        // workgroup without one only delays the dispatch of the others
+        const long long most = (long long)ctx->ilane_max_items * p.rep_count;
+        g = (int)std::max(8LL, std::min((long long)g, (most + 7) / 8 * 8));
         if (ev_start)
             hipExtLaunchKernelGGL(mc_pair_kernel<true>, dim3(g), dim3(kWG), 0, stream, ev_start, ev_stop, 0, p);
         else
@@ -4088,10 +4269,11 @@ bool resident_active(const cgppd_ctx *ctx)
     const bool chosen = ctx->pair_layout == CGPPD_LAYOUT_RESIDENT ||
                         (ctx->pair_layout == CGPPD_LAYOUT_AUTO && resident_small(ctx));
     return chosen && !ctx->n_flex && !ctx->da && !ctx->rep_crowd && ctx->N <= kResidentRes && ctx->M <= kResidentMol &&
-           groups_of(ctx) == 1 && !ctx->timing && resident_lds_bytes(ctx->N, ctx->M, ctx->max_n) + kTableBytes + 1024 <= 65536;
+           groups_of(ctx) == 1 && resident_lds_bytes(ctx->N, ctx->M, ctx->max_n) + kTableBytes + 1024 <= 65536;
 }
 
-// n_steps resident MC steps, at most 16384 per launch.
+// n_steps resident MC steps, at most kResidentChunk per launch.
+constexpr int kResidentChunk = 16384;
 int launch_resident_steps(cgppd_ctx *ctx, int n_steps)
 {
     PairParams p = base_params(ctx);
@@ -4099,7 +4281,7 @@ int launch_resident_steps(cgppd_ctx *ctx, int n_steps)
     const size_t lds = resident_lds_bytes(ctx->N, ctx->M, ctx->max_n);
     for (int done = 0; done < n_steps;) {
         ResidentParams rp;
-        rp.steps = std::min(n_steps - done, 16384);
+        rp.steps = std::min(n_steps - done, kResidentChunk);
         hipLaunchKernelGGL(mc_resident_kernel, dim3(ctx->R), dim3(kWG), lds, ctx->stream, p, rp);
         CG_HIP(ctx, hipGetLastError());
         done += rp.steps;
@@ -4797,6 +4979,20 @@ int cgppd_mc_prepare(cgppd_ctx *ctx)
     if ((rc = zero_dev(ctx, ctx->d_emat, (size_t)ctx->R * ctx->M * ctx->M))) return rc;
     if ((rc = launch_full(ctx, 0, ctx->R, ctx->M, true, nullptr))) return rc;
     if ((rc = launch_intra(ctx, 0, ctx->R, true, nullptr, nullptr, nullptr))) return rc;
+    // the most i-lane items one proposal can have (ilane_items): systems
+    // with flexible molecules size the pair kernel's grid by it
+    {
+        const int JC = (ctx->N + kJChunk - 1) / kJChunk;
+        int most = 0;
+        for (int m = 0; m < ctx->M; ++m) {
+            const int n = ctx->mol_off[m + 1] - ctx->mol_off[m], nib = (n + kIBlock - 1) / kIBlock;
+            int it = ctx->n_flex && ctx->N == n ? 0 : nib * JC;
+            if (!ctx->rigid_only && ctx->n_flex && ctx->mol_flex_host[m])
+                it += std::max((n + kJChunk - 1) / kJChunk, (intra_units(n) + kWavesPerWG - 1) / kWavesPerWG);
+            most = std::max(most, it);
+        }
+        ctx->ilane_max_items = most;
+    }
     PairParams p = base_params(ctx);
     hipLaunchKernelGGL(prepare_finish_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p);
     CG_HIP(ctx, hipGetLastError());
@@ -4826,6 +5022,20 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
     if (n_steps < 0) return fail(ctx, CGPPD_E_ARG, "negative step count");
     CG_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_stage1(ctx))) return rc;
+    if (ctx->timing && n_steps > 0 && resident_active(ctx)) {
+        // the resident kernel runs the steps in ceil(n_steps / kResidentChunk)
+        // launches: their span on the context stream, per MC step
+        if ((rc = ensure_events(ctx, 2))) return rc;
+        CG_HIP(ctx, hipEventRecord(ctx->events[0], ctx->stream));
+        if ((rc = launch_resident_steps(ctx, n_steps))) return rc;
+        CG_HIP(ctx, hipEventRecord(ctx->events[1], ctx->stream));
+        CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        CG_HIP(ctx, hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+        ctx->t_mean_ms = (double)ms / n_steps;
+        ctx->t_launches = (n_steps + kResidentChunk - 1) / kResidentChunk;
+        return CGPPD_OK;
+    }
     if (ctx->timing && n_steps > 0) {
         if ((rc = ensure_events(ctx, (size_t)2 * n_steps))) return rc;
         for (int k = 0; k < n_steps; ++k) {

@@ -265,8 +265,11 @@ int cgppd_get_energy_matrix(cgppd_ctx *ctx, int replica, double *mat);
  * cgppd_set_launch: workgroups per CU of the persistent pair kernel
  * (0 = automatic).  cgppd_last_kernel_ms: mean duration in ms of the pair
  * kernel launches of the last cgppd_mc_steps / cgppd_run call, measured with
- * HIP events on the context stream (0 when not measured);
- * cgppd_set_timing(ctx, 1) enables the measurement. */
+ * HIP events on the context stream (0 when not measured); with the
+ * resident layout (CGPPD_LAYOUT_RESIDENT) active, the span of the resident
+ * kernel's launches divided by the MC steps they ran, and `launches` counts
+ * those launches (one per 16384 steps).  cgppd_set_timing(ctx, 1) enables
+ * the measurement. */
 int cgppd_set_launch(cgppd_ctx *ctx, int wg_per_cu);
 
 /* Layout of the pair kernel for systems without flexible molecules.

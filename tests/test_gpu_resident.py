@@ -166,3 +166,20 @@ def test_resident_rows_of_more_than_64_molecules(gpu_engine_factory):
     _, _, full = e.energy_all()
     np.testing.assert_allclose(e.stats()["energy"], full, rtol=1e-5, atol=1e-8)
     e.close()
+
+
+def test_resident_timing_keeps_the_resident_kernel(gpu_engine_factory):
+    """Timed steps still run the resident kernel (the oracle's trajectory)
+    and report its span per MC step."""
+    spec = small_spec(4)
+    e = resident_engine(gpu_engine_factory, spec, 21)
+    st = oracle.Remc(spec, seed=21)
+    e.set_timing(True)
+    e.mc_steps(70)
+    ms, launches = e.last_kernel_ms()
+    e.set_timing(False)
+    st.mc_steps(70)
+    assert e.pair_layout() == "resident"
+    assert launches == 1 and 0.0 < ms < 1.0
+    assert_matches(e, st, spec)
+    e.close()
