@@ -86,15 +86,17 @@ KERNEL_NAMES = {"ilanes": "mc_pair_kernel", "jlanes": "mc_row_kernel", "jlanes4"
                 "resident": "mc_resident_kernel"}
 
 
-def committed_traffic(kernel, workload):
-    """HBM bytes per pair-kernel launch from the committed PMC summary
+def committed_traffic(kernel, workload, key="bytes_per_launch"):
+    """HBM bytes per pair-kernel launch (key bytes_per_launch) or, for the
+    resident kernel, whose one launch runs many MC steps, per MC step (key
+    bytes_per_step), from the committed PMC summary
     profiles/<kernel>_hbm_traffic_<workload>.txt (line
-    'bytes_per_launch <workload> <value>'), or None."""
+    '<key> <workload> <value>'), or None."""
     path = ROOT / "profiles" / f"{kernel}_hbm_traffic_{workload}.txt"
     try:
         for line in path.read_text().splitlines():
             f = line.split()
-            if len(f) == 3 and f[0] == "bytes_per_launch" and f[1] == workload:
+            if len(f) == 3 and f[0] == key and f[1] == workload:
                 return float(f[2])
     except OSError:
         pass
@@ -280,13 +282,26 @@ def da_leg(args, layout, wg_per_cu):
 
 def energy_error(e, spec):
     """Running totals vs a fresh FP64 CPU evaluation of replica 0."""
-    import oracle
-
     s = e.stats()
     pos, _, _ = e.coordinates(0)
-    flj, fdh = oracle.energy_fp64(spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps, pos, spec.box)
-    want = oracle.total(flj, fdh)
+    want = fp64_energy(spec, pos)
     return abs(s["energy"][0] - want) / max(abs(want), 1e-30)
+
+
+def fp64_energy(spec, pos):
+    """FP64 oracle energy (kcal/mol) of one replica's coordinates: the
+    inter-molecular pairs plus, for flexible molecules, their internal
+    energy (intra-molecular pairs and bonded terms)."""
+    import oracle
+
+    args = (spec.mol_off, spec.types, spec.ff.charge, spec.ff.sigma, spec.ff.eps)
+    want = oracle.total(*oracle.energy_fp64(*args, pos, spec.box))
+    if getattr(spec, "flexible", False):
+        fl = oracle.Flex(spec, spec.torsion_v, spec.torsion_delta)
+        for m in range(spec.n_mol):
+            if fl.mol_flexible[m]:
+                want += oracle.internal_total(oracle.internal(fl, m, *args, pos, spec.box, 0, True))
+    return want
 
 
 def main():
@@ -410,10 +425,14 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": committed_traffic(KERNEL_NAMES[layout], spec.name),
+            "traffic": committed_traffic(KERNEL_NAMES[layout], spec.name,
+                                         "bytes_per_step" if layout == "resident" else "bytes_per_launch"),
             "kernel_ms": kern_ms,
             "pairs_per_launch": pairs_timed / n_timed,
             "flops_per_pair": FLOPS_PER_PAIR,
+            # the resident kernel runs all n_timed steps in one launch: its
+            # time, pairs and traffic are given per MC step
+            "per": "MC step" if layout == "resident" else "launch",
         },
     }
     if not args.no_cpu_baseline:
