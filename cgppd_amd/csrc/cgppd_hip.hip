@@ -836,6 +836,57 @@ __device__ __forceinline__ double torsion_term(float4 a, float4 b, float4 c, flo
 // coordinates differ between src and old), the terms without it are skipped.
 // GLOBAL_TABLE: the indirect-neighbour pairs read the scaled table from
 // global memory (kernels without the LDS table), else the LDS table tab.
+constexpr int kBondedBatch = 4;
+
+// The bonded terms of flexible molecule m in one index space, the costliest
+// kinds first (torsions, angles, bonds, indirect pairs), each kind's range
+// padded to whole waves so that a wave evaluates one kind at a time.
+struct BondedSpace {
+    int i0, to, ao, bo, io, nt, na, nb, qa, qb, qi, total;
+    __device__ BondedSpace(const PairParams &p, int m)
+    {
+        i0 = p.mol_off[m];
+        to = p.tors_off[m], ao = p.angle_off[m], bo = p.bond_off[m], io = p.ind_off[m];
+        nt = p.tors_off[m + 1] - to, na = p.angle_off[m + 1] - ao;
+        nb = p.bond_off[m + 1] - bo;
+        const int ni = p.ind_off[m + 1] - io;
+        qa = (nt + kWave - 1) & ~(kWave - 1), qb = qa + ((na + kWave - 1) & ~(kWave - 1));
+        qi = qb + ((nb + kWave - 1) & ~(kWave - 1)), total = qi + ni;
+    }
+};
+
+// The kind (0 torsion, 1 angle, 2 bond, 3 indirect pair, -1 none) and
+// residues of the terms q0 + u * blockDim.x, u < kBondedBatch (unused
+// residue slots name the molecule's first residue).
+struct BondedBatch {
+    int kind[kBondedBatch];
+    int4 e[kBondedBatch];
+};
+
+__device__ __forceinline__ void bonded_batch(const PairParams &p, const BondedSpace &sp, int q0, BondedBatch &bt)
+{
+    const int i0 = sp.i0;
+#pragma unroll
+    for (int u = 0; u < kBondedBatch; ++u) {
+        const int q = q0 + u * blockDim.x;
+        bt.kind[u] = q >= sp.total ? -1
+                     : q < sp.qa   ? (q < sp.nt ? 0 : -1)
+                     : q < sp.qb   ? (q - sp.qa < sp.na ? 1 : -1)
+                     : q < sp.qi   ? (q - sp.qb < sp.nb ? 2 : -1)
+                                   : 3;
+        bt.e[u] = make_int4(i0, i0, i0, i0);
+        if (bt.kind[u] == 0) {
+            bt.e[u] = p.tors[sp.to + q];
+        } else if (bt.kind[u] == 1) {
+            const int4 a = p.angles[sp.ao + q - sp.qa];
+            bt.e[u] = make_int4(a.x, a.y, a.z, i0);
+        } else if (bt.kind[u] >= 2) {
+            const int2 b = bt.kind[u] == 2 ? p.bonds[sp.bo + q - sp.qb] : p.ind[sp.io + q - sp.qi];
+            bt.e[u] = make_int4(b.x, b.y, i0, i0);
+        }
+    }
+}
+
 template <bool GLOBAL_TABLE>
 __device__ void bonded_terms(const float4 *__restrict__ src, int i0, int m, const PairParams &p,
                              const char *__restrict__ tab, double t[5], const float4 *__restrict__ old = nullptr,
@@ -843,40 +894,20 @@ __device__ void bonded_terms(const float4 *__restrict__ src, int i0, int m, cons
 {
 #pragma clang fp contract(off)
     for (int k = 0; k < 5; ++k) t[k] = 0.0;
-    // one index space over the terms, the costliest kinds first (torsions,
-    // angles, bonds, indirect pairs), each kind's range padded to whole
-    // waves so that a wave evaluates one kind at a time: thread t takes
-    // entries t, t + block, ..., kBatch at a time (their indices, then their
-    // coordinates, then the arithmetic)
-    const int to = p.tors_off[m], ao = p.angle_off[m], bo = p.bond_off[m], io = p.ind_off[m];
-    const int nt = p.tors_off[m + 1] - to, na = p.angle_off[m + 1] - ao;
-    const int nb = p.bond_off[m + 1] - bo, ni = p.ind_off[m + 1] - io;
-    const int qa = (nt + kWave - 1) & ~(kWave - 1), qb = qa + ((na + kWave - 1) & ~(kWave - 1));
-    const int qi = qb + ((nb + kWave - 1) & ~(kWave - 1)), total = qi + ni;
+    // thread t takes entries t, t + block, ..., kBondedBatch at a time
+    // (their indices, then their coordinates, then the arithmetic)
+    const BondedSpace sp(p, m);
     const int g = i0 + only;  // device index of the only moved residue
-    constexpr int kBatch = 4;
-    for (int q0 = threadIdx.x; q0 < total; q0 += kBatch * blockDim.x) {
+    constexpr int kBatch = kBondedBatch;
+    for (int q0 = threadIdx.x; q0 < sp.total; q0 += kBatch * blockDim.x) {
+        BondedBatch bt;
+        bonded_batch(p, sp, q0, bt);
         int kind[kBatch];
         int4 e[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const int q = q0 + u * blockDim.x;
-            kind[u] = q >= total ? -1
-                      : q < qa   ? (q < nt ? 0 : -1)
-                      : q < qb   ? (q - qa < na ? 1 : -1)
-                      : q < qi   ? (q - qb < nb ? 2 : -1)
-                                 : 3;
-            // unused residue slots name the molecule's first residue
-            e[u] = make_int4(i0, i0, i0, i0);
-            if (kind[u] == 0) {
-                e[u] = p.tors[to + q];
-            } else if (kind[u] == 1) {
-                const int4 a = p.angles[ao + q - qa];
-                e[u] = make_int4(a.x, a.y, a.z, i0);
-            } else if (kind[u] >= 2) {
-                const int2 b = kind[u] == 2 ? p.bonds[bo + q - qb] : p.ind[io + q - qi];
-                e[u] = make_int4(b.x, b.y, i0, i0);
-            }
+            kind[u] = bt.kind[u];
+            e[u] = bt.e[u];
         }
         if (only >= 0) {
             // one residue moved and nothing else did: the other terms'
@@ -2202,8 +2233,8 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
     // words alone (src/Molecule.cpp:451-466, src/Replica.cpp:309-334)
     const int64_t next = propose_only ? step : step + 1;
     const Proposal nx = draw_proposal(p.seed, r + p.rep_rng, next, M);
-    const int kind2 = p.n_flex && p.mol_flex[nx.mol] ? (M > 1 ? mulhi_n(nx.kind_word, 4) : 1 + mulhi_n(nx.kind_word, 3))
-                                                     : 0;
+    const bool next_flex = p.n_flex && p.mol_flex[nx.mol];
+    const int kind2 = next_flex ? (M > 1 ? mulhi_n(nx.kind_word, 4) : 1 + mulhi_n(nx.kind_word, 3)) : 0;
     bool planned = false, copied = false;
     if (!propose_only) {
         // every load of the decision and of applying it is issued first
@@ -2283,12 +2314,22 @@ __global__ __launch_bounds__(kWG) void mc_finish_kernel(PairParams p, int propos
             }
             d += v - mat[(size_t)m * M + J];
         }
-        d = wave_total(d);
-        if (lane == 0) red[wave] = d;
-        __syncthreads();
+        // with at most 64 molecules every entry is wave 0's: thread 0 takes
+        // the decision from its wave's sum without a barrier, so waves 1-3
+        // meet wave 0 only once, after the decision (the other waves' sums
+        // are zero, so the value is the same)
+        const bool one_wave = M <= kWave;
+        if (!one_wave || wave == 0) d = wave_total(d);
+        if (!one_wave) {
+            if (lane == 0) red[wave] = d;
+            __syncthreads();
+        }
         if (t == 0) {
             double delta = 0.0;
-            for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
+            if (one_wave)
+                delta += d;
+            else
+                for (int w = 0; w < kWavesPerWG; ++w) delta += red[w];
             const Proposal pr = draw_proposal(p.seed, r + p.rep_rng, step, M);
             int acc = 0;
             if (p.da && !s1) {
