@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--layout", default="auto", choices=["auto", "ilanes", "jlanes", "jlanes4", "resident"],
                     help="pair-kernel layout (auto: tuned)")
     ap.add_argument("--no-autotune", action="store_true", help="skip the pair-kernel autotuner")
+    ap.add_argument("--window", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="MC steps per speculative window (0: automatic, 1: one step per launch pair)")
     ap.add_argument("--no-image-shift", action="store_true", help="per-pair minimum image everywhere")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -325,6 +327,7 @@ def main():
         e.set_pair_layout(args.layout)
     if args.no_image_shift:
         e.set_image_shift(False)
+    e.set_window(args.window)
     e.mc_prepare()
     R = spec.n_rep
     tuned = None
@@ -374,10 +377,15 @@ def main():
     p1 = e.pair_evaluations()
     n_timed = 2000 if layout == "resident" else 500
     e.mc_steps(n_timed)
-    kern_ms, _ = e.last_kernel_ms()
+    kern_ms, launches = e.last_kernel_ms()
     pairs_timed = e.pair_evaluations() - p1
     e.set_timing(False)
-    achieved = pairs_timed / n_timed * FLOPS_PER_PAIR / (kern_ms * 1e-3) / 1e12
+    # per launch of the row kernel (a window launch evaluates the rows of
+    # several MC steps; the pairs are the algorithmic ones, n_m (N - n_m)
+    # per step, without the energies between a window's proposals), or,
+    # for the resident kernel, per MC step
+    per_unit = n_timed if layout == "resident" else max(1, launches)
+    achieved = pairs_timed / per_unit * FLOPS_PER_PAIR / (kern_ms * 1e-3) / 1e12
 
     # acceptance of the timed steps (counters follow the temperatures, so
     # the sums over replicas are unaffected by exchanges)
@@ -406,6 +414,7 @@ def main():
             "flexible": bool(getattr(spec, "flexible", False)),
             "image_shift": not args.no_image_shift,
             "pair_layout": layout,
+            "window": e.window(),
             "wg_per_cu": e.pair_wg_per_cu(),
             "autotuned": tuned is not None,
         },
@@ -428,7 +437,8 @@ def main():
             "traffic": committed_traffic(KERNEL_NAMES[layout], spec.name,
                                          "bytes_per_step" if layout == "resident" else "bytes_per_launch"),
             "kernel_ms": kern_ms,
-            "pairs_per_launch": pairs_timed / n_timed,
+            "pairs_per_launch": pairs_timed / per_unit,
+            "mc_steps_per_launch": n_timed / per_unit,
             "flops_per_pair": FLOPS_PER_PAIR,
             # the resident kernel runs all n_timed steps in one launch: its
             # time, pairs and traffic are given per MC step

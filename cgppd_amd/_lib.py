@@ -58,6 +58,8 @@ SIGNATURES = {
     "cgppd_get_replica_groups": (C.c_int, [_ctxp, _i32p]),
     "cgppd_autotune": (C.c_int, [_ctxp, C.c_int, _i32p, _i32p, _f64p]),
     "cgppd_get_pair_launch": (C.c_int, [_ctxp, _i32p, _i32p]),
+    "cgppd_set_window": (C.c_int, [_ctxp, C.c_int]),
+    "cgppd_get_window": (C.c_int, [_ctxp, _i32p]),
     "cgppd_autotune_table": (C.c_int, [_ctxp, _f64p]),
     "cgppd_set_image_shift": (C.c_int, [_ctxp, C.c_int]),
     "cgppd_set_repulsive_crowding": (C.c_int, [_ctxp, C.c_int]),

@@ -114,15 +114,20 @@ struct RowCoef {
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
-                                       uint32_t k1, uint32_t out[4])
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+    return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+__host__ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                                uint32_t k1, uint32_t out[4])
 {
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
         uint32_t lo0 = 0xD2511F53u * c0;
-        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t hi0 = mulhi32(0xD2511F53u, c0);
         uint32_t lo1 = 0xCD9E8D57u * c2;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t hi1 = mulhi32(0xCD9E8D57u, c2);
         uint32_t n0 = hi1 ^ c1 ^ k0;
         uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
@@ -170,6 +175,49 @@ __device__ __forceinline__ Proposal draw_proposal(uint64_t seed, int r, int64_t 
     p.axis[2] = unit32(v[2]) - 0.5;
     p.u = unit53(w[2], w[3]);
     return p;
+}
+
+// The molecule draw_proposal picks at MC step `step` of replica r (the
+// host replays it to count the windows of a run).
+__host__ __device__ __forceinline__ int draw_mol(uint64_t seed, int r, int64_t step, int n_mol)
+{
+    uint32_t w[4];
+    philox((uint32_t)step, (uint32_t)((uint64_t)step >> 32), (uint32_t)r, kTagMC, (uint32_t)seed,
+           (uint32_t)(seed >> 32), w);
+    return (int)(((uint64_t)w[0] * (uint64_t)n_mol) >> 32);
+}
+
+// Speculative windows of MC steps (rigid systems).  A window is a run of
+// consecutive MC steps of one replica that move distinct molecules: each
+// step's proposal then depends only on the state at the window's start
+// (its molecule has not moved earlier in the window), so the delta-E rows
+// of all of them are evaluated in one launch against that state, together
+// with the energies between the proposals themselves, and the decisions
+// are then taken one after another in step order (a proposal's row entry
+// for a molecule accepted earlier in the window is replaced by the
+// proposal-proposal energy).  The decisions, coordinates and energies are
+// those of the one-step-per-launch chain.  A window holds at most kMaxWin
+// steps, ends before the first repeated molecule and never crosses `stop`
+// (the end of the caller's run of steps).  mol_at(k) is the molecule of
+// step `step + k`.
+constexpr int kMaxWin = 8;
+template <typename F>
+__host__ __device__ __forceinline__ int window_len(F mol_at, int64_t step, int64_t stop, int W)
+{
+    int m[kMaxWin];
+    int len = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxWin; ++k) {
+        if (k >= W || step + k >= stop) break;
+        const int x = mol_at(k);
+        bool seen = false;
+#pragma unroll
+        for (int q = 0; q < kMaxWin; ++q) seen |= q < k && m[q] == x;
+        if (seen) break;
+        m[k] = x;
+        len = k + 1;
+    }
+    return len;
 }
 
 struct Move {
@@ -310,6 +358,21 @@ struct PairParams {
     int *prop_items;      // R: items of the i-lane pair kernel for the pending proposal
     int4 *prop_hdr;       // R: {molecule, first residue, residues, bounding radius bits} of the proposal
     int row_j;            // j residues per j-block of the row kernel (64 x residues per lane)
+    // speculative windows (win_log2 = 0: one MC step per launch).  Slot s
+    // of replica r is window entry v = r W + s (W = 1 << win_log2); slot 0's
+    // proposal is the pending proposal above (prop_pos, prop_hdr, ...), the
+    // others live in wprop_pos / wprop_rel.
+    int win_log2;
+    int4 *whdr;           // R W: {molecule, first residue, residues, bounding radius bits}
+    float4 *wcen;         // R W: proposed centre; w = residues of the earlier slots (int bits)
+    int *wlen;            // R: slots of the current window (0: the replica has reached wstop)
+    long long *wstop;     // R: step at which the current run of steps ends
+    float4 *wprop_pos;    // R (W - 1) max_n (+ 8 records of padding): slots 1..W-1
+    float4 *wprop_rel;    // R (W - 1) max_n
+    // window rows: entry v (M + W) + J, J < M the molecule row, M + q the
+    // energy between slot v's and slot q's proposals (q < s)
+    unsigned long long *wrowfix;
+    double *wrowbuf;
     // flexible molecules (null / 0 when the system has none).  Residue
     // indices are device indices (global); lists are CSR per molecule.
     int n_flex;                 // number of flexible molecules
@@ -1557,12 +1620,32 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
     }
 }
 
+// Proposal records of window slot s of replica r (slot 0: the pending
+// proposal).
+__device__ __forceinline__ float4 *slot_pos(const PairParams &p, int r, int s)
+{
+    return s == 0 ? p.prop_pos + (size_t)r * p.max_n
+                  : p.wprop_pos + ((size_t)r * ((1 << p.win_log2) - 1) + (s - 1)) * p.max_n;
+}
+__device__ __forceinline__ float4 *slot_rel(const PairParams &p, int r, int s)
+{
+    return s == 0 ? p.prop_rel + (size_t)r * p.max_n
+                  : p.wprop_rel + ((size_t)r * ((1 << p.win_log2) - 1) + (s - 1)) * p.max_n;
+}
+
+// Proposal headers the row kernel keeps in LDS (at most this many).
+constexpr int kHdrLds = 128;
+
 // CROWD: the system has crowders under repulsive crowding (a separate
 // instantiation, so that systems without them keep the registers of the
-// plain kernel)
-template <int JPL, bool CROWD>
+// plain kernel).  WIN: speculative windows; the kernel's "proposals" are
+// the R W window slots v = r W + s, each evaluated against replica r's
+// stored residues outside its molecule and against the proposals of the
+// slots before it (J = M + q in the window row).
+template <int JPL, bool CROWD, bool WIN>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4 : 8, 8))) void mc_row_kernel(PairParams p)
 {
+    static_assert(!(CROWD && WIN), "windows run without repulsive crowding");
     // Wave priority: the SIMDs issue from the oldest wave first, so a wave
     // still in its set-up (a chain of dependent loads and a few hundred
     // instructions) would wait behind the pair loops of older waves and
@@ -1571,45 +1654,58 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     __builtin_amdgcn_s_setprio(2);
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
     __shared__ float qtab[CGPPD_AA_COUNT];
-    // dynamic LDS (row_lds_bytes): the replicas' unit offsets, then for at
-    // most 64 replicas their proposal headers and centres
+    // dynamic LDS (row_lds_bytes): the proposals' unit offsets, then for at
+    // most kHdrLds proposals their headers and centres
     extern __shared__ __attribute__((aligned(16))) long long row_lds[];
+    const int lw = WIN ? p.win_log2 : 0;
     const int R = p.rep_count;
-    const bool hdr_lds = R <= kWave;
+    const int V = R << lw;  // proposals: replicas, or window slots
+    const bool hdr_lds = V <= kHdrLds;
     long long *unit_off = row_lds;
-    int4 *s_hdr = reinterpret_cast<int4 *>(row_lds + ((R + 2) & ~1));
-    float4 *s_cen = reinterpret_cast<float4 *>(s_hdr + (hdr_lds ? R : 0));
+    int4 *s_hdr = reinterpret_cast<int4 *>(row_lds + ((V + 2) & ~1));
+    float4 *s_cen = reinterpret_cast<float4 *>(s_hdr + (hdr_lds ? V : 0));
+    const int4 *g_hdr = WIN ? p.whdr : p.prop_hdr;
+    const float4 *g_cen = WIN ? p.wcen : p.prop_center;
     // every set-up load is issued before the LDS stores: the table (entry
     // e = tj * 20 + ti, i = the moved residue, goes to row ti, column tj),
-    // the charges, and the first 64 replicas' headers and centres
+    // the charges, and the first 64 proposals' headers and centres
     static_assert(CGPPD_AA_COUNT * CGPPD_AA_COUNT <= 2 * kWG, "two table entries per thread");
     // (unconditional, so that none waits for another; only wave 0 uses
     // the headers)
     const int e0 = threadIdx.x, e1 = threadIdx.x + kWG;
-    const int rr = min((int)threadIdx.x & (kWave - 1), R - 1);
+    const int rr = min((int)threadIdx.x & (kWave - 1), V - 1);
     const RowCoef t0 = p.row_table[e0];
     const RowCoef t1 = p.row_table[min(e1, CGPPD_AA_COUNT * CGPPD_AA_COUNT - 1)];
     const float q0 = p.charge[min((int)threadIdx.x, CGPPD_AA_COUNT - 1)];
-    const int4 hd0 = p.prop_hdr[rr];
-    const float4 cen0 = p.prop_center[rr];
+    const int4 hd0 = g_hdr[rr];
+    const float4 cen0 = g_cen[rr];
+    const int wl0 = WIN ? p.wlen[rr >> lw] : 0;
     constexpr int kEntries = CGPPD_AA_COUNT * CGPPD_AA_COUNT;
     const int e1c = min(e1, kEntries - 1);  // (threads past the table rewrite its last entry)
     *reinterpret_cast<RowCoef *>(tab + (((e0 % CGPPD_AA_COUNT) << kRowShift) | ((e0 / CGPPD_AA_COUNT) << kColShift))) = t0;
     *reinterpret_cast<RowCoef *>(tab + (((e1c % CGPPD_AA_COUNT) << kRowShift) | ((e1c / CGPPD_AA_COUNT) << kColShift))) = t1;
     qtab[min((int)threadIdx.x, CGPPD_AA_COUNT - 1)] = q0;  // (likewise)
-    // unit offsets of the replicas: an inclusive scan, 64 replicas at a
+    // unit offsets of the proposals: an inclusive scan, 64 proposals at a
     // time, from the proposal headers (the first 64 loaded above).  Every
     // wave scans (the loads then have an unconditional use and are not
     // deferred); wave 0 stores.
     {
         const int lane0 = threadIdx.x & (kWave - 1);
         const bool w0 = threadIdx.x < kWave;
-        auto units = [&](int r, int4 hd) -> long long {
-            if (r >= R || (p.da && !p.s1[r])) return 0;
-            return (long long)((p.n_res - hd.z + JPL * kWave - 1) / (JPL * kWave)) * hd.z;
+        // a proposal's j residues: the other molecules' residues, plus,
+        // for a window slot, the residues of the slots before it (cen.w)
+        auto units = [&](int v, int4 hd, float4 cen, int wl) -> long long {
+            if (v >= V) return 0;
+            if (WIN) {
+                if ((v & ((1 << lw) - 1)) >= wl) return 0;
+            } else if (p.da && !p.s1[v]) {
+                return 0;
+            }
+            const int nj = p.n_res - hd.z + (WIN ? __float_as_int(cen.w) : 0);
+            return (long long)((nj + JPL * kWave - 1) / (JPL * kWave)) * hd.z;
         };
-        long long c = wave_scan_i64(units(lane0, hd0));
-        if (w0 && lane0 < R) {
+        long long c = wave_scan_i64(units(lane0, hd0, cen0, wl0));
+        if (w0 && lane0 < V) {
             unit_off[lane0 + 1] = c;
             if (hdr_lds) {
                 s_hdr[lane0] = hd0;
@@ -1617,16 +1713,24 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
             }
         }
         long long carry = readlane_b64(c, kWave - 1);
-        for (int base = kWave; base < R; base += kWave) {
-            const int r = base + lane0;
-            c = wave_scan_i64(units(r, p.prop_hdr[min(r, R - 1)]));
-            if (w0 && r < R) unit_off[r + 1] = carry + c;
+        for (int base = kWave; base < V; base += kWave) {
+            const int v = base + lane0, vc = min(v, V - 1);
+            const int4 hd = g_hdr[vc];
+            const float4 cen = g_cen[vc];
+            c = wave_scan_i64(units(v, hd, cen, WIN ? p.wlen[vc >> lw] : 0));
+            if (w0 && v < V) {
+                unit_off[v + 1] = carry + c;
+                if (hdr_lds) {
+                    s_hdr[v] = hd;
+                    s_cen[v] = cen;
+                }
+            }
             carry += readlane_b64(c, kWave - 1);
         }
         if (threadIdx.x == 0) unit_off[0] = 0;
     }
     __syncthreads();
-    const long long total = unit_off[R];
+    const long long total = unit_off[V];
     const int G = gridDim.x, b = blockIdx.x;
     const int lb = (p.xcd_map && (G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
     const int lane = threadIdx.x & (kWave - 1);
@@ -1639,66 +1743,101 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     long long P = wv == 0 ? 0 : (long long)floor((double)wv * per);
     const long long P1 = wv + 1 == W ? total : (long long)floor((double)(wv + 1) * per);
     if (P >= P1) return;
-    // replica holding unit P: the number of replicas that end at or before
-    // P (unit_off is non-decreasing and P < unit_off[R]), 64 at a time
-    int r = 0;
-    for (int base = 0; base < R; base += kWave) {
+    // proposal holding unit P: the number of proposals that end at or
+    // before P (unit_off is non-decreasing and P < unit_off[V]), 64 at a time
+    int v = 0;
+    for (int base = 0; base < V; base += kWave) {
         const int k = base + lane;
-        const unsigned long long before = __ballot(k < R && unit_off[k + 1] <= P);
-        r = base + __popcll(before);
+        const unsigned long long before = __ballot(k < V && unit_off[k + 1] <= P);
+        v = base + __popcll(before);
         if (before != ~0ull) break;
     }
-    r = __builtin_amdgcn_readfirstlane(r);
+    v = __builtin_amdgcn_readfirstlane(v);
     const int M = p.n_mol;
-    // position of unit P inside its replica: j-block jb, residue ia of m
+    // position of unit P inside its proposal: j-block jb, residue ia of m
     // (one division for the first visit, then the visits advance)
     int jb = -1, ia = 0;
     while (P < P1) {
-        if (jb < 0 || P >= unit_off[r + 1]) {
-            while (P >= unit_off[r + 1]) ++r;
+        if (jb < 0 || P >= unit_off[v + 1]) {
+            while (P >= unit_off[v + 1]) ++v;
             jb = -1;
         }
+        // replica r and window slot s of proposal v
+        const int r = v >> lw, slot = v & ((1 << lw) - 1);
         // the proposal: molecule, first residue, size, bounding radius, centre
         int4 hd;
         float4 cm;
         if (hdr_lds) {
-            hd = s_hdr[r];
-            cm = s_cen[r];
+            hd = s_hdr[v];
+            cm = s_cen[v];
         } else {
-            hd = p.prop_hdr[r];
-            cm = p.prop_center[r];
+            hd = g_hdr[v];
+            cm = g_cen[v];
         }
         const int i0 = __builtin_amdgcn_readfirstlane(hd.y);
         const int n_m = __builtin_amdgcn_readfirstlane(hd.z);
         const float rm = __int_as_float(__builtin_amdgcn_readfirstlane(hd.w));
+        const int ext = WIN ? __builtin_amdgcn_readfirstlane(__float_as_int(cm.w)) : 0;
         cm.x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cm.x)));
         cm.y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cm.y)));
         cm.z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cm.z)));
         if (jb < 0) {
-            // units per replica fit in 32 bits (set_replicas bounds them)
-            const unsigned local = (unsigned)(P - unit_off[r]);
+            // units per proposal fit in 32 bits (set_replicas bounds them)
+            const unsigned local = (unsigned)(P - unit_off[v]);
             jb = __builtin_amdgcn_readfirstlane((int)(local / (unsigned)n_m));
             ia = __builtin_amdgcn_readfirstlane((int)(local % (unsigned)n_m));
         }
         const int ib = __builtin_amdgcn_readfirstlane((int)min((long long)n_m, (long long)ia + (P1 - P)));
         // the first i record's word (its run), requested before the j loads
-        const float4 *irec = p.prop_pos + (size_t)r * p.max_n;
+        const float4 *irec = WIN ? slot_pos(p, r, slot) : p.prop_pos + (size_t)r * p.max_n;
         const uint32_t code0 = __float_as_uint(scalar_rec(irec, ia).w);
-        // this lane's j residues (jb * 64 JPL + 64 h + lane, h < JPL)
-        const int nj = p.n_res - n_m;
+        // this lane's j residues (jb * 64 JPL + 64 h + lane, h < JPL): the
+        // stored residues outside m, then (window slots) the earlier
+        // slots' proposed residues
+        const int nmain = p.n_res - n_m;
+        const int nj = nmain + ext;
         LaneJ<JPL> lj;
         bool valid[JPL];
         int J[JPL];
         float4 rj[JPL];
         // (every load issued before any is used; lanes past the last j
         // residue read the last residue and are masked)
+        if (!WIN || (jb + 1) * (JPL * kWave) <= nmain) {
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) {
-            const int jp = jb * (JPL * kWave) + h * kWave + lane;
-            valid[h] = jp < nj;
-            const int j = min(jp + (jp >= i0 ? n_m : 0), p.n_res - 1);
-            rj[h] = p.pos[(size_t)r * p.stride + j];
-            J[h] = p.mol_of[j];
+            for (int h = 0; h < JPL; ++h) {
+                const int jp = jb * (JPL * kWave) + h * kWave + lane;
+                valid[h] = jp < nj;
+                const int j = min(jp + (jp >= i0 ? n_m : 0), p.n_res - 1);
+                rj[h] = p.pos[(size_t)r * p.stride + j];
+                J[h] = p.mol_of[j];
+            }
+        } else {
+            // a j-block reaching into the earlier slots' proposals: slot q
+            // holds window j indices [nmain + cum_q, nmain + cum_q + n_q)
+#pragma unroll
+            for (int h = 0; h < JPL; ++h) {
+                const int jp = jb * (JPL * kWave) + h * kWave + lane;
+                valid[h] = jp < nj;
+                if (jp < nmain) {
+                    const int j = jp + (jp >= i0 ? n_m : 0);
+                    rj[h] = p.pos[(size_t)r * p.stride + j];
+                    J[h] = p.mol_of[j];
+                } else if (jp >= nj) {
+                    rj[h] = p.pos[(size_t)r * p.stride + p.n_res - 1];
+                    J[h] = 0;
+                } else {
+                    // 0 <= k < ext: slot >= 1 and slot q < slot holds k
+                    const int k = jp - nmain;
+                    int q = 0, cum = 0;
+                    for (; q < slot - 1; ++q) {
+                        const int nq = g_hdr[(r << lw) + q].z;
+                        if (k < cum + nq) break;
+                        cum += nq;
+                    }
+                    rj[h] = slot_pos(p, r, q)[k - cum];
+                    J[h] = M + q;
+                }
+            }
         }
 #pragma unroll
         for (int h = 0; h < JPL; ++h) {
@@ -1802,23 +1941,24 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         __builtin_amdgcn_s_setprio(2);
         // FP64 reduction per molecule J: across the lanes J[h] is
         // non-decreasing for each h, and every J[h] <= every J[h + 1]
-        double v[JPL];
+        double vs[JPL];
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) v[h] = valid[h] ? (double)sa[h] - (double)ca[h] : 0.0;
+        for (int h = 0; h < JPL; ++h) vs[h] = valid[h] ? (double)sa[h] - (double)ca[h] : 0.0;
         // (lane 0 holds the block's first j residue: when it has none, no
         // lane has)
         int Jc = __builtin_amdgcn_readfirstlane(J[0]);
+        unsigned long long *const fix_row = WIN ? p.wrowfix + (size_t)v * (M + (1 << lw)) : p.rowfix + (size_t)r * M;
+        double *const dbl_row = WIN ? p.wrowbuf + (size_t)v * (M + (1 << lw)) : p.rowbuf + (size_t)r * M;
         while (Jc != 0x7fffffff) {
             double part = 0.0;
 #pragma unroll
-            for (int h = 0; h < JPL; ++h) part += J[h] == Jc ? v[h] : 0.0;
+            for (int h = 0; h < JPL; ++h) part += J[h] == Jc ? vs[h] : 0.0;
             const double t = wave_total(part);
             if (lane == 0) {
-                const size_t at = (size_t)r * M + Jc;
                 if (fabs(t) < kFixMax)
-                    atomicAdd(p.rowfix + at, (unsigned long long)__double2ll_rn(t * kFixScale));
+                    atomicAdd(fix_row + Jc, (unsigned long long)__double2ll_rn(t * kFixScale));
                 else
-                    unsafeAtomicAdd(p.rowbuf + at, t);
+                    unsafeAtomicAdd(dbl_row + Jc, t);
             }
             int next = -1;
 #pragma unroll
@@ -1830,7 +1970,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
             Jc = next < 0 ? 0x7fffffff : next;
         }
         P += ib - ia;
-        // the next visit starts at the next j-block of this replica, or at
+        // the next visit starts at the next j-block of this proposal, or at
         // the first unit of a later one
         if (ib == n_m) {
             ++jb;
@@ -2553,6 +2693,221 @@ __global__ __launch_bounds__(kWG) void mc_finish_rigid_kernel(PairParams p)
         p.prop_n[r] = n_2;
         p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
         record_proposal(p, r, m2, j_beg, n_2, kMovedNone, 0, p.radius[(size_t)r * M + m2], 0.0);
+    }
+}
+
+// Value of window row entry J of slot s of replica r: the fixed-point sum
+// plus the FP64 part (partials too large for the fixed point).
+__device__ __forceinline__ double window_row(const PairParams &p, int r, int s, int J)
+{
+    const size_t at = ((size_t)(r << p.win_log2) + s) * (p.n_mol + (1 << p.win_log2)) + J;
+    double v = p.wrowbuf[at];
+    const unsigned long long f = p.wrowfix[at];
+    if (f) v += (double)(long long)f * kFixInv;
+    return v;
+}
+
+// One workgroup per replica.  begin_steps >= 0: the start of a run of
+// begin_steps MC steps: sets the run's end and lays out the first window
+// (slot 0 is the pending proposal; slots 1.. are laid out from the current
+// state).  begin_steps < 0: takes the decisions of the current window in
+// step order (src/Replica.cpp:385-410, FP64 delta and exponent), applies
+// the accepted moves and the matrix rows and columns, and lays out the
+// next window (slot 0 always, so that a proposal is pending when the run
+// ends).  A replica that has reached the run's end does nothing.
+//
+// Decision of slot s: with D_s = sum_J (row_s[J] - mat[m_s][J]) over the
+// window-start state, and for each earlier slot q that was accepted the
+// replacement of its entry row_s[m_q] - mat[m_s][m_q] by
+// X_sq - row_q[m_s] (X_sq the energy between the two proposals, row_q[m_s]
+// the energy between q's proposal and m_s as stored):
+//   delta_s = D_s + sum_{q < s accepted} (X_sq - row_q[m_s] - row_s[m_q] + mat[m_s][m_q]).
+__global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_steps)
+{
+#pragma clang fp contract(off)
+    __shared__ double s_D[kMaxWin];
+    __shared__ double s_corr[kMaxWin * kMaxWin];
+    __shared__ double s_u[kMaxWin];
+    __shared__ int4 s_hd[kMaxWin];
+    __shared__ float4 s_cn[kMaxWin];
+    __shared__ int s_next[kMaxWin];    // molecules of the next window's steps
+    __shared__ Proposal s_prop[kMaxWin];
+    __shared__ Move s_mv[kMaxWin];
+    const int r = blockIdx.x, t = threadIdx.x;
+    const int lane = t & (kWave - 1), wave = t / kWave;
+    const int lw = p.win_log2, W = 1 << lw;
+    const int M = p.n_mol, N = p.n_res;
+    const bool begin = begin_steps >= 0;
+    const int64_t step = p.steps[r];
+    const int act = begin ? 0 : p.wlen[r];
+    if (!begin && act == 0) return;
+    const int64_t stop = begin ? step + begin_steps : p.wstop[r];
+    const int64_t next = step + act;
+    const int rr = r + p.rep_rng;
+    // phase A: the window's headers and uniforms, the next window's draws
+    if (t < act) {
+        s_hd[t] = p.whdr[(r << lw) + t];
+        s_cn[t] = p.wcen[(r << lw) + t];
+    } else if (t >= kWave && t < kWave + W) {
+        const int k = t - kWave;
+        const Proposal pr = draw_proposal(p.seed, rr, next + k, M);
+        s_prop[k] = pr;
+        s_next[k] = pr.mol;
+    } else if (t >= 2 * kWave && t < 2 * kWave + act) {
+        s_u[t - 2 * kWave] = draw_proposal(p.seed, rr, step + (t - 2 * kWave), M).u;
+    }
+    __syncthreads();
+    unsigned accm = 0u;
+    if (act > 0) {
+        // phase B: D_s (wave w takes slots w, w + 4) and the corrections
+        for (int s = wave; s < act; s += kWavesPerWG) {
+            const int ms = s_hd[s].x;
+            const double *mrow = p.emat + ((size_t)r * M + ms) * M;
+            double d = 0.0;
+            for (int J = lane; J < M; J += kWave) d += window_row(p, r, s, J) - mrow[J];
+            d = wave_total(d);
+            if (lane == 0) s_D[s] = d;
+        }
+        if (t < W * W) {
+            const int s = t >> lw, q = t & (W - 1);
+            if (q < s && s < act) {
+                const int ms = s_hd[s].x, mq = s_hd[q].x;
+                const double X = window_row(p, r, s, M + q);
+                const double a = window_row(p, r, q, ms);
+                const double b = window_row(p, r, s, mq);
+                const double c = p.emat[((size_t)r * M + ms) * M + mq];
+                s_corr[t] = X - a - (b - c);
+            }
+        }
+        __syncthreads();
+        // phase C: the decisions in step order (every thread takes the same
+        // ones from the same values)
+        const double temp = p.temperature[r];
+        double dE[kMaxWin];
+        int kinds[kMaxWin];
+#pragma unroll
+        for (int s = 0; s < kMaxWin; ++s) {
+            dE[s] = 0.0;
+            kinds[s] = 2;
+            if (s < act) {
+                double d = s_D[s];
+#pragma unroll
+                for (int q = 0; q < kMaxWin; ++q)
+                    if (q < s && (accm >> q & 1u)) d += s_corr[(s << lw) + q];
+                const int kind = d < 0.0 ? 0 : s_u[s] < exp(-d * 4184.0 / ((double)kRgas * temp)) ? 1 : 2;
+                if (kind < 2) accm |= 1u << s;
+                dE[s] = d;
+                kinds[s] = kind;
+            }
+        }
+        if (t == 0) {
+            double e = p.energy[r];
+            long long c0 = 0, c1 = 0, c2 = 0;
+            unsigned long long pairs = 0ull;
+#pragma unroll
+            for (int s = 0; s < kMaxWin; ++s)
+                if (s < act) {
+                    if (kinds[s] < 2) e += dE[s];
+                    c0 += kinds[s] == 0;
+                    c1 += kinds[s] == 1;
+                    c2 += kinds[s] == 2;
+                    const unsigned long long nm = (unsigned long long)s_hd[s].z;
+                    pairs += nm * ((unsigned long long)N - nm);
+                }
+            p.energy[r] = e;
+            p.counters[3 * (size_t)r] += c0;
+            p.counters[3 * (size_t)r + 1] += c1;
+            p.counters[3 * (size_t)r + 2] += c2;
+            p.steps[r] = next;
+            atomicAdd(p.pair_evals, pairs);
+        }
+        // apply (src/Molecule.cpp:468-504): coordinates of the accepted
+        // slots, and their matrix rows and columns; an entry between two
+        // accepted slots is the energy between their proposals
+        float4 *pos = p.pos + (size_t)r * p.stride;
+        float4 *rel = p.rel + (size_t)r * p.stride;
+        double *mat = p.emat + (size_t)r * M * M;
+        for (int s = 0; s < act; ++s) {
+            if (!(accm >> s & 1u)) continue;
+            const int ms = s_hd[s].x, i0 = s_hd[s].y, ns = s_hd[s].z;
+            const float4 *sp = slot_pos(p, r, s), *sq = slot_rel(p, r, s);
+            for (int k = t; k < ns; k += kWG) {
+                pos[i0 + k] = sp[k];
+                rel[i0 + k] = sq[k];
+            }
+            if (t == 0) p.center[(size_t)r * M + ms] = make_float4(s_cn[s].x, s_cn[s].y, s_cn[s].z, 0.0f);
+            for (int J = t; J < M; J += kWG) {
+                double v = -1.0;
+                bool pair = false;
+                for (int q = 0; q < act; ++q)
+                    if (q != s && (accm >> q & 1u) && s_hd[q].x == J) {
+                        v = q < s ? window_row(p, r, s, M + q) : window_row(p, r, q, M + s);
+                        pair = true;
+                    }
+                if (!pair) v = window_row(p, r, s, J);
+                mat[(size_t)ms * M + J] = v;
+                mat[(size_t)J * M + ms] = v;
+            }
+        }
+        // the rows are cleared for the next window (every read of them
+        // above comes before the barrier below)
+    }
+    __syncthreads();
+    if (act > 0) {
+        const int MW = M + W;
+        for (int k = t; k < act * MW; k += kWG) {
+            const size_t at = (size_t)(r << lw) * MW + k;
+            p.wrowfix[at] = 0ull;
+            p.wrowbuf[at] = 0.0;
+        }
+    }
+    // phase D: the next window, laid out from the state after the apply
+    const int len = window_len([&](int k) { return s_next[k]; }, next, stop, W);
+    const int first = begin ? 1 : 0;
+    const int n_lay = begin ? len : max(len, 1);
+    if (t < n_lay && t >= first) {
+        const int m2 = s_next[t];
+        const float4 c2 = p.center[(size_t)r * M + m2];
+        s_mv[t] = make_move(s_prop[t], c2, p.tstep[r], p.rcos[r], p.rsin[r], p.box);
+    }
+    __syncthreads();
+    int ext = 0;
+    for (int s = 0; s < n_lay; ++s) {
+        const int m2 = s_next[s];
+        const int j_beg = p.mol_off[m2], n_2 = p.mol_off[m2 + 1] - j_beg;
+        if (s >= first) {
+            const Move mv = s_mv[s];
+            const float4 *rl = p.rel + (size_t)r * p.stride + j_beg;
+            float4 *qp = slot_pos(p, r, s), *qq = slot_rel(p, r, s);
+            for (int k = t; k < n_2; k += kWG) {
+                float4 rn, pn;
+                move_residue(mv, rl[k], rn, pn);
+                qp[k] = pn;
+                qq[k] = rn;
+            }
+            if (t == 0) {
+                const float rad = p.radius[(size_t)r * M + m2];
+                p.whdr[(r << lw) + s] = make_int4(m2, j_beg, n_2, __float_as_int(rad));
+                p.wcen[(r << lw) + s] = make_float4(mv.c[0], mv.c[1], mv.c[2], __int_as_float(ext));
+                if (s == 0) {
+                    p.prop_mol[r] = m2;
+                    p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
+                    p.prop_n[r] = n_2;
+                    p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+                    record_proposal(p, r, m2, j_beg, n_2, kMovedNone, 0, rad, 0.0);
+                }
+            }
+        } else if (t == 0) {
+            // slot 0 of a run's first window: the pending proposal
+            p.whdr[r << lw] = p.prop_hdr[r];
+            const float4 c = p.prop_center[r];
+            p.wcen[r << lw] = make_float4(c.x, c.y, c.z, __int_as_float(0));
+        }
+        ext += n_2;
+    }
+    if (t == 0) {
+        p.wlen[r] = len;
+        if (begin) p.wstop[r] = stop;
     }
 }
 
@@ -3434,10 +3789,28 @@ struct cgppd_ctx {
     double *d_s_dS = nullptr;
     long long *d_s1_rej = nullptr;
 
+    // speculative windows (rigid systems): the caller's choice (0:
+    // automatic, 1: off, else slots per window), the window buffers (for
+    // win_alloc slots), and the MC steps every replica has taken (they all
+    // advance together; the host replays the molecule draws from it to
+    // count the launches a run of steps needs)
+    int window = 0;
+    int win_alloc = 0;
+    int4 *d_whdr = nullptr;
+    float4 *d_wcen = nullptr;
+    int *d_wlen = nullptr;
+    long long *d_wstop = nullptr;
+    float4 *d_wprop_pos = nullptr, *d_wprop_rel = nullptr;
+    unsigned long long *d_wrowfix = nullptr;
+    double *d_wrowbuf = nullptr;
+    long long h_step = 0;
+
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
     int graph_groups = 0;
+    int graph_win = 0;  // window slots of the captured graph (1: one step per launch pair)
+    int graph_L = 0;    // window launch pairs in the captured graph
 
     // replica groups: the MC steps of each group run on a stream of their
     // own (group 0 on the context stream), so that one group's finish
@@ -3515,11 +3888,28 @@ void destroy_graph(cgppd_ctx *ctx)
     ctx->graph_exec = nullptr;
     ctx->graph = nullptr;
     ctx->graph_steps = 0;
+    ctx->graph_win = 0;
+    ctx->graph_L = 0;
+}
+
+void free_windows(cgppd_ctx *ctx)
+{
+    dev_free(ctx->d_whdr);
+    dev_free(ctx->d_wcen);
+    dev_free(ctx->d_wlen);
+    dev_free(ctx->d_wstop);
+    dev_free(ctx->d_wprop_pos);
+    dev_free(ctx->d_wprop_rel);
+    dev_free(ctx->d_wrowfix);
+    dev_free(ctx->d_wrowbuf);
+    ctx->win_alloc = 0;
 }
 
 void free_replicas(cgppd_ctx *ctx)
 {
     destroy_graph(ctx);
+    free_windows(ctx);
+    ctx->h_step = 0;
     dev_free(ctx->d_pos);
     dev_free(ctx->d_rel);
     dev_free(ctx->d_center);
@@ -3816,6 +4206,7 @@ int sync_topology(cgppd_ctx *ctx)
 
 int pair_layout_of(const cgppd_ctx *ctx);
 int row_lanes_of(int layout);
+int window_of(const cgppd_ctx *ctx);
 
 PairParams base_params(cgppd_ctx *ctx)
 {
@@ -3869,6 +4260,16 @@ PairParams base_params(cgppd_ctx *ctx)
     p.prop_items = ctx->d_prop_items;
     p.prop_hdr = ctx->d_prop_hdr;
     p.row_j = kWave * row_lanes_of(pair_layout_of(ctx));
+    p.win_log2 = 0;
+    for (int w = window_of(ctx); w > 1; w >>= 1) ++p.win_log2;
+    p.whdr = ctx->d_whdr;
+    p.wcen = ctx->d_wcen;
+    p.wlen = ctx->d_wlen;
+    p.wstop = ctx->d_wstop;
+    p.wprop_pos = ctx->d_wprop_pos;
+    p.wprop_rel = ctx->d_wprop_rel;
+    p.wrowfix = ctx->d_wrowfix;
+    p.wrowbuf = ctx->d_wrowbuf;
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
     p.temperature = ctx->d_temp;
@@ -4126,11 +4527,13 @@ int pair_wg_per_cu(const cgppd_ctx *ctx, int layout)
 
 // ev_start/ev_stop (optional) are recorded by the dispatch itself, so their
 // elapsed time is the kernel's execution time without separate event packets.
-// Dynamic LDS of the row kernel for R replicas: R + 1 unit offsets (padded
-// to 16 bytes), then, for R <= 64, R proposal headers and R centres.
-size_t row_lds_bytes(int R)
+// Dynamic LDS of the row kernel for V proposals (replicas, or window
+// slots): V + 1 unit offsets (padded to 16 bytes), then, for V <= kHdrLds,
+// V proposal headers and V centres.
+size_t row_lds_bytes(int V)
 {
-    return sizeof(long long) * (size_t)((R + 2) & ~1) + (R <= kWave ? (size_t)R * (sizeof(int4) + sizeof(float4)) : 0);
+    return sizeof(long long) * (size_t)((V + 2) & ~1) +
+           (V <= kHdrLds ? (size_t)V * (sizeof(int4) + sizeof(float4)) : 0);
 }
 
 // Number of replica groups the MC steps use: the caller's choice, else 1;
@@ -4225,9 +4628,11 @@ int launch_pair_p(cgppd_ctx *ctx, PairParams p, hipStream_t stream, hipEvent_t e
             hipLaunchKernelGGL(mc_pair_kernel<false>, dim3(g), dim3(kWG), 0, stream, p);
     } else {
         const bool crowd = p.crowd_mol < p.n_mol;
-        auto kern = layout == CGPPD_LAYOUT_JLANES4 ? (crowd ? mc_row_kernel<4, true> : mc_row_kernel<4, false>)
-                                                   : (crowd ? mc_row_kernel<2, true> : mc_row_kernel<2, false>);
-        const size_t lds = row_lds_bytes(p.rep_count);
+        const bool win = p.win_log2 > 0;
+        auto kern = layout == CGPPD_LAYOUT_JLANES4
+                        ? (win ? mc_row_kernel<4, false, true> : crowd ? mc_row_kernel<4, true, false> : mc_row_kernel<4, false, false>)
+                        : (win ? mc_row_kernel<2, false, true> : crowd ? mc_row_kernel<2, true, false> : mc_row_kernel<2, false, false>);
+        const size_t lds = row_lds_bytes(p.rep_count << p.win_log2);
         if (ev_start)
             hipExtLaunchKernelGGL(kern, dim3(g), dim3(kWG), lds, stream, ev_start, ev_stop, 0, p);
         else
@@ -4313,6 +4718,99 @@ bool resident_active(const cgppd_ctx *ctx)
            groups_of(ctx) == 1 && resident_lds_bytes(ctx->N, ctx->M, ctx->max_n) + kTableBytes + 1024 <= 65536;
 }
 
+// Slots per speculative window: windows serve rigid systems on the j-lane
+// row kernel without delayed acceptance, repulsive crowding or replica
+// groups (1 elsewhere: one MC step per launch pair); the caller's choice,
+// else kAutoWindow.
+constexpr int kAutoWindow = 4;
+int window_of(const cgppd_ctx *ctx)
+{
+    if (ctx->window == 1 || ctx->n_flex || ctx->da || ctx->rep_crowd || groups_of(ctx) != 1 || resident_active(ctx))
+        return 1;
+    const int layout = pair_layout_of(ctx);
+    if (layout != CGPPD_LAYOUT_JLANES && layout != CGPPD_LAYOUT_JLANES4) return 1;
+    return ctx->window > 1 ? ctx->window : kAutoWindow;
+}
+
+// Window buffers for W slots (allocated and cleared outside any capture).
+int ensure_windows(cgppd_ctx *ctx, int W)
+{
+    if (W <= 1 || ctx->win_alloc == W) return CGPPD_OK;
+    destroy_graph(ctx);
+    free_windows(ctx);
+    const size_t R = ctx->R, M = ctx->M;
+    const size_t rows = R * W * (M + W);
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_whdr, R * W))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wcen, R * W))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wlen, R))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wstop, R))) return rc;
+    // 8 records of padding: the row kernel reads i records in groups
+    if ((rc = dev_alloc(ctx, &ctx->d_wprop_pos, R * (W - 1) * ctx->max_n + 8))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wprop_rel, R * (W - 1) * ctx->max_n))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wrowfix, rows))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wrowbuf, rows))) return rc;
+    CG_HIP(ctx, hipMemset(ctx->d_wrowfix, 0, sizeof(unsigned long long) * rows));
+    CG_HIP(ctx, hipMemset(ctx->d_wrowbuf, 0, sizeof(double) * rows));
+    CG_HIP(ctx, hipMemset(ctx->d_wlen, 0, sizeof(int) * R));
+    CG_HIP(ctx, hipMemset(ctx->d_wprop_pos, 0, sizeof(float4) * (R * (W - 1) * ctx->max_n + 8)));
+    ctx->win_alloc = W;
+    return CGPPD_OK;
+}
+
+// Windows replica r needs for the steps [step0, step0 + n): the molecule
+// draws replayed on the host (draw_mol), cut as window_len cuts them.
+long long windows_of_replica(const cgppd_ctx *ctx, int r, long long step0, int n, int W)
+{
+    const long long stop = step0 + n;
+    long long s = step0, count = 0;
+    while (s < stop) {
+        const long long at = s;
+        s += window_len([&](int k) { return draw_mol(ctx->seed, r, at + k, ctx->M); }, at, stop, W);
+        ++count;
+    }
+    return count;
+}
+
+// Launch pairs a run of n steps from step0 needs: the most windows of any
+// replica.
+int windows_needed(const cgppd_ctx *ctx, long long step0, int n, int W)
+{
+    long long most = 0;
+    for (int r = 0; r < ctx->R; ++r) most = std::max(most, windows_of_replica(ctx, r, step0, n, W));
+    return (int)most;
+}
+
+int launch_window_begin(cgppd_ctx *ctx, int n_steps)
+{
+    const PairParams p = base_params(ctx);
+    hipLaunchKernelGGL(mc_window_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, n_steps);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+// One window launch pair: the rows of every replica's window, then the
+// decisions and the next windows.
+int launch_window_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr)
+{
+    int rc = launch_pair(ctx, ev_start, ev_stop);
+    if (rc) return rc;
+    const PairParams p = base_params(ctx);
+    hipLaunchKernelGGL(mc_window_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, -1);
+    CG_HIP(ctx, hipGetLastError());
+    return CGPPD_OK;
+}
+
+// n_steps MC steps in windows from ctx->h_step (which the caller advances).
+int launch_window_steps(cgppd_ctx *ctx, int n_steps)
+{
+    if (n_steps <= 0) return CGPPD_OK;
+    const int L = windows_needed(ctx, ctx->h_step, n_steps, window_of(ctx));
+    int rc = launch_window_begin(ctx, n_steps);
+    for (int k = 0; k < L && !rc; ++k) rc = launch_window_pair(ctx);
+    return rc;
+}
+
 // n_steps resident MC steps, at most kResidentChunk per launch.
 constexpr int kResidentChunk = 16384;
 int launch_resident_steps(cgppd_ctx *ctx, int n_steps)
@@ -4340,6 +4838,7 @@ int launch_mc_steps(cgppd_ctx *ctx, int n_steps)
     int rc;
     if (G == 1) {
         if (resident_active(ctx)) return n_steps > 0 ? launch_resident_steps(ctx, n_steps) : CGPPD_OK;
+        if (window_of(ctx) > 1) return launch_window_steps(ctx, n_steps);
         for (int k = 0; k < n_steps; ++k)
             if ((rc = launch_mc_step(ctx))) return rc;
         return CGPPD_OK;
@@ -5045,6 +5544,8 @@ int cgppd_mc_prepare(cgppd_ctx *ctx)
     return CGPPD_OK;
 }
 
+static int mc_steps_launch(cgppd_ctx *ctx, int n_steps, int W);
+
 static int ensure_events(cgppd_ctx *ctx, size_t n)
 {
     while (ctx->events.size() < n) {
@@ -5063,6 +5564,35 @@ int cgppd_mc_steps(cgppd_ctx *ctx, int n_steps)
     if (n_steps < 0) return fail(ctx, CGPPD_E_ARG, "negative step count");
     CG_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure_stage1(ctx))) return rc;
+    const int W = window_of(ctx);
+    if ((rc = ensure_windows(ctx, W))) return rc;
+    rc = mc_steps_launch(ctx, n_steps, W);
+    if (rc == CGPPD_OK) ctx->h_step += n_steps;
+    return rc;
+}
+
+// The launches of cgppd_mc_steps (the caller advances ctx->h_step).
+static int mc_steps_launch(cgppd_ctx *ctx, int n_steps, int W)
+{
+    int rc;
+    if (ctx->timing && n_steps > 0 && W > 1) {
+        // the row kernel of each window launch pair
+        const int L = windows_needed(ctx, ctx->h_step, n_steps, W);
+        if ((rc = ensure_events(ctx, (size_t)2 * L))) return rc;
+        if ((rc = launch_window_begin(ctx, n_steps))) return rc;
+        for (int k = 0; k < L; ++k)
+            if ((rc = launch_window_pair(ctx, ctx->events[2 * k], ctx->events[2 * k + 1]))) return rc;
+        CG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double sum = 0.0;
+        for (int k = 0; k < L; ++k) {
+            float ms = 0.f;
+            CG_HIP(ctx, hipEventElapsedTime(&ms, ctx->events[2 * k], ctx->events[2 * k + 1]));
+            sum += ms;
+        }
+        ctx->t_mean_ms = L ? sum / L : 0.0;
+        ctx->t_launches = L;
+        return CGPPD_OK;
+    }
     if (ctx->timing && n_steps > 0 && resident_active(ctx)) {
         // the resident kernel runs the steps in ceil(n_steps / kResidentChunk)
         // launches: their span on the context stream, per MC step
@@ -5107,6 +5637,52 @@ int cgppd_exchange(cgppd_ctx *ctx)
     return launch_exchange(ctx);
 }
 
+// Exchange intervals in speculative windows: the graph holds the run's
+// start and graph_L window launch pairs; an interval whose replicas need
+// more windows (counted on the host from the molecule draws) gets the rest
+// as direct launches, and one that needs fewer leaves the graph's last
+// pairs without work.  The exchange follows each interval.
+static int run_windows_graph(cgppd_ctx *ctx, int n_intervals, int n, int W)
+{
+    int rc = ensure_windows(ctx, W);
+    if (rc) return rc;
+    if (!ctx->graph_exec || ctx->graph_steps != n || ctx->graph_win != W || ctx->graph_groups != 1) {
+        destroy_graph(ctx);
+        // the fewest windows of the first intervals of this run (at most 4)
+        int L = 0;
+        for (int k = 0; k < std::max(1, std::min(n_intervals, 4)); ++k) {
+            const int l = windows_needed(ctx, ctx->h_step + (long long)k * n, n, W);
+            L = k == 0 ? l : std::min(L, l);
+        }
+        L = std::max(1, L - 1);
+        CG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int lrc = launch_window_begin(ctx, n);
+        for (int k = 0; k < L && !lrc; ++k) lrc = launch_window_pair(ctx);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (lrc) {
+            if (g) (void)hipGraphDestroy(g);
+            return lrc;
+        }
+        if (ce != hipSuccess) return fail(ctx, CGPPD_E_HIP, "graph capture: %s", hipGetErrorString(ce));
+        ctx->graph = g;
+        CG_HIP(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+        ctx->graph_steps = n;
+        ctx->graph_groups = 1;
+        ctx->graph_win = W;
+        ctx->graph_L = L;
+    }
+    for (int k = 0; k < n_intervals; ++k) {
+        const int need = windows_needed(ctx, ctx->h_step, n, W);
+        CG_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        for (int e = ctx->graph_L; e < need; ++e)
+            if ((rc = launch_window_pair(ctx))) return rc;
+        if ((rc = launch_exchange(ctx))) return rc;
+        ctx->h_step += n;
+    }
+    return CGPPD_OK;
+}
+
 int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_graph)
 {
     int rc = check_ready(ctx, true);
@@ -5124,8 +5700,10 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
         }
         return CGPPD_OK;
     }
+    const int W = window_of(ctx);
+    if (W > 1) return run_windows_graph(ctx, n_intervals, steps_per_interval, W);
     const int G = groups_of(ctx);
-    if (!ctx->graph_exec || ctx->graph_steps != steps_per_interval || ctx->graph_groups != G) {
+    if (!ctx->graph_exec || ctx->graph_steps != steps_per_interval || ctx->graph_groups != G || ctx->graph_win != 1) {
         destroy_graph(ctx);
         if ((rc = ensure_group_streams(ctx, G))) return rc;
         CG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -5143,8 +5721,10 @@ int cgppd_run(cgppd_ctx *ctx, int n_intervals, int steps_per_interval, int use_g
         CG_HIP(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
         ctx->graph_steps = steps_per_interval;
         ctx->graph_groups = G;
+        ctx->graph_win = 1;
     }
     for (int k = 0; k < n_intervals; ++k) CG_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    ctx->h_step += (long long)n_intervals * steps_per_interval;
     return CGPPD_OK;
 }
 
@@ -5268,6 +5848,23 @@ int cgppd_set_pair_layout(cgppd_ctx *ctx, int layout)
     return CGPPD_OK;
 }
 
+int cgppd_set_window(cgppd_ctx *ctx, int slots)
+{
+    if (!ctx) return CGPPD_E_ARG;
+    if (slots != 0 && slots != 1 && slots != 2 && slots != 4 && slots != 8)
+        return fail(ctx, CGPPD_E_ARG, "window slots must be 0 (automatic), 1 (off), 2, 4 or 8 (got %d)", slots);
+    if (slots != ctx->window) destroy_graph(ctx);
+    ctx->window = slots;
+    return CGPPD_OK;
+}
+
+int cgppd_get_window(cgppd_ctx *ctx, int *slots)
+{
+    if (!ctx || !slots) return CGPPD_E_ARG;
+    *slots = ctx->have_sys ? window_of(ctx) : 1;
+    return CGPPD_OK;
+}
+
 int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu)
 {
     if (!ctx) return CGPPD_E_ARG;
@@ -5313,9 +5910,23 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     double best = 1e300;
     for (auto &row : ctx->tune_ms)
         for (double &v : row) v = 0.0;
+    bool windowed = false;
     for (int li = 0; li < n_layouts && rc == CGPPD_OK; ++li) {
+        ctx->pair_layout = ctx->n_flex ? CGPPD_LAYOUT_AUTO : layouts[li];
+        // a layout that runs speculative windows is timed on the window
+        // laid out from the current state, and its time per launch is
+        // divided by the mean window length of the next 1000 steps
+        const int Wc = window_of(ctx);
+        double steps_per_launch = 1.0;
+        if (Wc > 1) {
+            if ((rc = ensure_windows(ctx, Wc))) break;
+            if ((rc = launch_window_begin(ctx, 1 << 20))) break;
+            double windows = 0.0;
+            for (int r = 0; r < ctx->R; ++r) windows += (double)windows_of_replica(ctx, r, ctx->h_step, 1000, Wc);
+            steps_per_launch = 1000.0 * ctx->R / windows;
+            windowed = true;
+        }
         for (int w = 2; w <= 8 && rc == CGPPD_OK; ++w) {
-            ctx->pair_layout = ctx->n_flex ? CGPPD_LAYOUT_AUTO : layouts[li];
             ctx->wg_per_cu = w;
             // one untimed launch, then the timed ones
             if ((rc = launch_pair(ctx))) break;
@@ -5335,7 +5946,7 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
                 }
                 sum += ms;
             }
-            const double mean = sum / launches;
+            const double mean = sum / launches / steps_per_launch;
             ctx->tune_ms[ctx->n_flex ? 2 : li][w - 2] = mean;
             if (rc == CGPPD_OK && mean < best) {
                 best = mean;
@@ -5361,6 +5972,13 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     }
     ctx->pair_layout = keep_layout;
     ctx->wg_per_cu = keep_wg;
+    // the timed window launches added into the window rows, which are zero
+    // between windows
+    if (windowed && rc == CGPPD_OK) {
+        const size_t rows = (size_t)ctx->R * ctx->win_alloc * (ctx->M + ctx->win_alloc);
+        if ((rc = zero_dev(ctx, ctx->d_wrowbuf, rows)) == CGPPD_OK)
+            rc = zero_dev(ctx, reinterpret_cast<double *>(ctx->d_wrowfix), rows);
+    }
     const hipError_t back =
         hipMemcpyAsync(ctx->d_rowbuf, row_copy, row_n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
     const hipError_t done = hipStreamSynchronize(ctx->stream);

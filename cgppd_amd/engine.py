@@ -314,6 +314,16 @@ class Engine:
         code = self.LAYOUTS[layout] if isinstance(layout, str) else int(layout)
         self._check(self.lib.cgppd_set_pair_layout(self.h, code), "set_pair_layout")
 
+    def set_window(self, slots):
+        """Speculative windows of MC steps: 0 automatic, 1 off, 2, 4 or 8 slots."""
+        self._check(self.lib.cgppd_set_window(self.h, int(slots)), "set_window")
+
+    def window(self):
+        """Slots per window the next MC steps use (1: one step per launch pair)."""
+        w = C.c_int()
+        self._check(self.lib.cgppd_get_window(self.h, C.byref(w)), "get_window")
+        return w.value
+
     def _pair_launch(self):
         lay, wg = C.c_int(), C.c_int()
         self._check(self.lib.cgppd_get_pair_launch(self.h, C.byref(lay), C.byref(wg)), "get_pair_launch")

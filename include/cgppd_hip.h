@@ -318,14 +318,34 @@ int cgppd_get_replica_groups(cgppd_ctx *ctx, int *groups);
 /* The layout and workgroups per CU the next pair-kernel launch uses. */
 int cgppd_get_pair_launch(cgppd_ctx *ctx, int *layout, int *wg_per_cu);
 
+/* Speculative windows of MC steps (new; the reference evaluates one MC step
+ * per E_TiledKernel launch, src/Replica.cpp:350-414 with
+ * src/CudaFunctions.cu:198-277).  `slots` = 0: automatic (4), 1: off, or 2,
+ * 4 or 8 steps per window.  A window is a run of consecutive MC steps of
+ * one replica that move distinct molecules (it ends before the first
+ * repeated molecule and at the end of the cgppd_mc_steps / cgppd_run
+ * interval); the delta-E rows of all its proposals and the energies
+ * between the proposals are evaluated in one row-kernel launch against the
+ * window's starting state, and the decisions are then taken in step order,
+ * so the trajectory is that of one step per launch (up to the FP64
+ * summation order of the delta-E).  Windows serve systems of rigid
+ * molecules on the j-lane layouts without delayed acceptance, repulsive
+ * crowding or replica groups; other systems run one step per launch pair.
+ * cgppd_get_window reports the slots the next MC steps use (1: none). */
+int cgppd_set_window(cgppd_ctx *ctx, int slots);
+int cgppd_get_window(cgppd_ctx *ctx, int *slots);
+
 /* Autotuner (replaces the reference's fixed block size, same lines as
  * above): times `launches` pair-kernel launches on the pending proposals
  * (after cgppd_mc_prepare) for every layout and workgroups-per-CU
  * candidate, keeps the fastest for this context and reports it (any
  * pointer may be NULL).  The MC state is unchanged. */
 int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, double *best_ms);
-/* The last autotune's mean kernel times in ms, ms[3*7]: rows j-lanes,
- * j-lanes x4, i-lanes; columns 2..8 workgroups per CU (0 = not timed). */
+/* The last autotune's mean kernel times in ms per MC step of every replica,
+ * ms[3*7]: rows j-lanes, j-lanes x4, i-lanes; columns 2..8 workgroups per
+ * CU (0 = not timed).  A layout that runs speculative windows is timed per
+ * window launch and divided by the mean window length of the next 1000
+ * steps. */
 int cgppd_autotune_table(cgppd_ctx *ctx, double *ms);
 int cgppd_set_timing(cgppd_ctx *ctx, int enable);
 int cgppd_last_kernel_ms(cgppd_ctx *ctx, double *mean_ms, int64_t *launches);
