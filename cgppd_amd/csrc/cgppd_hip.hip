@@ -295,6 +295,119 @@ __device__ void move_residue(const Move &mv, float4 rel, float4 &rel_new, float4
     pos_new.w = rel.w;
 }
 
+// The Philox draw of one MC step with its molecule's residue range (the
+// window kernel lays these out one window ahead).
+struct WinDraw {
+    int mol, rotate, beg, n;  // molecule, rotation (else translation), first residue, residues
+    float t[3];               // translation: the float increments of the centre
+    double q[4];              // rotation: the normalised quaternion
+    double u;                 // Metropolis uniform
+};
+
+// The parts of make_move that do not depend on the molecule's centre
+// (the same operations, so that window moves are bit-identical to it).
+__device__ void move_direction(const Proposal &p, float tstep, double rcos, double rsin, WinDraw &d)
+{
+#pragma clang fp contract(off)
+    d.rotate = p.rotate;
+    d.t[0] = d.t[1] = d.t[2] = 0.0f;
+    d.q[0] = 1.0;
+    d.q[1] = d.q[2] = d.q[3] = 0.0;
+    if (!p.rotate) {
+        float n0 = (float)p.axis[0], n1 = (float)p.axis[1], n2 = (float)p.axis[2];
+        float mag = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+        if (mag != 0.0f) {
+            n0 = n0 / mag;
+            n1 = n1 / mag;
+            n2 = n2 / mag;
+        }
+        d.t[0] = (float)((double)n0 * (double)tstep);
+        d.t[1] = (float)((double)n1 * (double)tstep);
+        d.t[2] = (float)((double)n2 * (double)tstep);
+    } else {
+        double ax = p.axis[0], ay = p.axis[1], az = p.axis[2];
+        double m = sqrt(ax * ax + ay * ay + az * az);
+        ax /= m;
+        ay /= m;
+        az /= m;
+        double w = rcos, x = rsin * ax, y = rsin * ay, z = rsin * az;
+        double qm = sqrt(w * w + x * x + y * y + z * z);
+        d.q[0] = w / qm;
+        d.q[1] = x / qm;
+        d.q[2] = y / qm;
+        d.q[3] = z / qm;
+    }
+}
+
+// The move of a window draw about the molecule's centre c_old (make_move).
+__device__ Move move_at(const WinDraw &d, float4 c_old, float box)
+{
+#pragma clang fp contract(off)
+    Move mv;
+    mv.rotate = d.rotate;
+    if (!d.rotate) {
+        mv.c[0] = fmodf(c_old.x + d.t[0], box);
+        mv.c[1] = fmodf(c_old.y + d.t[1], box);
+        mv.c[2] = fmodf(c_old.z + d.t[2], box);
+    } else {
+        mv.c[0] = c_old.x;
+        mv.c[1] = c_old.y;
+        mv.c[2] = c_old.z;
+    }
+    mv.q[0] = d.q[0];
+    mv.q[1] = d.q[1];
+    mv.q[2] = d.q[2];
+    mv.q[3] = d.q[3];
+    return mv;
+}
+
+// The quaternion's rotation terms as move_residue forms them, once per move.
+struct RotTerms {
+    double a, b, c, d, e, f, g, h, i;
+};
+__device__ RotTerms rot_terms(const Move &mv)
+{
+#pragma clang fp contract(off)
+    const double w = mv.q[0], x = mv.q[1], y = mv.q[2], z = mv.q[3];
+    const double w2 = w * w, x2 = x * x, y2 = y * y, z2 = z * z;
+    const double wx = w * x, wy = w * y, wz = w * z, xy = x * y, xz = x * z, yz = y * z;
+    RotTerms k;
+    k.a = w2 + x2 - y2 - z2;
+    k.b = xy - wz;
+    k.c = wy + xz;
+    k.d = wz + xy;
+    k.e = yz - wx;
+    k.f = w2 - x2 + y2 - z2;
+    k.g = xz - wy;
+    k.h = yz + wx;
+    k.i = w2 - x2 - y2 + z2;
+    return k;
+}
+
+// move_residue with the rotation terms formed beforehand (the same
+// operations on the same values).
+__device__ __forceinline__ void move_residue_terms(const Move &mv, const RotTerms &k, float4 rel, float4 &rel_new,
+                                                   float4 &pos_new)
+{
+#pragma clang fp contract(off)
+    if (!mv.rotate) {
+        rel_new = rel;
+        pos_new.x = rel.x + mv.c[0];
+        pos_new.y = rel.y + mv.c[1];
+        pos_new.z = rel.z + mv.c[2];
+    } else {
+        const double vx = rel.x, vy = rel.y, vz = rel.z;
+        rel_new.x = (float)(vx * k.a + (vy * k.b + vz * k.c) * 2.0);
+        rel_new.y = (float)((vx * k.d + vz * k.e) * 2.0 + vy * k.f);
+        rel_new.z = (float)((vx * k.g + vy * k.h) * 2.0 + vz * k.i);
+        rel_new.w = rel.w;
+        pos_new.x = mv.c[0] + rel_new.x;
+        pos_new.y = mv.c[1] + rel_new.y;
+        pos_new.z = mv.c[2] + rel_new.z;
+    }
+    pos_new.w = rel.w;
+}
+
 // ---------------------------------------------------------------------------
 // Kernel parameters
 // ---------------------------------------------------------------------------
@@ -373,6 +486,8 @@ struct PairParams {
     // energy between slot v's and slot q's proposals (q < s)
     unsigned long long *wrowfix;
     double *wrowbuf;
+    double *wu;                  // R W: Metropolis uniforms of the current window's steps
+    struct WinDraw *wd;          // R W: draws of the W steps after the current window
     // flexible molecules (null / 0 when the system has none).  Residue
     // indices are device indices (global); lists are CSR per molecule.
     int n_flex;                 // number of flexible molecules
@@ -2707,6 +2822,18 @@ __device__ __forceinline__ double window_row(const PairParams &p, int r, int s, 
     return v;
 }
 
+__device__ __forceinline__ WinDraw win_draw(const PairParams &p, int r, int rr, int64_t step)
+{
+    const Proposal pr = draw_proposal(p.seed, rr, step, p.n_mol);
+    WinDraw d;
+    d.mol = pr.mol;
+    d.beg = p.mol_off[pr.mol];
+    d.n = p.mol_off[pr.mol + 1] - d.beg;
+    move_direction(pr, p.tstep[r], p.rcos[r], p.rsin[r], d);
+    d.u = pr.u;
+    return d;
+}
+
 // One workgroup per replica.  begin_steps >= 0: the start of a run of
 // begin_steps MC steps: sets the run's end and lays out the first window
 // (slot 0 is the pending proposal; slots 1.. are laid out from the current
@@ -2722,17 +2849,39 @@ __device__ __forceinline__ double window_row(const PairParams &p, int r, int s, 
 // X_sq - row_q[m_s] (X_sq the energy between the two proposals, row_q[m_s]
 // the energy between q's proposal and m_s as stored):
 //   delta_s = D_s + sum_{q < s accepted} (X_sq - row_q[m_s] - row_s[m_q] + mat[m_s][m_q]).
+// Metropolis accepts when u < exp(-delta / RT); the test is taken on
+// log(u) (computed while the loads are in flight) unless the two sides are
+// within 1e-12 of each other, where the exponential decides as written.
+//
+// The kernel is a chain of latencies, so every load is issued before the
+// decisions and every store after them, in two levels: (1) the window's
+// headers, uniforms and rows and the next window's draws (laid out by the
+// previous launch), (2) what depends on their molecules: the matrix
+// entries of the delta-E sums, the correction terms, this window's
+// proposed relative positions (for the apply) and the next molecules'
+// current relative positions and centres (for the next proposals), staged
+// in LDS.  A proposed position is the proposed centre plus the proposed
+// relative position (the order of move_residue's float additions), so only
+// relative positions are read.  The code is kept compact (one site per
+// loop): every launch runs it from a cold instruction cache.
+constexpr int kWinStage = 1536;  // residues staged in LDS per kind (this window's, the next window's)
 __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_steps)
 {
 #pragma clang fp contract(off)
     __shared__ double s_D[kMaxWin];
     __shared__ double s_corr[kMaxWin * kMaxWin];
-    __shared__ double s_u[kMaxWin];
+    __shared__ double s_X[kMaxWin * kMaxWin];
+    __shared__ double s_u[kMaxWin], s_lu[kMaxWin];
     __shared__ int4 s_hd[kMaxWin];
     __shared__ float4 s_cn[kMaxWin];
-    __shared__ int s_next[kMaxWin];    // molecules of the next window's steps
-    __shared__ Proposal s_prop[kMaxWin];
-    __shared__ Move s_mv[kMaxWin];
+    __shared__ WinDraw s_nd[kMaxWin];      // the next window's draws
+    __shared__ float s_nrad[kMaxWin];
+    __shared__ Move s_mv[2][kMaxWin];      // from the stored centre, and from the proposed centre of
+                                           // the slot of this window that moves the same molecule
+    __shared__ RotTerms s_rt[kMaxWin];     // the next slots' rotation terms
+    __shared__ unsigned s_accm;
+    __shared__ float4 s_ar[kWinStage];     // this window's proposed relative positions
+    __shared__ float4 s_nr[kWinStage];     // the next molecules' stored relative positions
     const int r = blockIdx.x, t = threadIdx.x;
     const int lane = t & (kWave - 1), wave = t / kWave;
     const int lw = p.win_log2, W = 1 << lw;
@@ -2744,170 +2893,281 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
     const int64_t stop = begin ? step + begin_steps : p.wstop[r];
     const int64_t next = step + act;
     const int rr = r + p.rep_rng;
-    // phase A: the window's headers and uniforms, the next window's draws
-    if (t < act) {
+    const int first = begin ? 1 : 0;
+    float4 *pos = p.pos + (size_t)r * p.stride;
+    float4 *rel = p.rel + (size_t)r * p.stride;
+    double *mat = p.emat + (size_t)r * M * M;
+    const double beta = 4184.0 / ((double)kRgas * p.temperature[r]);
+    // ---- level 1: the window and the next window's draws
+    double e_old = 0.0;
+    long long c0 = 0, c1 = 0, c2 = 0;
+    if (t == 0 && act > 0) {
+        e_old = p.energy[r];
+        c0 = p.counters[3 * (size_t)r];
+        c1 = p.counters[3 * (size_t)r + 1];
+        c2 = p.counters[3 * (size_t)r + 2];
+    }
+    if (t < W) {
         s_hd[t] = p.whdr[(r << lw) + t];
         s_cn[t] = p.wcen[(r << lw) + t];
+        s_u[t] = p.wu[(r << lw) + t];
     } else if (t >= kWave && t < kWave + W) {
         const int k = t - kWave;
-        const Proposal pr = draw_proposal(p.seed, rr, next + k, M);
-        s_prop[k] = pr;
-        s_next[k] = pr.mol;
-    } else if (t >= 2 * kWave && t < 2 * kWave + act) {
-        s_u[t - 2 * kWave] = draw_proposal(p.seed, rr, step + (t - 2 * kWave), M).u;
+        s_nd[k] = begin ? win_draw(p, r, rr, next + k) : p.wd[(r << lw) + k];
+    }
+    // rows: wave w takes slots w and w + 4, its lanes J, J + 64, ... (the
+    // values stay in registers for the matrix update)
+    constexpr int kRowRegs = 4;
+    double rv[2][kRowRegs];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int s = wave + it * kWavesPerWG;
+#pragma unroll
+        for (int u = 0; u < kRowRegs; ++u) {
+            const int J = lane + u * kWave;
+            rv[it][u] = s < act && J < M ? window_row(p, r, s, J) : 0.0;
+        }
     }
     __syncthreads();
-    unsigned accm = 0u;
-    if (act > 0) {
-        // phase B: D_s (wave w takes slots w, w + 4) and the corrections
-        for (int s = wave; s < act; s += kWavesPerWG) {
-            const int ms = s_hd[s].x;
-            const double *mrow = p.emat + ((size_t)r * M + ms) * M;
-            double d = 0.0;
-            for (int J = lane; J < M; J += kWave) d += window_row(p, r, s, J) - mrow[J];
-            d = wave_total(d);
-            if (lane == 0) s_D[s] = d;
+    // ---- level 2
+    const int len = window_len([&](int k) { return s_nd[k].mol; }, next, stop, W);
+    const int n_lay = begin ? len : max(len, 1);
+    // residue prefixes: this window's slots, the next window's laid-out slots
+    int cum[kMaxWin + 1], ncum[kMaxWin + 1];
+    cum[0] = ncum[0] = 0;
+#pragma unroll
+    for (int s = 0; s < kMaxWin; ++s) {
+        cum[s + 1] = cum[s] + (s < act ? s_hd[s].z : 0);
+        ncum[s + 1] = ncum[s] + (s >= first && s < n_lay ? s_nd[s].n : 0);
+    }
+    const int n_items = cum[kMaxWin], n_next = ncum[kMaxWin];
+    // the slot of this window that moves next slot j's molecule, or -1
+    auto slot_moving = [&](int j) {
+        int a = -1;
+        for (int q = 0; q < act; ++q)
+            if (s_hd[q].x == s_nd[j].mol) a = q;
+        return a;
+    };
+    // slot of item i in a prefix, and a prefix entry (selects, so that the
+    // prefixes stay in registers)
+    auto slot_of = [&](const int (&c)[kMaxWin + 1], int i) {
+        int s = 0;
+#pragma unroll
+        for (int q = 1; q < kMaxWin; ++q) s += i >= c[q] ? 1 : 0;
+        return s;
+    };
+    auto at = [&](const int (&c)[kMaxWin + 1], int s) {
+        int v = 0;
+#pragma unroll
+        for (int q = 0; q <= kMaxWin; ++q) v = q == s ? c[q] : v;
+        return v;
+    };
+    if (t < 2 * kMaxWin) {
+        // the next slots' moves: from the stored centre (var 0), and, when
+        // this window moves the same molecule, from its proposed centre
+        const int j = t & (kMaxWin - 1), var = t / kMaxWin;
+        const int a = j < n_lay ? slot_moving(j) : -1;
+        if (j >= first && j < n_lay && (var == 0 || a >= 0)) {
+            const WinDraw &d = s_nd[j];
+            const float4 c2 = var == 0 ? p.center[(size_t)r * M + d.mol] : s_cn[a];
+            if (var == 0) s_nrad[j] = p.radius[(size_t)r * M + d.mol];
+            const Move mv = move_at(d, c2, p.box);
+            s_mv[var][j] = mv;
+            if (var == 0) s_rt[j] = rot_terms(mv);
         }
-        if (t < W * W) {
-            const int s = t >> lw, q = t & (W - 1);
-            if (q < s && s < act) {
-                const int ms = s_hd[s].x, mq = s_hd[q].x;
-                const double X = window_row(p, r, s, M + q);
-                const double a = window_row(p, r, q, ms);
-                const double b = window_row(p, r, s, mq);
-                const double c = p.emat[((size_t)r * M + ms) * M + mq];
-                s_corr[t] = X - a - (b - c);
+    } else if (t >= kWave && t < kWave + kMaxWin) {
+        // the draws after the next window (for the launch after this one),
+        // and the next window's uniforms
+        const int k = t - kWave;
+        if (k < W) p.wd[(r << lw) + k] = win_draw(p, r, rr, next + len + k);
+        if (k < len) p.wu[(r << lw) + k] = s_nd[k].u;
+    } else if (t >= kWave + kMaxWin && t < kWave + kMaxWin + act) {
+        const int k = t - kWave - kMaxWin;
+        s_lu[k] = log(s_u[k]);
+    } else if (begin && t == kWave + 2 * kMaxWin) {
+        // slot 0 of a run's first window: the pending proposal
+        const int4 h = p.prop_hdr[r];
+        const float4 c = p.prop_center[r];
+        p.whdr[r << lw] = h;
+        p.wcen[r << lw] = make_float4(c.x, c.y, c.z, __int_as_float(0));
+    }
+    if (t >= 2 * kWave && t < 2 * kWave + W * W) {
+        const int s = (t - 2 * kWave) >> lw, q = (t - 2 * kWave) & (W - 1);
+        if (q < s && s < act) {
+            const int ms = s_hd[s].x, mq = s_hd[q].x;
+            const double X = window_row(p, r, s, M + q);
+            const double a = window_row(p, r, q, ms);
+            const double b = window_row(p, r, s, mq);
+            const double c = mat[(size_t)ms * M + mq];
+            s_X[(s << lw) + q] = X;
+            s_corr[(s << lw) + q] = X - a - (b - c);
+        }
+    }
+    // D_s
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int s = wave + it * kWavesPerWG;
+        if (s >= act) continue;
+        const double *mrow = mat + (size_t)s_hd[s].x * M;
+        double d = 0.0;
+#pragma unroll
+        for (int u = 0; u < kRowRegs; ++u) {
+            const int J = lane + u * kWave;
+            if (J < M) d += rv[it][u] - mrow[J];
+        }
+        for (int J = lane + kRowRegs * kWave; J < M; J += kWave) d += window_row(p, r, s, J) - mrow[J];
+        d = wave_total(d);
+        if (lane == 0) s_D[s] = d;
+    }
+    // staged: this window's proposed relative positions, the next
+    // molecules' stored ones (three per thread per pass, loads first)
+    for (int base = 0; base < kWinStage && (base < n_items || base < n_next); base += 3 * kWG) {
+        auto load_a = [&](int i) {
+            const int s = slot_of(cum, i);
+            return i < n_items && i < kWinStage ? slot_rel(p, r, s)[i - at(cum, s)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        };
+        auto load_b = [&](int i) {
+            const int j = slot_of(ncum, i);
+            return i < n_next && i < kWinStage ? rel[s_nd[j].beg + i - at(ncum, j)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        };
+        const int i0 = base + t, i1 = i0 + kWG, i2 = i1 + kWG;
+        const float4 a0 = load_a(i0), a1 = load_a(i1), a2 = load_a(i2);
+        const float4 b0 = load_b(i0), b1 = load_b(i1), b2 = load_b(i2);
+        if (i0 < kWinStage) {
+            s_ar[i0] = a0;
+            s_nr[i0] = b0;
+        }
+        if (i1 < kWinStage) {
+            s_ar[i1] = a1;
+            s_nr[i1] = b1;
+        }
+        if (i2 < kWinStage) {
+            s_ar[i2] = a2;
+            s_nr[i2] = b2;
+        }
+    }
+    __syncthreads();
+    // ---- decisions in step order (thread 0), then every store
+    if (t == 0) {
+        unsigned accm = 0u;
+        double e = e_old;
+        unsigned long long pairs = 0ull;
+        for (int s = 0; s < act; ++s) {
+            double d = s_D[s];
+            for (int q = 0; q < s; ++q)
+                if (accm >> q & 1u) d += s_corr[(s << lw) + q];
+            int kind;
+            if (d < 0.0) {
+                kind = 0;
+            } else {
+                const double x = -d * beta;
+                const double gap = x - s_lu[s];
+                if (s_u[s] != 0.0 && fabs(gap) > 1e-12 * (1.0 + fabs(x)))
+                    kind = gap > 0.0 ? 1 : 2;
+                else
+                    kind = s_u[s] < exp(-d * 4184.0 / ((double)kRgas * p.temperature[r])) ? 1 : 2;
             }
-        }
-        __syncthreads();
-        // phase C: the decisions in step order (every thread takes the same
-        // ones from the same values)
-        const double temp = p.temperature[r];
-        double dE[kMaxWin];
-        int kinds[kMaxWin];
-#pragma unroll
-        for (int s = 0; s < kMaxWin; ++s) {
-            dE[s] = 0.0;
-            kinds[s] = 2;
-            if (s < act) {
-                double d = s_D[s];
-#pragma unroll
-                for (int q = 0; q < kMaxWin; ++q)
-                    if (q < s && (accm >> q & 1u)) d += s_corr[(s << lw) + q];
-                const int kind = d < 0.0 ? 0 : s_u[s] < exp(-d * 4184.0 / ((double)kRgas * temp)) ? 1 : 2;
-                if (kind < 2) accm |= 1u << s;
-                dE[s] = d;
-                kinds[s] = kind;
+            if (kind < 2) {
+                accm |= 1u << s;
+                e += d;
             }
+            c0 += kind == 0;
+            c1 += kind == 1;
+            c2 += kind == 2;
+            const unsigned long long nm = (unsigned long long)s_hd[s].z;
+            pairs += nm * ((unsigned long long)N - nm);
         }
-        if (t == 0) {
-            double e = p.energy[r];
-            long long c0 = 0, c1 = 0, c2 = 0;
-            unsigned long long pairs = 0ull;
-#pragma unroll
-            for (int s = 0; s < kMaxWin; ++s)
-                if (s < act) {
-                    if (kinds[s] < 2) e += dE[s];
-                    c0 += kinds[s] == 0;
-                    c1 += kinds[s] == 1;
-                    c2 += kinds[s] == 2;
-                    const unsigned long long nm = (unsigned long long)s_hd[s].z;
-                    pairs += nm * ((unsigned long long)N - nm);
-                }
+        s_accm = accm;
+        if (act > 0) {
             p.energy[r] = e;
-            p.counters[3 * (size_t)r] += c0;
-            p.counters[3 * (size_t)r + 1] += c1;
-            p.counters[3 * (size_t)r + 2] += c2;
+            p.counters[3 * (size_t)r] = c0;
+            p.counters[3 * (size_t)r + 1] = c1;
+            p.counters[3 * (size_t)r + 2] = c2;
             p.steps[r] = next;
             atomicAdd(p.pair_evals, pairs);
         }
-        // apply (src/Molecule.cpp:468-504): coordinates of the accepted
-        // slots, and their matrix rows and columns; an entry between two
-        // accepted slots is the energy between their proposals
-        float4 *pos = p.pos + (size_t)r * p.stride;
-        float4 *rel = p.rel + (size_t)r * p.stride;
-        double *mat = p.emat + (size_t)r * M * M;
-        for (int s = 0; s < act; ++s) {
-            if (!(accm >> s & 1u)) continue;
-            const int ms = s_hd[s].x, i0 = s_hd[s].y, ns = s_hd[s].z;
-            const float4 *sp = slot_pos(p, r, s), *sq = slot_rel(p, r, s);
-            for (int k = t; k < ns; k += kWG) {
-                pos[i0 + k] = sp[k];
-                rel[i0 + k] = sq[k];
-            }
-            if (t == 0) p.center[(size_t)r * M + ms] = make_float4(s_cn[s].x, s_cn[s].y, s_cn[s].z, 0.0f);
-            for (int J = t; J < M; J += kWG) {
-                double v = -1.0;
-                bool pair = false;
-                for (int q = 0; q < act; ++q)
-                    if (q != s && (accm >> q & 1u) && s_hd[q].x == J) {
-                        v = q < s ? window_row(p, r, s, M + q) : window_row(p, r, q, M + s);
-                        pair = true;
-                    }
-                if (!pair) v = window_row(p, r, s, J);
-                mat[(size_t)ms * M + J] = v;
-                mat[(size_t)J * M + ms] = v;
-            }
-        }
-        // the rows are cleared for the next window (every read of them
-        // above comes before the barrier below)
-    }
-    __syncthreads();
-    if (act > 0) {
-        const int MW = M + W;
-        for (int k = t; k < act * MW; k += kWG) {
-            const size_t at = (size_t)(r << lw) * MW + k;
-            p.wrowfix[at] = 0ull;
-            p.wrowbuf[at] = 0.0;
-        }
-    }
-    // phase D: the next window, laid out from the state after the apply
-    const int len = window_len([&](int k) { return s_next[k]; }, next, stop, W);
-    const int first = begin ? 1 : 0;
-    const int n_lay = begin ? len : max(len, 1);
-    if (t < n_lay && t >= first) {
-        const int m2 = s_next[t];
-        const float4 c2 = p.center[(size_t)r * M + m2];
-        s_mv[t] = make_move(s_prop[t], c2, p.tstep[r], p.rcos[r], p.rsin[r], p.box);
-    }
-    __syncthreads();
-    int ext = 0;
-    for (int s = 0; s < n_lay; ++s) {
-        const int m2 = s_next[s];
-        const int j_beg = p.mol_off[m2], n_2 = p.mol_off[m2 + 1] - j_beg;
-        if (s >= first) {
-            const Move mv = s_mv[s];
-            const float4 *rl = p.rel + (size_t)r * p.stride + j_beg;
-            float4 *qp = slot_pos(p, r, s), *qq = slot_rel(p, r, s);
-            for (int k = t; k < n_2; k += kWG) {
-                float4 rn, pn;
-                move_residue(mv, rl[k], rn, pn);
-                qp[k] = pn;
-                qq[k] = rn;
-            }
-            if (t == 0) {
-                const float rad = p.radius[(size_t)r * M + m2];
-                p.whdr[(r << lw) + s] = make_int4(m2, j_beg, n_2, __float_as_int(rad));
-                p.wcen[(r << lw) + s] = make_float4(mv.c[0], mv.c[1], mv.c[2], __int_as_float(ext));
-                if (s == 0) {
-                    p.prop_mol[r] = m2;
-                    p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
-                    p.prop_n[r] = n_2;
-                    p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
-                    record_proposal(p, r, m2, j_beg, n_2, kMovedNone, 0, rad, 0.0);
-                }
-            }
-        } else if (t == 0) {
-            // slot 0 of a run's first window: the pending proposal
-            p.whdr[r << lw] = p.prop_hdr[r];
-            const float4 c = p.prop_center[r];
-            p.wcen[r << lw] = make_float4(c.x, c.y, c.z, __int_as_float(0));
-        }
-        ext += n_2;
-    }
-    if (t == 0) {
         p.wlen[r] = len;
         if (begin) p.wstop[r] = stop;
+    }
+    __syncthreads();
+    const unsigned accm = s_accm;
+    // apply (src/Molecule.cpp:468-504): the accepted slots' coordinates
+    // (residues past the staged ones are read now, before the barrier that
+    // precedes the next window's writes into the slot buffers)
+    for (int i = t; i < n_items; i += kWG) {
+        const int s = slot_of(cum, i);
+        if (accm >> s & 1u) {
+            const int k = i - at(cum, s);
+            const float4 q = i < kWinStage ? s_ar[i] : slot_rel(p, r, s)[k], c = s_cn[s];
+            rel[s_hd[s].y + k] = q;
+            pos[s_hd[s].y + k] = make_float4(c.x + q.x, c.y + q.y, c.z + q.z, q.w);
+        }
+    }
+    if (t < act && (accm >> t & 1u))
+        p.center[(size_t)r * M + s_hd[t].x] = make_float4(s_cn[t].x, s_cn[t].y, s_cn[t].z, 0.0f);
+    // the matrix rows and columns of the accepted slots; an entry between
+    // two accepted slots is the energy between their proposals
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int s = wave + it * kWavesPerWG;
+        if (s >= act || !(accm >> s & 1u)) continue;
+        const int ms = s_hd[s].x;
+        for (int J = lane; J < M; J += kWave) {
+            const int u = J / kWave;
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < kRowRegs; ++q)
+                if (u == q) v = rv[it][q];
+            if (u >= kRowRegs) v = window_row(p, r, s, J);
+            for (int q = 0; q < act; ++q)
+                if (q != s && (accm >> q & 1u) && s_hd[q].x == J) v = q < s ? s_X[(s << lw) + q] : s_X[(q << lw) + s];
+            mat[(size_t)ms * M + J] = v;
+            mat[(size_t)J * M + ms] = v;
+        }
+    }
+    // the rows are cleared for the next window
+    for (int k = t; k < act * (M + W); k += kWG) {
+        const size_t at = (size_t)(r << lw) * (M + W) + k;
+        p.wrowfix[at] = 0ull;
+        p.wrowbuf[at] = 0.0;
+    }
+    // ---- the next window (src/Molecule.cpp:146-266), from the state after
+    // the apply: a next molecule that this window moved and accepted starts
+    // from that proposal.  Past the staged residues the relative positions
+    // after the apply are read, behind a barrier.
+    if (n_items > kWinStage || n_next > kWinStage) __syncthreads();
+    for (int i = t; i < n_next; i += kWG) {
+        const int j = slot_of(ncum, i), k = i - at(ncum, j);
+        const int a = slot_moving(j);
+        const bool from_slot = a >= 0 && (accm >> a & 1u);
+        const int ia = from_slot ? at(cum, a) + k : 0;
+        float4 q;
+        if (from_slot && ia < kWinStage)
+            q = s_ar[ia];
+        else if (!from_slot && i < kWinStage)
+            q = s_nr[i];
+        else
+            q = rel[s_nd[j].beg + k];
+        float4 rn, pn;
+        move_residue_terms(s_mv[from_slot ? 1 : 0][j], s_rt[j], q, rn, pn);
+        slot_pos(p, r, j)[k] = pn;
+        slot_rel(p, r, j)[k] = rn;
+    }
+    if (t < n_lay && t >= first) {
+        const int j = t;
+        const int a = slot_moving(j);
+        const Move &mv = s_mv[a >= 0 && (accm >> a & 1u) ? 1 : 0][j];
+        const int m2 = s_nd[j].mol, j_beg = s_nd[j].beg, n_2 = s_nd[j].n;
+        const float rad = s_nrad[j];
+        p.whdr[(r << lw) + j] = make_int4(m2, j_beg, n_2, __float_as_int(rad));
+        p.wcen[(r << lw) + j] = make_float4(mv.c[0], mv.c[1], mv.c[2], __int_as_float(at(ncum, j) + (first ? s_nd[0].n : 0)));
+        if (j == 0) {
+            p.prop_mol[r] = m2;
+            p.prop_nib[r] = (n_2 + kIBlock - 1) / kIBlock;
+            p.prop_n[r] = n_2;
+            p.prop_center[r] = make_float4(mv.c[0], mv.c[1], mv.c[2], 0.0f);
+            record_proposal(p, r, m2, j_beg, n_2, kMovedNone, 0, rad, 0.0);
+        }
     }
 }
 
@@ -3803,6 +4063,8 @@ struct cgppd_ctx {
     float4 *d_wprop_pos = nullptr, *d_wprop_rel = nullptr;
     unsigned long long *d_wrowfix = nullptr;
     double *d_wrowbuf = nullptr;
+    double *d_wu = nullptr;
+    WinDraw *d_wd = nullptr;
     long long h_step = 0;
 
     hipGraphExec_t graph_exec = nullptr;
@@ -3823,6 +4085,8 @@ struct cgppd_ctx {
     int pair_layout = CGPPD_LAYOUT_AUTO;  // caller's choice
     int tuned_layout = 0, tuned_wg = 0;    // set by cgppd_autotune
     double tune_ms[3][7] = {};             // its timings: j-lanes, j-lanes x4, i-lanes; 2..8 per CU (0: not run)
+    int tune_win[3][7] = {};               // the window slots of each timing
+    int tuned_window = 0;                  // the autotuner's window slots (0: not tuned)
     bool timing = false;
     std::vector<hipEvent_t> events;
     double t_mean_ms = 0.0;
@@ -3902,6 +4166,8 @@ void free_windows(cgppd_ctx *ctx)
     dev_free(ctx->d_wprop_rel);
     dev_free(ctx->d_wrowfix);
     dev_free(ctx->d_wrowbuf);
+    dev_free(ctx->d_wu);
+    dev_free(ctx->d_wd);
     ctx->win_alloc = 0;
 }
 
@@ -4270,6 +4536,8 @@ PairParams base_params(cgppd_ctx *ctx)
     p.wprop_rel = ctx->d_wprop_rel;
     p.wrowfix = ctx->d_wrowfix;
     p.wrowbuf = ctx->d_wrowbuf;
+    p.wu = ctx->d_wu;
+    p.wd = ctx->d_wd;
     p.emat = ctx->d_emat;
     p.energy = ctx->d_energy;
     p.temperature = ctx->d_temp;
@@ -4729,7 +4997,10 @@ int window_of(const cgppd_ctx *ctx)
         return 1;
     const int layout = pair_layout_of(ctx);
     if (layout != CGPPD_LAYOUT_JLANES && layout != CGPPD_LAYOUT_JLANES4) return 1;
-    return ctx->window > 1 ? ctx->window : kAutoWindow;
+    if (ctx->window > 1) return ctx->window;
+    // the autotuner's choice holds for the layout it was tuned with
+    if (ctx->tuned_window > 0 && ctx->tuned_layout == layout) return ctx->tuned_window;
+    return kAutoWindow;
 }
 
 // Window buffers for W slots (allocated and cleared outside any capture).
@@ -4750,6 +5021,8 @@ int ensure_windows(cgppd_ctx *ctx, int W)
     if ((rc = dev_alloc(ctx, &ctx->d_wprop_rel, R * (W - 1) * ctx->max_n))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_wrowfix, rows))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_wrowbuf, rows))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wu, R * W))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_wd, R * W))) return rc;
     CG_HIP(ctx, hipMemset(ctx->d_wrowfix, 0, sizeof(unsigned long long) * rows));
     CG_HIP(ctx, hipMemset(ctx->d_wrowbuf, 0, sizeof(double) * rows));
     CG_HIP(ctx, hipMemset(ctx->d_wlen, 0, sizeof(int) * R));
@@ -5911,13 +6184,33 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     for (auto &row : ctx->tune_ms)
         for (double &v : row) v = 0.0;
     bool windowed = false;
+    const int keep_window = ctx->window;
+    int best_win = 0;
     for (int li = 0; li < n_layouts && rc == CGPPD_OK; ++li) {
         ctx->pair_layout = ctx->n_flex ? CGPPD_LAYOUT_AUTO : layouts[li];
+        // window candidates: with the automatic choice, 1, 4 and 8 slots
+        // where the layout runs windows, else the caller's
+        int cands[3] = {0, 0, 0}, n_cands = 0;
+        ctx->window = keep_window == 0 ? 4 : keep_window;
+        if (keep_window == 0 && window_of(ctx) > 1) {
+            cands[0] = 1;
+            cands[1] = 4;
+            cands[2] = 8;
+            n_cands = 3;
+        } else {
+            cands[0] = keep_window;
+            n_cands = 1;
+        }
+        for (int ci = 0; ci < n_cands && rc == CGPPD_OK; ++ci) {
+        ctx->window = cands[ci];
         // a layout that runs speculative windows is timed on the window
-        // laid out from the current state, and its time per launch is
+        // laid out from the current state; the time per MC step is the
+        // launch time plus the decision kernel's (an estimate, from traces
+        // of a 40-chain system: 6 us for one step, 15 us for a window),
         // divided by the mean window length of the next 1000 steps
         const int Wc = window_of(ctx);
         double steps_per_launch = 1.0;
+        const double finish_ms = Wc > 1 ? 0.015 : 0.006;
         if (Wc > 1) {
             if ((rc = ensure_windows(ctx, Wc))) break;
             if ((rc = launch_window_begin(ctx, 1 << 20))) break;
@@ -5946,15 +6239,22 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
                 }
                 sum += ms;
             }
-            const double mean = sum / launches / steps_per_launch;
-            ctx->tune_ms[ctx->n_flex ? 2 : li][w - 2] = mean;
+            const double mean = (sum / launches + finish_ms) / steps_per_launch;
+            double &slot = ctx->tune_ms[ctx->n_flex ? 2 : li][w - 2];
+            if (slot == 0.0 || mean < slot) {
+                slot = mean;
+                ctx->tune_win[ctx->n_flex ? 2 : li][w - 2] = Wc;
+            }
             if (rc == CGPPD_OK && mean < best) {
                 best = mean;
                 best_l = ctx->n_flex ? CGPPD_LAYOUT_ILANES : layouts[li];
                 best_w = w;
+                best_win = Wc;
             }
         }
+        }
     }
+    ctx->window = keep_window;
     // ties (within 1 % of the best, same layout) go to more workgroups per
     // CU: with the finish kernel between launches, the fuller grid measured
     // faster in replayed intervals (1.074 M against 1.058 M MC steps/s on a
@@ -5966,6 +6266,7 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
             if (v > 0.0 && v <= best * 1.01) {
                 best_w = w;
                 best = v;
+                best_win = ctx->tune_win[li][w - 2];
                 break;
             }
         }
@@ -5987,6 +6288,7 @@ int cgppd_autotune(cgppd_ctx *ctx, int launches, int *layout, int *wg_per_cu, do
     if (back != hipSuccess || done != hipSuccess) return fail(ctx, CGPPD_E_HIP, "autotune: restoring the rows failed");
     ctx->tuned_layout = best_l;
     ctx->tuned_wg = best_w;
+    ctx->tuned_window = best_win;
     destroy_graph(ctx);
     if (layout) *layout = best_l;
     if (wg_per_cu) *wg_per_cu = best_w;

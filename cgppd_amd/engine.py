@@ -339,7 +339,8 @@ class Engine:
 
     def autotune(self, launches=20):
         """Times the pair-kernel candidates on the pending proposals and keeps
-        the fastest; returns {"layout", "wg_per_cu", "ms"}."""
+        the fastest; returns {"layout", "wg_per_cu", "window", "ms"} (ms per
+        MC step of all replicas) and the per-candidate table."""
         lay, wg, ms = C.c_int(), C.c_int(), C.c_double()
         self._check(self.lib.cgppd_autotune(self.h, int(launches), C.byref(lay), C.byref(wg), C.byref(ms)),
                     "autotune")
@@ -349,7 +350,8 @@ class Engine:
         timings = {f"{lay}/{w}": round(float(table[i * 7 + w - 2]) * 1e3, 2)
                    for i, lay in enumerate(("jlanes", "jlanes4", "ilanes")) for w in range(2, 9)
                    if table[i * 7 + w - 2] > 0}
-        return {"layout": names[lay.value], "wg_per_cu": wg.value, "ms": ms.value, "us_by_candidate": timings}
+        return {"layout": names[lay.value], "wg_per_cu": wg.value, "window": self.window(), "ms": ms.value,
+                "us_by_candidate": timings}
 
     def set_repulsive_crowding(self, on):
         """(6/r)^12 below 15 A and no DH for pairs with a crowder molecule

@@ -114,6 +114,9 @@ def test_crowded40_zero_surrogate_is_plain_metropolis(gpu_engine_factory):
     spec = synth.config(1)
     plain = gpu_engine_factory()
     plain.load_spec(spec, seed=21)
+    # one MC step per launch pair, as delayed acceptance runs (windows sum
+    # the delta-E in another FP64 order)
+    plain.set_window(1)
     plain.mc_prepare()
     da = gpu_engine_factory()
     da.load_spec(spec, seed=21)
