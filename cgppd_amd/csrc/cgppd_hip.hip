@@ -2865,7 +2865,12 @@ __device__ __forceinline__ WinDraw win_draw(const PairParams &p, int r, int rr, 
 // relative positions are read.  The code is kept compact (one site per
 // loop): every launch runs it from a cold instruction cache.
 constexpr int kWinStage = 1536;  // residues staged in LDS per kind (this window's, the next window's)
-__global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_steps)
+// The decision kernel's workgroup: 16 waves, so that the residues of a
+// window are spread thin and every slot has a wave of its own.
+constexpr int kWinWaves = 16;
+constexpr int kWinWG = kWinWaves * kWave;
+constexpr int kWinIt = (kMaxWin + kWinWaves - 1) / kWinWaves;  // slots per wave
+__global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int begin_steps)
 {
 #pragma clang fp contract(off)
     __shared__ double s_D[kMaxWin];
@@ -2918,10 +2923,10 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
     // rows: wave w takes slots w and w + 4, its lanes J, J + 64, ... (the
     // values stay in registers for the matrix update)
     constexpr int kRowRegs = 4;
-    double rv[2][kRowRegs];
+    double rv[kWinIt][kRowRegs];
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const int s = wave + it * kWavesPerWG;
+    for (int it = 0; it < kWinIt; ++it) {
+        const int s = wave + it * kWinWaves;
 #pragma unroll
         for (int u = 0; u < kRowRegs; ++u) {
             const int J = lane + u * kWave;
@@ -3005,8 +3010,8 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
     }
     // D_s
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const int s = wave + it * kWavesPerWG;
+    for (int it = 0; it < kWinIt; ++it) {
+        const int s = wave + it * kWinWaves;
         if (s >= act) continue;
         const double *mrow = mat + (size_t)s_hd[s].x * M;
         double d = 0.0;
@@ -3021,7 +3026,7 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
     }
     // staged: this window's proposed relative positions, the next
     // molecules' stored ones (three per thread per pass, loads first)
-    for (int base = 0; base < kWinStage && (base < n_items || base < n_next); base += 3 * kWG) {
+    for (int base = 0; base < kWinStage && (base < n_items || base < n_next); base += 3 * kWinWG) {
         auto load_a = [&](int i) {
             const int s = slot_of(cum, i);
             return i < n_items && i < kWinStage ? slot_rel(p, r, s)[i - at(cum, s)] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -3030,7 +3035,7 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
             const int j = slot_of(ncum, i);
             return i < n_next && i < kWinStage ? rel[s_nd[j].beg + i - at(ncum, j)] : make_float4(0.f, 0.f, 0.f, 0.f);
         };
-        const int i0 = base + t, i1 = i0 + kWG, i2 = i1 + kWG;
+        const int i0 = base + t, i1 = i0 + kWinWG, i2 = i1 + kWinWG;
         const float4 a0 = load_a(i0), a1 = load_a(i1), a2 = load_a(i2);
         const float4 b0 = load_b(i0), b1 = load_b(i1), b2 = load_b(i2);
         if (i0 < kWinStage) {
@@ -3094,7 +3099,7 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
     // apply (src/Molecule.cpp:468-504): the accepted slots' coordinates
     // (residues past the staged ones are read now, before the barrier that
     // precedes the next window's writes into the slot buffers)
-    for (int i = t; i < n_items; i += kWG) {
+    for (int i = t; i < n_items; i += kWinWG) {
         const int s = slot_of(cum, i);
         if (accm >> s & 1u) {
             const int k = i - at(cum, s);
@@ -3108,8 +3113,8 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
     // the matrix rows and columns of the accepted slots; an entry between
     // two accepted slots is the energy between their proposals
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const int s = wave + it * kWavesPerWG;
+    for (int it = 0; it < kWinIt; ++it) {
+        const int s = wave + it * kWinWaves;
         if (s >= act || !(accm >> s & 1u)) continue;
         const int ms = s_hd[s].x;
         for (int J = lane; J < M; J += kWave) {
@@ -3126,7 +3131,7 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
         }
     }
     // the rows are cleared for the next window
-    for (int k = t; k < act * (M + W); k += kWG) {
+    for (int k = t; k < act * (M + W); k += kWinWG) {
         const size_t at = (size_t)(r << lw) * (M + W) + k;
         p.wrowfix[at] = 0ull;
         p.wrowbuf[at] = 0.0;
@@ -3136,7 +3141,7 @@ __global__ __launch_bounds__(kWG) void mc_window_kernel(PairParams p, int begin_
     // from that proposal.  Past the staged residues the relative positions
     // after the apply are read, behind a barrier.
     if (n_items > kWinStage || n_next > kWinStage) __syncthreads();
-    for (int i = t; i < n_next; i += kWG) {
+    for (int i = t; i < n_next; i += kWinWG) {
         const int j = slot_of(ncum, i), k = i - at(ncum, j);
         const int a = slot_moving(j);
         const bool from_slot = a >= 0 && (accm >> a & 1u);
@@ -5057,7 +5062,7 @@ int windows_needed(const cgppd_ctx *ctx, long long step0, int n, int W)
 int launch_window_begin(cgppd_ctx *ctx, int n_steps)
 {
     const PairParams p = base_params(ctx);
-    hipLaunchKernelGGL(mc_window_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, n_steps);
+    hipLaunchKernelGGL(mc_window_kernel, dim3(ctx->R), dim3(kWinWG), 0, ctx->stream, p, n_steps);
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
@@ -5069,7 +5074,7 @@ int launch_window_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t
     int rc = launch_pair(ctx, ev_start, ev_stop);
     if (rc) return rc;
     const PairParams p = base_params(ctx);
-    hipLaunchKernelGGL(mc_window_kernel, dim3(ctx->R), dim3(kWG), 0, ctx->stream, p, -1);
+    hipLaunchKernelGGL(mc_window_kernel, dim3(ctx->R), dim3(kWinWG), 0, ctx->stream, p, -1);
     CG_HIP(ctx, hipGetLastError());
     return CGPPD_OK;
 }
