@@ -2902,7 +2902,8 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
     float4 *pos = p.pos + (size_t)r * p.stride;
     float4 *rel = p.rel + (size_t)r * p.stride;
     double *mat = p.emat + (size_t)r * M * M;
-    const double beta = 4184.0 / ((double)kRgas * p.temperature[r]);
+    const double temp = p.temperature[r];
+    const double beta = 4184.0 / ((double)kRgas * temp);
     // ---- level 1: the window and the next window's draws
     double e_old = 0.0;
     long long c0 = 0, c1 = 0, c2 = 0;
@@ -3052,29 +3053,44 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
         }
     }
     __syncthreads();
-    // ---- decisions in step order (thread 0), then every store
-    if (t == 0) {
+    // ---- decisions in step order (wave 0: lane l holds slot l's delta-E,
+    // its uniform and its correction terms; each acceptance adds its
+    // column of corrections to the later slots), then every store
+    if (wave == 0) {
+        double dl = 0.0, lul = 0.0, ul = 1.0;
+        double cl[kMaxWin];
+        if (lane < act) {
+            dl = s_D[lane];
+            lul = s_lu[lane];
+            ul = s_u[lane];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxWin; ++q) cl[q] = lane < act && q < lane ? s_corr[(lane << lw) + q] : 0.0;
         unsigned accm = 0u;
         double e = e_old;
         unsigned long long pairs = 0ull;
         for (int s = 0; s < act; ++s) {
-            double d = s_D[s];
-            for (int q = 0; q < s; ++q)
-                if (accm >> q & 1u) d += s_corr[(s << lw) + q];
+            const double d = __longlong_as_double(readlane_b64(__double_as_longlong(dl), s));
+            const double lu = __longlong_as_double(readlane_b64(__double_as_longlong(lul), s));
+            const double u = __longlong_as_double(readlane_b64(__double_as_longlong(ul), s));
             int kind;
             if (d < 0.0) {
                 kind = 0;
             } else {
                 const double x = -d * beta;
-                const double gap = x - s_lu[s];
-                if (s_u[s] != 0.0 && fabs(gap) > 1e-12 * (1.0 + fabs(x)))
+                const double gap = x - lu;
+                if (u != 0.0 && fabs(gap) > 1e-12 * (1.0 + fabs(x)))
                     kind = gap > 0.0 ? 1 : 2;
                 else
-                    kind = s_u[s] < exp(-d * 4184.0 / ((double)kRgas * p.temperature[r])) ? 1 : 2;
+                    kind = u < exp(-d * 4184.0 / ((double)kRgas * temp)) ? 1 : 2;
             }
             if (kind < 2) {
                 accm |= 1u << s;
                 e += d;
+                double c = 0.0;
+#pragma unroll
+                for (int q = 0; q < kMaxWin; ++q) c = q == s ? cl[q] : c;
+                dl += c;
             }
             c0 += kind == 0;
             c1 += kind == 1;
@@ -3082,8 +3098,8 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
             const unsigned long long nm = (unsigned long long)s_hd[s].z;
             pairs += nm * ((unsigned long long)N - nm);
         }
-        s_accm = accm;
-        if (act > 0) {
+        if (lane == 0) s_accm = accm;
+        if (lane == 0 && act > 0) {
             p.energy[r] = e;
             p.counters[3 * (size_t)r] = c0;
             p.counters[3 * (size_t)r + 1] = c1;
@@ -3091,8 +3107,10 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
             p.steps[r] = next;
             atomicAdd(p.pair_evals, pairs);
         }
-        p.wlen[r] = len;
-        if (begin) p.wstop[r] = stop;
+        if (lane == 0) {
+            p.wlen[r] = len;
+            if (begin) p.wstop[r] = stop;
+        }
     }
     __syncthreads();
     const unsigned accm = s_accm;
