@@ -89,9 +89,9 @@ KERNEL_NAMES = {"ilanes": "mc_pair_kernel", "jlanes": "mc_row_kernel", "jlanes4"
 
 
 def committed_traffic(kernel, workload, key="bytes_per_launch"):
-    """HBM bytes per pair-kernel launch (key bytes_per_launch) or, for the
-    resident kernel, whose one launch runs many MC steps, per MC step (key
-    bytes_per_step), from the committed PMC summary
+    """HBM bytes per pair-kernel launch (key bytes_per_launch_w<window
+    slots>) or, for the resident kernel, whose one launch runs many MC
+    steps, per MC step (key bytes_per_step), from the committed PMC summary
     profiles/<kernel>_hbm_traffic_<workload>.txt (line
     '<key> <workload> <value>'), or None."""
     path = ROOT / "profiles" / f"{kernel}_hbm_traffic_{workload}.txt"
@@ -435,7 +435,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": committed_traffic(KERNEL_NAMES[layout], spec.name,
-                                         "bytes_per_step" if layout == "resident" else "bytes_per_launch"),
+                                         "bytes_per_step" if layout == "resident"
+                                         else f"bytes_per_launch_w{e.window()}"),
             "kernel_ms": kern_ms,
             "pairs_per_launch": pairs_timed / per_unit,
             "mc_steps_per_launch": n_timed / per_unit,
