@@ -5097,13 +5097,22 @@ int launch_window_pair(cgppd_ctx *ctx, hipEvent_t ev_start = nullptr, hipEvent_t
     return CGPPD_OK;
 }
 
-// n_steps MC steps in windows from ctx->h_step (which the caller advances).
+// n_steps MC steps in windows from ctx->h_step (which the caller advances),
+// as runs of at most kWindowChunk steps: the host counts the next run's
+// windows while the device works through the previous one (a run ends
+// windows at its end, which leaves the chain unchanged).
+constexpr int kWindowChunk = 8192;
 int launch_window_steps(cgppd_ctx *ctx, int n_steps)
 {
-    if (n_steps <= 0) return CGPPD_OK;
-    const int L = windows_needed(ctx, ctx->h_step, n_steps, window_of(ctx));
-    int rc = launch_window_begin(ctx, n_steps);
-    for (int k = 0; k < L && !rc; ++k) rc = launch_window_pair(ctx);
+    const int W = window_of(ctx);
+    int rc = CGPPD_OK;
+    for (int done = 0; done < n_steps && !rc;) {
+        const int n = std::min(n_steps - done, kWindowChunk);
+        const int L = windows_needed(ctx, ctx->h_step + done, n, W);
+        rc = launch_window_begin(ctx, n);
+        for (int k = 0; k < L && !rc; ++k) rc = launch_window_pair(ctx);
+        done += n;
+    }
     return rc;
 }
 

@@ -181,3 +181,60 @@ def test_delayed_acceptance_keeps_exact_energies(gpu_engine_factory):
     for r in range(spec.n_rep):
         assert abs(s["energy"][r] - full[r]) <= 1e-6 * max(1.0, abs(full[r])), r
     e.close()
+
+
+def test_trained_surrogate_gain_over_the_method_without_it(gpu_engine_factory):
+    """What the trained network gains over delayed acceptance without it (an
+    all-zero network, which passes every proposal to the exact test): from
+    the same state and the same Philox moves, stage one rejects a share of
+    the proposals before their exact delta-E rows, so the exact rows (and
+    residue pairs) evaluated drop by that share, and the proposals that
+    reach stage two are accepted more often than plain proposals.  The
+    chain stays exact (running totals equal full evaluations).  Whether the
+    saving pays for stage one's own cost is measured by bench.py's
+    delayed-acceptance leg, not asserted here."""
+    spec = small_spec(8)
+    spec.translate_step, spec.rotate_step = 8.0, 0.8
+    e = engine_with_system(gpu_engine_factory, spec, seed=5)
+    e.surrogate_init(seed=4)
+    for _ in range(10):
+        e.mc_steps(50)
+        e.surrogate_collect()
+    e.surrogate_train(400, batch=256, lr=2e-3, seed=9)
+    trained = e.surrogate_params()
+    start = [e.coordinates(r) for r in range(spec.n_rep)]
+    st0 = e.stats()
+    ts, rs = spec.step_sizes()
+    e.close()
+    zero = {k: np.zeros_like(v) for k, v in trained.items()}
+
+    def leg(params):
+        f = gpu_engine_factory()
+        f.load_spec(spec, seed=5)
+        f.set_replicas(np.array([c for _, _, c in start]), np.array([rel for _, rel, _ in start]), st0["temperature"],
+                       ts, rs, st0["label"], seed=77)
+        f.mc_prepare()
+        f.surrogate_set_params(params)
+        f.set_delayed_acceptance(True)
+        p0 = f.pair_evaluations()
+        f.run(4, 100, use_graph=True)
+        s = f.stats()
+        out = {"rejected": int(f.stage1_rejections().sum()), "pairs": int(f.pair_evaluations() - p0),
+               "accepted": int(s["counters"][:, :2].sum()), "steps": int(s["steps"].sum())}
+        _, _, full = f.energy_all()
+        for r in range(spec.n_rep):
+            assert abs(s["energy"][r] - full[r]) <= 1e-6 * max(1.0, abs(full[r])), r
+        f.close()
+        return out
+
+    with_net, without = leg(trained), leg(zero)
+    saved = 1.0 - with_net["pairs"] / without["pairs"]
+    reach = with_net["steps"] - with_net["rejected"]
+    print(f"trained network: {with_net['rejected']} of {with_net['steps']} proposals rejected at stage one, "
+          f"exact residue pairs {with_net['pairs']} against {without['pairs']} without it ({saved:.1%} saved); "
+          f"stage-two acceptance {with_net['accepted'] / reach:.3f} against plain {without['accepted'] / without['steps']:.3f}",
+          flush=True)
+    assert without["rejected"] == 0
+    assert with_net["rejected"] > 0.05 * with_net["steps"]
+    assert with_net["pairs"] < 0.95 * without["pairs"]
+    assert with_net["accepted"] / reach > without["accepted"] / without["steps"]
