@@ -2968,26 +2968,7 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
         for (int q = 0; q <= kMaxWin; ++q) v = q == s ? c[q] : v;
         return v;
     };
-    if (t < 2 * kMaxWin) {
-        // the next slots' moves: from the stored centre (var 0), and, when
-        // this window moves the same molecule, from its proposed centre
-        const int j = t & (kMaxWin - 1), var = t / kMaxWin;
-        const int a = j < n_lay ? slot_moving(j) : -1;
-        if (j >= first && j < n_lay && (var == 0 || a >= 0)) {
-            const WinDraw &d = s_nd[j];
-            const float4 c2 = var == 0 ? p.center[(size_t)r * M + d.mol] : s_cn[a];
-            if (var == 0) s_nrad[j] = p.radius[(size_t)r * M + d.mol];
-            const Move mv = move_at(d, c2, p.box);
-            s_mv[var][j] = mv;
-            if (var == 0) s_rt[j] = rot_terms(mv);
-        }
-    } else if (t >= kWave && t < kWave + kMaxWin) {
-        // the draws after the next window (for the launch after this one),
-        // and the next window's uniforms
-        const int k = t - kWave;
-        if (k < W) p.wd[(r << lw) + k] = win_draw(p, r, rr, next + len + k);
-        if (k < len) p.wu[(r << lw) + k] = s_nd[k].u;
-    } else if (t >= kWave + kMaxWin && t < kWave + kMaxWin + act) {
+    if (t >= kWave + kMaxWin && t < kWave + kMaxWin + act) {
         const int k = t - kWave - kMaxWin;
         s_lu[k] = log(s_u[k]);
     } else if (begin && t == kWave + 2 * kMaxWin) {
@@ -3055,8 +3036,34 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
     __syncthreads();
     // ---- decisions in step order (wave 0: lane l holds slot l's delta-E,
     // its uniform and its correction terms; each acceptance adds its
-    // column of corrections to the later slots), then every store
-    if (wave == 0) {
+    // column of corrections to the later slots), while wave 2 forms the
+    // next slots' moves and wave 3 draws the steps after the next window;
+    // then every store
+    if (wave == 2) {
+        if (lane < 2 * kMaxWin) {
+            // the next slots' moves: from the stored centre (var 0), and,
+            // when this window moves the same molecule, from its proposed
+            // centre
+            const int j = lane & (kMaxWin - 1), var = lane / kMaxWin;
+            const int a = j < n_lay ? slot_moving(j) : -1;
+            if (j >= first && j < n_lay && (var == 0 || a >= 0)) {
+                const WinDraw &d = s_nd[j];
+                const float4 c2 = var == 0 ? p.center[(size_t)r * M + d.mol] : s_cn[a];
+                if (var == 0) s_nrad[j] = p.radius[(size_t)r * M + d.mol];
+                const Move mv = move_at(d, c2, p.box);
+                s_mv[var][j] = mv;
+                if (var == 0) s_rt[j] = rot_terms(mv);
+            }
+        }
+    } else if (wave == 3) {
+        if (lane < kMaxWin) {
+            // the draws after the next window (for the launch after this
+            // one), and the next window's uniforms
+            const int k = lane;
+            if (k < W) p.wd[(r << lw) + k] = win_draw(p, r, rr, next + len + k);
+            if (k < len) p.wu[(r << lw) + k] = s_nd[k].u;
+        }
+    } else if (wave == 0) {
         double dl = 0.0, lul = 0.0, ul = 1.0;
         double cl[kMaxWin];
         if (lane < act) {
@@ -5012,8 +5019,12 @@ bool resident_active(const cgppd_ctx *ctx)
 // Slots per speculative window: windows serve rigid systems on the j-lane
 // row kernel without delayed acceptance, repulsive crowding or replica
 // groups (1 elsewhere: one MC step per launch pair); the caller's choice,
-// else kAutoWindow.
+// else the autotuner's, else kAutoWindow for systems of at least
+// kAutoWindowMol molecules (with fewer, a window ends on a repeated
+// molecule after about two steps, which does not repay the decision
+// kernel).
 constexpr int kAutoWindow = 4;
+constexpr int kAutoWindowMol = 8;
 int window_of(const cgppd_ctx *ctx)
 {
     if (ctx->window == 1 || ctx->n_flex || ctx->da || ctx->rep_crowd || groups_of(ctx) != 1 || resident_active(ctx))
@@ -5023,7 +5034,7 @@ int window_of(const cgppd_ctx *ctx)
     if (ctx->window > 1) return ctx->window;
     // the autotuner's choice holds for the layout it was tuned with
     if (ctx->tuned_window > 0 && ctx->tuned_layout == layout) return ctx->tuned_window;
-    return kAutoWindow;
+    return ctx->M >= kAutoWindowMol ? kAutoWindow : 1;
 }
 
 // Window buffers for W slots (allocated and cleared outside any capture).
