@@ -2892,9 +2892,11 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
     const int lw = p.win_log2, W = 1 << lw;
     const int M = p.n_mol, N = p.n_res;
     const bool begin = begin_steps >= 0;
+    // (every load of the first level is issued before a replica that has
+    // reached its run's end leaves; none of them depends on the window's
+    // length)
     const int64_t step = p.steps[r];
     const int act = begin ? 0 : p.wlen[r];
-    if (!begin && act == 0) return;
     const int64_t stop = begin ? step + begin_steps : p.wstop[r];
     const int64_t next = step + act;
     const int rr = r + p.rep_rng;
@@ -2907,7 +2909,7 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
     // ---- level 1: the window and the next window's draws
     double e_old = 0.0;
     long long c0 = 0, c1 = 0, c2 = 0;
-    if (t == 0 && act > 0) {
+    if (t == 0) {
         e_old = p.energy[r];
         c0 = p.counters[3 * (size_t)r];
         c1 = p.counters[3 * (size_t)r + 1];
@@ -2919,7 +2921,7 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
         s_u[t] = p.wu[(r << lw) + t];
     } else if (t >= kWave && t < kWave + W) {
         const int k = t - kWave;
-        s_nd[k] = begin ? win_draw(p, r, rr, next + k) : p.wd[(r << lw) + k];
+        s_nd[k] = begin ? win_draw(p, r, rr, step + k) : p.wd[(r << lw) + k];
     }
     // rows: wave w takes slots w and w + 4, its lanes J, J + 64, ... (the
     // values stay in registers for the matrix update)
@@ -2931,9 +2933,10 @@ __global__ __launch_bounds__(kWinWG) void mc_window_kernel(PairParams p, int beg
 #pragma unroll
         for (int u = 0; u < kRowRegs; ++u) {
             const int J = lane + u * kWave;
-            rv[it][u] = s < act && J < M ? window_row(p, r, s, J) : 0.0;
+            rv[it][u] = s < W && J < M ? window_row(p, r, s, J) : 0.0;
         }
     }
+    if (!begin && act == 0) return;
     __syncthreads();
     // ---- level 2
     const int len = window_len([&](int k) { return s_nd[k].mol; }, next, stop, W);
