@@ -2,25 +2,30 @@
 //
 // Design (one GPU, HBM-resident replicas):
 //   * Topology (molecule offsets, residue types) is shared by all replicas;
-//     per replica the device holds float4 positions {x,y,z,code}, float4
+//     per replica the device holds float4 positions {x,y,z,word}, float4
 //     relative positions, float4 molecule centres, the molecule-pair energy
 //     matrix (FP64, kcal/mol), the running total, temperature, step sizes,
-//     move counters and the Philox step counter.
-//   * One persistent pair kernel serves both the full energy (rows I vs all
-//     J>I) and the MC step (row of the moved molecule vs all J).  Work items
-//     are (replica, i-block of 64 residues, j-chunk of 256 residues); a 256-
-//     thread workgroup stages the j-chunk and the 20x20 pair table in LDS,
-//     each wave owns 64 i residues (one per lane) and 64 of the j residues,
-//     and every j is read as a wave-uniform LDS broadcast.  Pair terms are
-//     FP32 with Kahan compensation per lane; each (i-block, molecule J)
-//     segment is reduced across the wave in FP64 and added atomically to the
-//     per-replica row buffer.
-//   * MC steps are decided on the device: every workgroup re-derives the
-//     replica's proposal from the counter-based Philox stream, the workgroup
-//     completing a replica's last item performs Metropolis in FP64, applies
-//     the move and updates the matrix row/column.  A sweep is one launch per
-//     MC step; an exchange interval is captured once in a hipGraph.
-//   * No MFMA: the pair sum is a table-lookup n-body reduction.
+//     move counters and the Philox step counter.  Within each molecule the
+//     device order puts charged residues first, then sorts by type, so that
+//     a molecule is a few constant-type runs.
+//   * Full energies (full_energy_kernel, intra_energy_kernel) and the MC
+//     delta-E rows share the FP32 pair terms with per-lane Kahan sums and
+//     FP64 reductions per molecule pair.
+//   * An MC step of every replica: the row kernel (mc_row_kernel, j
+//     residues in the lanes, the moved molecule's residues read as
+//     wave-uniform records; or mc_pair_kernel, i residues in the lanes, for
+//     systems with flexible molecules) evaluates each replica's pending
+//     proposal against all other molecules into fixed-point row buffers;
+//     the finish kernel (mc_finish_rigid_kernel / mc_finish_kernel) takes
+//     the Metropolis decision in FP64, applies the move and lays out the
+//     next proposal.  Rigid systems run speculative windows of up to 8
+//     steps per launch pair instead (mc_row_kernel<., ., true> and
+//     mc_window_kernel); small rigid systems run many steps per launch in
+//     one workgroup per replica (mc_resident_kernel).
+//   * Exchange intervals are captured once in a hipGraph and replayed; the
+//     autotuner picks the layout, grid and window size.
+//   * No MFMA in the pair sums (a table-lookup n-body reduction); the
+//     learned surrogate's MLP kernels (mlp.h) run on BF16 MFMA.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
