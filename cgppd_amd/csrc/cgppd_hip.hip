@@ -444,6 +444,7 @@ struct PairParams {
     float4 *center;      // R * n_mol
     float *radius;       // R * n_mol, max |relative position| of each molecule
     int fast_image;      // use the per-segment image shift where it is exact
+    float lj_reach;      // above every pair type's switch distance (2^(1/6) sigma_ij of E_ij > 0)
     // full-energy outputs (raw LJ/DH)
     double *out_lj, *out_dh;  // R
     double *out_mat;          // R * M * M (may be null)
@@ -1642,8 +1643,10 @@ __device__ __forceinline__ float image_axis(float d, float box, float inv_box, i
 }
 
 // One pair term of the row kernel: wave-uniform i record (xi, yi, zi),
-// this lane's j (xj, yj, zj); qq = q_i q_j DH for charged i runs.
-template <int IMG, bool CHARGED>
+// this lane's j (xj, yj, zj); qq = q_i q_j DH for charged i runs.  FAR: the
+// visit's pairs all lie beyond every switch distance (r^2 >= rc^2), where
+// the term is the attractive form.
+template <int IMG, bool CHARGED, bool FAR>
 __device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj, float yj, float zj,
                                           const RowCoef &cf, float qq, float box, float inv_box)
 {
@@ -1660,8 +1663,8 @@ __device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj
     }
     const float u = rinv2 * rinv2 * rinv2;
     const float att = fmaf(cf.B, u, -cf.C) * u;
-    const float rep = cf.K - att;
-    float e = r2 < cf.rc2 ? rep : att;
+    float e = att;
+    if (!FAR) e = r2 < cf.rc2 ? cf.K - att : att;
     if (CHARGED) e = fmaf(qq * __builtin_amdgcn_exp2f(r2 * rinv * kDHExp2), rinv, e);
     return e;
 }
@@ -1683,7 +1686,7 @@ struct LaneJ {
 // skipped).  Groups of 4 measured 1.3 % faster than groups of 8 on a
 // 40-chain system (2 to 5 time the same).
 constexpr int kRowGroup = 4;
-template <int IMG, bool CHARGED, int JPL>
+template <int IMG, bool CHARGED, bool FAR, int JPL>
 __device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, const LaneJ<JPL> &lj,
                                         const RowCoef (&cf)[JPL], const float (&qq)[JPL], float box, float inv_box,
                                         float (&s)[JPL], float (&c)[JPL])
@@ -1696,21 +1699,21 @@ __device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, 
         float g[JPL];
 #pragma unroll
         for (int h = 0; h < JPL; ++h)
-            g[h] = row_term<IMG, CHARGED>(r[0].x, r[0].y, r[0].z, lj.x[h], lj.y[h], lj.z[h], cf[h], qq[h], box, inv_box);
+            g[h] = row_term<IMG, CHARGED, FAR>(r[0].x, r[0].y, r[0].z, lj.x[h], lj.y[h], lj.z[h], cf[h], qq[h], box, inv_box);
 #pragma unroll
         for (int u = 1; u < kRowGroup; ++u)
             if (u < n) {
 #pragma unroll
                 for (int h = 0; h < JPL; ++h)
-                    g[h] += row_term<IMG, CHARGED>(r[u].x, r[u].y, r[u].z, lj.x[h], lj.y[h], lj.z[h], cf[h], qq[h],
-                                                   box, inv_box);
+                    g[h] += row_term<IMG, CHARGED, FAR>(r[u].x, r[u].y, r[u].z, lj.x[h], lj.y[h], lj.z[h], cf[h],
+                                                        qq[h], box, inv_box);
             }
 #pragma unroll
         for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], g[h]);
     }
 }
 
-template <int IMG, int JPL>
+template <int IMG, bool FAR, int JPL>
 __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec,
                                          const float *__restrict__ qtab, uint32_t code0, int ia, int ib,
                                          const LaneJ<JPL> &lj, const char *__restrict__ tab, float (&s)[JPL],
@@ -1730,11 +1733,11 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
             const float qi = qtab[(code & kRowMask) >> kRowShift] * p.dh_factor;
 #pragma unroll
             for (int h = 0; h < JPL; ++h) qq[h] = qi * lj.q[h];
-            row_run<IMG, true, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
+            row_run<IMG, true, FAR, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
         } else {
 #pragma unroll
             for (int h = 0; h < JPL; ++h) qq[h] = 0.0f;
-            row_run<IMG, false, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
+            row_run<IMG, false, FAR, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
         }
         a = b;
     }
@@ -1986,12 +1989,20 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         // |cm - x_j| < L/2 - rm (with margin) for every lane the plain
         // difference is the minimum image, and elsewhere
         // min(|dx|, L - |dx|) is (|dx| < L needs rm < L/2, with margin)
+        //
+        // far visit: every j residue lies beyond rm + lj_reach of cm, so
+        // every pair is past its switch distance (the pair loops then skip
+        // the repulsive branch); shifted visits that are not far take the
+        // per-pair rint path
         int img = kImgFull;
+        bool far = false;
         float sh[JPL][3];
         const float reach = rm * 1.0001f + 0.01f;
         if (p.fast_image && reach < 0.45f * p.box) {
             const float lim = 0.5f * p.box - reach;
+            const float fr = reach + p.lj_reach;
             int need = 0;
+            bool near = false;
 #pragma unroll
             for (int h = 0; h < JPL; ++h) {
                 sh[h][0] = sh[h][1] = sh[h][2] = 0.0f;
@@ -2003,8 +2014,11 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
                 sh[h][1] = p.box * ky;
                 sh[h][2] = p.box * kz;
                 need |= (fabsf(ex) < lim ? 0 : 1) | (fabsf(ey) < lim ? 0 : 2) | (fabsf(ez) < lim ? 0 : 4);
+                near |= fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= fr * fr;
             }
             img = (__any(need & 1) ? 1 : 0) | (__any(need & 2) ? 2 : 0) | (__any(need & 4) ? 4 : 0);
+            far = !__any(near);
+            if (img != 0 && !far) img = kImgFull;
         }
         const bool shift = img != kImgFull;
 #pragma unroll
@@ -2017,17 +2031,19 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
 #pragma unroll
         for (int h = 0; h < JPL; ++h) sa[h] = ca[h] = 0.0f;
         __builtin_amdgcn_s_setprio(0);
-        switch (m_crowd ? -1 : img) {
+        // (img 1-7 are far visits: see above)
+        switch (m_crowd ? -1 : img == 0 && !far ? kImgFull + 1 : img) {
         case -1: break;  // every pair of the visit is a crowder pair
-        case 0: row_runs<0, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 1: row_runs<1, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 2: row_runs<2, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 3: row_runs<3, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 4: row_runs<4, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 5: row_runs<5, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 6: row_runs<6, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 7: row_runs<7, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        default: row_runs<kImgFull, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 0: row_runs<0, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 1: row_runs<1, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 2: row_runs<2, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 3: row_runs<3, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 4: row_runs<4, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 5: row_runs<5, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 6: row_runs<6, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 7: row_runs<7, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case kImgFull + 1: row_runs<0, false, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        default: row_runs<kImgFull, false, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
         }
         if (crowd_visit) {
             // crowder pairs: the stored j coordinates (reloaded: the
@@ -4022,6 +4038,7 @@ struct cgppd_ctx {
     float4 *d_pos = nullptr, *d_rel = nullptr, *d_center = nullptr;
     float *d_radius = nullptr;
     bool fast_image = true;
+    float lj_reach = 0.0f;   // PairParams::lj_reach of the current table
     bool rep_crowd = false;  // repulsive crowding (cgppd_set_repulsive_crowding)
     bool rigid_only = false;  // cgppd_set_rigid_potential_only: no internal energy
     double *d_rowbuf = nullptr, *d_emat = nullptr, *d_energy = nullptr, *d_temp = nullptr;
@@ -4270,6 +4287,7 @@ int build_tables(cgppd_ctx *ctx)
 {
     std::vector<Coef> raw(400), scaled(400);
     std::vector<RowCoef> rowt(400);
+    float rc_max = 0.0f;
     for (int tj = 0; tj < 20; ++tj)
         for (int ti = 0; ti < 20; ++ti) {
             // src/Potential.cpp:14-17, float arithmetic as in the reference
@@ -4280,6 +4298,7 @@ int build_tables(cgppd_ctx *ctx)
             c.A = -4.0f * Eij;
             const float rc = sij * ctx->r0f;
             c.rc2 = Eij > 0.0f ? rc * rc : -1.0f;
+            if (Eij > 0.0f) rc_max = std::max(rc_max, rc);
             c.qq = ctx->charge[ti] * ctx->charge[tj];
             raw[tj * 20 + ti] = c;
             Coef s = c;
@@ -4294,6 +4313,9 @@ int build_tables(cgppd_ctx *ctx)
             w.K = -0.5f * s.A;
             rowt[tj * 20 + ti] = w;
         }
+    // a pair farther apart than this is past every switch distance (with a
+    // margin far above the FP32 rounding of r^2 and of rc^2)
+    ctx->lj_reach = rc_max * 1.001f + 0.05f;
     int rc;
     if ((rc = dev_alloc(ctx, &ctx->d_table_raw, 400))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_table_scaled, 400))) return rc;
@@ -4541,6 +4563,7 @@ PairParams base_params(cgppd_ctx *ctx)
     p.center = ctx->d_center;
     p.radius = ctx->d_radius;
     p.fast_image = ctx->fast_image ? 1 : 0;
+    p.lj_reach = ctx->lj_reach;
     p.out_lj = ctx->d_out;
     p.out_dh = ctx->d_out + ctx->R;
     p.out_mat = nullptr;

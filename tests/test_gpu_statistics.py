@@ -10,7 +10,9 @@ steps after a 20 000-step warm-up; at four temperature ranks chosen by a fixed r
 rank nearest 300 K and every fifth rank above it) the interaction energy
 and the centre-to-centre distance of the replica at that temperature are
 compared by two-sample Kolmogorov-Smirnov tests whose sample sizes are the
-effective ones, n / tau_int (Sokal's automatic window, c = 5).  Each
+effective ones (the delayed-acceptance chain, whose stage-one rejections
+can lengthen its autocorrelation at the hottest rank to about 100 samples,
+is sampled over twice as many steps), n / tau_int (Sokal's automatic window, c = 5).  Each
 comparison needs at least 100 effective samples on both sides and a
 p-value above 1e-3 (eight comparisons per pair of chains: a family-wise
 false-alarm rate below 1 %).  A comparison across temperatures shows that
@@ -86,14 +88,14 @@ class OracleChain:
         return en, di
 
 
-def sample(chain, ranks):
+def sample(chain, ranks, chunks=CHUNKS):
     chain.advance(WARMUP)
     import time
     t0 = time.perf_counter()
     E, D = [], []
-    for c in range(CHUNKS):
+    for c in range(chunks):
         if c % 2000 == 0:
-            print(f"{type(chain).__name__}: chunk {c} of {CHUNKS}, {time.perf_counter() - t0:.1f} s", flush=True)
+            print(f"{type(chain).__name__}: chunk {c} of {chunks}, {time.perf_counter() - t0:.1f} s", flush=True)
         chain.advance(CHUNK)
         en, di = chain.record(ranks)
         E.append(en)
@@ -161,7 +163,7 @@ def test_delayed_acceptance_matches_plain(system, device_plain, gpu_engine_facto
         e.surrogate_collect()
     e.surrogate_train(400, batch=1024, lr=2e-3, seed=9)
     e.set_delayed_acceptance(True)
-    da = sample(DeviceChain(e, spec), ranks)
+    da = sample(DeviceChain(e, spec), ranks, chunks=2 * CHUNKS)
     rej = e.stage1_rejections().sum()
     print(f"stage-one rejections {rej}", flush=True)
     assert rej > 0
