@@ -30,6 +30,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -108,11 +109,13 @@ struct Coef {
 // needs the fewest operations per pair: with u = 1/r^6,
 //   attractive branch  A (s^12 - s^6) = u (B u - C),  B = A sigma^12, C = A sigma^6
 //   repulsive branch   -A/2 - A (s^12 - s^6) = K - u (B u - C),  K = -A/2
-// (A = -4 E_ij RT_to_kcal).  The Debye-Hueckel coefficient q_i q_j DH is
-// formed from per-residue charges instead (one product per run and lane).
+// (A = -4 E_ij RT_to_kcal).  The repulsive branch holds where E_ij > 0 and
+// r < 2^(1/6) sigma, i.e. (sigma/r)^6 > 1/2; with A < 0 that is
+// C u + K < 0.  Pairs with E_ij <= 0 store K = FLT_MAX, for which the test
+// never holds (K itself is then unused).
 struct RowCoef {
     float B, C;
-    float rc2;   // as in Coef
+    float qq;    // q_i q_j DH factor
     float K;
 };
 
@@ -1642,18 +1645,25 @@ __device__ __forceinline__ float image_axis(float d, float box, float inv_box, i
     return d;
 }
 
-// One pair term of the row kernel: wave-uniform i record (xi, yi, zi),
-// this lane's j (xj, yj, zj); qq = q_i q_j DH for charged i runs.  FAR: the
-// visit's pairs all lie beyond every switch distance (r^2 >= rc^2), where
-// the term is the attractive form.
-template <int IMG, bool CHARGED, bool FAR>
-__device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj, float yj, float zj,
-                                          const RowCoef &cf, float qq, float box, float inv_box)
+// Squared minimum-image distance of a row-kernel pair: wave-uniform i
+// record (xi, yi, zi), this lane's j (xj, yj, zj).
+template <int IMG>
+__device__ __forceinline__ float row_r2(float xi, float yi, float zi, float xj, float yj, float zj, float box,
+                                        float inv_box)
 {
     const float dx = image_axis<IMG>(xi - xj, box, inv_box, 1);
     const float dy = image_axis<IMG>(yi - yj, box, inv_box, 2);
     const float dz = image_axis<IMG>(zi - zj, box, inv_box, 4);
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+// g plus one pair term of the row kernel at squared distance r2; the
+// Debye-Hueckel term for charged i residues only.  FAR: the visit's pairs
+// all lie beyond every switch distance, where the term is the attractive
+// form.
+template <bool CHARGED, bool FAR>
+__device__ __forceinline__ float row_term(float r2, const RowCoef &cf, float g)
+{
     float rinv = 0.0f, rinv2;
     if (CHARGED) {
         rinv = __builtin_amdgcn_rsqf(r2);
@@ -1662,84 +1672,79 @@ __device__ __forceinline__ float row_term(float xi, float yi, float zi, float xj
         rinv2 = __builtin_amdgcn_rcpf(r2);
     }
     const float u = rinv2 * rinv2 * rinv2;
-    const float att = fmaf(cf.B, u, -cf.C) * u;
-    float e = att;
-    if (!FAR) e = r2 < cf.rc2 ? cf.K - att : att;
-    if (CHARGED) e = fmaf(qq * __builtin_amdgcn_exp2f(r2 * rinv * kDHExp2), rinv, e);
+    const float w = fmaf(cf.B, u, -cf.C);
+    float e;
+    if (FAR) {
+        e = fmaf(w, u, g);
+    } else {
+        const float att = w * u;
+        e = g + (fmaf(cf.C, u, cf.K) < 0.0f ? cf.K - att : att);
+    }
+    if (CHARGED) e = fmaf(cf.qq * __builtin_amdgcn_exp2f(r2 * rinv * kDHExp2), rinv, e);
     return e;
 }
 
-// The JPL j residues a lane holds (64 apart) with their table entries and
-// Debye-Hueckel coefficients for the current run.
+// The JPL j residues a lane holds (64 apart) with their table columns.
 template <int JPL>
 struct LaneJ {
     float x[JPL], y[JPL], z[JPL];
     uint32_t col[JPL];
-    float q[JPL];
 };
 
-// A run [a, b) of i residues of one type (wave-uniform) against the lane's
-// j residues: terms summed plainly in groups of up to kRowGroup per j, group
-// sums added into Kahan-compensated accumulators.  Records are read
-// kRowGroup at a time; the proposal buffer is padded by 8 records so that a
-// short group may read past the molecule's last residue (those terms are
-// skipped).  Groups of 4 measured 1.3 % faster than groups of 8 on a
-// 40-chain system (2 to 5 time the same).
+// The i residues [ia, ib) of the proposal against the lane's j residues:
+// terms summed plainly in groups of kRowGroup consecutive i residues per j
+// (across type changes: residue types change at almost every residue of a
+// protein chain), group sums added into Kahan-compensated accumulators.
+// Each i residue's word (the record's w lane) gives its type, hence the
+// lane's table entry (for far visits its first two or three words), and
+// whether it is charged (Debye-Hueckel terms only then).  Records are read kRowGroup at a time through the scalar cache;
+// the proposal buffer is padded by 8 records so that a short group may read
+// past the molecule's last residue (those terms are skipped).
 constexpr int kRowGroup = 4;
-template <int IMG, bool CHARGED, bool FAR, int JPL>
-__device__ __forceinline__ void row_run(const float4 *__restrict__ irec, int a, int b, const LaneJ<JPL> &lj,
-                                        const RowCoef (&cf)[JPL], const float (&qq)[JPL], float box, float inv_box,
-                                        float (&s)[JPL], float (&c)[JPL])
+template <int IMG, bool FAR, int JPL>
+__device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri, const LaneJ<JPL> &lj,
+                                            const char *__restrict__ tab, float (&g)[JPL])
 {
-    for (int i = a; i < b; i += kRowGroup) {
-        f4_t r[kRowGroup];
+    const uint32_t code = __float_as_uint(ri.w);
+    RowCoef cf[JPL];
+    float r2[JPL];
 #pragma unroll
-        for (int u = 0; u < kRowGroup; ++u) r[u] = scalar_rec(irec, i + u);
-        const int n = b - i;
-        float g[JPL];
+    for (int h = 0; h < JPL; ++h) cf[h] = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[h]));
 #pragma unroll
-        for (int h = 0; h < JPL; ++h)
-            g[h] = row_term<IMG, CHARGED, FAR>(r[0].x, r[0].y, r[0].z, lj.x[h], lj.y[h], lj.z[h], cf[h], qq[h], box, inv_box);
+    for (int h = 0; h < JPL; ++h) r2[h] = row_r2<IMG>(ri.x, ri.y, ri.z, lj.x[h], lj.y[h], lj.z[h], p.box, p.inv_box);
+    // q_i q_j is read with B and C by one LDS load ahead of the charge
+    // branch (else the load sinks into the branch and the address is
+    // formed again there); its wait comes after the distances
 #pragma unroll
-        for (int u = 1; u < kRowGroup; ++u)
-            if (u < n) {
+    for (int h = 0; h < JPL; ++h) asm volatile("" : "+v"(cf[h].qq) : "v"(r2[h]));
+    if (code & kChargedBit) {
 #pragma unroll
-                for (int h = 0; h < JPL; ++h)
-                    g[h] += row_term<IMG, CHARGED, FAR>(r[u].x, r[u].y, r[u].z, lj.x[h], lj.y[h], lj.z[h], cf[h],
-                                                        qq[h], box, inv_box);
-            }
+        for (int h = 0; h < JPL; ++h) g[h] = row_term<true, FAR>(r2[h], cf[h], g[h]);
+    } else {
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], g[h]);
+        for (int h = 0; h < JPL; ++h) g[h] = row_term<false, FAR>(r2[h], cf[h], g[h]);
     }
 }
 
 template <int IMG, bool FAR, int JPL>
-__device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec,
-                                         const float *__restrict__ qtab, uint32_t code0, int ia, int ib,
+__device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__restrict__ irec, int ia, int ib,
                                          const LaneJ<JPL> &lj, const char *__restrict__ tab, float (&s)[JPL],
                                          float (&c)[JPL])
 {
-    int a = ia;
-    while (a < ib) {
-        // the run [a, b) of one type: the first record's word gives the
-        // type, the charge flag and the rest of the run
-        const uint32_t code = a == ia ? code0 : __float_as_uint(scalar_rec(irec, a).w);
-        const int b = min(ib, a + max(1, (int)(code >> kRunShift & kRunMax)));
-        RowCoef cf[JPL];
-        float qq[JPL];
+    for (int i = ia; i < ib; i += kRowGroup) {
+        f4_t r[kRowGroup];
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) cf[h] = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[h]));
-        if (code & kChargedBit) {
-            const float qi = qtab[(code & kRowMask) >> kRowShift] * p.dh_factor;
+        for (int u = 0; u < kRowGroup; ++u) r[u] = scalar_rec(irec, i + u);
+        const int n = ib - i;
+        float g[JPL];
 #pragma unroll
-            for (int h = 0; h < JPL; ++h) qq[h] = qi * lj.q[h];
-            row_run<IMG, true, FAR, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
-        } else {
+        for (int h = 0; h < JPL; ++h) g[h] = 0.0f;
+        row_residue<IMG, FAR, JPL>(p, r[0], lj, tab, g);
 #pragma unroll
-            for (int h = 0; h < JPL; ++h) qq[h] = 0.0f;
-            row_run<IMG, false, FAR, JPL>(irec, a, b, lj, cf, qq, p.box, p.inv_box, s, c);
-        }
-        a = b;
+        for (int u = 1; u < kRowGroup; ++u)
+            if (u < n) row_residue<IMG, FAR, JPL>(p, r[u], lj, tab, g);
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], g[h]);
     }
 }
 
@@ -1776,7 +1781,6 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     // at high priority, the pair loops at normal priority.
     __builtin_amdgcn_s_setprio(2);
     __shared__ __attribute__((aligned(16))) char tab[kTableBytes];
-    __shared__ float qtab[CGPPD_AA_COUNT];
     // dynamic LDS (row_lds_bytes): the proposals' unit offsets, then for at
     // most kHdrLds proposals their headers and centres
     extern __shared__ __attribute__((aligned(16))) long long row_lds[];
@@ -1791,7 +1795,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     const float4 *g_cen = WIN ? p.wcen : p.prop_center;
     // every set-up load is issued before the LDS stores: the table (entry
     // e = tj * 20 + ti, i = the moved residue, goes to row ti, column tj),
-    // the charges, and the first 64 proposals' headers and centres
+    // and the first 64 proposals' headers and centres
     static_assert(CGPPD_AA_COUNT * CGPPD_AA_COUNT <= 2 * kWG, "two table entries per thread");
     // (unconditional, so that none waits for another; only wave 0 uses
     // the headers)
@@ -1799,7 +1803,6 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     const int rr = min((int)threadIdx.x & (kWave - 1), V - 1);
     const RowCoef t0 = p.row_table[e0];
     const RowCoef t1 = p.row_table[min(e1, CGPPD_AA_COUNT * CGPPD_AA_COUNT - 1)];
-    const float q0 = p.charge[min((int)threadIdx.x, CGPPD_AA_COUNT - 1)];
     const int4 hd0 = g_hdr[rr];
     const float4 cen0 = g_cen[rr];
     const int wl0 = WIN ? p.wlen[rr >> lw] : 0;
@@ -1807,7 +1810,6 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
     const int e1c = min(e1, kEntries - 1);  // (threads past the table rewrite its last entry)
     *reinterpret_cast<RowCoef *>(tab + (((e0 % CGPPD_AA_COUNT) << kRowShift) | ((e0 / CGPPD_AA_COUNT) << kColShift))) = t0;
     *reinterpret_cast<RowCoef *>(tab + (((e1c % CGPPD_AA_COUNT) << kRowShift) | ((e1c / CGPPD_AA_COUNT) << kColShift))) = t1;
-    qtab[min((int)threadIdx.x, CGPPD_AA_COUNT - 1)] = q0;  // (likewise)
     // unit offsets of the proposals: an inclusive scan, 64 proposals at a
     // time, from the proposal headers (the first 64 loaded above).  Every
     // wave scans (the loads then have an unconditional use and are not
@@ -1911,9 +1913,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
             ia = __builtin_amdgcn_readfirstlane((int)(local % (unsigned)n_m));
         }
         const int ib = __builtin_amdgcn_readfirstlane((int)min((long long)n_m, (long long)ia + (P1 - P)));
-        // the first i record's word (its run), requested before the j loads
         const float4 *irec = WIN ? slot_pos(p, r, slot) : p.prop_pos + (size_t)r * p.max_n;
-        const uint32_t code0 = __float_as_uint(scalar_rec(irec, ia).w);
         // this lane's j residues (jb * 64 JPL + 64 h + lane, h < JPL): the
         // stored residues outside m, then (window slots) the earlier
         // slots' proposed residues
@@ -1969,7 +1969,6 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
                 J[h] = 0x7fffffff;
             }
             lj.col[h] = lane_col(rj[h]);
-            lj.q[h] = qtab[lj.col[h] >> kColShift];
         }
         // repulsive crowding: a visit pairing a crowder with anything (the
         // moved molecule is one, or some lane's molecule is) adds the
@@ -2034,16 +2033,16 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         // (img 1-7 are far visits: see above)
         switch (m_crowd ? -1 : img == 0 && !far ? kImgFull + 1 : img) {
         case -1: break;  // every pair of the visit is a crowder pair
-        case 0: row_runs<0, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 1: row_runs<1, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 2: row_runs<2, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 3: row_runs<3, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 4: row_runs<4, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 5: row_runs<5, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 6: row_runs<6, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case 7: row_runs<7, true, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        case kImgFull + 1: row_runs<0, false, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
-        default: row_runs<kImgFull, false, JPL>(p, irec, qtab, code0, ia, ib, lj, tab, sa, ca); break;
+        case 0: row_runs<0, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 1: row_runs<1, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 2: row_runs<2, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 3: row_runs<3, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 4: row_runs<4, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 5: row_runs<5, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 6: row_runs<6, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 7: row_runs<7, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case kImgFull + 1: row_runs<0, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        default: row_runs<kImgFull, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
         }
         if (crowd_visit) {
             // crowder pairs: the stored j coordinates (reloaded: the
@@ -4309,8 +4308,8 @@ int build_tables(cgppd_ctx *ctx)
             const float s6 = c.sig2 * c.sig2 * c.sig2;
             w.B = s.A * (s6 * s6);
             w.C = s.A * s6;
-            w.rc2 = c.rc2;
-            w.K = -0.5f * s.A;
+            w.qq = s.qq;
+            w.K = Eij > 0.0f ? -0.5f * s.A : FLT_MAX;
             rowt[tj * 20 + ti] = w;
         }
     // a pair farther apart than this is past every switch distance (with a
