@@ -1703,7 +1703,7 @@ struct LaneJ {
 constexpr int kRowGroup = 4;
 template <int IMG, bool FAR, int JPL>
 __device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri, const LaneJ<JPL> &lj,
-                                            const char *__restrict__ tab, float (&g)[JPL])
+                                            const char *__restrict__ tab, float (&gu)[JPL], float (&gc)[JPL])
 {
     const uint32_t code = __float_as_uint(ri.w);
     RowCoef cf[JPL];
@@ -1717,12 +1717,14 @@ __device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri,
     // formed again there); its wait comes after the distances
 #pragma unroll
     for (int h = 0; h < JPL; ++h) asm volatile("" : "+v"(cf[h].qq) : "v"(r2[h]));
+    // (each branch adds into an accumulator of its own, which it updates
+    // in place: one shared accumulator costs register copies at the join)
     if (code & kChargedBit) {
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) g[h] = row_term<true, FAR>(r2[h], cf[h], g[h]);
+        for (int h = 0; h < JPL; ++h) gc[h] = row_term<true, FAR>(r2[h], cf[h], gc[h]);
     } else {
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) g[h] = row_term<false, FAR>(r2[h], cf[h], g[h]);
+        for (int h = 0; h < JPL; ++h) gu[h] = row_term<false, FAR>(r2[h], cf[h], gu[h]);
     }
 }
 
@@ -1736,15 +1738,15 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
 #pragma unroll
         for (int u = 0; u < kRowGroup; ++u) r[u] = scalar_rec(irec, i + u);
         const int n = ib - i;
-        float g[JPL];
+        float gu[JPL], gc[JPL];
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) g[h] = 0.0f;
-        row_residue<IMG, FAR, JPL>(p, r[0], lj, tab, g);
+        for (int h = 0; h < JPL; ++h) gu[h] = gc[h] = 0.0f;
+        row_residue<IMG, FAR, JPL>(p, r[0], lj, tab, gu, gc);
 #pragma unroll
         for (int u = 1; u < kRowGroup; ++u)
-            if (u < n) row_residue<IMG, FAR, JPL>(p, r[u], lj, tab, g);
+            if (u < n) row_residue<IMG, FAR, JPL>(p, r[u], lj, tab, gu, gc);
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], g[h]);
+        for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], gu[h] + gc[h]);
     }
 }
 
