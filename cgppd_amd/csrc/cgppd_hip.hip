@@ -1618,42 +1618,57 @@ __global__ __launch_bounds__(kWG) void mc_pair_kernel(PairParams p)
 // proposal, read as a wave-uniform record through the scalar cache.  The
 // units of all replicas are laid end to end (replica, j-block, i) and every
 // wave takes an equal contiguous share, so the grid is balanced to within
-// one unit and waves never wait for one another.  Residue i's type is
-// uniform along m's constant-type runs, so each lane loads its table entry
-// once per run; uncharged runs skip the Debye-Hueckel term.  Per lane the
-// terms are summed in groups of four into a Kahan-compensated FP32
+// one unit and waves never wait for one another.  Each lane loads its
+// table entry per i residue (residue types change at almost every residue);
+// uncharged i residues skip the Debye-Hueckel term.  Per lane the terms are
+// summed in groups of kRowGroup i residues into a Kahan-compensated FP32
 // accumulator; at the end of a j-block visit the lanes of each molecule J
 // are reduced in FP64 and added atomically into the row.
 // ---------------------------------------------------------------------------
 
-// Image handling of the row kernel, wave-uniform per j-block visit.  The
-// lanes' j coordinates are shifted by their molecule's centre image (the
-// multiple of L nearest to the moved molecule's centre), so that on every
-// axis |dx| < L.  On an axis where some lane's molecule pair is not
-// confined to one image (the bounding-sphere test of segment_shift), the
-// per-pair minimum image is min(|dx|, L - |dx|): the same value as
-// dx - L rint(dx / L) for |dx| < L, in two operations.  kImgFull: the
-// bounding spheres are too large for |dx| < L; unshifted coordinates and
-// the per-pair rint on every axis.
+// Image handling of the row kernel, wave-uniform per j-block visit.  Each
+// lane's j residue is moved to its image nearest the proposal's centre cm:
+// e = cm - x_j with |e| <= L/2 on every axis, and every pair difference
+// d = x_i - x_j = (x_i - cm) + e lies within rm of e.  On an axis where
+// |e| < L/2 - rm for every lane, d is the minimum image.  Elsewhere (IMG
+// bit set) a lane with |e| >= L/2 - rm ("straddling": its pairs may cross
+// L/2) stores x_j + sign(e) L/2 and c = L/2, the others x_j and c = 0; the
+// pair loop forms t = x_i - (stored x_j) and c - |t|, whose square is the
+// square of the minimum image in one operation: for a straddling lane,
+// t = d - sign(e) L/2 with sign(e) d in (-L/2, L) (rm < L/2), where
+// L/2 - |t| is d on the near side and L - |d| on the far side; for the
+// others c - |t| = -|d|.  The straddling lanes' pairs lie at least
+// L/2 - 2 rm apart, where the rounding of the half-box shift is
+// immaterial.  kImgFull: rm too large for these bounds; unshifted
+// coordinates and the per-pair rint on every axis.
 constexpr int kImgFull = 8;
 
 template <int IMG>
-__device__ __forceinline__ float image_axis(float d, float box, float inv_box, int axis_bit)
+__device__ __forceinline__ float image_axis(float d, float c, float box, float inv_box, int axis_bit)
 {
     if (IMG == kImgFull) return fmaf(-box, rintf(d * inv_box), d);
-    if (IMG & axis_bit) return fminf(fabsf(d), box - fabsf(d));
+    if (IMG & axis_bit) return c - fabsf(d);
     return d;
 }
 
+// The JPL j residues a lane holds (64 apart) with their table columns and
+// the image constants of straddling lanes (see image_axis).
+template <int JPL>
+struct LaneJ {
+    float x[JPL], y[JPL], z[JPL];
+    float cx[JPL], cy[JPL], cz[JPL];
+    uint32_t col[JPL];
+};
+
 // Squared minimum-image distance of a row-kernel pair: wave-uniform i
-// record (xi, yi, zi), this lane's j (xj, yj, zj).
-template <int IMG>
-__device__ __forceinline__ float row_r2(float xi, float yi, float zi, float xj, float yj, float zj, float box,
+// record (xi, yi, zi), the lane's j residue h.
+template <int IMG, int JPL>
+__device__ __forceinline__ float row_r2(float xi, float yi, float zi, const LaneJ<JPL> &lj, int h, float box,
                                         float inv_box)
 {
-    const float dx = image_axis<IMG>(xi - xj, box, inv_box, 1);
-    const float dy = image_axis<IMG>(yi - yj, box, inv_box, 2);
-    const float dz = image_axis<IMG>(zi - zj, box, inv_box, 4);
+    const float dx = image_axis<IMG>(xi - lj.x[h], lj.cx[h], box, inv_box, 1);
+    const float dy = image_axis<IMG>(yi - lj.y[h], lj.cy[h], box, inv_box, 2);
+    const float dz = image_axis<IMG>(zi - lj.z[h], lj.cz[h], box, inv_box, 4);
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
 }
 
@@ -1684,13 +1699,6 @@ __device__ __forceinline__ float row_term(float r2, const RowCoef &cf, float g)
     return e;
 }
 
-// The JPL j residues a lane holds (64 apart) with their table columns.
-template <int JPL>
-struct LaneJ {
-    float x[JPL], y[JPL], z[JPL];
-    uint32_t col[JPL];
-};
-
 // The i residues [ia, ib) of the proposal against the lane's j residues:
 // terms summed plainly in groups of kRowGroup consecutive i residues per j
 // (across type changes: residue types change at almost every residue of a
@@ -1713,7 +1721,7 @@ __device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri,
 #pragma unroll
     for (int h = 0; h < JPL; ++h) cf[h] = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[h]));
 #pragma unroll
-    for (int h = 0; h < JPL; ++h) r2[h] = row_r2<IMG>(ri.x, ri.y, ri.z, lj.x[h], lj.y[h], lj.z[h], p.box, p.inv_box);
+    for (int h = 0; h < JPL; ++h) r2[h] = row_r2<IMG, JPL>(ri.x, ri.y, ri.z, lj, h, p.box, p.inv_box);
     // q_i q_j is read with B and C by one LDS load ahead of the charge
     // branch (else the load sinks into the branch and the address is
     // formed again there); its wait comes after the distances
@@ -1999,25 +2007,33 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         // per-pair rint path
         int img = kImgFull;
         bool far = false;
-        float sh[JPL][3];
+        // per lane and axis: the image shift, the straddling half-box
+        // shift and the image constant c
+        float sh[JPL][3], hs[JPL][3], hc[JPL][3];
         const float reach = rm * 1.0001f + 0.01f;
         if (p.fast_image && reach < 0.45f * p.box) {
-            const float lim = 0.5f * p.box - reach;
+            const float lim = 0.5f * p.box - reach, half = 0.5f * p.box;
             const float fr = reach + p.lj_reach;
             int need = 0;
             bool near = false;
 #pragma unroll
             for (int h = 0; h < JPL; ++h) {
-                sh[h][0] = sh[h][1] = sh[h][2] = 0.0f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) sh[h][a] = hs[h][a] = hc[h][a] = 0.0f;
                 if (!valid[h]) continue;
-                const float ox = cm.x - rj[h].x, oy = cm.y - rj[h].y, oz = cm.z - rj[h].z;
-                const float kx = rintf(ox * p.inv_box), ky = rintf(oy * p.inv_box), kz = rintf(oz * p.inv_box);
-                const float ex = fmaf(-p.box, kx, ox), ey = fmaf(-p.box, ky, oy), ez = fmaf(-p.box, kz, oz);
-                sh[h][0] = p.box * kx;
-                sh[h][1] = p.box * ky;
-                sh[h][2] = p.box * kz;
-                need |= (fabsf(ex) < lim ? 0 : 1) | (fabsf(ey) < lim ? 0 : 2) | (fabsf(ez) < lim ? 0 : 4);
-                near |= fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= fr * fr;
+                const float o[3] = {cm.x - rj[h].x, cm.y - rj[h].y, cm.z - rj[h].z};
+                float e[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float k = rintf(o[a] * p.inv_box);
+                    e[a] = fmaf(-p.box, k, o[a]);
+                    sh[h][a] = p.box * k;
+                    const bool st = !(fabsf(e[a]) < lim);
+                    need |= st ? 1 << a : 0;
+                    hs[h][a] = st ? copysignf(half, e[a]) : 0.0f;
+                    hc[h][a] = st ? half : 0.0f;
+                }
+                near |= fmaf(e[0], e[0], fmaf(e[1], e[1], e[2] * e[2])) <= fr * fr;
             }
             img = (__any(need & 1) ? 1 : 0) | (__any(need & 2) ? 2 : 0) | (__any(need & 4) ? 4 : 0);
             far = !__any(near);
@@ -2026,9 +2042,12 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4
         const bool shift = img != kImgFull;
 #pragma unroll
         for (int h = 0; h < JPL; ++h) {
-            lj.x[h] = shift ? rj[h].x + sh[h][0] : rj[h].x;
-            lj.y[h] = shift ? rj[h].y + sh[h][1] : rj[h].y;
-            lj.z[h] = shift ? rj[h].z + sh[h][2] : rj[h].z;
+            lj.x[h] = shift ? rj[h].x + sh[h][0] + hs[h][0] : rj[h].x;
+            lj.y[h] = shift ? rj[h].y + sh[h][1] + hs[h][1] : rj[h].y;
+            lj.z[h] = shift ? rj[h].z + sh[h][2] + hs[h][2] : rj[h].z;
+            lj.cx[h] = shift ? hc[h][0] : 0.0f;
+            lj.cy[h] = shift ? hc[h][1] : 0.0f;
+            lj.cz[h] = shift ? hc[h][2] : 0.0f;
         }
         float sa[JPL], ca[JPL];
 #pragma unroll
