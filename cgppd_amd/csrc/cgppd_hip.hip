@@ -549,11 +549,14 @@ __device__ __forceinline__ double wave_sum(double v)
 // 64-bit DPP move: lane l receives the value of the lane the GFX9 control
 // CTRL names (row_shr:n = 0x110 + n, row_bcast:15 = 0x142, row_bcast:31 =
 // 0x143), or 0 where that lane is out of its row or ROW_MASK disables l's row.
+// With every row enabled the bound control supplies the 0 (no zeroing move
+// before each DPP move); a disabled row keeps the old value, 0.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ long long dpp_b64(long long v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROW_MASK, 0xf, false);
+    constexpr bool all_rows = ROW_MASK == 0xf;
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, all_rows);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROW_MASK, 0xf, all_rows);
     return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
@@ -1782,8 +1785,10 @@ constexpr int kHdrLds = 128;
 // the R W window slots v = r W + s, each evaluated against replica r's
 // stored residues outside its molecule and against the proposals of the
 // slots before it (J = M + q in the window row).
+// (the crowder instantiations hold the crowder pass's state as well: below
+// 8 waves per SIMD rather than spill)
 template <int JPL, bool CROWD, bool WIN>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 ? 4 : 8, 8))) void mc_row_kernel(PairParams p)
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 || CROWD ? 4 : 8, 8))) void mc_row_kernel(PairParams p)
 {
     static_assert(!(CROWD && WIN), "windows run without repulsive crowding");
     // Wave priority: the SIMDs issue from the oldest wave first, so a wave
