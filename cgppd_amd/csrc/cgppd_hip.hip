@@ -2008,8 +2008,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 || 
         //
         // far visit: every j residue lies beyond rm + lj_reach of cm, so
         // every pair is past its switch distance (the pair loops then skip
-        // the repulsive branch); shifted visits that are not far take the
-        // per-pair rint path
+        // the repulsive branch)
         int img = kImgFull;
         bool far = false;
         // per lane and axis: the image shift, the straddling half-box
@@ -2042,7 +2041,6 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 || 
             }
             img = (__any(need & 1) ? 1 : 0) | (__any(need & 2) ? 2 : 0) | (__any(need & 4) ? 4 : 0);
             far = !__any(near);
-            if (img != 0 && !far) img = kImgFull;
         }
         const bool shift = img != kImgFull;
 #pragma unroll
@@ -2058,8 +2056,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 || 
 #pragma unroll
         for (int h = 0; h < JPL; ++h) sa[h] = ca[h] = 0.0f;
         __builtin_amdgcn_s_setprio(0);
-        // (img 1-7 are far visits: see above)
-        switch (m_crowd ? -1 : img == 0 && !far ? kImgFull + 1 : img) {
+        // far visits: cases 0-7 (the image axes); near visits: 16 + axes
+        switch (m_crowd ? -1 : img == kImgFull ? kImgFull : far ? img : 16 + img) {
         case -1: break;  // every pair of the visit is a crowder pair
         case 0: row_runs<0, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
         case 1: row_runs<1, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
@@ -2069,7 +2067,14 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 || 
         case 5: row_runs<5, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
         case 6: row_runs<6, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
         case 7: row_runs<7, true, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
-        case kImgFull + 1: row_runs<0, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 16: row_runs<0, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 17: row_runs<1, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 18: row_runs<2, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 19: row_runs<3, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 20: row_runs<4, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 21: row_runs<5, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 22: row_runs<6, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
+        case 23: row_runs<7, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
         default: row_runs<kImgFull, false, JPL>(p, irec, ia, ib, lj, tab, sa, ca); break;
         }
         if (crowd_visit) {
