@@ -1716,20 +1716,22 @@ __device__ __forceinline__ float row_term(float r2, const RowCoef &cf, float g)
 constexpr int kRowGroup = 8;
 template <int IMG, bool FAR, int JPL>
 __device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri, const LaneJ<JPL> &lj,
-                                            const char *__restrict__ tab, float (&gu)[JPL], float (&gc)[JPL])
+                                            const char *__restrict__ tab, uint32_t &cur, RowCoef (&cf)[JPL],
+                                            float (&gu)[JPL], float (&gc)[JPL])
 {
     const uint32_t code = __float_as_uint(ri.w);
-    RowCoef cf[JPL];
+    // the lane's table entries change with the residue type only (each
+    // molecule's residues are sorted by charge and type: runs of about 8
+    // residues); the loads are issued before the distances
+    const bool fresh = (code & kRowMask) != cur;
+    if (fresh) {
+        cur = code & kRowMask;
+#pragma unroll
+        for (int h = 0; h < JPL; ++h) cf[h] = *reinterpret_cast<const RowCoef *>(tab + (cur | lj.col[h]));
+    }
     float r2[JPL];
 #pragma unroll
-    for (int h = 0; h < JPL; ++h) cf[h] = *reinterpret_cast<const RowCoef *>(tab + ((code & kRowMask) | lj.col[h]));
-#pragma unroll
     for (int h = 0; h < JPL; ++h) r2[h] = row_r2<IMG, JPL>(ri.x, ri.y, ri.z, lj, h, p.box, p.inv_box);
-    // q_i q_j is read with B and C by one LDS load ahead of the charge
-    // branch (else the load sinks into the branch and the address is
-    // formed again there); its wait comes after the distances
-#pragma unroll
-    for (int h = 0; h < JPL; ++h) asm volatile("" : "+v"(cf[h].qq) : "v"(r2[h]));
     // (each branch adds into an accumulator of its own, which it updates
     // in place: one shared accumulator costs register copies at the join)
     if (code & kChargedBit) {
@@ -1746,6 +1748,8 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
                                          const LaneJ<JPL> &lj, const char *__restrict__ tab, float (&s)[JPL],
                                          float (&c)[JPL])
 {
+    uint32_t cur = ~0u;  // no residue type yet
+    RowCoef cf[JPL];
     for (int i = ia; i < ib; i += kRowGroup) {
         f4_t r[kRowGroup];
 #pragma unroll
@@ -1754,10 +1758,10 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
         float gu[JPL], gc[JPL];
 #pragma unroll
         for (int h = 0; h < JPL; ++h) gu[h] = gc[h] = 0.0f;
-        row_residue<IMG, FAR, JPL>(p, r[0], lj, tab, gu, gc);
+        row_residue<IMG, FAR, JPL>(p, r[0], lj, tab, cur, cf, gu, gc);
 #pragma unroll
         for (int u = 1; u < kRowGroup; ++u)
-            if (u < n) row_residue<IMG, FAR, JPL>(p, r[u], lj, tab, gu, gc);
+            if (u < n) row_residue<IMG, FAR, JPL>(p, r[u], lj, tab, cur, cf, gu, gc);
 #pragma unroll
         for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], gu[h] + gc[h]);
     }
