@@ -1703,17 +1703,19 @@ __device__ __forceinline__ float row_term(float r2, const RowCoef &cf, float g)
 }
 
 // The i residues [ia, ib) of the proposal against the lane's j residues:
-// terms summed plainly in groups of kRowGroup consecutive i residues per j
-// (across type changes: residue types change at almost every residue of a
-// protein chain), group sums added into Kahan-compensated accumulators.
-// Each i residue's word (the record's w lane) gives its type, hence the
-// lane's table entry (for far visits its first two or three words), and
-// whether it is charged (Debye-Hueckel terms only then).  Records are read
-// kRowGroup at a time through the scalar cache; the proposal buffers end
-// with 8 spare records, so that a short group may read past a molecule's
-// last residue (those terms are skipped).  Groups of 8 measured 2.7 %
-// faster than groups of 4 on a 40-chain system.
+// terms summed plainly per j over kRowFlush groups of kRowGroup consecutive
+// i residues (across type changes), the plain sums added into
+// Kahan-compensated accumulators.  Each i residue's word (the record's w
+// lane) gives its type, hence the lane's table entry (for far visits its
+// first two or three words), and whether it is charged (Debye-Hueckel
+// terms only then).  Records are read kRowGroup at a time through the
+// scalar cache; the proposal buffers end with 8 spare records, so that a
+// short group may read past a molecule's last residue (those terms are
+// skipped).  Groups of 8 measured 2.7 % faster than groups of 4 on a
+// 40-chain system, and one compensated add per two groups 0.7 % faster
+// than one per group.
 constexpr int kRowGroup = 8;
+constexpr int kRowFlush = 2;
 template <int IMG, bool FAR, int JPL>
 __device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri, const LaneJ<JPL> &lj,
                                             const char *__restrict__ tab, uint32_t &cur, RowCoef (&cf)[JPL],
@@ -1750,20 +1752,29 @@ __device__ __forceinline__ void row_runs(const PairParams &p, const float4 *__re
 {
     uint32_t cur = ~0u;  // no residue type yet
     RowCoef cf[JPL];
+    float gu[JPL], gc[JPL];
+#pragma unroll
+    for (int h = 0; h < JPL; ++h) gu[h] = gc[h] = 0.0f;
+    int k = 0;
     for (int i = ia; i < ib; i += kRowGroup) {
         f4_t r[kRowGroup];
 #pragma unroll
         for (int u = 0; u < kRowGroup; ++u) r[u] = scalar_rec(irec, i + u);
         const int n = ib - i;
-        float gu[JPL], gc[JPL];
-#pragma unroll
-        for (int h = 0; h < JPL; ++h) gu[h] = gc[h] = 0.0f;
         row_residue<IMG, FAR, JPL>(p, r[0], lj, tab, cur, cf, gu, gc);
 #pragma unroll
         for (int u = 1; u < kRowGroup; ++u)
             if (u < n) row_residue<IMG, FAR, JPL>(p, r[u], lj, tab, cur, cf, gu, gc);
+        // the plain sums enter the compensated ones every kRowFlush groups
+        // and after the last
+        if (++k == kRowFlush || n <= kRowGroup) {
 #pragma unroll
-        for (int h = 0; h < JPL; ++h) kahan_add(s[h], c[h], gu[h] + gc[h]);
+            for (int h = 0; h < JPL; ++h) {
+                kahan_add(s[h], c[h], gu[h] + gc[h]);
+                gu[h] = gc[h] = 0.0f;
+            }
+            k = 0;
+        }
     }
 }
 
