@@ -448,6 +448,8 @@ struct PairParams {
     float *radius;       // R * n_mol, max |relative position| of each molecule
     int fast_image;      // use the per-segment image shift where it is exact
     float lj_reach;      // above every pair type's switch distance (2^(1/6) sigma_ij of E_ij > 0)
+    float half_box;      // box / 2 and 0.45 box (kernel arguments, so that the row kernel keeps
+    float image_reach;   // them in scalar registers)
     // full-energy outputs (raw LJ/DH)
     double *out_lj, *out_dh;  // R
     double *out_mat;          // R * M * M (may be null)
@@ -2030,8 +2032,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(JPL > 2 || 
         // shift and the image constant c
         float sh[JPL][3], hs[JPL][3], hc[JPL][3];
         const float reach = rm * 1.0001f + 0.01f;
-        if (p.fast_image && reach < 0.45f * p.box) {
-            const float lim = 0.5f * p.box - reach, half = 0.5f * p.box;
+        if (p.fast_image && reach < p.image_reach) {
+            const float lim = p.half_box - reach, half = p.half_box;
             const float fr = reach + p.lj_reach;
             int need = 0;
             bool near = false;
@@ -4605,6 +4607,8 @@ PairParams base_params(cgppd_ctx *ctx)
     p.table_scaled = ctx->d_table_scaled;
     p.box = ctx->box;
     p.inv_box = 1.0f / ctx->box;
+    p.half_box = 0.5f * ctx->box;
+    p.image_reach = 0.45f * ctx->box;
     p.pos = ctx->d_pos;
     p.rel = ctx->d_rel;
     p.center = ctx->d_center;
