@@ -1718,6 +1718,8 @@ __device__ __forceinline__ float row_term(float r2, const RowCoef &cf, float g)
 // than one per group.
 constexpr int kRowGroup = 8;
 constexpr int kRowFlush = 2;
+// (a group starting at the last residue reads kRowGroup - 1 records past it)
+static_assert(kRowGroup - 1 <= 8, "the proposal buffers end with 8 spare records");
 template <int IMG, bool FAR, int JPL>
 __device__ __forceinline__ void row_residue(const PairParams &p, const f4_t &ri, const LaneJ<JPL> &lj,
                                             const char *__restrict__ tab, uint32_t &cur, RowCoef (&cf)[JPL],
