@@ -410,7 +410,7 @@ def main():
             "exchange_every": per,
             "moves": ("translate 2 A / rotate 0.2 rad / flex / local translate + crankshaft"
                       if getattr(spec, "flexible", False) else "rigid translate 2 A / rotate 0.2 rad"),
-            "pair_terms": "fp32, Kahan over 8-term groups per lane, fp64 row reduction",
+            "pair_terms": "fp32, plain sums of 16 terms into Kahan sums per lane, fp64 row reduction",
             "flexible": bool(getattr(spec, "flexible", False)),
             "image_shift": not args.no_image_shift,
             "pair_layout": layout,
