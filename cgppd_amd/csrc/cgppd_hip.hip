@@ -5203,8 +5203,9 @@ int launch_window_steps(cgppd_ctx *ctx, int n_steps)
     int rc = CGPPD_OK;
     for (int done = 0; done < n_steps && !rc;) {
         const int n = std::min(n_steps - done, kWindowChunk);
-        const int L = windows_needed(ctx, ctx->h_step + done, n, W);
         rc = launch_window_begin(ctx, n);
+        // (the host replays the draws while the start kernel runs)
+        const int L = rc ? 0 : windows_needed(ctx, ctx->h_step + done, n, W);
         for (int k = 0; k < L && !rc; ++k) rc = launch_window_pair(ctx);
         done += n;
     }
@@ -6073,8 +6074,10 @@ static int run_windows_graph(cgppd_ctx *ctx, int n_intervals, int n, int W)
         ctx->graph_L = L;
     }
     for (int k = 0; k < n_intervals; ++k) {
-        const int need = windows_needed(ctx, ctx->h_step, n, W);
+        // the graph's launches do not depend on the interval's window count:
+        // the host replays the draws while the graph runs
         CG_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        const int need = windows_needed(ctx, ctx->h_step, n, W);
         for (int e = ctx->graph_L; e < need; ++e)
             if ((rc = launch_window_pair(ctx))) return rc;
         if ((rc = launch_exchange(ctx))) return rc;
